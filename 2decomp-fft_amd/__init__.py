@@ -1,0 +1,312 @@
+"""Python binding of libd2dhip, the gfx950 FFT backend of the 2DECOMP&FFT
+single-GPU 3D FFT path.
+
+The C ABI (include/d2dhip.h) is the product; this module only binds it with
+ctypes so that tests and benchmarks can drive it.  Device memory comes from
+PyTorch tensors (their data_ptr) and work is queued on PyTorch's current
+stream.  Nothing here computes a transform on the host: if libd2dhip.so is
+missing the import of the binding fails.
+
+Names follow the reference's Fortran API (src/fft_common.f90:31-53,
+src/decomp_2d.f90:98-309): ``decomp_2d_fft_init`` returns a plan whose
+``fft_3d`` methods mirror ``decomp_2d_fft_3d``; ``transpose_*`` and the pack
+entry points mirror the reference's transposes and mem_split / mem_merge.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libd2dhip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "d2dhip.h")
+
+PHYSICAL_IN_X = 1
+PHYSICAL_IN_Z = 3
+FORWARD = -1
+BACKWARD = 1
+PREC_F64 = 64
+PREC_F32 = 32
+PREC_SPLIT_BF16 = 16
+FLAG_INPLACE = 0x1
+FLAG_SKIP_X = 0x2
+FLAG_SKIP_Y = 0x4
+FLAG_SKIP_Z = 0x8
+R8, C16, R4, C8 = 0, 1, 2, 3
+X_TO_Y, Y_TO_Z, Z_TO_Y, Y_TO_X = 0, 1, 2, 3
+SUCCESS = 0
+ERR_INVALID_VALUE = 1
+ERR_UNSUPPORTED = 2
+ERR_ALLOC = 3
+ERR_HIP = 4
+ERR_NULL = 5
+ERR_GRID = 6
+WARN_SKIP_IGNORED = 100
+MAX_PROCS = 64
+
+_lib = None
+
+
+class D2dError(RuntimeError):
+    """A non-zero status of the C ABI (the reference would decomp_2d_abort)."""
+
+    def __init__(self, status, message):
+        super().__init__("d2dhip status %d (%s): %s" % (status, status_string(status), message))
+        self.status = status
+
+
+class D2dWarning(UserWarning):
+    pass
+
+
+class Decomp(ctypes.Structure):
+    """Mirror of d2dhip_decomp (DECOMP_INFO, src/info.f90:19-47)."""
+    _fields_ = [
+        ("xst", ctypes.c_int * 3), ("xen", ctypes.c_int * 3), ("xsz", ctypes.c_int * 3),
+        ("yst", ctypes.c_int * 3), ("yen", ctypes.c_int * 3), ("ysz", ctypes.c_int * 3),
+        ("zst", ctypes.c_int * 3), ("zen", ctypes.c_int * 3), ("zsz", ctypes.c_int * 3),
+        ("p_row", ctypes.c_int), ("p_col", ctypes.c_int),
+        ("coord_row", ctypes.c_int), ("coord_col", ctypes.c_int),
+        ("x1dist", ctypes.c_int * MAX_PROCS), ("y1dist", ctypes.c_int * MAX_PROCS),
+        ("y2dist", ctypes.c_int * MAX_PROCS), ("z2dist", ctypes.c_int * MAX_PROCS),
+        ("x1cnts", ctypes.c_longlong * MAX_PROCS), ("y1cnts", ctypes.c_longlong * MAX_PROCS),
+        ("x1disp", ctypes.c_longlong * MAX_PROCS), ("y1disp", ctypes.c_longlong * MAX_PROCS),
+        ("y2cnts", ctypes.c_longlong * MAX_PROCS), ("z2cnts", ctypes.c_longlong * MAX_PROCS),
+        ("y2disp", ctypes.c_longlong * MAX_PROCS), ("z2disp", ctypes.c_longlong * MAX_PROCS),
+        ("x1count", ctypes.c_longlong), ("y1count", ctypes.c_longlong),
+        ("y2count", ctypes.c_longlong), ("z2count", ctypes.c_longlong),
+        ("even", ctypes.c_int),
+    ]
+
+
+class Modes(ctypes.Structure):
+    _fields_ = [
+        ("nmodes", ctypes.c_int),
+        ("kx", ctypes.c_int * 64), ("ky", ctypes.c_int * 64), ("kz", ctypes.c_int * 64),
+        ("amp", ctypes.c_double * 64),
+        ("sigma", ctypes.c_double),
+        ("noise_seed", ctypes.c_ulonglong),
+    ]
+
+
+def lib():
+    """Loads libd2dhip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libd2dhip.so is not built: run `python 2decomp-fft_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ci, cu, cll = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_longlong
+        pl = ctypes.c_void_p
+        sig = {
+            "d2dhip_status_string": ([ci], ctypes.c_char_p),
+            "d2dhip_last_message": ([], ctypes.c_char_p),
+            "d2dhip_version": ([], ci),
+            "d2dhip_decomp_info_init": ([ci, ci, ci, ci, ci, ci, ci, ctypes.POINTER(Decomp)], ci),
+            "d2dhip_best_2d_grid": ([ci, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
+            "d2dhip_plan_create": ([ctypes.POINTER(pl), ci, ci, ci, ci, ci, cu, vp], ci),
+            "d2dhip_plan_destroy": ([pl], ci),
+            "d2dhip_plan_set_stream": ([pl, vp], ci),
+            "d2dhip_plan_get_sizes": ([pl, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
+            "d2dhip_plan_device_bytes": ([pl], cll),
+            "d2dhip_exec_c2c": ([pl, vp, vp, ci], ci),
+            "d2dhip_exec_r2c": ([pl, vp, vp], ci),
+            "d2dhip_exec_c2r": ([pl, vp, vp], ci),
+            "d2dhip_transpose": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), vp], ci),
+            "d2dhip_pack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
+            "d2dhip_unpack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
+            "d2dhip_pack_buffer_elems": ([ci, ctypes.POINTER(Decomp), ci], cll),
+            "d2dhip_fill_uniform": ([vp, cll, ci, ctypes.c_ulonglong, vp], ci),
+            "d2dhip_host_fill_uniform": ([vp, cll, ci, ctypes.c_ulonglong], None),
+            "d2dhip_modes_make": ([ctypes.POINTER(Modes), ci, ci, ci, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong], ci),
+            "d2dhip_fill_modal": ([vp, ci, ci, ci, ci, ctypes.POINTER(Modes), vp], ci),
+            "d2dhip_host_fill_modal": ([vp, ci, ci, ci, ci, ctypes.POINTER(Modes)], None),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def header_functions():
+    """Names of the functions include/d2dhip.h declares."""
+    import re
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"\b(d2dhip_\w+)\s*\(", text)))
+
+
+def status_string(status):
+    return lib().d2dhip_status_string(status).decode()
+
+
+def last_message():
+    return lib().d2dhip_last_message().decode()
+
+
+def check(status):
+    if status == SUCCESS:
+        return status
+    if status >= 100:
+        import warnings
+        warnings.warn(last_message(), D2dWarning, stacklevel=3)
+        return status
+    raise D2dError(status, last_message())
+
+
+def _stream(stream):
+    if stream is not None:
+        return ctypes.c_void_p(stream)
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    """Device address of a torch tensor (or a plain int address)."""
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+# ---- decomposition ----------------------------------------------------------
+
+def decomp_info_init(nx, ny, nz, p_row=1, p_col=1, coord_row=0, coord_col=0):
+    """DECOMP_INFO of one rank (decomp_info_init, src/decomp_2d.f90:382-490)."""
+    d = Decomp()
+    check(lib().d2dhip_decomp_info_init(nx, ny, nz, p_row, p_col, coord_row, coord_col,
+                                        ctypes.byref(d)))
+    return d
+
+
+def best_2d_grid(nproc):
+    r, c = ctypes.c_int(), ctypes.c_int()
+    check(lib().d2dhip_best_2d_grid(nproc, ctypes.byref(r), ctypes.byref(c)))
+    return r.value, c.value
+
+
+# ---- FFT engine ----------------------------------------------------------------
+
+class FFTEngine:
+    """One decomp_2d_fft engine (src/fft_common.f90:141-242) at one rank."""
+
+    def __init__(self, pencil, nx, ny, nz, prec=PREC_F64, inplace=False, skip_xyz=None,
+                 stream=None):
+        flags = FLAG_INPLACE if inplace else 0
+        if skip_xyz is not None:
+            for bit, s in zip((FLAG_SKIP_X, FLAG_SKIP_Y, FLAG_SKIP_Z), skip_xyz):
+                if s:
+                    flags |= bit
+        self._plan = ctypes.c_void_p()
+        self.format = pencil
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.prec = prec
+        self.flags = flags
+        st = ctypes.c_void_p(stream) if stream is not None else ctypes.c_void_p()
+        check(lib().d2dhip_plan_create(ctypes.byref(self._plan), pencil, nx, ny, nz, prec, flags,
+                                       st))
+        self._explicit_stream = stream is not None
+
+    def _sync_stream(self):
+        if not self._explicit_stream:
+            check(lib().d2dhip_plan_set_stream(self._plan, _stream(None)))
+
+    def get_sizes(self):
+        ph = (ctypes.c_int * 3)()
+        sp = (ctypes.c_int * 3)()
+        check(lib().d2dhip_plan_get_sizes(self._plan, ph, sp))
+        return tuple(ph), tuple(sp)
+
+    def device_bytes(self):
+        return lib().d2dhip_plan_device_bytes(self._plan)
+
+    def fft_c2c(self, inp, out, isign):
+        self._sync_stream()
+        return check(lib().d2dhip_exec_c2c(self._plan, _ptr(inp), _ptr(out), isign))
+
+    def fft_r2c(self, inp, out):
+        self._sync_stream()
+        return check(lib().d2dhip_exec_r2c(self._plan, _ptr(inp), _ptr(out)))
+
+    def fft_c2r(self, inp, out):
+        self._sync_stream()
+        return check(lib().d2dhip_exec_c2r(self._plan, _ptr(inp), _ptr(out)))
+
+    def fin(self):
+        if self._plan:
+            lib().d2dhip_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.fin()
+        except Exception:
+            pass
+
+
+def decomp_2d_fft_init(pencil, nx, ny, nz, **kw):
+    return FFTEngine(pencil, nx, ny, nz, **kw)
+
+
+# ---- transposes and packs -------------------------------------------------------
+
+def transpose(direction, elem, src, dst, decomp, stream=None):
+    return check(lib().d2dhip_transpose(direction, elem, _ptr(src), _ptr(dst),
+                                        ctypes.byref(decomp), _stream(stream)))
+
+
+def transpose_x_to_y(src, dst, decomp, elem=C16, stream=None):
+    return transpose(X_TO_Y, elem, src, dst, decomp, stream)
+
+
+def transpose_y_to_z(src, dst, decomp, elem=C16, stream=None):
+    return transpose(Y_TO_Z, elem, src, dst, decomp, stream)
+
+
+def transpose_z_to_y(src, dst, decomp, elem=C16, stream=None):
+    return transpose(Z_TO_Y, elem, src, dst, decomp, stream)
+
+
+def transpose_y_to_x(src, dst, decomp, elem=C16, stream=None):
+    return transpose(Y_TO_X, elem, src, dst, decomp, stream)
+
+
+def pack(direction, elem, src, send, decomp, even=False, stream=None):
+    return check(lib().d2dhip_pack(direction, elem, _ptr(src), _ptr(send), ctypes.byref(decomp),
+                                   int(bool(even)), _stream(stream)))
+
+
+def unpack(direction, elem, recv, dst, decomp, even=False, stream=None):
+    return check(lib().d2dhip_unpack(direction, elem, _ptr(recv), _ptr(dst),
+                                     ctypes.byref(decomp), int(bool(even)), _stream(stream)))
+
+
+def pack_buffer_elems(direction, decomp, even=False):
+    return lib().d2dhip_pack_buffer_elems(direction, ctypes.byref(decomp), int(bool(even)))
+
+
+# ---- synthetic inputs ------------------------------------------------------------
+
+def fill_uniform(dst, n, elem, seed, stream=None):
+    return check(lib().d2dhip_fill_uniform(_ptr(dst), n, elem, seed, _stream(stream)))
+
+
+def host_fill_uniform(np_array, elem, seed):
+    n = np_array.size
+    lib().d2dhip_host_fill_uniform(np_array.ctypes.data_as(ctypes.c_void_p), n, elem, seed)
+
+
+def modes_make(nmodes, kmax, amp_kind, amp_lo, amp_hi, sigma, seed):
+    m = Modes()
+    check(lib().d2dhip_modes_make(ctypes.byref(m), nmodes, kmax, amp_kind, amp_lo, amp_hi, sigma,
+                                  seed))
+    return m
+
+
+def fill_modal(dst, nx, ny, nz, elem, modes, stream=None):
+    return check(lib().d2dhip_fill_modal(_ptr(dst), nx, ny, nz, elem, ctypes.byref(modes),
+                                         _stream(stream)))
+
+
+def host_fill_modal(np_array, nx, ny, nz, elem, modes):
+    lib().d2dhip_host_fill_modal(np_array.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, elem,
+                                 ctypes.byref(modes))

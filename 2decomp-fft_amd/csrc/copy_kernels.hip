@@ -1,0 +1,208 @@
+// copy_kernels.hip -- alltoallv pack/unpack maps and synthetic-input fills.
+//
+// The eight mem_split_* / mem_merge_* maps of the reference
+// (src/transpose_x_to_y.f90:266-500, src/transpose_y_to_z.f90:370-606,
+// src/transpose_z_to_y.f90:370-606, src/transpose_y_to_x.f90:266-500) are
+// all block copies that keep natural (i fastest) order inside each
+// destination block: element (i,j,k) of a pencil (n1,n2,n3) whose block axis
+// value lies in block m = [b1, b1 + w) sits in the flat buffer at
+//   init_m + (i-b1) + j*w + k*n2*w      (block on i)
+//   init_m + i + (j-b1)*n1 + k*w*n1     (block on j)
+//   init_m + i + j*n1 + (k-b1)*n1*n2    (block on k)
+// with init_m the alltoallv displacement (or m*count for the padded EVEN
+// layout).  One kernel covers every map: threads walk the pencil with i
+// fastest, so both the pencil side and each block's run in the buffer are
+// read and written in consecutive addresses.
+#include <hip/hip_runtime.h>
+
+#include "gen.h"
+#include "internal.h"
+
+namespace d2dhip {
+
+struct PackArgs {
+  const void *src;
+  void *dst;
+  int n1, n2, n3;
+  int ax;     // block axis: 0 = i, 1 = j, 2 = k
+  int split;  // 1: pencil -> buffer (mem_split), 0: buffer -> pencil (mem_merge)
+  int np;
+  int b1[D2DHIP_MAX_PROCS];
+  int w[D2DHIP_MAX_PROCS];
+  long long init[D2DHIP_MAX_PROCS];
+};
+
+struct alignas(16) E16 {
+  unsigned long long a, b;
+};
+
+template <typename E>
+__global__ void __launch_bounds__(256) pack_kernel(PackArgs a, long long units, int cpr) {
+  const E *src = reinterpret_cast<const E *>(a.src);
+  E *dst = reinterpret_cast<E *>(a.dst);
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const long long row = u / cpr;
+    const int chunk = (int)(u - row * cpr);
+    const int i = chunk * 256 + (int)threadIdx.x;
+    if (i >= a.n1) continue;
+    const int j = (int)(row % a.n2);
+    const int k = (int)(row / a.n2);
+    const int key = a.ax == 0 ? i : (a.ax == 1 ? j : k);
+    int m = 0;
+    while (m + 1 < a.np && key >= a.b1[m + 1]) ++m;
+    const int b1 = a.b1[m], w = a.w[m];
+    long long pos = a.init[m];
+    if (a.ax == 0)
+      pos += (long long)(i - b1) + (long long)j * w + (long long)k * a.n2 * w;
+    else if (a.ax == 1)
+      pos += (long long)i + (long long)(j - b1) * a.n1 + (long long)k * w * a.n1;
+    else
+      pos += (long long)i + (long long)j * a.n1 + (long long)(k - b1) * a.n1 * a.n2;
+    const long long lin = (long long)i + (long long)a.n1 * row;
+    if (a.split)
+      dst[pos] = src[lin];
+    else
+      dst[lin] = src[pos];
+  }
+}
+
+int launch_pack(int dir, int split, int esize, const void *src, void *dst, const d2dhip_decomp *d,
+                int even, hipStream_t st) {
+  PackArgs a;
+  a.src = src;
+  a.dst = dst;
+  // The pencil each map reads (split) or writes (merge), the block axis, and
+  // the dist / displacement arrays, as in the reference's calls
+  // (e.g. src/transpose_x_to_y.f90:94-95 and :132-133).
+  const int *sz;
+  const int *dist;
+  const long long *disp;
+  long long count;
+  int np;
+  switch (dir) {
+    case D2DHIP_X_TO_Y:
+      np = d->p_row;
+      sz = split ? d->xsz : d->ysz;
+      a.ax = split ? 0 : 1;
+      dist = split ? d->x1dist : d->y1dist;
+      disp = split ? d->x1disp : d->y1disp;
+      count = split ? d->x1count : d->y1count;
+      break;
+    case D2DHIP_Y_TO_Z:
+      np = d->p_col;
+      sz = split ? d->ysz : d->zsz;
+      a.ax = split ? 1 : 2;
+      dist = split ? d->y2dist : d->z2dist;
+      disp = split ? d->y2disp : d->z2disp;
+      count = split ? d->y2count : d->z2count;
+      break;
+    case D2DHIP_Z_TO_Y:
+      np = d->p_col;
+      sz = split ? d->zsz : d->ysz;
+      a.ax = split ? 2 : 1;
+      dist = split ? d->z2dist : d->y2dist;
+      disp = split ? d->z2disp : d->y2disp;
+      count = split ? d->z2count : d->y2count;
+      break;
+    default:
+      np = d->p_row;
+      sz = split ? d->ysz : d->xsz;
+      a.ax = split ? 1 : 0;
+      dist = split ? d->y1dist : d->x1dist;
+      disp = split ? d->y1disp : d->x1disp;
+      count = split ? d->y1count : d->x1count;
+      break;
+  }
+  if (np < 1 || np > D2DHIP_MAX_PROCS)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "process count out of range");
+  a.n1 = sz[0];
+  a.n2 = sz[1];
+  a.n3 = sz[2];
+  a.split = split;
+  a.np = np;
+  int b = 0;
+  long long wsum = 0;
+  for (int m = 0; m < np; ++m) {
+    a.b1[m] = b;
+    a.w[m] = dist[m];
+    a.init[m] = even ? (long long)m * count : disp[m];
+    b += dist[m];
+    wsum += dist[m];
+  }
+  const int axlen = a.ax == 0 ? a.n1 : (a.ax == 1 ? a.n2 : a.n3);
+  if (wsum != axlen)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition does not match the pencil");
+  if ((long long)a.n1 * a.n2 * a.n3 == 0) return D2DHIP_SUCCESS;
+  const int cpr = (a.n1 + 255) / 256;
+  const long long units = (long long)a.n2 * a.n3 * cpr;
+  const long long grid = units < (1LL << 20) ? units : (1LL << 20);
+  switch (esize) {
+    case 4:
+      hipLaunchKernelGGL(pack_kernel<unsigned int>, dim3((unsigned)grid), dim3(256), 0, st, a,
+                         units, cpr);
+      break;
+    case 8:
+      hipLaunchKernelGGL(pack_kernel<unsigned long long>, dim3((unsigned)grid), dim3(256), 0, st,
+                         a, units, cpr);
+      break;
+    default:
+      hipLaunchKernelGGL(pack_kernel<E16>, dim3((unsigned)grid), dim3(256), 0, st, a, units, cpr);
+      break;
+  }
+  return hip_check(hipGetLastError(), "pack_kernel launch");
+}
+
+// ---- synthetic inputs ------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(256) fill_uniform_kernel(T *dst, long long nv,
+                                                           unsigned long long seed) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < nv; i += (long long)gridDim.x * 256)
+    dst[i] = (T)gen_uniform_pm1(seed, (unsigned long long)i);
+}
+
+int launch_fill_uniform(void *dst, long long n, int elem, unsigned long long seed,
+                        hipStream_t st) {
+  const bool cplx = (elem == D2DHIP_C16 || elem == D2DHIP_C8);
+  const long long nv = cplx ? 2 * n : n;
+  if (nv == 0) return D2DHIP_SUCCESS;
+  long long grid = (nv + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  if (elem == D2DHIP_R8 || elem == D2DHIP_C16)
+    hipLaunchKernelGGL(fill_uniform_kernel<double>, dim3((unsigned)grid), dim3(256), 0, st,
+                       (double *)dst, nv, seed);
+  else
+    hipLaunchKernelGGL(fill_uniform_kernel<float>, dim3((unsigned)grid), dim3(256), 0, st,
+                       (float *)dst, nv, seed);
+  return hip_check(hipGetLastError(), "fill_uniform launch");
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) fill_modal_kernel(T *dst, int nx, int ny, int nz,
+                                                         d2dhip_modes m) {
+  const long long n = (long long)nx * ny * nz;
+  for (long long idx = blockIdx.x * 256LL + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * 256) {
+    const long long i = idx % nx;
+    const long long r = idx / nx;
+    const long long j = r % ny;
+    const long long k = r / ny;
+    dst[idx] = (T)gen_modal_value(&m, nx, ny, nz, i, j, k, idx);
+  }
+}
+
+int launch_fill_modal(void *dst, int nx, int ny, int nz, int elem, const d2dhip_modes *m,
+                      hipStream_t st) {
+  const long long n = (long long)nx * ny * nz;
+  long long grid = (n + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  if (elem == D2DHIP_R8)
+    hipLaunchKernelGGL(fill_modal_kernel<double>, dim3((unsigned)grid), dim3(256), 0, st,
+                       (double *)dst, nx, ny, nz, *m);
+  else
+    hipLaunchKernelGGL(fill_modal_kernel<float>, dim3((unsigned)grid), dim3(256), 0, st,
+                       (float *)dst, nx, ny, nz, *m);
+  return hip_check(hipGetLastError(), "fill_modal launch");
+}
+
+}  // namespace d2dhip

@@ -1,0 +1,279 @@
+// fft_kernels.hip -- batched 1D FFT stages along one pencil axis (gfx950).
+//
+// One workgroup transforms a tile of TX columns of length n.  The tile lives
+// in LDS between Stockham passes; within a pass each thread holds up to
+// kRegCap complex values in registers (E/R butterflies of radix R).  The
+// first pass reads its butterfly inputs straight from HBM and the last pass
+// writes straight to HBM, so a stage costs one read and one write of the
+// pencil.  Consecutive threads take consecutive tile columns (the fastest
+// tile index), which makes HBM accesses of strided axes 16*TX-byte runs and
+// those of contiguous lines TX runs of consecutive points.
+//
+// Stockham autosort pass (decimation in time) with radix R over
+// sub-transforms of length L (product of the earlier radices): butterfly b
+// has k = b mod L, g = b div L, inputs x[b + s*n/R] (s < R) scaled by
+// exp(-2 pi i s k / (L R)), and outputs y[g*L*R + k + s*L].
+//
+// Real transforms pair two real columns into one complex column
+// (z = a + i b) and separate / merge them with the Hermitian identities
+//   A[k] = (Z[k] + conj Z[n-k]) / 2,   B[k] = (Z[k] - conj Z[n-k]) / (2i),
+// which holds for every n, odd or even.
+//
+// The stage descriptor sits in device memory and is read with scalar loads;
+// each radix's pass is a separate (non-inlined) function so that its
+// register arrays are allocated on their own.
+#include <hip/hip_runtime.h>
+
+#include "codelets.h"
+#include "internal.h"
+
+namespace d2dhip {
+
+// Input value of tile column c0 (global), outer index c1, point j.
+template <typename T, int IO>
+__device__ __forceinline__ cpx<T> gload(const AxisDesc *__restrict__ d, const AxisIo &io,
+                                        long long c0, long long c1, int j) {
+  if constexpr (IO == IO_R2C) {
+    const T *in = reinterpret_cast<const T *>(io.in);
+    const long long base = c1 * d->in_s1 + (long long)j * d->in_sj;
+    const long long ca = 2 * c0;
+    cpx<T> v = {T(0), T(0)};
+    if (ca < d->C0r) v.x = in[base + ca * d->in_s0];
+    if (ca + 1 < d->C0r) v.y = in[base + (ca + 1) * d->in_s0];
+    return v;
+  } else {
+    cpx<T> v = {T(0), T(0)};
+    if (c0 < d->C0)
+      v = reinterpret_cast<const cpx<T> *>(io.in)[c0 * d->in_s0 + c1 * d->in_s1 +
+                                                  (long long)j * d->in_sj];
+    if (io.conj) v.y = -v.y;
+    return v;
+  }
+}
+
+// Store of an output point of the (conjugated, for inverses) transform.
+template <typename T, int IO>
+__device__ __forceinline__ void gstore(const AxisDesc *__restrict__ d, const AxisIo &io,
+                                       long long c0, long long c1, int j, cpx<T> v) {
+  if constexpr (IO == IO_C2R) {
+    // the pre-phase stored conj(Z): z = conj(result) = a + i b
+    T *out = reinterpret_cast<T *>(io.out);
+    const long long base = c1 * d->out_s1 + (long long)j * d->out_sj;
+    const long long ca = 2 * c0;
+    if (ca < d->C0r) out[base + ca * d->out_s0] = v.x;
+    if (ca + 1 < d->C0r) out[base + (ca + 1) * d->out_s0] = -v.y;
+  } else {
+    if (c0 < d->C0) {
+      if (io.conj) v.y = -v.y;
+      reinterpret_cast<cpx<T> *>(io.out)[c0 * d->out_s0 + c1 * d->out_s1 +
+                                         (long long)j * d->out_sj] = v;
+    }
+  }
+}
+
+// One Stockham pass of radix R.  `sg` reads the pass inputs from HBM (first
+// pass), `dg` writes the outputs to HBM (last pass); otherwise LDS.  Ends
+// with a barrier whenever it wrote LDS.
+template <typename T, int E, int IO, int R>
+__device__ __noinline__ void run_pass(const AxisDesc *__restrict__ d, AxisIo io, int p, bool sg,
+                                      bool dg, cpx<T> *lds, long long c0b, long long c1) {
+  constexpr int NB = E / R;
+  static_assert(NB >= 1, "radix larger than the register capacity");
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int n = d->n, nb = n / R, L = d->pass[p].L;
+  const float invL = d->pass[p].invL;
+  const int TX = d->TX, lg = d->log2TX, tmask = TX - 1;
+  const int ldj = d->ldj, ldt = d->ldt;
+  const int total = TX * nb;
+  cpx<T> v[NB][R];
+
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int w = tid + q * nthr;
+    if (w < total) {
+      const int t = w & tmask, b = w >> lg;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        const int j = b + s * nb;
+        v[q][s] = sg ? gload<T, IO>(d, io, c0b + t, c1, j) : lds[j * ldj + t * ldt];
+      }
+    }
+  }
+  if (!sg && !dg) __syncthreads();  // all LDS reads before LDS writes
+
+  const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(d->tw) + d->pass[p].tw_off;
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int w = tid + q * nthr;
+    if (w < total) {
+      const int t = w & tmask, b = w >> lg;
+      int g = (int)((float)b * invL);
+      int k = b - g * L;
+      if (k < 0) {
+        --g;
+        k += L;
+      } else if (k >= L) {
+        ++g;
+        k -= L;
+      }
+      if (L > 1) {
+#pragma unroll
+        for (int s = 1; s < R; ++s) v[q][s] = cmul(v[q][s], tw[k * (R - 1) + (s - 1)]);
+      }
+      dft<T, R>(v[q]);
+      const int jb = g * L * R + k;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        const int j = jb + s * L;
+        if (dg)
+          gstore<T, IO>(d, io, c0b + t, c1, j, v[q][s]);
+        else
+          lds[j * ldj + t * ldt] = v[q][s];
+      }
+    }
+  }
+  if (!dg) __syncthreads();
+}
+
+template <typename T, int E, int IO>
+__device__ __forceinline__ void dispatch_pass(const AxisDesc *__restrict__ d, const AxisIo &io,
+                                              int p, bool sg, bool dg, cpx<T> *lds,
+                                              long long c0b, long long c1) {
+  switch (d->pass[p].r) {
+    case 1: run_pass<T, E, IO, 1>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 2: run_pass<T, E, IO, 2>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 3: run_pass<T, E, IO, 3>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 4: run_pass<T, E, IO, 4>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 5: run_pass<T, E, IO, 5>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 6: run_pass<T, E, IO, 6>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 7: run_pass<T, E, IO, 7>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 8: run_pass<T, E, IO, 8>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 9: run_pass<T, E, IO, 9>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 10: run_pass<T, E, IO, 10>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 11: run_pass<T, E, IO, 11>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 12: run_pass<T, E, IO, 12>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 13: run_pass<T, E, IO, 13>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 15: run_pass<T, E, IO, 15>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 16: run_pass<T, E, IO, 16>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 17: run_pass<T, E, IO, 17>(d, io, p, sg, dg, lds, c0b, c1); break;
+    default: break;  // the planner only emits the radices above
+  }
+}
+
+template <typename T, int E, int IO>
+__global__ void __launch_bounds__(512)
+    axis_fft_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
+  const AxisIo io = {in, out, conj};
+  const long long tiles0 = d->tiles0;
+  const long long tile = (long long)blockIdx.x + (long long)blockIdx.y * gridDim.x;
+  if (tile >= tiles0 * d->C1) return;  // whole block, before any barrier
+  const long long c1 = tile / tiles0;
+  const long long c0b = (tile - c1 * tiles0) * d->TX;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int n = d->n, TX = d->TX, lg = d->log2TX, tmask = TX - 1;
+  const int ldj = d->ldj, ldt = d->ldt, nh = d->nh;
+
+  if constexpr (IO == IO_C2R) {
+    // Merge the two half-spectra of columns (2c0, 2c0+1) into conj(Z) over
+    // all n points; the imaginary parts of the zero and Nyquist modes are
+    // ignored, as a complex-to-real transform does.
+    const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in);
+    for (int w = tid; w < TX * nh; w += nthr) {
+      const int t = w & tmask, k = w >> lg;
+      const long long ca = 2 * (c0b + t);
+      const long long base = c1 * d->in_s1 + (long long)k * d->in_sj;
+      cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
+      if (ca < d->C0r) A = src[base + ca * d->in_s0];
+      if (ca + 1 < d->C0r) B = src[base + (ca + 1) * d->in_s0];
+      if (k == 0 || 2 * k == n) {
+        A.y = T(0);
+        B.y = T(0);
+      }
+      lds[k * ldj + t * ldt] = cpx<T>{A.x - B.y, -(A.y + B.x)};
+      if (k > 0 && 2 * k < n) lds[(n - k) * ldj + t * ldt] = cpx<T>{A.x + B.y, A.y - B.x};
+    }
+    __syncthreads();
+  }
+
+  const int np = d->npass;
+  for (int p = 0; p < np; ++p) {
+    const bool sg = (p == 0) && (IO != IO_C2R);
+    const bool dg = (p == np - 1) && (IO != IO_R2C) && (np >= 2);
+    dispatch_pass<T, E, IO>(d, io, p, sg, dg, lds, c0b, c1);
+  }
+
+  if constexpr (IO == IO_R2C) {
+    cpx<T> *dst = reinterpret_cast<cpx<T> *>(out);
+    const T half = T(0.5);
+    for (int w = tid; w < TX * nh; w += nthr) {
+      const int t = w & tmask, k = w >> lg;
+      const long long ca = 2 * (c0b + t);
+      if (ca >= d->C0r) continue;
+      const cpx<T> zk = lds[k * ldj + t * ldt];
+      const int km = (k == 0) ? 0 : n - k;
+      const cpx<T> zm = lds[km * ldj + t * ldt];
+      const long long base = c1 * d->out_s1 + (long long)k * d->out_sj;
+      dst[base + ca * d->out_s0] = cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half};
+      if (ca + 1 < d->C0r)
+        dst[base + (ca + 1) * d->out_s0] = cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half};
+    }
+  } else {
+    if (np < 2) {
+      for (int w = tid; w < TX * n; w += nthr) {
+        const int t = w & tmask, j = w >> lg;
+        gstore<T, IO>(d, io, c0b + t, c1, j, lds[j * ldj + t * ldt]);
+      }
+    }
+  }
+}
+
+template <typename T, int IO>
+static hipError_t set_lds_limit() {
+  return hipFuncSetAttribute((const void *)axis_fft_kernel<T, kRegCap, IO>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+int axis_kernel_prepare(int prec_bytes) {
+  hipError_t e;
+  if (prec_bytes == 8) {
+    e = set_lds_limit<double, IO_C2C>();
+    if (e == hipSuccess) e = set_lds_limit<double, IO_R2C>();
+    if (e == hipSuccess) e = set_lds_limit<double, IO_C2R>();
+  } else {
+    e = set_lds_limit<float, IO_C2C>();
+    if (e == hipSuccess) e = set_lds_limit<float, IO_R2C>();
+    if (e == hipSuccess) e = set_lds_limit<float, IO_C2R>();
+  }
+  return hip_check(e, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
+}
+
+// Launches one axis stage.  The tiles0 * C1 tiles are spread over gridDim.x
+// (at most 2^20 blocks) and gridDim.y; surplus blocks exit at once.
+template <typename T, int IO>
+static int launch_t(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, const AxisShape &s,
+                    hipStream_t st) {
+  const long long ntiles = (long long)h.tiles0 * h.C1;
+  if (ntiles <= 0) return D2DHIP_SUCCESS;
+  const long long gx = ntiles < (1LL << 20) ? ntiles : (1LL << 20);
+  const long long gy = (ntiles + gx - 1) / gx;
+  if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "axis stage has too many tiles");
+  hipLaunchKernelGGL((axis_fft_kernel<T, kRegCap, IO>), dim3((unsigned)gx, (unsigned)gy),
+                     dim3(s.nthreads), s.lds_bytes, st, dd, io.in, io.out, io.conj);
+  return hip_check(hipGetLastError(), "axis_fft_kernel launch");
+}
+
+int launch_axis(int prec_bytes, int io, const AxisDesc *d_desc, const AxisDesc &h_desc,
+                const AxisIo &io_ptrs, const AxisShape &s, hipStream_t st) {
+  if (prec_bytes == 8) {
+    if (io == IO_C2C) return launch_t<double, IO_C2C>(d_desc, h_desc, io_ptrs, s, st);
+    if (io == IO_R2C) return launch_t<double, IO_R2C>(d_desc, h_desc, io_ptrs, s, st);
+    return launch_t<double, IO_C2R>(d_desc, h_desc, io_ptrs, s, st);
+  }
+  if (io == IO_C2C) return launch_t<float, IO_C2C>(d_desc, h_desc, io_ptrs, s, st);
+  if (io == IO_R2C) return launch_t<float, IO_R2C>(d_desc, h_desc, io_ptrs, s, st);
+  return launch_t<float, IO_C2R>(d_desc, h_desc, io_ptrs, s, st);
+}
+
+}  // namespace d2dhip

@@ -1,0 +1,95 @@
+// internal.h -- shared declarations of libd2dhip (host side + kernel args).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/d2dhip.h"
+
+namespace d2dhip {
+
+constexpr int kMaxPass = 16;
+// Register capacity (complex values per thread per Stockham pass) of the
+// general axis kernel.  18 holds one radix-16/17/18 butterfly, two radix-8/9,
+// four radix-4 and so on.
+constexpr int kRegCap = 18;
+
+enum IoKind : int { IO_C2C = 0, IO_R2C = 1, IO_C2R = 2 };
+
+struct PassInfo {
+  int r;       // radix
+  int L;       // product of the radices of the earlier passes
+  int tw_off;  // first twiddle (complex) of this pass
+  float invL;  // 1/L for the quotient estimate
+};
+
+// One batched 1D FFT stage along one axis of a pencil.
+//
+// The stage sees the pencil as columns: column (c0, c1) starts at
+// c0*s0 + c1*s1 and its n points are sj apart (all in elements of the
+// array's own type).  A workgroup owns a tile of TX consecutive c0 of one
+// c1 and all n points of each, staged in LDS.  For IO_R2C / IO_C2R the tile
+// columns are pairs of real columns (2*c0, 2*c0+1) transformed together as
+// one complex column and split / merged by Hermitian symmetry.
+struct AxisDesc {
+  const void *tw;  // per-pass twiddles, forward sign, complex of the plan type
+  int n;
+  int npass;
+  PassInfo pass[kMaxPass];
+  int TX, log2TX;
+  int tiles0;         // ceil(C0 / TX)
+  long long C0;       // tile-level columns along c0
+  long long C0r;      // real columns along c0 (paired IO), else C0
+  long long C1;       // columns along c1
+  long long in_s0, in_s1, in_sj;
+  long long out_s0, out_s1, out_sj;
+  int ldj, ldt;       // LDS strides (complex units) of point j and column t
+  int nh;             // n/2 + 1
+};
+
+// Per-launch arguments: the descriptor lives in device memory (built once per
+// plan), the data pointers and the direction come with each launch.
+struct AxisIo {
+  const void *in;
+  void *out;
+  int conj;  // 1: inverse transform (conjugate on load and store)
+};
+
+// Launch shape chosen by the planner for one axis stage.
+struct AxisShape {
+  int TX = 1;
+  int nthreads = 64;
+  size_t lds_bytes = 0;
+  int ldj = 1, ldt = 1;
+};
+
+// Per-axis plan: the radix schedule and twiddles of one FFT length.
+struct AxisPlan {
+  int n = 0;
+  std::vector<int> radices;
+  std::vector<PassInfo> passes;
+  void *d_tw = nullptr;  // device twiddles (cpx<T>)
+  size_t tw_bytes = 0;
+};
+
+// status helpers (api.cpp)
+int set_error(int status, const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+
+// axis kernels (fft_kernels.hip)
+int launch_axis(int prec_bytes, int io, const AxisDesc *d_desc, const AxisDesc &h_desc,
+                const AxisIo &io_ptrs, const AxisShape &s, hipStream_t st);
+int axis_kernel_prepare(int prec_bytes);  // raise LDS limits once
+
+// copy / pack kernels (copy_kernels.hip)
+int launch_pack(int dir, int split, int esize, const void *src, void *dst,
+                const d2dhip_decomp *d, int even, hipStream_t st);
+int launch_fill_uniform(void *dst, long long n, int elem, unsigned long long seed,
+                        hipStream_t st);
+int launch_fill_modal(void *dst, int nx, int ny, int nz, int elem,
+                      const d2dhip_modes *m, hipStream_t st);
+
+}  // namespace d2dhip

@@ -1,0 +1,184 @@
+/*
+ * d2dhip.h -- C ABI of libd2dhip, the MI355X (gfx950) FFT backend for the
+ * 2DECOMP&FFT single-GPU 3D FFT path.
+ *
+ * Every entry point returns an int status: 0 on success, 1..99 on error (the
+ * caller maps it to decomp_2d_abort, as the reference does with
+ * `istat /= 0` after each vendor call, src/fft_cufft.f90:86-94), and >= 100
+ * for warnings after a completed call (the caller maps them to
+ * decomp_2d_warning, src/decomp_2d_mpi.f90:259-299).
+ *
+ * Data layout is the reference's: Fortran order, i fastest; complex values
+ * are interleaved (re, im).  Device pointers are plain HIP device addresses.
+ * Streams are hipStream_t passed as void* (NULL = the null stream).
+ * Calls are stream-ordered and return without synchronising.
+ */
+#ifndef D2DHIP_H
+#define D2DHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants (src/decomp_2d_constants.f90:77-93) ---------------------- */
+#define D2DHIP_FFT_BACKEND 5 /* new id beside generic=0 .. cufft=4 (:77-81) */
+#define D2DHIP_PHYSICAL_IN_X 1 /* PHYSICAL_IN_X (:92) */
+#define D2DHIP_PHYSICAL_IN_Z 3 /* PHYSICAL_IN_Z (:93) */
+#define D2DHIP_FORWARD (-1)    /* DECOMP_2D_FFT_FORWARD (:86) */
+#define D2DHIP_BACKWARD 1      /* DECOMP_2D_FFT_BACKWARD (:87) */
+
+/* arithmetic/storage precision of a plan */
+#define D2DHIP_PREC_F64 64        /* real64 storage and arithmetic (drop-in) */
+#define D2DHIP_PREC_F32 32        /* real32 storage and arithmetic */
+#define D2DHIP_PREC_SPLIT_BF16 16 /* real32 storage, bf16 hi+lo MFMA stages */
+
+/* plan flags (optional arguments of decomp_2d_fft_init, src/fft_common.f90:141-193) */
+#define D2DHIP_FLAG_INPLACE 0x1u /* opt_inplace: c2c/c2r may overwrite input */
+#define D2DHIP_FLAG_SKIP_X 0x2u  /* opt_skip_XYZ_c2c(1) */
+#define D2DHIP_FLAG_SKIP_Y 0x4u  /* opt_skip_XYZ_c2c(2) */
+#define D2DHIP_FLAG_SKIP_Z 0x8u  /* opt_skip_XYZ_c2c(3) */
+
+/* element kinds for transposes and packs */
+#define D2DHIP_R8 0  /* real(real64)    */
+#define D2DHIP_C16 1 /* complex(real64) */
+#define D2DHIP_R4 2  /* real(real32)    */
+#define D2DHIP_C8 3  /* complex(real32) */
+
+/* transpose / pack directions */
+#define D2DHIP_X_TO_Y 0
+#define D2DHIP_Y_TO_Z 1
+#define D2DHIP_Z_TO_Y 2
+#define D2DHIP_Y_TO_X 3
+
+/* status codes */
+#define D2DHIP_SUCCESS 0
+#define D2DHIP_ERR_INVALID_VALUE 1 /* bad argument (nx <= 0, bad pencil, ...) */
+#define D2DHIP_ERR_UNSUPPORTED 2   /* valid but not implemented (see message) */
+#define D2DHIP_ERR_ALLOC 3         /* device allocation failed */
+#define D2DHIP_ERR_HIP 4           /* a HIP runtime call failed */
+#define D2DHIP_ERR_NULL 5          /* NULL plan or pointer */
+#define D2DHIP_ERR_GRID 6          /* grid too small for p_row x p_col (errorcode 6,
+                                      src/decomp_2d.f90:399-405) */
+#define D2DHIP_ERR_PLAN_MISMATCH 7 /* exec kind does not match the plan */
+#define D2DHIP_WARN_SKIP_IGNORED 100 /* "r2c / c2r transform can not be skipped"
+                                        (src/fft_generic.f90:222-223) */
+
+#define D2DHIP_MAX_PROCS 64
+
+typedef struct d2dhip_plan_s *d2dhip_plan;
+
+/* Mirror of DECOMP_INFO (src/info.f90:19-47) for one rank of a
+ * p_row x p_col grid.  Starts/ends are 1-based as in the reference. */
+typedef struct {
+  int xst[3], xen[3], xsz[3];
+  int yst[3], yen[3], ysz[3];
+  int zst[3], zen[3], zsz[3];
+  int p_row, p_col;       /* dims(1), dims(2) */
+  int coord_row, coord_col; /* coord(1), coord(2) of this rank */
+  int x1dist[D2DHIP_MAX_PROCS], y1dist[D2DHIP_MAX_PROCS];
+  int y2dist[D2DHIP_MAX_PROCS], z2dist[D2DHIP_MAX_PROCS];
+  long long x1cnts[D2DHIP_MAX_PROCS], y1cnts[D2DHIP_MAX_PROCS];
+  long long x1disp[D2DHIP_MAX_PROCS], y1disp[D2DHIP_MAX_PROCS];
+  long long y2cnts[D2DHIP_MAX_PROCS], z2cnts[D2DHIP_MAX_PROCS];
+  long long y2disp[D2DHIP_MAX_PROCS], z2disp[D2DHIP_MAX_PROCS];
+  long long x1count, y1count, y2count, z2count; /* EVEN padded counts */
+  int even;                                     /* evenly divisible grid */
+} d2dhip_decomp;
+
+/* ---- status ------------------------------------------------------------- */
+const char *d2dhip_status_string(int status);
+/* Detail text of the last non-zero status returned on this host thread. */
+const char *d2dhip_last_message(void);
+/* Library version as major*10000 + minor*100 + patch. */
+int d2dhip_version(void);
+
+/* ---- decomposition metadata (host only) ----------------------------------
+ * Replaces decomp_info_init / get_dist / partition / prepare_buffer,
+ * src/decomp_2d.f90:382-490 and :1016-1206, for rank (coord_row, coord_col). */
+int d2dhip_decomp_info_init(int nx, int ny, int nz, int p_row, int p_col,
+                            int coord_row, int coord_col, d2dhip_decomp *d);
+/* Replaces best_2d_grid, src/decomp_2d_init_fin.f90:270-300 (the process
+ * grid the reference picks when p_row = p_col = 0). */
+int d2dhip_best_2d_grid(int nproc, int *p_row, int *p_col);
+
+/* ---- FFT plans -----------------------------------------------------------
+ * d2dhip_plan_create replaces decomp_2d_fft_engine_init + init_fft_engine
+ * (src/fft_common.f90:141-242; cuFFT plan set-up src/fft_cufft.f90:263-431)
+ * for one rank (p_row = p_col = 1).  format is PHYSICAL_IN_X or _Z; prec one
+ * of D2DHIP_PREC_*; flags D2DHIP_FLAG_*.  The plan owns its twiddles and
+ * workspace. */
+int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz,
+                       int prec, unsigned flags, void *stream);
+/* Replaces decomp_2d_fft_engine_fin / finalize_fft_engine
+ * (src/fft_common.f90:274-295). */
+int d2dhip_plan_destroy(d2dhip_plan plan);
+int d2dhip_plan_set_stream(d2dhip_plan plan, void *stream);
+/* Physical (ph) x-pencil size and spectral (sp) size of the r2c/c2r output:
+ * (nx/2+1,ny,nz) for PHYSICAL_IN_X, (nx,ny,nz/2+1) for PHYSICAL_IN_Z
+ * (decomp_2d_fft_get_size, src/fft_common.f90:301-327). */
+int d2dhip_plan_get_sizes(d2dhip_plan plan, int ph_sz[3], int sp_sz[3]);
+/* Device bytes the plan holds (twiddles + workspace). */
+long long d2dhip_plan_device_bytes(d2dhip_plan plan);
+
+/* decomp_2d_fft_3d(in, out, isign): fft_3d_c2c, src/fft_common_3d.f90:9-116.
+ * in is preserved unless the plan has D2DHIP_FLAG_INPLACE; in == out allowed. */
+int d2dhip_exec_c2c(d2dhip_plan plan, void *in, void *out, int isign);
+/* decomp_2d_fft_3d(in_r, out_c): fft_3d_r2c, src/fft_common_3d.f90:121-189. */
+int d2dhip_exec_r2c(d2dhip_plan plan, const void *in_r, void *out_c);
+/* decomp_2d_fft_3d(in_c, out_r): fft_3d_c2r, src/fft_common_3d.f90:194-296.
+ * in_c is preserved unless the plan has D2DHIP_FLAG_INPLACE. */
+int d2dhip_exec_c2r(d2dhip_plan plan, void *in_c, void *out_r);
+
+/* ---- transposes and alltoallv packing ------------------------------------
+ * d2dhip_transpose replaces transpose_x_to_y / y_to_z / z_to_y / y_to_x for
+ * one rank (src/transpose_x_to_y.f90:41-50 and the matching branches of the
+ * other three files): with dims = (1,1) it is the copy dst = src of the
+ * source pencil.  For p_row*p_col > 1 it returns D2DHIP_ERR_UNSUPPORTED (the
+ * exchange itself needs a communicator). */
+int d2dhip_transpose(int dir, int elem, const void *src, void *dst,
+                     const d2dhip_decomp *d, void *stream);
+/* Send-buffer pack of one rank: mem_split_xy / _yz / _zy / _yx
+ * (src/transpose_x_to_y.f90:266-323, src/transpose_y_to_z.f90:370-428,
+ * src/transpose_z_to_y.f90:370-428, src/transpose_y_to_x.f90:266-323).
+ * even != 0 selects the padded MPI_ALLTOALL offsets m*count (EVEN build). */
+int d2dhip_pack(int dir, int elem, const void *src, void *send,
+                const d2dhip_decomp *d, int even, void *stream);
+/* Receive-buffer unpack of one rank: mem_merge_xy / _yz / _zy / _yx
+ * (src/transpose_x_to_y.f90:384-441, src/transpose_y_to_z.f90:489-547,
+ * src/transpose_z_to_y.f90:489-547, src/transpose_y_to_x.f90:384-441). */
+int d2dhip_unpack(int dir, int elem, const void *recv, void *dst,
+                  const d2dhip_decomp *d, int even, void *stream);
+/* Elements a pack/unpack buffer must hold for (dir, d, even). */
+long long d2dhip_pack_buffer_elems(int dir, const d2dhip_decomp *d, int even);
+
+/* ---- synthetic inputs (no reference counterpart; used by tests/benchmarks) -
+ * SplitMix64 stream keyed by seed, counter = 0-based Fortran linear index;
+ * uniform values 2u-1 in [-1,1).  Complex elements use counters 2*idx (re)
+ * and 2*idx+1 (im).  elem is D2DHIP_R8/C16/R4/C8. */
+int d2dhip_fill_uniform(void *dst, long long n, int elem,
+                        unsigned long long seed, void *stream);
+void d2dhip_host_fill_uniform(void *dst, long long n, int elem,
+                              unsigned long long seed);
+/* Modal field sum_m a_m cos(2 pi (kx i/nx + ky j/ny + kz k/nz)) + sigma*xi,
+ * i,j,k 0-based, with nmodes integer wavevectors uniform in [-kmax,kmax]^3;
+ * amp_kind 0: a_m ~ N(0,1); amp_kind 1: |a_m| = 10^U(amp_lo,amp_hi) with a
+ * random sign.  xi ~ N(0,1) white noise.  Real output (elem R8 or R4). */
+typedef struct {
+  int nmodes;
+  int kx[64], ky[64], kz[64];
+  double amp[64];
+  double sigma;
+  unsigned long long noise_seed;
+} d2dhip_modes;
+int d2dhip_modes_make(d2dhip_modes *m, int nmodes, int kmax, int amp_kind,
+                      double amp_lo, double amp_hi, double sigma,
+                      unsigned long long seed);
+int d2dhip_fill_modal(void *dst, int nx, int ny, int nz, int elem,
+                      const d2dhip_modes *m, void *stream);
+void d2dhip_host_fill_modal(void *dst, int nx, int ny, int nz, int elem,
+                            const d2dhip_modes *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* D2DHIP_H */

@@ -1,0 +1,196 @@
+"""ctypes front-end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, and only there as the checker.  The shipped
+library never imports it.
+
+Arrays follow the reference's Fortran layout: a pencil of global shape
+(nx, ny, nz) is a numpy array of shape (nx, ny, nz) in Fortran order, i.e.
+index i is fastest in memory.  Complex arrays are complex128.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+PHYSICAL_IN_X = 1
+PHYSICAL_IN_Z = 3
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        dp = ctypes.c_void_p
+        ci = ctypes.c_int
+        L.oracle_spcfft.argtypes = [dp, ci, ci, dp]
+        for name in ("oracle_fft3d_c2c",):
+            getattr(L, name).argtypes = [ci, ci, ci, ci, dp, dp, ci, dp]
+            getattr(L, name).restype = ci
+        for name in ("oracle_fft3d_r2c", "oracle_fft3d_c2r"):
+            getattr(L, name).argtypes = [ci, ci, ci, ci, dp, dp, dp]
+            getattr(L, name).restype = ci
+        L.oracle_dft3d_ld.argtypes = [ci, ci, ci, dp, dp, ci]
+        L.oracle_sample_stage_times.argtypes = [ci, ci, ci, ci, ci, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_sample_stage_times.restype = ci
+        L.oracle_distribute.argtypes = [ci, ci, dp, dp, dp]
+        L.oracle_distribute.restype = ci
+        L.oracle_decomp_info_init.argtypes = [ci, ci, ci, ci, ci, ci, ci, dp]
+        L.oracle_decomp_info_init.restype = ci
+        L.oracle_mem_split.argtypes = [ci, dp, ci, ci, ci, dp, dp, ci, ci]
+        L.oracle_mem_merge.argtypes = [ci, dp, ci, ci, ci, dp, dp, ci, ci]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _fortran(a, dtype):
+    return np.asfortranarray(a, dtype=dtype)
+
+
+def spcfft(u, isign):
+    """Glassman 1D FFT of a complex vector (src/glassman.f90:29-67)."""
+    u = np.array(u, dtype=np.complex128)
+    work = np.empty_like(u)
+    lib().oracle_spcfft(_ptr(u), u.size, int(isign), _ptr(work))
+    return u
+
+
+def _skip_arr(skip):
+    if skip is None:
+        return None
+    return (ctypes.c_int * 3)(*[int(bool(s)) for s in skip])
+
+
+def fft3d_c2c(x, isign, fmt=PHYSICAL_IN_X, skip=None):
+    """One-rank decomp_2d_fft_3d c2c (src/fft_common_3d.f90:9-116)."""
+    x = _fortran(x, np.complex128)
+    nx, ny, nz = x.shape
+    out = np.empty((nx, ny, nz), dtype=np.complex128, order="F")
+    rc = lib().oracle_fft3d_c2c(fmt, nx, ny, nz, _ptr(x), _ptr(out), int(isign), _skip_arr(skip))
+    if rc:
+        raise ValueError("oracle_fft3d_c2c failed: %d" % rc)
+    return out
+
+
+def spectral_shape(fmt, nx, ny, nz):
+    """Shape of the r2c output (src/fft_common.f90:210-216)."""
+    return (nx // 2 + 1, ny, nz) if fmt == PHYSICAL_IN_X else (nx, ny, nz // 2 + 1)
+
+
+def fft3d_r2c(x, fmt=PHYSICAL_IN_X, skip=None):
+    """One-rank decomp_2d_fft_3d r2c (src/fft_common_3d.f90:121-189)."""
+    x = _fortran(x, np.float64)
+    nx, ny, nz = x.shape
+    out = np.empty(spectral_shape(fmt, nx, ny, nz), dtype=np.complex128, order="F")
+    rc = lib().oracle_fft3d_r2c(fmt, nx, ny, nz, _ptr(x), _ptr(out), _skip_arr(skip))
+    if rc:
+        raise ValueError("oracle_fft3d_r2c failed: %d" % rc)
+    return out
+
+
+def fft3d_c2r(c, shape, fmt=PHYSICAL_IN_X, skip=None):
+    """One-rank decomp_2d_fft_3d c2r (src/fft_common_3d.f90:194-296)."""
+    nx, ny, nz = shape
+    c = _fortran(c, np.complex128)
+    assert c.shape == spectral_shape(fmt, nx, ny, nz)
+    out = np.empty((nx, ny, nz), dtype=np.float64, order="F")
+    rc = lib().oracle_fft3d_c2r(fmt, nx, ny, nz, _ptr(c), _ptr(out), _skip_arr(skip))
+    if rc:
+        raise ValueError("oracle_fft3d_c2r failed: %d" % rc)
+    return out
+
+
+def dft3d_ld(x, isign):
+    """Direct 3D DFT accumulated in long double (tiny grids only)."""
+    x = _fortran(x, np.complex128)
+    out = np.empty_like(x, order="F")
+    nx, ny, nz = x.shape
+    lib().oracle_dft3d_ld(nx, ny, nz, _ptr(x), _ptr(out), int(isign))
+    return out
+
+
+def sample_stage_times(nx, ny, nz, lines, nthreads=0):
+    t = (ctypes.c_double * 6)()
+    used = lib().oracle_sample_stage_times(nx, ny, nz, lines, nthreads, t)
+    if used < 0:
+        raise MemoryError("oracle_sample_stage_times could not reserve the grid")
+    return list(t), used
+
+
+# ---- decomposition metadata -------------------------------------------------
+
+class OracleDecomp(ctypes.Structure):
+    _fields_ = [
+        ("xst", ctypes.c_int * 3), ("xen", ctypes.c_int * 3), ("xsz", ctypes.c_int * 3),
+        ("yst", ctypes.c_int * 3), ("yen", ctypes.c_int * 3), ("ysz", ctypes.c_int * 3),
+        ("zst", ctypes.c_int * 3), ("zen", ctypes.c_int * 3), ("zsz", ctypes.c_int * 3),
+        ("p_row", ctypes.c_int), ("p_col", ctypes.c_int),
+        ("x1dist", ctypes.c_int * 64), ("y1dist", ctypes.c_int * 64),
+        ("y2dist", ctypes.c_int * 64), ("z2dist", ctypes.c_int * 64),
+        ("x1cnts", ctypes.c_int64 * 64), ("y1cnts", ctypes.c_int64 * 64),
+        ("x1disp", ctypes.c_int64 * 64), ("y1disp", ctypes.c_int64 * 64),
+        ("y2cnts", ctypes.c_int64 * 64), ("z2cnts", ctypes.c_int64 * 64),
+        ("y2disp", ctypes.c_int64 * 64), ("z2disp", ctypes.c_int64 * 64),
+        ("x1count", ctypes.c_int64), ("y1count", ctypes.c_int64),
+        ("y2count", ctypes.c_int64), ("z2count", ctypes.c_int64),
+        ("even", ctypes.c_int),
+    ]
+
+
+def distribute(data1, proc):
+    st = np.zeros(proc, np.int32)
+    en = np.zeros(proc, np.int32)
+    sz = np.zeros(proc, np.int32)
+    rc = lib().oracle_distribute(data1, proc, _ptr(st), _ptr(en), _ptr(sz))
+    if rc:
+        raise ValueError("distribute failed: %d" % rc)
+    return st, en, sz
+
+
+def decomp_info(nx, ny, nz, p_row, p_col, coord_row=0, coord_col=0):
+    d = OracleDecomp()
+    rc = lib().oracle_decomp_info_init(nx, ny, nz, p_row, p_col, coord_row, coord_col,
+                                       ctypes.byref(d))
+    if rc:
+        raise ValueError("decomp_info_init failed: %d" % rc)
+    return d
+
+
+XY, YZ, ZY, YX = 0, 1, 2, 3
+
+
+def mem_split(direction, src, d, even=False):
+    """Send buffer of one rank (mem_split_* of src/transpose_*.f90)."""
+    src = np.asfortranarray(src)
+    n1, n2, n3 = src.shape
+    nbuf = max(src.size, _padded_size(direction, d, 1) if even else 0)
+    out = np.zeros(nbuf, dtype=src.dtype)
+    lib().oracle_mem_split(direction, _ptr(src), n1, n2, n3, _ptr(out), ctypes.byref(d),
+                           src.itemsize, int(bool(even)))
+    return out
+
+
+def mem_merge(direction, buf, dst_shape, dtype, d, even=False):
+    """Receive-buffer unpack of one rank (mem_merge_* of src/transpose_*.f90)."""
+    buf = np.ascontiguousarray(buf, dtype=dtype)
+    n1, n2, n3 = dst_shape
+    out = np.zeros(dst_shape, dtype=dtype, order="F")
+    lib().oracle_mem_merge(direction, _ptr(buf), n1, n2, n3, _ptr(out), ctypes.byref(d),
+                           out.itemsize, int(bool(even)))
+    return out
+
+
+def _padded_size(direction, d, split):
+    if direction in (XY, YX):
+        return int(d.x1count) * d.p_row
+    return int(d.y2count) * d.p_col
