@@ -107,6 +107,9 @@ def lib():
             "d2dhip_plan_set_stream": ([pl, vp], ci),
             "d2dhip_plan_get_sizes": ([pl, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
             "d2dhip_plan_device_bytes": ([pl], cll),
+            "d2dhip_plan_profile": ([pl, ci], ci),
+            "d2dhip_plan_profile_report": ([pl, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ci)], ci),
             "d2dhip_exec_c2c": ([pl, vp, vp, ci], ci),
             "d2dhip_exec_r2c": ([pl, vp, vp], ci),
             "d2dhip_exec_c2r": ([pl, vp, vp], ci),
@@ -218,6 +221,19 @@ class FFTEngine:
 
     def device_bytes(self):
         return lib().d2dhip_plan_device_bytes(self._plan)
+
+    STAGES = ("c2c_x", "c2c_y", "c2c_z", "r2c", "spec_y", "spec_last", "c2r")
+
+    def profile(self, enable=True):
+        """Bracket every kernel launch with a hipEvent pair (stage timing)."""
+        check(lib().d2dhip_plan_profile(self._plan, int(bool(enable))))
+
+    def profile_report(self):
+        """{stage: (total_ms, launches)} since the last report."""
+        ms = (ctypes.c_double * 7)()
+        n = (ctypes.c_int * 7)()
+        check(lib().d2dhip_plan_profile_report(self._plan, ms, n))
+        return {self.STAGES[i]: (ms[i], n[i]) for i in range(7) if n[i]}
 
     def fft_c2c(self, inp, out, isign):
         self._sync_stream()

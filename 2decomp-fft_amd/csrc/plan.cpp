@@ -39,6 +39,14 @@ struct d2dhip_plan_s {
   AxisDesc *d_desc = nullptr;
   void *wk = nullptr;  // c2r workspace (spectral pencil)
   size_t wk_bytes = 0;
+  // stage timing: an event pair around every launch while enabled
+  int prof = 0;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
 };
 
 namespace {
@@ -318,10 +326,34 @@ int make_stages(d2dhip_plan_s *p) {
                    "hipMemcpy(stage descriptors)");
 }
 
+hipEvent_t take_event(d2dhip_plan_s *p) {
+  if (!p->pool.empty()) {
+    hipEvent_t e = p->pool.back();
+    p->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
 int run(d2dhip_plan_s *p, int idx, const void *in, void *out, int conj) {
   const AxisIo io = {in, out, conj};
-  return launch_axis(p->rb, p->stage_io[idx], p->d_desc + idx, p->h_desc[idx], io, p->shape[idx],
-                     p->stream);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (p->prof) {
+    a = take_event(p);
+    b = take_event(p);
+    if (!a || !b) return set_error(D2DHIP_ERR_HIP, "hipEventCreate failed");
+    int st = hip_check(hipEventRecord(a, p->stream), "hipEventRecord");
+    if (st) return st;
+  }
+  int st = launch_axis(p->rb, p->stage_io[idx], p->d_desc + idx, p->h_desc[idx], io,
+                       p->shape[idx], p->stream);
+  if (p->prof) {
+    if (!st) st = hip_check(hipEventRecord(b, p->stream), "hipEventRecord");
+    p->recs.push_back({idx, a, b});
+  }
+  return st;
 }
 
 }  // namespace
@@ -375,6 +407,11 @@ int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, in
 
 int d2dhip_plan_destroy(d2dhip_plan p) {
   if (!p) return D2DHIP_SUCCESS;
+  for (auto &r : p->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : p->pool) (void)hipEventDestroy(e);
   for (auto &a : p->ax)
     if (a.d_tw) (void)hipFree(a.d_tw);
   if (p->wk) (void)hipFree(p->wk);
@@ -402,6 +439,34 @@ int d2dhip_plan_get_sizes(d2dhip_plan p, int ph_sz[3], int sp_sz[3]) {
     sp_sz[2] = p->format == D2DHIP_PHYSICAL_IN_Z ? p->nz / 2 + 1 : p->nz;
   }
   return D2DHIP_SUCCESS;
+}
+
+int d2dhip_plan_profile(d2dhip_plan p, int enable) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  p->prof = enable ? 1 : 0;
+  return D2DHIP_SUCCESS;
+}
+
+int d2dhip_plan_profile_report(d2dhip_plan p, double ms[7], int launches[7]) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  for (int i = 0; i < d2dhip_plan_s::kStages; ++i) {
+    if (ms) ms[i] = 0.0;
+    if (launches) launches[i] = 0;
+  }
+  int st = D2DHIP_SUCCESS;
+  for (auto &r : p->recs) {
+    if (!st) st = hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
+    float t = 0.f;
+    if (!st) st = hip_check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
+    if (!st) {
+      if (ms) ms[r.stage] += t;
+      if (launches) launches[r.stage] += 1;
+    }
+    p->pool.push_back(r.a);
+    p->pool.push_back(r.b);
+  }
+  p->recs.clear();
+  return st;
 }
 
 long long d2dhip_plan_device_bytes(d2dhip_plan p) {

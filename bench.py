@@ -1,0 +1,200 @@
+"""Benchmark of the single-GPU 3D FFT path: r2c + c2r round trip at 1024^3.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--n 1024] [--no-cpu]
+
+One step is decomp_2d_fft_3d(r2c) followed by decomp_2d_fft_3d(c2r) on a
+1024^3 FP64 x-pencil (PHYSICAL_IN_X) that already sits in HBM: the modal
+field of 32 Fourier modes with wavenumbers in [-128,128]^3, amplitudes
+10^U(-6,2) with random signs, plus N(0, 1e-6) noise, seed 3 (synthetic).
+
+Printed as one JSON line: ms per fwd+bwd (`value`), the 3-pass HBM model's
+effective bandwidth and roofline fraction, per-stage times from hipEvents
+recorded around every kernel launch of the timed region, the FP32 variant,
+and a CPU baseline: the Glassman restatement of the reference's generic
+backend (oracle/) timed on a bounded sample of the same workload's lines on
+the host cores and scaled to the full round trip.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "fwd+bwd 3D FFT ms & HBM-roofline % at 1024³ r2c, FP64 vs split-BF16 MFMA"
+
+
+def stage_bytes(n, rb):
+    """Algorithmic bytes of each stage launch (one read + one write of its
+    pencils), r2c/c2r PHYSICAL_IN_X at n^3 with rb bytes per real."""
+    nxc = n // 2 + 1
+    s_real = n ** 3 * rb
+    s_spec = nxc * n * n * 2 * rb
+    return {"r2c": s_real + s_spec, "spec_y": 2 * s_spec, "spec_last": 2 * s_spec,
+            "c2r": s_spec + s_real}
+
+
+def round_trip_bytes(n, rb):
+    b = stage_bytes(n, rb)
+    return 2 * (b["r2c"] + b["spec_y"] + b["spec_last"])
+
+
+def time_round_trips(torch, eng, x, spec, y, steps, warmup, profile):
+    for _ in range(warmup):
+        eng.fft_r2c(x, spec)
+        eng.fft_c2r(spec, y)
+    torch.cuda.synchronize()
+    if profile:
+        eng.profile(True)
+        eng.profile_report()  # drop anything recorded before the timed region
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        eng.fft_r2c(x, spec)
+        eng.fft_c2r(spec, y)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    rep = eng.profile_report() if profile else {}
+    if profile:
+        eng.profile(False)
+    return ms, rep
+
+
+def cpu_baseline(n, budget_lines):
+    import oracle as O
+    ncores = min(os.cpu_count() or 1, 16)
+    t0 = time.time()
+    t, used = O.sample_stage_times(n, n, n, budget_lines, ncores)
+    wall = time.time() - t0
+    nxc = n // 2 + 1
+    lines = [n * n, nxc * n, nxc * n, nxc * n, nxc * n, n * n]  # x, y, z, z, y, x
+    total_s = sum(tl * nl for tl, nl in zip(t, lines))
+    return {
+        "value": total_s * 1e3,
+        "unit": "ms per fwd+bwd (extrapolated)",
+        "cores": used,
+        "kind": "port",
+        "sample": ("Glassman restatement of the generic backend (oracle/glassman.c), "
+                   "%d lines of each of the 6 stages of the %d^3 r2c+c2r round trip at their "
+                   "real strides, %d OpenMP threads, %.1f s wall, scaled by lines per stage"
+                   % (budget_lines, n, used, wall)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-lines", type=int, default=16384)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs the 3D FFT path on one GPU only; "
+              "--gpus %d is not supported (multi-GPU pencils need the alltoallv exchange, "
+              "which is out of scope)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import importlib
+    import torch
+    d2d = importlib.import_module("2decomp-fft_amd")
+    n = args.n
+    nxc = n // 2 + 1
+    print("bench: n=%d steps=%d warmup=%d" % (n, args.steps, args.warmup), file=sys.stderr,
+          flush=True)
+
+    # ---- FP64 drop-in -------------------------------------------------------
+    modes = d2d.modes_make(32, 128, 1, -6.0, 2.0, 1e-6, 3)
+    x = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
+    d2d.fill_modal(x, n, n, n, d2d.R8, modes)
+    spec = torch.empty(nxc * n * n, dtype=torch.complex128, device="cuda")
+    y = torch.empty_like(x)
+    eng = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, prec=d2d.PREC_F64)
+    ms, rep = time_round_trips(torch, eng, x, spec, y, args.steps, args.warmup, profile=True)
+    print("bench: fp64 %.3f ms per fwd+bwd" % ms, file=sys.stderr, flush=True)
+    rt_err = float(torch.linalg.vector_norm(y / n ** 3 - x) / torch.linalg.vector_norm(x))
+    spec64 = spec.clone()
+
+    bytes_rt = round_trip_bytes(n, 8)
+    gbs = bytes_rt / (ms * 1e-3) / 1e9
+    sb = stage_bytes(n, 8)
+    stages = {}
+    for name, (tot, cnt) in rep.items():
+        avg = tot / cnt
+        stages[name] = {"avg_ms": round(avg, 4), "launches": cnt,
+                        "GBps": round(sb[name] / (avg * 1e-3) / 1e9, 1)}
+    dom = max(rep.items(), key=lambda kv: kv[1][0])[0]
+    dom_avg = rep[dom][0] / rep[dom][1]
+    achieved = sb[dom] / (dom_avg * 1e-3) / 1e9
+
+    # device-to-device copy bandwidth of the same bytes, for context
+    cpy = torch.empty_like(spec)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    cpy.copy_(spec)
+    e0.record()
+    for _ in range(5):
+        cpy.copy_(spec)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * spec.numel() * 16 / (e0.elapsed_time(e1) / 5 * 1e-3) / 1e9
+    del cpy
+
+    # ---- FP32 VALU variant ---------------------------------------------------
+    variants = {}
+    del y
+    eng32 = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, prec=d2d.PREC_F32)
+    x32 = x.float()
+    spec32 = torch.empty(nxc * n * n, dtype=torch.complex64, device="cuda")
+    y32 = torch.empty_like(x32)
+    ms32, _ = time_round_trips(torch, eng32, x32, spec32, y32, args.steps, args.warmup, False)
+    err32 = float(torch.linalg.vector_norm(spec32.to(torch.complex128) - spec64) /
+                  torch.linalg.vector_norm(spec64))
+    maxabs32 = float((spec32.to(torch.complex128) - spec64).abs().max())
+    variants["fp32"] = {"ms": round(ms32, 3), "rel_l2_vs_fp64": err32, "max_abs_vs_fp64": maxabs32,
+                        "GBps_3pass": round(round_trip_bytes(n, 4) / (ms32 * 1e-3) / 1e9, 1),
+                        "speedup_vs_fp64": round(ms / ms32, 3)}
+    variants["split_bf16"] = {"status": "not built yet"}
+    del x32, spec32, y32, eng32
+
+    line = {
+        "metric": METRIC,
+        "value": round(ms, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "r2c+c2r FP64 %d^3 PHYSICAL_IN_X, modal field seed 3" % n,
+                   "grid": [n, n, n], "format": "PHYSICAL_IN_X", "precision": "f64"},
+        "GBps_3pass": round(gbs, 1),
+        "hbm_frac_3pass": round(gbs / PEAK_HBM_GBS, 4),
+        "copy_GBps": round(copy_gbs, 1),
+        "round_trip_rel_l2": rt_err,
+        "stages": stages,
+        "roofline": {"bound": "hbm", "kernel": "axis_fft_kernel (%s stage)" % dom,
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+        "variants": variants,
+    }
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(n, args.cpu_lines)
+        except Exception as exc:  # report, do not hide
+            line["cpu_baseline"] = {"error": repr(exc)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -117,6 +117,15 @@ int d2dhip_plan_set_stream(d2dhip_plan plan, void *stream);
  * (nx/2+1,ny,nz) for PHYSICAL_IN_X, (nx,ny,nz/2+1) for PHYSICAL_IN_Z
  * (decomp_2d_fft_get_size, src/fft_common.f90:301-327). */
 int d2dhip_plan_get_sizes(d2dhip_plan plan, int ph_sz[3], int sp_sz[3]);
+/* Stage timing, the counterpart of the reference's profiler regions
+ * (decomp_profiler_start/end, src/profiler_caliper.f90:156-177): while
+ * enabled, every kernel launch of the plan is bracketed by a hipEvent pair
+ * on the plan's stream.  The report synchronises on those events, returns
+ * the summed milliseconds and launch counts per stage and clears them.
+ * Stages: 0-2 c2c along x, y, z; 3 the r2c stage; 4 the spectral y stage;
+ * 5 the other spectral c2c stage (z for PHYSICAL_IN_X, x for _Z); 6 c2r. */
+int d2dhip_plan_profile(d2dhip_plan plan, int enable);
+int d2dhip_plan_profile_report(d2dhip_plan plan, double ms[7], int launches[7]);
 /* Device bytes the plan holds (twiddles + workspace). */
 long long d2dhip_plan_device_bytes(d2dhip_plan plan);
 

@@ -344,7 +344,8 @@ int oracle_fft3d_c2r(int format, int nx, int ny, int nz, const double *in,
  * Times the generic backend's six per-line stages of an r2c+c2r round trip
  * (PHYSICAL_IN_X) on `lines` lines each, gathered and scattered at the real
  * strides of the full grid.  The full-size arrays are reserved with calloc
- * so that only the pages the sampled lines touch are ever committed. */
+ * so that only the pages the sampled lines touch are ever committed; each
+ * stage runs twice and the second (warm-page) run is timed. */
 #include <time.h>
 static double now_s(void) {
   struct timespec ts;
@@ -376,7 +377,8 @@ int oracle_sample_stage_times(int nx, int ny, int nz, int lines, int nthreads,
   /* Lines are taken as consecutive runs in the fastest batch index so the
    * touched pages stay few; each stage's lines match the reference's loop
    * order (src/fft_generic.f90 loops). */
-  for (int s = 0; s < 6; ++s) {
+  for (int s = 0; s < 6; ++s)
+  for (int rep = 0; rep < 2; ++rep) { /* rep 0 touches the pages, rep 1 is timed */
     const int nl = lines;
     double t0 = now_s();
 #pragma omp parallel
