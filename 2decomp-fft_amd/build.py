@@ -58,7 +58,32 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+FORTRAN_EXE = os.path.join(FORTRAN, "fft_hip_tests")
+
+
+def build_fortran():
+    """The Fortran modules (decomp_2d / decomp_2d_fft over the C ABI) and the
+    test program that runs the reference's own checks through them.  Skipped
+    with a notice when amdflang is absent."""
+    flang = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(flang):
+        print("build: amdflang not found, Fortran shim not built", file=sys.stderr)
+        return None
+    mod = os.path.join(OBJ, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [flang, "-cpp", "-O2", "-module-dir", mod,
+           os.path.join(FORTRAN, "decomp_2d_hip.F90"), os.path.join(FORTRAN, "fft_hip_tests.F90"),
+           "-L" + HERE, "-ld2dhip", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib", "-o", FORTRAN_EXE]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJ)
+    if r.returncode != 0:
+        raise RuntimeError("amdflang failed:\n" + r.stderr)
+    return FORTRAN_EXE
 
 
 if __name__ == "__main__":

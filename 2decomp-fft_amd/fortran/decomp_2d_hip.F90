@@ -1,0 +1,845 @@
+!! decomp_2d_hip.F90 -- the 2DECOMP&FFT Fortran API at one rank over libd2dhip.
+!!
+!! Modules decomp_2d_constants, decomp_2d and decomp_2d_fft with the public
+!! names of the reference (src/decomp_2d_constants.f90, src/decomp_2d.f90:98-309,
+!! src/fft_common.f90:31-53) for p_row = p_col = 1, calling the C ABI of
+!! include/d2dhip.h through ISO_C_BINDING.  decomp_2d_fft_3d takes host arrays
+!! (staged through plan-side device buffers) as the reference's CPU backends
+!! do; the *_dev variants take device addresses (type(c_ptr)) as the GPU
+!! backend's OpenACC `host_data use_device` calls do (src/fft_cufft.f90:502-504).
+!! A non-zero status of the C ABI becomes decomp_2d_abort, a status >= 100
+!! decomp_2d_warning, as in the reference's `istat /= 0` checks
+!! (src/fft_cufft.f90:86-94).
+
+module decomp_2d_constants
+
+   use, intrinsic :: iso_fortran_env, only: real32, real64
+
+   implicit none
+
+   integer, parameter, public :: mytype = KIND(0._real64)
+   integer, parameter, public :: mytype_single = KIND(0._real32)
+
+   integer, parameter, public :: D2D_FFT_BACKEND_GENERIC = 0
+   integer, parameter, public :: D2D_FFT_BACKEND_FFTW3 = 1
+   integer, parameter, public :: D2D_FFT_BACKEND_FFTW3_F03 = 2
+   integer, parameter, public :: D2D_FFT_BACKEND_MKL = 3
+   integer, parameter, public :: D2D_FFT_BACKEND_CUFFT = 4
+   integer, parameter, public :: D2D_FFT_BACKEND_HIP = 5
+
+   integer, parameter, public :: DECOMP_2D_FFT_FORWARD = -1
+   integer, parameter, public :: DECOMP_2D_FFT_BACKWARD = 1
+
+   integer, parameter, public :: PHYSICAL_IN_X = 1
+   integer, parameter, public :: PHYSICAL_IN_Z = 3
+
+   logical, parameter, public :: DECOMP_2D_FFT_INPLACE = .false.
+
+end module decomp_2d_constants
+
+module d2dhip_c
+
+   use, intrinsic :: iso_c_binding
+
+   implicit none
+
+   integer(c_int), parameter :: D2DHIP_PREC_F64 = 64, D2DHIP_PREC_F32 = 32
+   integer(c_int), parameter :: D2DHIP_FLAG_INPLACE = 1, D2DHIP_FLAG_SKIP_X = 2
+   integer(c_int), parameter :: D2DHIP_FLAG_SKIP_Y = 4, D2DHIP_FLAG_SKIP_Z = 8
+   integer(c_int), parameter :: D2DHIP_R8 = 0, D2DHIP_C16 = 1
+   integer(c_int), parameter :: HIP_H2D = 1, HIP_D2H = 2, HIP_D2D = 3
+
+   type, bind(C) :: d2dhip_decomp
+      integer(c_int) :: xst(3), xen(3), xsz(3)
+      integer(c_int) :: yst(3), yen(3), ysz(3)
+      integer(c_int) :: zst(3), zen(3), zsz(3)
+      integer(c_int) :: p_row, p_col, coord_row, coord_col
+      integer(c_int) :: x1dist(64), y1dist(64), y2dist(64), z2dist(64)
+      integer(c_long_long) :: x1cnts(64), y1cnts(64), x1disp(64), y1disp(64)
+      integer(c_long_long) :: y2cnts(64), z2cnts(64), y2disp(64), z2disp(64)
+      integer(c_long_long) :: x1count, y1count, y2count, z2count
+      integer(c_int) :: even
+   end type d2dhip_decomp
+
+   interface
+      integer(c_int) function d2dhip_decomp_info_init(nx, ny, nz, p_row, p_col, cr, cc, d) &
+         bind(C, name="d2dhip_decomp_info_init")
+         import :: c_int, d2dhip_decomp
+         integer(c_int), value :: nx, ny, nz, p_row, p_col, cr, cc
+         type(d2dhip_decomp), intent(out) :: d
+      end function
+      integer(c_int) function d2dhip_best_2d_grid(nproc, p_row, p_col) &
+         bind(C, name="d2dhip_best_2d_grid")
+         import :: c_int
+         integer(c_int), value :: nproc
+         integer(c_int), intent(out) :: p_row, p_col
+      end function
+      integer(c_int) function d2dhip_plan_create(plan, fmt, nx, ny, nz, prec, flags, stream) &
+         bind(C, name="d2dhip_plan_create")
+         import :: c_int, c_ptr
+         type(c_ptr), intent(out) :: plan
+         integer(c_int), value :: fmt, nx, ny, nz, prec, flags
+         type(c_ptr), value :: stream
+      end function
+      integer(c_int) function d2dhip_plan_destroy(plan) bind(C, name="d2dhip_plan_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan
+      end function
+      integer(c_int) function d2dhip_exec_c2c(plan, din, dout, isign) &
+         bind(C, name="d2dhip_exec_c2c")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan, din, dout
+         integer(c_int), value :: isign
+      end function
+      integer(c_int) function d2dhip_exec_r2c(plan, din, dout) bind(C, name="d2dhip_exec_r2c")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan, din, dout
+      end function
+      integer(c_int) function d2dhip_exec_c2r(plan, din, dout) bind(C, name="d2dhip_exec_c2r")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan, din, dout
+      end function
+      integer(c_int) function d2dhip_transpose(dir, elem, src, dst, d, stream) &
+         bind(C, name="d2dhip_transpose")
+         import :: c_int, c_ptr, d2dhip_decomp
+         integer(c_int), value :: dir, elem
+         type(c_ptr), value :: src, dst, stream
+         type(d2dhip_decomp), intent(in) :: d
+      end function
+      type(c_ptr) function d2dhip_last_message() bind(C, name="d2dhip_last_message")
+         import :: c_ptr
+      end function
+      ! HIP runtime
+      integer(c_int) function hipMalloc(p, bytes) bind(C, name="hipMalloc")
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), intent(out) :: p
+         integer(c_size_t), value :: bytes
+      end function
+      integer(c_int) function hipFree(p) bind(C, name="hipFree")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: p
+      end function
+      integer(c_int) function hipMemcpy(dst, src, bytes, kind) bind(C, name="hipMemcpy")
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: dst, src
+         integer(c_size_t), value :: bytes
+         integer(c_int), value :: kind
+      end function
+      integer(c_int) function hipDeviceSynchronize() bind(C, name="hipDeviceSynchronize")
+         import :: c_int
+      end function
+   end interface
+
+contains
+
+   function c_message() result(msg)
+      character(len=:), allocatable :: msg
+      character(kind=c_char), pointer :: s(:)
+      type(c_ptr) :: p
+      integer :: n
+      p = d2dhip_last_message()
+      call c_f_pointer(p, s, [4096])
+      n = 0
+      do while (n < 4096)
+         if (s(n + 1) == c_null_char) exit
+         n = n + 1
+      end do
+      allocate (character(len=n) :: msg)
+      msg = transfer(s(1:n), msg)
+   end function c_message
+
+end module d2dhip_c
+
+module decomp_2d
+
+   use, intrinsic :: iso_c_binding
+   use decomp_2d_constants
+   use d2dhip_c
+
+   implicit none
+
+   private
+
+   ! DECOMP_INFO (src/info.f90:19-47), one rank
+   type, public :: decomp_info
+      integer, dimension(3) :: xst, xen, xsz
+      integer, dimension(3) :: yst, yen, ysz
+      integer, dimension(3) :: zst, zen, zsz
+      type(d2dhip_decomp) :: c
+   end type decomp_info
+
+   integer, save, public :: nrank = 0, nproc = 1
+   integer, save, public :: nx_global = 0, ny_global = 0, nz_global = 0
+   integer, save, dimension(2), public :: dims = 1
+   integer, save, dimension(3), public :: xstart, xend, xsize, ystart, yend, ysize, &
+                                          zstart, zend, zsize
+   type(decomp_info), target, save, public :: decomp_main
+
+   public :: decomp_2d_init, decomp_2d_finalize, decomp_info_init, decomp_info_finalize, &
+             decomp_2d_abort, decomp_2d_warning, get_decomp_dims, &
+             transpose_x_to_y, transpose_y_to_z, transpose_z_to_y, transpose_y_to_x, &
+             alloc_x, alloc_y, alloc_z
+
+   interface transpose_x_to_y
+      module procedure tr_real_short, tr_real_xy, tr_cplx_short, tr_cplx_xy
+   end interface
+   interface transpose_y_to_z
+      module procedure tr_real_short, tr_real_yz, tr_cplx_short, tr_cplx_yz
+   end interface
+   interface transpose_z_to_y
+      module procedure tr_real_short, tr_real_zy, tr_cplx_short, tr_cplx_zy
+   end interface
+   interface transpose_y_to_x
+      module procedure tr_real_short, tr_real_yx, tr_cplx_short, tr_cplx_yx
+   end interface
+   interface alloc_x
+      module procedure alloc_x_real, alloc_x_cplx
+   end interface
+   interface alloc_y
+      module procedure alloc_y_real, alloc_y_cplx
+   end interface
+   interface alloc_z
+      module procedure alloc_z_real, alloc_z_cplx
+   end interface
+
+contains
+
+   ! decomp_2d_init (src/decomp_2d_init_fin.f90:15-184) for one rank:
+   ! p_row = p_col = 0 runs best_2d_grid(1) = 1 x 1.
+   subroutine decomp_2d_init(nx, ny, nz, p_row, p_col)
+      integer, intent(in) :: nx, ny, nz, p_row, p_col
+      integer :: pr, pc
+      integer(c_int) :: r, c, st
+      if (nx <= 0) call decomp_2d_abort(__FILE__, __LINE__, nx, "Invalid value for nx")
+      if (ny <= 0) call decomp_2d_abort(__FILE__, __LINE__, ny, "Invalid value for ny")
+      if (nz <= 0) call decomp_2d_abort(__FILE__, __LINE__, nz, "Invalid value for nz")
+      pr = p_row
+      pc = p_col
+      if (pr <= 0 .or. pc <= 0) then
+         st = d2dhip_best_2d_grid(int(nproc, c_int), r, c)
+         pr = r
+         pc = c
+      end if
+      if (pr * pc /= nproc) call decomp_2d_abort(__FILE__, __LINE__, pr * pc, &
+                                                 "Invalid 2D processor grid - nproc /= p_row*p_col")
+      dims = [pr, pc]
+      nx_global = nx
+      ny_global = ny
+      nz_global = nz
+      call decomp_info_init(nx, ny, nz, decomp_main)
+      xstart = decomp_main%xst; xend = decomp_main%xen; xsize = decomp_main%xsz
+      ystart = decomp_main%yst; yend = decomp_main%yen; ysize = decomp_main%ysz
+      zstart = decomp_main%zst; zend = decomp_main%zen; zsize = decomp_main%zsz
+   end subroutine decomp_2d_init
+
+   subroutine decomp_2d_finalize()
+      nx_global = 0
+      ny_global = 0
+      nz_global = 0
+   end subroutine decomp_2d_finalize
+
+   function get_decomp_dims()
+      integer, dimension(2) :: get_decomp_dims
+      get_decomp_dims = dims
+   end function get_decomp_dims
+
+   subroutine decomp_info_init(nx, ny, nz, decomp)
+      integer, intent(in) :: nx, ny, nz
+      type(decomp_info), intent(inout) :: decomp
+      integer(c_int) :: st
+      st = d2dhip_decomp_info_init(int(nx, c_int), int(ny, c_int), int(nz, c_int), &
+                                   int(dims(1), c_int), int(dims(2), c_int), 0_c_int, 0_c_int, &
+                                   decomp%c)
+      if (st /= 0) call decomp_2d_abort(__FILE__, __LINE__, int(st), c_message())
+      decomp%xst = decomp%c%xst; decomp%xen = decomp%c%xen; decomp%xsz = decomp%c%xsz
+      decomp%yst = decomp%c%yst; decomp%yen = decomp%c%yen; decomp%ysz = decomp%c%ysz
+      decomp%zst = decomp%c%zst; decomp%zen = decomp%c%zen; decomp%zsz = decomp%c%zsz
+   end subroutine decomp_info_init
+
+   subroutine decomp_info_finalize(decomp)
+      type(decomp_info), intent(inout) :: decomp
+      decomp%xsz = 0
+   end subroutine decomp_info_finalize
+
+   ! decomp_2d_abort (src/decomp_2d_mpi.f90:145-191): report and stop.
+   subroutine decomp_2d_abort(file, line, errorcode, msg)
+      character(len=*), intent(in) :: file, msg
+      integer, intent(in) :: line, errorcode
+      write (*, *) '2DECOMP&FFT ERROR - errorcode: ', errorcode
+      write (*, *) 'ERROR MESSAGE: '//msg
+      write (*, *) 'File: '//file, ' line: ', line
+      error stop 1
+   end subroutine decomp_2d_abort
+
+   ! decomp_2d_warning (src/decomp_2d_mpi.f90:259-299)
+   subroutine decomp_2d_warning(file, line, warncode, msg)
+      character(len=*), intent(in) :: file, msg
+      integer, intent(in) :: line, warncode
+      write (*, *) '2DECOMP&FFT WARNING - errorcode: ', warncode
+      write (*, *) 'ERROR MESSAGE: '//msg
+      write (*, *) 'File: '//file, ' line: ', line
+   end subroutine decomp_2d_warning
+
+   ! One-rank transposes: dst = src (src/transpose_x_to_y.f90:41-50 and the
+   ! matching branches of the other three files).
+   subroutine tr_real_short(src, dst)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      dst = src
+   end subroutine
+   subroutine tr_cplx_short(src, dst)
+      complex(mytype), dimension(:, :, :), intent(in) :: src
+      complex(mytype), dimension(:, :, :), intent(out) :: dst
+      dst = src
+   end subroutine
+   subroutine tr_real_xy(src, dst, decomp)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%xsz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      dst = src
+   end subroutine
+   subroutine tr_real_yz(src, dst, decomp)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      dst = src
+   end subroutine
+   subroutine tr_real_zy(src, dst, decomp)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%zsz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      dst = src
+   end subroutine
+   subroutine tr_real_yx(src, dst, decomp)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      dst = src
+   end subroutine
+   subroutine tr_cplx_xy(src, dst, decomp)
+      complex(mytype), dimension(:, :, :), intent(in) :: src
+      complex(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%xsz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      dst = src
+   end subroutine
+   subroutine tr_cplx_yz(src, dst, decomp)
+      complex(mytype), dimension(:, :, :), intent(in) :: src
+      complex(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      dst = src
+   end subroutine
+   subroutine tr_cplx_zy(src, dst, decomp)
+      complex(mytype), dimension(:, :, :), intent(in) :: src
+      complex(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%zsz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      dst = src
+   end subroutine
+   subroutine tr_cplx_yx(src, dst, decomp)
+      complex(mytype), dimension(:, :, :), intent(in) :: src
+      complex(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      dst = src
+   end subroutine
+
+   ! alloc_x / alloc_y / alloc_z (src/alloc.f90:10-135): natural (i,j,k) order,
+   ! global index bounds when opt_global is true.
+   subroutine alloc_x_real(var, decomp, opt_global)
+      real(mytype), allocatable, dimension(:, :, :), intent(inout) :: var
+      type(decomp_info), intent(in) :: decomp
+      logical, intent(in), optional :: opt_global
+      if (gbl(opt_global)) then
+         allocate (var(decomp%xst(1):decomp%xen(1), decomp%xst(2):decomp%xen(2), decomp%xst(3):decomp%xen(3)))
+      else
+         allocate (var(decomp%xsz(1), decomp%xsz(2), decomp%xsz(3)))
+      end if
+   end subroutine
+   subroutine alloc_x_cplx(var, decomp, opt_global)
+      complex(mytype), allocatable, dimension(:, :, :), intent(inout) :: var
+      type(decomp_info), intent(in) :: decomp
+      logical, intent(in), optional :: opt_global
+      if (gbl(opt_global)) then
+         allocate (var(decomp%xst(1):decomp%xen(1), decomp%xst(2):decomp%xen(2), decomp%xst(3):decomp%xen(3)))
+      else
+         allocate (var(decomp%xsz(1), decomp%xsz(2), decomp%xsz(3)))
+      end if
+   end subroutine
+   subroutine alloc_y_real(var, decomp, opt_global)
+      real(mytype), allocatable, dimension(:, :, :), intent(inout) :: var
+      type(decomp_info), intent(in) :: decomp
+      logical, intent(in), optional :: opt_global
+      if (gbl(opt_global)) then
+         allocate (var(decomp%yst(1):decomp%yen(1), decomp%yst(2):decomp%yen(2), decomp%yst(3):decomp%yen(3)))
+      else
+         allocate (var(decomp%ysz(1), decomp%ysz(2), decomp%ysz(3)))
+      end if
+   end subroutine
+   subroutine alloc_y_cplx(var, decomp, opt_global)
+      complex(mytype), allocatable, dimension(:, :, :), intent(inout) :: var
+      type(decomp_info), intent(in) :: decomp
+      logical, intent(in), optional :: opt_global
+      if (gbl(opt_global)) then
+         allocate (var(decomp%yst(1):decomp%yen(1), decomp%yst(2):decomp%yen(2), decomp%yst(3):decomp%yen(3)))
+      else
+         allocate (var(decomp%ysz(1), decomp%ysz(2), decomp%ysz(3)))
+      end if
+   end subroutine
+   subroutine alloc_z_real(var, decomp, opt_global)
+      real(mytype), allocatable, dimension(:, :, :), intent(inout) :: var
+      type(decomp_info), intent(in) :: decomp
+      logical, intent(in), optional :: opt_global
+      if (gbl(opt_global)) then
+         allocate (var(decomp%zst(1):decomp%zen(1), decomp%zst(2):decomp%zen(2), decomp%zst(3):decomp%zen(3)))
+      else
+         allocate (var(decomp%zsz(1), decomp%zsz(2), decomp%zsz(3)))
+      end if
+   end subroutine
+   subroutine alloc_z_cplx(var, decomp, opt_global)
+      complex(mytype), allocatable, dimension(:, :, :), intent(inout) :: var
+      type(decomp_info), intent(in) :: decomp
+      logical, intent(in), optional :: opt_global
+      if (gbl(opt_global)) then
+         allocate (var(decomp%zst(1):decomp%zen(1), decomp%zst(2):decomp%zen(2), decomp%zst(3):decomp%zen(3)))
+      else
+         allocate (var(decomp%zsz(1), decomp%zsz(2), decomp%zsz(3)))
+      end if
+   end subroutine
+
+   logical function gbl(opt_global)
+      logical, intent(in), optional :: opt_global
+      gbl = .false.
+      if (present(opt_global)) gbl = opt_global
+   end function gbl
+
+end module decomp_2d
+
+module decomp_2d_fft
+
+   use, intrinsic :: iso_c_binding
+   use decomp_2d_constants
+   use decomp_2d
+   use d2dhip_c
+
+   implicit none
+
+   private
+
+   integer, parameter, public :: D2D_FFT_BACKEND = D2D_FFT_BACKEND_HIP
+
+   ! decomp_2d_fft_engine (src/fft_common.f90 engine API; plan owned by libd2dhip)
+   type, public :: decomp_2d_fft_engine
+      type(c_ptr), private :: plan = c_null_ptr
+      type(c_ptr), private :: d_a = c_null_ptr, d_b = c_null_ptr
+      integer(c_size_t), private :: nbytes = 0
+      integer, private :: format = PHYSICAL_IN_X
+      logical, private :: initialised = .false.
+      integer, private :: nx_fft = 0, ny_fft = 0, nz_fft = 0
+      logical, private :: inplace = .false.
+      logical, private :: skip(3) = .false.
+      integer, private :: slot = 0  ! index in the module's engine array, 0 if external
+      type(decomp_info), public :: ph, sp
+   contains
+      procedure, public :: init => engine_init
+      procedure, public :: fin => engine_fin
+      procedure, public :: use_it => engine_use_it
+      generic, public :: fft => e_c2c, e_r2c, e_c2r
+      procedure, private :: e_c2c => engine_fft_c2c
+      procedure, private :: e_r2c => engine_fft_r2c
+      procedure, private :: e_c2r => engine_fft_c2r
+   end type decomp_2d_fft_engine
+
+   integer, save :: n_grid = 0
+   integer, save :: cur_slot = 0
+   type(decomp_2d_fft_engine), allocatable, target, save :: fft_engines(:)
+   ! the engine decomp_2d_fft_3d works with (use_it switches it, as the
+   ! module-level pointers of src/fft_common.f90:446-479 do)
+   type(decomp_2d_fft_engine), pointer, save :: cur_e => null()
+
+   public :: decomp_2d_fft_init, decomp_2d_fft_3d, decomp_2d_fft_finalize, &
+             decomp_2d_fft_get_size, decomp_2d_fft_get_ph, decomp_2d_fft_get_sp, &
+             decomp_2d_fft_get_ngrid, decomp_2d_fft_set_ngrid, decomp_2d_fft_use_grid, &
+             decomp_2d_fft_get_engine, decomp_2d_fft_get_format, decomp_2d_fft_get_inplace, &
+             decomp_2d_fft_get_inplace_r2c, decomp_2d_fft_get_inplace_c2r, &
+             decomp_2d_fft_3d_dev, decomp_2d_fft_3d_r2c_dev, decomp_2d_fft_3d_c2r_dev
+
+   interface decomp_2d_fft_init
+      module procedure fft_init_noarg
+      module procedure fft_init_arg
+      module procedure fft_init_one_grid
+      module procedure fft_init_multiple_grids
+   end interface
+
+   interface decomp_2d_fft_3d
+      module procedure fft_3d_c2c
+      module procedure fft_3d_r2c
+      module procedure fft_3d_c2r
+   end interface
+
+   interface decomp_2d_fft_3d_dev
+      module procedure fft_3d_c2c_dev
+   end interface
+
+contains
+
+   subroutine check(st, file, line)
+      integer(c_int), intent(in) :: st
+      character(len=*), intent(in) :: file
+      integer, intent(in) :: line
+      if (st == 0) return
+      if (st >= 100) then
+         call decomp_2d_warning(file, line, int(st), c_message())
+      else
+         call decomp_2d_abort(file, line, int(st), c_message())
+      end if
+   end subroutine check
+
+   subroutine fft_init_noarg
+      call fft_init_arg(PHYSICAL_IN_X)
+   end subroutine
+
+   subroutine fft_init_arg(pencil, opt_skip_XYZ_c2c)
+      integer, intent(in) :: pencil
+      logical, dimension(3), intent(in), optional :: opt_skip_XYZ_c2c
+      call fft_init_one_grid(pencil, nx_global, ny_global, nz_global, &
+                             opt_skip_XYZ_c2c=opt_skip_XYZ_c2c)
+   end subroutine
+
+   subroutine fft_init_one_grid(pencil, nx, ny, nz, opt_inplace, opt_inplace_r2c, &
+                                opt_inplace_c2r, opt_skip_XYZ_c2c)
+      integer, intent(in) :: pencil, nx, ny, nz
+      logical, intent(in), optional :: opt_inplace, opt_inplace_r2c, opt_inplace_c2r
+      logical, dimension(3), intent(in), optional :: opt_skip_XYZ_c2c
+      call decomp_2d_fft_set_ngrid(1)
+      call fft_init_multiple_grids(pencil, nx, ny, nz, 1, opt_inplace, opt_inplace_r2c, &
+                                   opt_inplace_c2r, opt_skip_XYZ_c2c)
+   end subroutine
+
+   subroutine fft_init_multiple_grids(pencil, nx, ny, nz, igrid, opt_inplace, &
+                                      opt_inplace_r2c, opt_inplace_c2r, opt_skip_XYZ_c2c)
+      integer, intent(in) :: pencil, nx, ny, nz, igrid
+      logical, intent(in), optional :: opt_inplace, opt_inplace_r2c, opt_inplace_c2r
+      logical, dimension(3), intent(in), optional :: opt_skip_XYZ_c2c
+      if (igrid < 1 .or. igrid > n_grid) &
+         call decomp_2d_abort(__FILE__, __LINE__, igrid, "Invalid value for igrid")
+      call fft_engines(igrid)%init(pencil, nx, ny, nz, opt_inplace, opt_inplace_r2c, &
+                                   opt_inplace_c2r, opt_skip_XYZ_c2c)
+      fft_engines(igrid)%slot = igrid
+      cur_slot = igrid
+      cur_e => fft_engines(igrid)
+   end subroutine
+
+   ! decomp_2d_fft_engine_init (src/fft_common.f90:141-242)
+   subroutine engine_init(engine, pencil, nx, ny, nz, opt_inplace, opt_inplace_r2c, &
+                          opt_inplace_c2r, opt_skip_XYZ_c2c)
+      class(decomp_2d_fft_engine), intent(inout) :: engine
+      integer, intent(in) :: pencil, nx, ny, nz
+      logical, intent(in), optional :: opt_inplace, opt_inplace_r2c, opt_inplace_c2r
+      logical, dimension(3), intent(in), optional :: opt_skip_XYZ_c2c
+      integer(c_int) :: flags, st
+      integer(c_size_t) :: ncplx
+      if (engine%initialised) &
+         call decomp_2d_abort(__FILE__, __LINE__, 4, 'FFT engine should only be initialised once')
+      if (nx <= 0) call decomp_2d_abort(__FILE__, __LINE__, nx, "Invalid value for nx")
+      if (ny <= 0) call decomp_2d_abort(__FILE__, __LINE__, ny, "Invalid value for ny")
+      if (nz <= 0) call decomp_2d_abort(__FILE__, __LINE__, nz, "Invalid value for nz")
+      if (present(opt_inplace_r2c)) &
+         call decomp_2d_abort(__FILE__, __LINE__, -1, 'In-place r2c transform is not yet available')
+      if (present(opt_inplace_c2r)) &
+         call decomp_2d_abort(__FILE__, __LINE__, -1, 'In-place c2r transform is not yet available')
+      if (pencil /= PHYSICAL_IN_X .and. pencil /= PHYSICAL_IN_Z) &
+         call decomp_2d_abort(__FILE__, __LINE__, pencil, "Invalid value for pencil")
+      engine%format = pencil
+      engine%nx_fft = nx
+      engine%ny_fft = ny
+      engine%nz_fft = nz
+      engine%inplace = DECOMP_2D_FFT_INPLACE
+      if (present(opt_inplace)) engine%inplace = opt_inplace
+      engine%skip = .false.
+      if (present(opt_skip_XYZ_c2c)) engine%skip = opt_skip_XYZ_c2c
+      call decomp_info_init(nx, ny, nz, engine%ph)
+      if (pencil == PHYSICAL_IN_X) then
+         call decomp_info_init(nx / 2 + 1, ny, nz, engine%sp)
+      else
+         call decomp_info_init(nx, ny, nz / 2 + 1, engine%sp)
+      end if
+      flags = 0
+      if (engine%inplace) flags = ior(flags, D2DHIP_FLAG_INPLACE)
+      if (engine%skip(1)) flags = ior(flags, D2DHIP_FLAG_SKIP_X)
+      if (engine%skip(2)) flags = ior(flags, D2DHIP_FLAG_SKIP_Y)
+      if (engine%skip(3)) flags = ior(flags, D2DHIP_FLAG_SKIP_Z)
+      st = d2dhip_plan_create(engine%plan, int(pencil, c_int), int(nx, c_int), int(ny, c_int), &
+                              int(nz, c_int), D2DHIP_PREC_F64, flags, c_null_ptr)
+      call check(st, __FILE__, __LINE__)
+      ! staging buffers for host-array calls: the larger of the complex
+      ! physical pencil and the complex spectral pencil
+      ncplx = max(int(nx, c_size_t) * ny * nz, int(product(engine%sp%xsz), c_size_t))
+      engine%nbytes = 16_c_size_t * ncplx
+      call check(hipMalloc(engine%d_a, engine%nbytes), __FILE__, __LINE__)
+      call check(hipMalloc(engine%d_b, engine%nbytes), __FILE__, __LINE__)
+      engine%initialised = .true.
+   end subroutine engine_init
+
+   subroutine engine_fin(engine)
+      class(decomp_2d_fft_engine), intent(inout) :: engine
+      integer(c_int) :: st
+      if (.not. engine%initialised) return
+      st = d2dhip_plan_destroy(engine%plan)
+      st = hipFree(engine%d_a)
+      st = hipFree(engine%d_b)
+      engine%plan = c_null_ptr
+      engine%initialised = .false.
+   end subroutine engine_fin
+
+   subroutine engine_use_it(engine, opt_force)
+      class(decomp_2d_fft_engine), intent(in), target :: engine
+      logical, intent(in), optional :: opt_force
+      logical :: force
+      force = .false.
+      if (present(opt_force)) force = opt_force
+      if (.not. engine%initialised .and. .not. force) &
+         call decomp_2d_abort(__FILE__, __LINE__, -1, "FFT engine is not ready")
+      select type (engine)
+      type is (decomp_2d_fft_engine)
+         cur_e => engine
+         cur_slot = engine%slot
+      end select
+   end subroutine engine_use_it
+
+   function current() result(e)
+      type(decomp_2d_fft_engine), pointer :: e
+      if (.not. associated(cur_e)) &
+         call decomp_2d_abort(__FILE__, __LINE__, -1, 'FFT library must be initialised first')
+      e => cur_e
+   end function current
+
+   subroutine decomp_2d_fft_finalize
+      integer :: i
+      do i = 1, n_grid
+         call fft_engines(i)%fin()
+      end do
+      n_grid = 0
+      cur_slot = 0
+      nullify (cur_e)
+      if (allocated(fft_engines)) deallocate (fft_engines)
+   end subroutine
+
+   subroutine decomp_2d_fft_set_ngrid(ngrd)
+      integer, intent(in) :: ngrd
+      type(decomp_2d_fft_engine), allocatable :: tmp(:)
+      if (ngrd < 1) call decomp_2d_abort(__FILE__, __LINE__, ngrd, "Invalid value for n_grid")
+      if (allocated(fft_engines)) then
+         call move_alloc(fft_engines, tmp)
+         allocate (fft_engines(ngrd))
+         fft_engines(1:min(ngrd, n_grid)) = tmp(1:min(ngrd, n_grid))
+         deallocate (tmp)
+      else
+         allocate (fft_engines(ngrd))
+      end if
+      n_grid = ngrd
+      if (cur_slot > 0 .and. cur_slot <= n_grid) then
+         cur_e => fft_engines(cur_slot)
+      else if (cur_slot > 0) then
+         cur_slot = 0
+         nullify (cur_e)
+      end if
+   end subroutine
+
+   integer function decomp_2d_fft_get_ngrid()
+      decomp_2d_fft_get_ngrid = n_grid
+   end function
+
+   subroutine decomp_2d_fft_use_grid(igrid)
+      integer, intent(in), optional :: igrid
+      if (present(igrid)) then
+         if (igrid < 1 .or. igrid > n_grid) &
+            call decomp_2d_abort(__FILE__, __LINE__, igrid, "Invalid value for igrid")
+         call fft_engines(igrid)%use_it()
+      else
+         if (n_grid < 1) call decomp_2d_abort(__FILE__, __LINE__, n_grid, &
+                                              "The FFT module was not initialised")
+         call fft_engines(1)%use_it()
+      end if
+   end subroutine
+
+   function decomp_2d_fft_get_engine(igrid)
+      type(decomp_2d_fft_engine), pointer :: decomp_2d_fft_get_engine
+      integer, intent(in), optional :: igrid
+      integer :: g
+      g = 1
+      if (present(igrid)) g = igrid
+      if (g < 1 .or. g > n_grid) call decomp_2d_abort(__FILE__, __LINE__, g, "Invalid value for igrid")
+      decomp_2d_fft_get_engine => fft_engines(g)
+   end function
+
+   function decomp_2d_fft_get_ph()
+      type(decomp_info), pointer :: decomp_2d_fft_get_ph
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      decomp_2d_fft_get_ph => e%ph
+   end function
+
+   function decomp_2d_fft_get_sp()
+      type(decomp_info), pointer :: decomp_2d_fft_get_sp
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      decomp_2d_fft_get_sp => e%sp
+   end function
+
+   ! decomp_2d_fft_get_size (src/fft_common.f90:301-327)
+   subroutine decomp_2d_fft_get_size(istart, iend, isize)
+      integer, dimension(3), intent(out) :: istart, iend, isize
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      if (e%format == PHYSICAL_IN_X) then
+         istart = e%sp%zst
+         iend = e%sp%zen
+         isize = e%sp%zsz
+      else
+         istart = e%sp%xst
+         iend = e%sp%xen
+         isize = e%sp%xsz
+      end if
+   end subroutine
+
+   integer function decomp_2d_fft_get_format()
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      decomp_2d_fft_get_format = e%format
+   end function
+   logical function decomp_2d_fft_get_inplace()
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      decomp_2d_fft_get_inplace = e%inplace
+   end function
+   logical function decomp_2d_fft_get_inplace_r2c()
+      decomp_2d_fft_get_inplace_r2c = .false.
+   end function
+   logical function decomp_2d_fft_get_inplace_c2r()
+      decomp_2d_fft_get_inplace_c2r = .false.
+   end function
+
+   ! ---- transforms on host arrays (staged through the engine's device buffers)
+
+   subroutine fft_3d_c2c(in, out, isign)
+      complex(mytype), dimension(:, :, :), intent(inout), target, contiguous :: in
+      complex(mytype), dimension(:, :, :), intent(out), target, contiguous :: out
+      integer, intent(in) :: isign
+      call run_c2c(current(), in, out, isign)
+   end subroutine
+
+   subroutine fft_3d_r2c(in_r, out_c)
+      real(mytype), dimension(:, :, :), intent(in), target, contiguous :: in_r
+      complex(mytype), dimension(:, :, :), intent(out), target, contiguous :: out_c
+      call run_r2c(current(), in_r, out_c)
+   end subroutine
+
+   subroutine fft_3d_c2r(in_c, out_r)
+      complex(mytype), dimension(:, :, :), intent(inout), target, contiguous :: in_c
+      real(mytype), dimension(:, :, :), intent(out), target, contiguous :: out_r
+      call run_c2r(current(), in_c, out_r)
+   end subroutine
+
+   subroutine run_c2c(e, in, out, isign)
+      type(decomp_2d_fft_engine), intent(inout) :: e
+      complex(mytype), dimension(:, :, :), intent(inout), target, contiguous :: in
+      complex(mytype), dimension(:, :, :), intent(out), target, contiguous :: out
+      integer, intent(in) :: isign
+      integer(c_size_t) :: b
+      b = 16_c_size_t * size(in, kind=c_size_t)
+      if (size(out, kind=c_size_t) /= size(in, kind=c_size_t) .or. b > e%nbytes) &
+         call decomp_2d_abort(__FILE__, __LINE__, 1, "c2c array sizes do not match the engine")
+      call check(hipMemcpy(e%d_a, c_loc(in), b, HIP_H2D), __FILE__, __LINE__)
+      call check(d2dhip_exec_c2c(e%plan, e%d_a, e%d_b, int(isign, c_int)), __FILE__, __LINE__)
+      call check(hipMemcpy(c_loc(out), e%d_b, b, HIP_D2H), __FILE__, __LINE__)
+   end subroutine
+
+   subroutine run_r2c(e, in_r, out_c)
+      type(decomp_2d_fft_engine), intent(inout) :: e
+      real(mytype), dimension(:, :, :), intent(in), target, contiguous :: in_r
+      complex(mytype), dimension(:, :, :), intent(out), target, contiguous :: out_c
+      call check(hipMemcpy(e%d_a, c_loc(in_r), 8_c_size_t * size(in_r, kind=c_size_t), HIP_H2D), &
+                 __FILE__, __LINE__)
+      call check(d2dhip_exec_r2c(e%plan, e%d_a, e%d_b), __FILE__, __LINE__)
+      call check(hipMemcpy(c_loc(out_c), e%d_b, 16_c_size_t * size(out_c, kind=c_size_t), HIP_D2H), &
+                 __FILE__, __LINE__)
+   end subroutine
+
+   subroutine run_c2r(e, in_c, out_r)
+      type(decomp_2d_fft_engine), intent(inout) :: e
+      complex(mytype), dimension(:, :, :), intent(inout), target, contiguous :: in_c
+      real(mytype), dimension(:, :, :), intent(out), target, contiguous :: out_r
+      call check(hipMemcpy(e%d_a, c_loc(in_c), 16_c_size_t * size(in_c, kind=c_size_t), HIP_H2D), &
+                 __FILE__, __LINE__)
+      call check(d2dhip_exec_c2r(e%plan, e%d_a, e%d_b), __FILE__, __LINE__)
+      call check(hipMemcpy(c_loc(out_r), e%d_b, 8_c_size_t * size(out_r, kind=c_size_t), HIP_D2H), &
+                 __FILE__, __LINE__)
+   end subroutine
+
+   ! ---- transforms on device addresses (no staging; stream-ordered then synced)
+
+   subroutine fft_3d_c2c_dev(d_in, d_out, isign)
+      type(c_ptr), intent(in) :: d_in, d_out
+      integer, intent(in) :: isign
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      call check(d2dhip_exec_c2c(e%plan, d_in, d_out, int(isign, c_int)), __FILE__, __LINE__)
+      call check(hipDeviceSynchronize(), __FILE__, __LINE__)
+   end subroutine
+
+   subroutine decomp_2d_fft_3d_r2c_dev(d_in_r, d_out_c)
+      type(c_ptr), intent(in) :: d_in_r, d_out_c
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      call check(d2dhip_exec_r2c(e%plan, d_in_r, d_out_c), __FILE__, __LINE__)
+      call check(hipDeviceSynchronize(), __FILE__, __LINE__)
+   end subroutine
+
+   subroutine decomp_2d_fft_3d_c2r_dev(d_in_c, d_out_r)
+      type(c_ptr), intent(in) :: d_in_c, d_out_r
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      call check(d2dhip_exec_c2r(e%plan, d_in_c, d_out_r), __FILE__, __LINE__)
+      call check(hipDeviceSynchronize(), __FILE__, __LINE__)
+   end subroutine
+
+   ! ---- engine-object wrappers (src/fft_common.f90:560-598)
+
+   subroutine engine_fft_c2c(engine, in, out, isign)
+      class(decomp_2d_fft_engine), intent(inout), target :: engine
+      complex(mytype), dimension(:, :, :), intent(inout), target, contiguous :: in
+      complex(mytype), dimension(:, :, :), intent(out), target, contiguous :: out
+      integer, intent(in) :: isign
+      select type (engine)
+      type is (decomp_2d_fft_engine)
+         call run_c2c(engine, in, out, isign)
+      end select
+   end subroutine
+
+   subroutine engine_fft_r2c(engine, in, out)
+      class(decomp_2d_fft_engine), intent(inout), target :: engine
+      real(mytype), dimension(:, :, :), intent(inout), target, contiguous :: in
+      complex(mytype), dimension(:, :, :), intent(out), target, contiguous :: out
+      select type (engine)
+      type is (decomp_2d_fft_engine)
+         call run_r2c(engine, in, out)
+      end select
+   end subroutine
+
+   subroutine engine_fft_c2r(engine, in, out)
+      class(decomp_2d_fft_engine), intent(inout), target :: engine
+      complex(mytype), dimension(:, :, :), intent(inout), target, contiguous :: in
+      real(mytype), dimension(:, :, :), intent(out), target, contiguous :: out
+      select type (engine)
+      type is (decomp_2d_fft_engine)
+         call run_c2r(engine, in, out)
+      end select
+   end subroutine
+
+end module decomp_2d_fft

@@ -1,0 +1,212 @@
+!! fft_hip_tests.F90 -- the reference's own FFT and transpose checks, run in
+!! Fortran through the decomp_2d / decomp_2d_fft modules over libd2dhip at
+!! one rank (no MPI).  Mirrors:
+!!   examples/fft_physical_x/fft_c2c_x.f90 and fft_r2c_x.f90 (and the _z
+!!     copies): analytic inputs, ntest forward+backward pairs with 1/N,
+!!     mean |err| per point <= 50 * epsilon * ntest;
+!!   examples/fft_multiple_grids/fft_multiple_grids.f90: three grids plus a
+!!     local engine object, decomp_2d_init on (nx+1, ny+1, nz+1);
+!!   examples/test2d/test2d.f90: index field through the four transposes,
+!!     exact compare.
+!! Exits with a non-zero status (decomp_2d_abort / error stop) on failure.
+program fft_hip_tests
+
+   use, intrinsic :: iso_c_binding
+   use decomp_2d_constants
+   use decomp_2d
+   use decomp_2d_fft
+   use d2dhip_c, only: hipMalloc, hipFree, hipMemcpy, HIP_H2D, HIP_D2H
+
+   implicit none
+
+   integer, parameter :: nx = 17, ny = 13, nz = 11, ntest = 10
+   type(decomp_2d_fft_engine) :: local_engine
+   integer :: igrid
+
+   call decomp_2d_init(nx + 1, ny + 1, nz + 1, 0, 0)
+   call check_transposes()
+
+   ! single grid, both formats (fft_c2c_x / fft_r2c_x / fft_*_z)
+   call decomp_2d_fft_init(PHYSICAL_IN_X, nx, ny, nz)
+   call ntest_c2c(ntest, nx, ny, nz, 'c2c X')
+   call ntest_r2c(ntest, nx, ny, nz, 'r2c X')
+   call decomp_2d_fft_finalize
+   call decomp_2d_fft_init(PHYSICAL_IN_Z, nx, ny, nz)
+   call ntest_c2c(ntest, nx, ny, nz, 'c2c Z')
+   call ntest_r2c(ntest, nx, ny, nz, 'r2c Z')
+   call check_device_path(nx, ny, nz)
+   call decomp_2d_fft_finalize
+
+   ! multiple grids (fft_multiple_grids.f90:48-66)
+   call decomp_2d_fft_set_ngrid(3)
+   call decomp_2d_fft_init(PHYSICAL_IN_X, nx, ny, nz, 1)
+   call decomp_2d_fft_init(PHYSICAL_IN_Z, nx, ny, nz, 2)
+   call decomp_2d_fft_init(PHYSICAL_IN_X, nx, ny + 2, nz + 16, 3)
+   call local_engine%init(PHYSICAL_IN_Z, nx + 16, ny + 2, nz)
+   do igrid = 1, 3
+      call decomp_2d_fft_use_grid(igrid)
+      call grid_tests(ntest, igrid)
+   end do
+   call local_engine%use_it()
+   call ntest_c2c(ntest, nx + 16, ny + 2, nz, 'c2c local engine')
+   call ntest_r2c(ntest, nx + 16, ny + 2, nz, 'r2c local engine')
+   call local_engine%fin()
+   call decomp_2d_fft_finalize
+   call decomp_2d_finalize
+   write (*, *) 'fft_hip_tests completed'
+
+contains
+
+   subroutine grid_tests(nt, g)
+      integer, intent(in) :: nt, g
+      type(decomp_info), pointer :: ph
+      character(len=32) :: tag
+      ph => decomp_2d_fft_get_ph()
+      write (tag, '(a,i0)') 'grid ', g
+      call ntest_c2c(nt, ph%xsz(1), ph%xsz(2), ph%xsz(3), trim(tag)//' c2c')
+      call ntest_r2c(nt, ph%xsz(1), ph%xsz(2), ph%xsz(3), trim(tag)//' r2c')
+   end subroutine grid_tests
+
+   ! examples/fft_physical_x/fft_c2c_x.f90:64-154
+   subroutine ntest_c2c(nt, mx, my, mz, tag)
+      integer, intent(in) :: nt, mx, my, mz
+      character(len=*), intent(in) :: tag
+      complex(mytype), allocatable, dimension(:, :, :) :: in, out
+      real(mytype) :: dr, di, error
+      integer :: i, j, k, m
+      allocate (in(mx, my, mz), out(mx, my, mz))
+      do k = 1, mz
+         do j = 1, my
+            do i = 1, mx
+               dr = real(i, mytype) / real(mx, mytype) * real(j, mytype) &
+                    / real(my, mytype) * real(k, mytype) / real(mz, mytype)
+               in(i, j, k) = cmplx(dr, dr, mytype)
+            end do
+         end do
+      end do
+      do m = 1, nt
+         call decomp_2d_fft_3d(in, out, DECOMP_2D_FFT_FORWARD)
+         call decomp_2d_fft_3d(out, in, DECOMP_2D_FFT_BACKWARD)
+         in = in / (real(mx, mytype) * real(my, mytype) * real(mz, mytype))
+      end do
+      error = 0._mytype
+      do k = 1, mz
+         do j = 1, my
+            do i = 1, mx
+               dr = real(i, mytype) / real(mx, mytype) * real(j, mytype) &
+                    / real(my, mytype) * real(k, mytype) / real(mz, mytype)
+               di = dr - aimag(in(i, j, k))
+               dr = dr - real(in(i, j, k), mytype)
+               error = error + sqrt(dr * dr + di * di)
+            end do
+         end do
+      end do
+      error = error / (real(mx, mytype) * real(my, mytype) * real(mz, mytype))
+      write (*, *) tag, ' error / mesh point: ', error
+      if (error > epsilon(error) * 50 * nt) &
+         call decomp_2d_abort(__FILE__, __LINE__, int(log10(error)), tag)
+      deallocate (in, out)
+   end subroutine ntest_c2c
+
+   ! examples/fft_physical_x/fft_r2c_x.f90:66-152
+   subroutine ntest_r2c(nt, mx, my, mz, tag)
+      integer, intent(in) :: nt, mx, my, mz
+      character(len=*), intent(in) :: tag
+      real(mytype), allocatable, dimension(:, :, :) :: in_r
+      complex(mytype), allocatable, dimension(:, :, :) :: out
+      integer, dimension(3) :: fft_start, fft_end, fft_size
+      real(mytype) :: dr, error
+      integer :: i, j, k, m
+      call decomp_2d_fft_get_size(fft_start, fft_end, fft_size)
+      allocate (in_r(mx, my, mz))
+      allocate (out(fft_size(1), fft_size(2), fft_size(3)))
+      do k = 1, mz
+         do j = 1, my
+            do i = 1, mx
+               in_r(i, j, k) = real(i, mytype) / real(mx, mytype) * real(j, mytype) &
+                               / real(my, mytype) * real(k, mytype) / real(mz, mytype)
+            end do
+         end do
+      end do
+      do m = 1, nt
+         call decomp_2d_fft_3d(in_r, out)
+         call decomp_2d_fft_3d(out, in_r)
+         in_r = in_r / (real(mx, mytype) * real(my, mytype) * real(mz, mytype))
+      end do
+      error = 0._mytype
+      do k = 1, mz
+         do j = 1, my
+            do i = 1, mx
+               dr = real(i, mytype) / real(mx, mytype) * real(j, mytype) &
+                    / real(my, mytype) * real(k, mytype) / real(mz, mytype)
+               error = error + abs(in_r(i, j, k) - dr)
+            end do
+         end do
+      end do
+      error = error / (real(mx, mytype) * real(my, mytype) * real(mz, mytype))
+      write (*, *) tag, ' error / mesh point: ', error
+      if (error > epsilon(error) * 50 * nt) &
+         call decomp_2d_abort(__FILE__, __LINE__, int(log10(error)), tag)
+      deallocate (in_r, out)
+   end subroutine ntest_r2c
+
+   ! examples/test2d/test2d.f90:77-199: u = i + (j-1)nx + (k-1)nx*ny through
+   ! x -> y -> z -> y -> x, exact.
+   subroutine check_transposes()
+      real(mytype), allocatable, dimension(:, :, :) :: u1, u2, u3
+      integer :: i, j, k
+      call alloc_x(u1, decomp_main, .true.)
+      call alloc_y(u2, decomp_main, .true.)
+      call alloc_z(u3, decomp_main, .true.)
+      do k = xstart(3), xend(3)
+         do j = xstart(2), xend(2)
+            do i = xstart(1), xend(1)
+               u1(i, j, k) = real(i + (j - 1) * nx_global + (k - 1) * nx_global * ny_global, mytype)
+            end do
+         end do
+      end do
+      call transpose_x_to_y(u1, u2)
+      call transpose_y_to_z(u2, u3)
+      call transpose_z_to_y(u3, u2)
+      call transpose_y_to_x(u2, u1)
+      do k = xstart(3), xend(3)
+         do j = xstart(2), xend(2)
+            do i = xstart(1), xend(1)
+               if (u1(i, j, k) /= real(i + (j - 1) * nx_global + (k - 1) * nx_global * ny_global, mytype)) &
+                  call decomp_2d_abort(__FILE__, __LINE__, 10, "transpose test")
+            end do
+         end do
+      end do
+      write (*, *) 'transposes exact'
+   end subroutine check_transposes
+
+   ! Device-resident data (the GPU backend's use_device path): r2c then c2r
+   ! on hipMalloc'ed buffers, round trip to 1/N.
+   subroutine check_device_path(mx, my, mz)
+      integer, intent(in) :: mx, my, mz
+      real(mytype), allocatable, target :: a(:, :, :), b(:, :, :)
+      type(c_ptr) :: d_r, d_c
+      integer(c_size_t) :: nr, nc
+      integer, dimension(3) :: fs, fe, fz
+      real(mytype) :: err
+      integer(c_int) :: st
+      allocate (a(mx, my, mz), b(mx, my, mz))
+      call random_number(a)
+      call decomp_2d_fft_get_size(fs, fe, fz)
+      nr = 8_c_size_t * mx * my * mz
+      nc = 16_c_size_t * fz(1) * fz(2) * fz(3)
+      st = hipMalloc(d_r, nr)
+      st = ior(st, hipMalloc(d_c, nc))
+      st = ior(st, hipMemcpy(d_r, c_loc(a), nr, HIP_H2D))
+      if (st /= 0) call decomp_2d_abort(__FILE__, __LINE__, int(st), "hip")
+      call decomp_2d_fft_3d_r2c_dev(d_r, d_c)
+      call decomp_2d_fft_3d_c2r_dev(d_c, d_r)
+      st = hipMemcpy(c_loc(b), d_r, nr, HIP_D2H)
+      err = maxval(abs(b / (real(mx, mytype) * my * mz) - a))
+      write (*, *) 'device path max error: ', err
+      if (err > 1e-13_mytype) call decomp_2d_abort(__FILE__, __LINE__, 11, "device path")
+      st = hipFree(d_r)
+      st = hipFree(d_c)
+   end subroutine check_device_path
+
+end program fft_hip_tests

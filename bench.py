@@ -22,6 +22,7 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))  # oracle/oracle.py (cpu_baseline only)
 
 PEAK_HBM_GBS = 8000.0
 METRIC = "fwd+bwd 3D FFT ms & HBM-roofline % at 1024³ r2c, FP64 vs split-BF16 MFMA"

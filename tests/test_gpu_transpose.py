@@ -54,7 +54,8 @@ def test_pack_unpack_bit_exact(d2d, torch_gpu, oracle, case, direction, elem, ev
         dst = torch_gpu.zeros(int(np.prod(dshape)), dtype=send.dtype, device="cuda")
         d2d.unpack(direction, elem, to_dev(torch_gpu, recv), dst, dl, even)
         got_u = dst.cpu().numpy().reshape(dshape, order="F")
-        np.testing.assert_array_equal(got_u.view(np.uint8), want_u.view(np.uint8))
+        np.testing.assert_array_equal(got_u.ravel(order="F").view(np.uint8),
+                                      want_u.ravel(order="F").view(np.uint8))
 
 
 def test_one_rank_transposes_are_copies(d2d, torch_gpu):
