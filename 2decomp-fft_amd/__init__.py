@@ -30,6 +30,7 @@ FLAG_INPLACE = 0x1
 FLAG_SKIP_X = 0x2
 FLAG_SKIP_Y = 0x4
 FLAG_SKIP_Z = 0x8
+FLAG_TUNE = 0x10
 R8, C16, R4, C8 = 0, 1, 2, 3
 X_TO_Y, Y_TO_Z, Z_TO_Y, Y_TO_X = 0, 1, 2, 3
 SUCCESS = 0
@@ -107,6 +108,9 @@ def lib():
             "d2dhip_plan_set_stream": ([pl, vp], ci),
             "d2dhip_plan_get_sizes": ([pl, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
             "d2dhip_plan_device_bytes": ([pl], cll),
+            "d2dhip_plan_stage_kernel": ([pl, ci, ctypes.c_char_p, ci], ci),
+            "d2dhip_plan_stage_candidate": ([pl, ci, ci, ctypes.c_char_p, ci], ci),
+            "d2dhip_plan_set_stage_kernel": ([pl, ci, ctypes.c_char_p], ci),
             "d2dhip_plan_profile": ([pl, ci], ci),
             "d2dhip_plan_profile_report": ([pl, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ci)], ci),
@@ -193,8 +197,10 @@ class FFTEngine:
     """One decomp_2d_fft engine (src/fft_common.f90:141-242) at one rank."""
 
     def __init__(self, pencil, nx, ny, nz, prec=PREC_F64, inplace=False, skip_xyz=None,
-                 stream=None):
+                 stream=None, tune=False):
         flags = FLAG_INPLACE if inplace else 0
+        if tune:
+            flags |= FLAG_TUNE
         if skip_xyz is not None:
             for bit, s in zip((FLAG_SKIP_X, FLAG_SKIP_Y, FLAG_SKIP_Z), skip_xyz):
                 if s:
@@ -218,6 +224,28 @@ class FFTEngine:
         sp = (ctypes.c_int * 3)()
         check(lib().d2dhip_plan_get_sizes(self._plan, ph, sp))
         return tuple(ph), tuple(sp)
+
+    def stage_kernels(self):
+        """{stage: kernel name} of the seven stage slots."""
+        out = {}
+        buf = ctypes.create_string_buffer(128)
+        for i, name in enumerate(self.STAGES):
+            check(lib().d2dhip_plan_stage_kernel(self._plan, i, buf, 128))
+            out[name] = buf.value.decode()
+        return out
+
+    def stage_candidates(self, stage):
+        """Kernel names able to run a stage slot ("general" first)."""
+        i = self.STAGES.index(stage) if isinstance(stage, str) else stage
+        names = []
+        buf = ctypes.create_string_buffer(128)
+        while lib().d2dhip_plan_stage_candidate(self._plan, i, len(names), buf, 128) == 0:
+            names.append(buf.value.decode())
+        return names
+
+    def set_stage_kernel(self, stage, name):
+        i = self.STAGES.index(stage) if isinstance(stage, str) else stage
+        check(lib().d2dhip_plan_set_stage_kernel(self._plan, i, name.encode()))
 
     def device_bytes(self):
         return lib().d2dhip_plan_device_bytes(self._plan)

@@ -1,0 +1,29 @@
+// fast_kernels.h -- registry of the compile-time specialised axis kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "internal.h"
+
+namespace d2dhip {
+
+struct FastVariant {
+  int rb = 8;      // bytes per real (8 FP64, 4 FP32)
+  int io = 0;      // IO_C2C / IO_R2C / IO_C2R
+  int lines = 0;   // 1: contiguous lines (x axis), 0: strided columns
+  int n = 0;       // FFT length
+  int TX = 1, NT = 64;
+  size_t lds_bytes = 0;
+  int ldj = 1, ldt = 1;
+  std::vector<int> radices;
+  int (*launch)(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, hipStream_t st) = nullptr;
+  hipError_t (*prepare)() = nullptr;
+  const char *name = "";
+};
+
+// All specialised variants compiled into the library.
+const std::vector<FastVariant> &fast_registry();
+
+}  // namespace d2dhip

@@ -1,0 +1,328 @@
+// fast_kernels.hip -- compile-time specialised axis stages for hot lengths.
+//
+// Same stage contract as axis_fft_kernel (fft_kernels.hip): one workgroup
+// owns a tile of TX columns of length N, the first Stockham pass reads HBM,
+// the last writes HBM, the passes in between exchange through LDS.  Here N,
+// TX, the block size NT and the radix sequence are template parameters, so
+// every register index, LDS offset, twiddle offset and k = b mod L / g = b
+// div L is resolved at compile time, each thread keeps all of its values of
+// a pass in registers, and LDS is addressed with ds_read/ds_write.
+//
+// Thread mapping: strided axes (COLS) put consecutive threads on
+// consecutive tile columns (t fastest: one 16*TX-byte run per point row);
+// contiguous lines (LINES) put consecutive threads on consecutive points of
+// one line (b fastest: 64 consecutive points per wave).
+#include <hip/hip_runtime.h>
+
+#include "codelets.h"
+#include "fast_kernels.h"
+#include "internal.h"
+
+namespace d2dhip {
+
+template <int... Rs>
+struct RList {
+  static constexpr int P = sizeof...(Rs);
+  static constexpr int r[P] = {Rs...};
+  static constexpr int L(int p) {
+    int l = 1;
+    for (int i = 0; i < p; ++i) l *= r[i];
+    return l;
+  }
+  static constexpr int tw_off(int p) {
+    int o = 0;
+    for (int i = 0; i < p; ++i) o += L(i) * (r[i] - 1);
+    return o;
+  }
+  static constexpr int prod() { return L(P); }
+};
+
+template <typename T, int N, int TX, int NT, bool LINES, class RL>
+struct FastCfg {
+  static constexpr int P = RL::P;
+  static constexpr int RPT = NT / TX;  // butterfly rows per tile column
+  static_assert(NT % TX == 0, "NT must be a multiple of TX");
+  static_assert(RL::prod() == N, "radices must multiply to N");
+  static constexpr int nb(int p) { return (N / RL::r[p] + RPT - 1) / RPT; }
+  static constexpr bool exact(int p) { return (N / RL::r[p]) % RPT == 0; }
+  static constexpr int emax() {
+    int e = 0;
+    for (int p = 0; p < P; ++p) e = e > nb(p) * RL::r[p] ? e : nb(p) * RL::r[p];
+    return e;
+  }
+  static constexpr int LDT = LINES ? ((N % 2 == 0) ? N + 1 : N) : 1;
+  static constexpr int LDJ = LINES ? 1 : TX;
+  static constexpr size_t lds_bytes() {
+    return (size_t)(LINES ? TX * LDT : TX * N) * sizeof(cpx<T>);
+  }
+};
+
+template <typename T, int IO>
+__device__ __forceinline__ cpx<T> fload(const AxisDesc *__restrict__ d, const void *in,
+                                        long long base, bool cvalid, bool bvalid, long long ca,
+                                        int j, int conj) {
+  if constexpr (IO == IO_R2C) {
+    const T *p = reinterpret_cast<const T *>(in);
+    const long long o = base + (long long)j * d->in_sj;
+    cpx<T> v = {T(0), T(0)};
+    if (ca < d->C0r) v.x = p[o];
+    if (ca + 1 < d->C0r) v.y = p[o + d->in_s0];
+    return v;
+  } else {
+    cpx<T> v = {T(0), T(0)};
+    if (cvalid) v = reinterpret_cast<const cpx<T> *>(in)[base + (long long)j * d->in_sj];
+    if (conj) v.y = -v.y;
+    return v;
+  }
+  (void)bvalid;
+}
+
+template <typename T, int IO>
+__device__ __forceinline__ void fstore(const AxisDesc *__restrict__ d, void *out, long long base,
+                                       bool cvalid, long long ca, int j, cpx<T> v, int conj) {
+  if constexpr (IO == IO_C2R) {
+    T *p = reinterpret_cast<T *>(out);
+    const long long o = base + (long long)j * d->out_sj;
+    if (ca < d->C0r) p[o] = v.x;
+    if (ca + 1 < d->C0r) p[o + d->out_s0] = -v.y;
+  } else {
+    if (cvalid) {
+      if (conj) v.y = -v.y;
+      reinterpret_cast<cpx<T> *>(out)[base + (long long)j * d->out_sj] = v;
+    }
+  }
+}
+
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL>
+__global__ void __launch_bounds__(NT)
+    fast_axis_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
+  using C = FastCfg<T, N, TX, NT, LINES, RL>;
+  constexpr int P = C::P, RPT = C::RPT, LDT = C::LDT, LDJ = C::LDJ, E = C::emax();
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
+
+  const long long tiles0 = d->tiles0;
+  const long long tile = (long long)blockIdx.x + (long long)blockIdx.y * gridDim.x;
+  if (tile >= tiles0 * d->C1) return;  // whole block, before any barrier
+  const long long c1 = tile / tiles0;
+  const long long c0b = (tile - c1 * tiles0) * TX;
+  const int tid = threadIdx.x;
+  const int t = LINES ? tid / RPT : tid % TX;
+  const int r0 = LINES ? tid % RPT : tid / TX;
+  const long long c0 = c0b + t;
+  const long long ca = 2 * c0;  // first real column of a pair (real IO)
+  const bool cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
+  const long long in_base = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1;
+  const long long out_base = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1;
+  const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(d->tw);
+  cpx<T> v[E];
+
+  // ---- inputs of pass 0 -------------------------------------------------
+  if constexpr (IO == IO_C2R) {
+    // conj(Z) of the merged half spectra, all N points, into LDS
+    constexpr int NH = N / 2 + 1;
+    const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in);
+    for (int w = tid; w < TX * NH; w += NT) {
+      const int tt = LINES ? w / NH : w % TX;
+      const int k = LINES ? w % NH : w / TX;
+      const long long cc = 2 * (c0b + tt);
+      const long long base = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
+      cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
+      if (cc < d->C0r) A = src[base];
+      if (cc + 1 < d->C0r) B = src[base + d->in_s0];
+      if (k == 0 || 2 * k == N) {
+        A.y = T(0);
+        B.y = T(0);
+      }
+      lds[k * LDJ + tt * LDT] = cpx<T>{A.x - B.y, -(A.y + B.x)};
+      if (k > 0 && 2 * k < N) lds[(N - k) * LDJ + tt * LDT] = cpx<T>{A.x + B.y, A.y - B.x};
+    }
+    __syncthreads();
+    constexpr int R = RL::r[0], NBR = N / R;
+#pragma unroll
+    for (int q = 0; q < C::nb(0); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(0) || b < NBR) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) v[q * R + s] = lds[(b + s * NBR) * LDJ + t * LDT];
+      }
+    }
+  } else {
+    constexpr int R = RL::r[0], NBR = N / R;
+#pragma unroll
+    for (int q = 0; q < C::nb(0); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(0) || b < NBR) {
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+          v[q * R + s] = fload<T, IO>(d, in, in_base, cvalid, true, ca, b + s * NBR, conj);
+      }
+    }
+  }
+
+  // ---- passes -------------------------------------------------------------
+  static_for<0, P>([&](auto pc) __attribute__((always_inline)) {
+    constexpr int p = decltype(pc)::value;
+    constexpr int R = RL::r[p], NBR = N / R, L = RL::L(p);
+    if constexpr (p > 0) {
+      // outputs of pass p-1 -> LDS -> inputs of pass p
+      constexpr int Rq = RL::r[p - 1], NBq = N / Rq, Lq = RL::L(p - 1);
+      if constexpr (p > 1 || IO == IO_C2R) __syncthreads();
+#pragma unroll
+      for (int q = 0; q < C::nb(p - 1); ++q) {
+        const int b = r0 + q * RPT;
+        if (C::exact(p - 1) || b < NBq) {
+          const int k = b % Lq, g = b / Lq;
+#pragma unroll
+          for (int s = 0; s < Rq; ++s)
+            lds[(g * Lq * Rq + k + s * Lq) * LDJ + t * LDT] = v[q * Rq + s];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < C::nb(p); ++q) {
+        const int b = r0 + q * RPT;
+        if (C::exact(p) || b < NBR) {
+#pragma unroll
+          for (int s = 0; s < R; ++s) v[q * R + s] = lds[(b + s * NBR) * LDJ + t * LDT];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < C::nb(p); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(p) || b < NBR) {
+        if constexpr (L > 1) {
+          const int k = b % L;
+          const cpx<T> *tp = tw + RL::tw_off(p) + k * (R - 1);
+#pragma unroll
+          for (int s = 1; s < R; ++s) v[q * R + s] = cmul(v[q * R + s], tp[s - 1]);
+        }
+        dft<T, R>(&v[q * R]);
+      }
+    }
+  });
+
+  // ---- outputs of the last pass ---------------------------------------------
+  constexpr int RZ = RL::r[P - 1], NBZ = N / RZ, LZ = RL::L(P - 1);
+  if constexpr (IO == IO_R2C) {
+    if constexpr (P > 1) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < C::nb(P - 1); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(P - 1) || b < NBZ) {
+        const int k = b % LZ, g = b / LZ;
+#pragma unroll
+        for (int s = 0; s < RZ; ++s) lds[(g * LZ * RZ + k + s * LZ) * LDJ + t * LDT] = v[q * RZ + s];
+      }
+    }
+    __syncthreads();
+    constexpr int NH = N / 2 + 1;
+    cpx<T> *dst = reinterpret_cast<cpx<T> *>(out);
+    const T half = T(0.5);
+    for (int w = tid; w < TX * NH; w += NT) {
+      const int tt = LINES ? w / NH : w % TX;
+      const int k = LINES ? w % NH : w / TX;
+      const long long cc = 2 * (c0b + tt);
+      if (cc >= d->C0r) continue;
+      const cpx<T> zk = lds[k * LDJ + tt * LDT];
+      const int km = (k == 0) ? 0 : N - k;
+      const cpx<T> zm = lds[km * LDJ + tt * LDT];
+      const long long base = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
+      dst[base] = cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half};
+      if (cc + 1 < d->C0r) dst[base + d->out_s0] = cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half};
+    }
+  } else {
+    static_assert(P >= 2, "single-pass lengths go through the general kernel");
+#pragma unroll
+    for (int q = 0; q < C::nb(P - 1); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(P - 1) || b < NBZ) {
+        const int k = b % LZ, g = b / LZ;
+#pragma unroll
+        for (int s = 0; s < RZ; ++s)
+          fstore<T, IO>(d, out, out_base, cvalid, ca, g * LZ * RZ + k + s * LZ, v[q * RZ + s], conj);
+      }
+    }
+  }
+}
+
+// ---- registry -----------------------------------------------------------------
+
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL>
+static int launch_fast(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, hipStream_t st) {
+  using C = FastCfg<T, N, TX, NT, LINES, RL>;
+  const long long ntiles = (long long)h.tiles0 * h.C1;
+  if (ntiles <= 0) return D2DHIP_SUCCESS;
+  const long long gx = ntiles < (1LL << 20) ? ntiles : (1LL << 20);
+  const long long gy = (ntiles + gx - 1) / gx;
+  if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "axis stage has too many tiles");
+  hipLaunchKernelGGL((fast_axis_kernel<T, N, TX, NT, IO, LINES, RL>), dim3((unsigned)gx, (unsigned)gy),
+                     dim3(NT), C::lds_bytes(), st, dd, io.in, io.out, io.conj);
+  return hip_check(hipGetLastError(), "fast_axis_kernel launch");
+}
+
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL>
+static hipError_t prepare_fast() {
+  using C = FastCfg<T, N, TX, NT, LINES, RL>;
+  return hipFuncSetAttribute((const void *)fast_axis_kernel<T, N, TX, NT, IO, LINES, RL>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
+}
+
+template <int... Rs>
+static std::vector<int> rvec(RList<Rs...>) {
+  return {Rs...};
+}
+
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL>
+static FastVariant make_variant(const char *name) {
+  using C = FastCfg<T, N, TX, NT, LINES, RL>;
+  FastVariant v;
+  v.rb = (int)sizeof(T);
+  v.io = IO;
+  v.lines = LINES ? 1 : 0;
+  v.n = N;
+  v.TX = TX;
+  v.NT = NT;
+  v.lds_bytes = C::lds_bytes();
+  v.ldj = C::LDJ;
+  v.ldt = C::LDT;
+  v.radices = rvec(RL{});
+  v.launch = &launch_fast<T, N, TX, NT, IO, LINES, RL>;
+  v.prepare = &prepare_fast<T, N, TX, NT, IO, LINES, RL>;
+  v.name = name;
+  return v;
+}
+
+#define D2D_VARIANTS(T, N, TX, NT, LINES, RL)                                             \
+  make_variant<T, N, TX, NT, IO_C2C, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " c2c"), \
+      make_variant<T, N, TX, NT, IO_R2C, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " r2c"), \
+      make_variant<T, N, TX, NT, IO_C2R, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " c2r")
+
+using R16_16_4 = RList<16, 16, 4>;
+using R8_8_8 = RList<8, 8, 8>;
+
+const std::vector<FastVariant> &fast_registry() {
+  static const std::vector<FastVariant> reg = {
+      // FP64, 1024: strided columns and contiguous lines
+      D2D_VARIANTS(double, 1024, 4, 256, false, R16_16_4),
+      D2D_VARIANTS(double, 1024, 2, 128, false, R16_16_4),
+      D2D_VARIANTS(double, 1024, 4, 256, true, R16_16_4),
+      D2D_VARIANTS(double, 1024, 2, 128, true, R16_16_4),
+      // FP64, 512
+      D2D_VARIANTS(double, 512, 8, 512, false, R8_8_8),
+      D2D_VARIANTS(double, 512, 4, 256, false, R8_8_8),
+      D2D_VARIANTS(double, 512, 4, 256, true, R8_8_8),
+      D2D_VARIANTS(double, 512, 8, 512, true, R8_8_8),
+      // FP32, 1024 and 512
+      D2D_VARIANTS(float, 1024, 8, 512, false, R16_16_4),
+      D2D_VARIANTS(float, 1024, 4, 256, false, R16_16_4),
+      D2D_VARIANTS(float, 1024, 8, 512, true, R16_16_4),
+      D2D_VARIANTS(float, 1024, 4, 256, true, R16_16_4),
+      D2D_VARIANTS(float, 512, 16, 512, false, R8_8_8),
+      D2D_VARIANTS(float, 512, 8, 256, true, R8_8_8),
+  };
+  return reg;
+}
+
+}  // namespace d2dhip

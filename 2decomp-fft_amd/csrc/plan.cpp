@@ -12,14 +12,38 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
+#include <sys/stat.h>
+
+#include "fast_kernels.h"
 #include "internal.h"
 
 using namespace d2dhip;
+
+// Twiddles of one radix schedule (per-pass tables, forward sign).
+struct TwTable {
+  void *d = nullptr;
+  size_t bytes = 0;
+  std::vector<PassInfo> passes;
+};
+
+// Geometry of one stage and the kernels that can run it.
+struct StageGeom {
+  int axis = 0, io = 0, n = 0;
+  long long C0r = 0, C1 = 0;
+  long long in_s0 = 0, in_s1 = 0, in_sj = 0, out_s0 = 0, out_s1 = 0, out_sj = 0;
+  long long in_elems = 0, out_elems = 0;  // elements (real or complex) of each side
+  std::vector<const FastVariant *> cands;  // specialised kernels for this stage
+  int choice = -1;                         // index into cands, -1 = general kernel
+};
 
 struct d2dhip_plan_s {
   int format = 1;
@@ -28,14 +52,15 @@ struct d2dhip_plan_s {
   int rb = 8;  // bytes per real of the plan's storage
   unsigned flags = 0;
   hipStream_t stream = nullptr;
-  AxisPlan ax[3];
+  std::map<std::vector<int>, TwTable> tw;
   // Stage descriptors: 0-2 c2c along x, y, z of the (nx,ny,nz) pencil;
-  // 3 the real-to-complex stage; 4, 5 the two c2c stages of the spectral
-  // pencil (y then the remaining axis); 6 the complex-to-real stage.
+  // 3 the real-to-complex stage; 4 the spectral y stage; 5 the other
+  // spectral c2c stage (z for PHYSICAL_IN_X, x for _Z); 6 the c2r stage.
   static constexpr int kStages = 7;
+  StageGeom geom[kStages];
   AxisDesc h_desc[kStages];
   AxisShape shape[kStages];
-  int stage_io[kStages];
+  const FastVariant *fast[kStages] = {};
   AxisDesc *d_desc = nullptr;
   void *wk = nullptr;  // c2r workspace (spectral pencil)
   size_t wk_bytes = 0;
@@ -50,6 +75,8 @@ struct d2dhip_plan_s {
 };
 
 namespace {
+
+int mkdir_one(const std::string &d) { return ::mkdir(d.c_str(), 0755); }
 
 // (cos, sin) of 2*pi*num/den in long double, reduced exactly to [0, pi/4].
 void unit_root(long long num, long long den, long double &c, long double &s) {
@@ -131,39 +158,39 @@ std::vector<int> factor_radices(int n, bool &ok) {
   return r;
 }
 
-int build_axis(AxisPlan &ap, int n, int rb) {
-  bool ok = true;
-  ap.n = n;
-  ap.radices = factor_radices(n, ok);
-  if (!ok)
-    return set_error(D2DHIP_ERR_UNSUPPORTED,
-                     "FFT length " + std::to_string(n) +
-                         " has a prime factor above 17 (Bluestein path not available)");
-  ap.passes.clear();
+// Per-pass twiddle tables of a radix schedule: pass p (radix r, earlier
+// product L) holds exp(-2 pi i s k / (L r)) at k*(r-1) + s-1, k < L, 0 < s < r.
+int get_twiddles(d2dhip_plan_s *p, const std::vector<int> &radices, const TwTable **out) {
+  auto it = p->tw.find(radices);
+  if (it != p->tw.end()) {
+    *out = &it->second;
+    return D2DHIP_SUCCESS;
+  }
+  if ((int)radices.size() > kMaxPass)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "too many FFT passes");
+  TwTable t;
   int L = 1, off = 0;
   std::vector<long double> twc, tws;
-  for (int r : ap.radices) {
+  for (int r : radices) {
     PassInfo pi;
     pi.r = r;
     pi.L = L;
     pi.tw_off = off;
     pi.invL = 1.0f / (float)L;
-    ap.passes.push_back(pi);
+    t.passes.push_back(pi);
     for (int k = 0; k < L; ++k)
       for (int s = 1; s < r; ++s) {
         long double c, sn;
         unit_root((long long)s * k, (long long)L * r, c, sn);
         twc.push_back(c);
-        tws.push_back(-sn);  // forward sign: exp(-2 pi i s k / (L r))
+        tws.push_back(-sn);
       }
     off += L * (r - 1);
     L *= r;
   }
-  if ((int)ap.passes.size() > kMaxPass)
-    return set_error(D2DHIP_ERR_UNSUPPORTED, "too many FFT passes");
-  const size_t cnt = std::max<size_t>(twc.size(), 1);
-  ap.tw_bytes = cnt * 2 * rb;
-  std::vector<unsigned char> host(ap.tw_bytes, 0);
+  const int rb = p->rb;
+  t.bytes = std::max<size_t>(twc.size(), 1) * 2 * rb;
+  std::vector<unsigned char> host(t.bytes, 0);
   for (size_t i = 0; i < twc.size(); ++i) {
     if (rb == 8) {
       double *h = reinterpret_cast<double *>(host.data());
@@ -175,13 +202,11 @@ int build_axis(AxisPlan &ap, int n, int rb) {
       h[2 * i + 1] = (float)tws[i];
     }
   }
-  int st = hip_check(hipMalloc(&ap.d_tw, ap.tw_bytes), "hipMalloc(twiddles)");
-  if (st) {
-    ap.d_tw = nullptr;
-    return D2DHIP_ERR_ALLOC;
-  }
-  return hip_check(hipMemcpy(ap.d_tw, host.data(), ap.tw_bytes, hipMemcpyHostToDevice),
-                   "hipMemcpy(twiddles)");
+  if (hipMalloc(&t.d, t.bytes) != hipSuccess) return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(twiddles)");
+  int st = hip_check(hipMemcpy(t.d, host.data(), t.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+  auto res = p->tw.emplace(radices, t);
+  *out = &res.first->second;
+  return st;
 }
 
 int ilog2(int v) {
@@ -194,8 +219,8 @@ int ilog2(int v) {
 // (lines = false) take TX columns of one 128-byte run per point; contiguous
 // lines (lines = true) take TX whole lines.  LDS is capped at 64 KiB so that
 // two workgroups fit a CU, unless a single column needs more.
-int choose_shape(const AxisPlan &ap, bool lines, int cb, long long C0, AxisShape &s) {
-  const int n = ap.n;
+int choose_shape(int n, const std::vector<PassInfo> &passes, bool lines, int cb, long long C0,
+                 AxisShape &s) {
   int txmax = lines ? 64 : std::max(1, 128 / cb);
   long long c0p2 = 1;
   while (c0p2 < C0 && c0p2 < txmax) c0p2 <<= 1;
@@ -205,7 +230,7 @@ int choose_shape(const AxisPlan &ap, bool lines, int cb, long long C0, AxisShape
     const size_t cap = pass == 0 ? 64 * 1024 : 160 * 1024;
     for (int tx = txmax; tx >= 1; tx >>= 1) {
       int nt = 64;
-      for (const PassInfo &p : ap.passes) {
+      for (const PassInfo &p : passes) {
         const int per = kRegCap / p.r;
         const long long need = ((long long)tx * (n / p.r) + per - 1) / per;
         nt = (int)std::max<long long>(nt, need);
@@ -259,71 +284,283 @@ StageIo cols_view(int axis, long long n1, long long n2, long long n3) {
   return v;
 }
 
-int make_stage(d2dhip_plan_s *p, int idx, int axis, int io, const StageIo &sio) {
-  const AxisPlan &ap = p->ax[axis];
+// Fills the descriptor and launch shape of stage idx for kernel choice c
+// (-1 = general kernel, else the c-th specialised candidate).
+int configure(d2dhip_plan_s *p, int idx, int c) {
+  StageGeom &g = p->geom[idx];
   const int cb = 2 * p->rb;
   AxisDesc &a = p->h_desc[idx];
-  std::memset(&a, 0, sizeof(a));
-  a.tw = ap.d_tw;
-  a.n = ap.n;
-  a.npass = (int)ap.passes.size();
-  for (int i = 0; i < a.npass; ++i) a.pass[i] = ap.passes[i];
-  a.C0r = sio.C0r;
-  a.C0 = (io == IO_C2C) ? sio.C0r : (sio.C0r + 1) / 2;
-  a.C1 = sio.C1;
   AxisShape &s = p->shape[idx];
-  int st = choose_shape(ap, axis == 0, cb, a.C0, s);
+  std::memset(&a, 0, sizeof(a));
+  s = AxisShape();
+  std::vector<int> radices;
+  const FastVariant *fv = (c >= 0) ? g.cands[c] : nullptr;
+  if (fv) {
+    radices = fv->radices;
+  } else {
+    bool ok = true;
+    radices = factor_radices(g.n, ok);
+    if (!ok)
+      return set_error(D2DHIP_ERR_UNSUPPORTED,
+                       "FFT length " + std::to_string(g.n) +
+                           " has a prime factor above 17 (Bluestein path not available)");
+  }
+  const TwTable *tt = nullptr;
+  int st = get_twiddles(p, radices, &tt);
   if (st) return st;
+  a.tw = tt->d;
+  a.n = g.n;
+  a.npass = (int)tt->passes.size();
+  for (int i = 0; i < a.npass; ++i) a.pass[i] = tt->passes[i];
+  a.C0r = g.C0r;
+  a.C0 = (g.io == IO_C2C) ? g.C0r : (g.C0r + 1) / 2;
+  a.C1 = g.C1;
+  if (fv) {
+    s.TX = fv->TX;
+    s.nthreads = fv->NT;
+    s.lds_bytes = fv->lds_bytes;
+    s.ldj = fv->ldj;
+    s.ldt = fv->ldt;
+    st = hip_check(fv->prepare(), "hipFuncSetAttribute(fast kernel)");
+    if (st) return st;
+  } else {
+    st = choose_shape(g.n, tt->passes, g.axis == 0, cb, a.C0, s);
+    if (st) return st;
+  }
   a.TX = s.TX;
   a.log2TX = ilog2(s.TX);
   const long long tiles0 = (a.C0 + s.TX - 1) / s.TX;
   if (tiles0 > (1LL << 30)) return set_error(D2DHIP_ERR_UNSUPPORTED, "pencil too large");
   a.tiles0 = (int)tiles0;
-  a.in_s0 = sio.in_s0;
-  a.in_s1 = sio.in_s1;
-  a.in_sj = sio.in_sj;
-  a.out_s0 = sio.out_s0;
-  a.out_s1 = sio.out_s1;
-  a.out_sj = sio.out_sj;
+  a.in_s0 = g.in_s0;
+  a.in_s1 = g.in_s1;
+  a.in_sj = g.in_sj;
+  a.out_s0 = g.out_s0;
+  a.out_s1 = g.out_s1;
+  a.out_sj = g.out_sj;
   a.ldj = s.ldj;
   a.ldt = s.ldt;
-  a.nh = ap.n / 2 + 1;
-  p->stage_io[idx] = io;
+  a.nh = g.n / 2 + 1;
+  p->fast[idx] = fv;
+  g.choice = c;
   return D2DHIP_SUCCESS;
+}
+
+int make_stage(d2dhip_plan_s *p, int idx, int axis, int io, const StageIo &sio, long long in_elems,
+               long long out_elems) {
+  StageGeom &g = p->geom[idx];
+  g.axis = axis;
+  g.io = io;
+  g.n = axis == 0 ? p->nx : (axis == 1 ? p->ny : p->nz);
+  g.C0r = sio.C0r;
+  g.C1 = sio.C1;
+  g.in_s0 = sio.in_s0;
+  g.in_s1 = sio.in_s1;
+  g.in_sj = sio.in_sj;
+  g.out_s0 = sio.out_s0;
+  g.out_s1 = sio.out_s1;
+  g.out_sj = sio.out_sj;
+  g.in_elems = in_elems;
+  g.out_elems = out_elems;
+  g.cands.clear();
+  if (!std::getenv("D2DHIP_NO_FAST")) {
+    for (const FastVariant &v : fast_registry())
+      if (v.rb == p->rb && v.io == io && v.lines == (axis == 0 ? 1 : 0) && v.n == g.n)
+        g.cands.push_back(&v);
+  }
+  return configure(p, idx, g.cands.empty() ? -1 : 0);
+}
+
+int upload_descs(d2dhip_plan_s *p) {
+  return hip_check(hipMemcpy(p->d_desc, p->h_desc, sizeof(AxisDesc) * d2dhip_plan_s::kStages,
+                             hipMemcpyHostToDevice),
+                   "hipMemcpy(stage descriptors)");
 }
 
 // Builds the seven stage descriptors of a plan and uploads them.
 int make_stages(d2dhip_plan_s *p) {
   const long long nx = p->nx, ny = p->ny, nz = p->nz;
+  const long long nph = nx * ny * nz;
   int st;
   for (int a = 0; a < 3; ++a)
-    if ((st = make_stage(p, a, a, IO_C2C, cols_view(a, nx, ny, nz)))) return st;
+    if ((st = make_stage(p, a, a, IO_C2C, cols_view(a, nx, ny, nz), nph, nph))) return st;
   if (p->format == D2DHIP_PHYSICAL_IN_X) {
-    const long long nxc = nx / 2 + 1;
+    const long long nxc = nx / 2 + 1, nsp = nxc * ny * nz;
     StageIo r = cols_view(0, nx, ny, nz);  // real lines of nx -> complex lines of nxc
     r.out_s0 = nxc;
-    if ((st = make_stage(p, 3, 0, IO_R2C, r))) return st;
-    if ((st = make_stage(p, 4, 1, IO_C2C, cols_view(1, nxc, ny, nz)))) return st;
-    if ((st = make_stage(p, 5, 2, IO_C2C, cols_view(2, nxc, ny, nz)))) return st;
+    if ((st = make_stage(p, 3, 0, IO_R2C, r, nph, nsp))) return st;
+    if ((st = make_stage(p, 4, 1, IO_C2C, cols_view(1, nxc, ny, nz), nsp, nsp))) return st;
+    if ((st = make_stage(p, 5, 2, IO_C2C, cols_view(2, nxc, ny, nz), nsp, nsp))) return st;
     StageIo c = cols_view(0, nx, ny, nz);
     c.in_s0 = nxc;
-    if ((st = make_stage(p, 6, 0, IO_C2R, c))) return st;
+    if ((st = make_stage(p, 6, 0, IO_C2R, c, nsp, nph))) return st;
   } else {
-    const long long nzc = nz / 2 + 1;
+    const long long nzc = nz / 2 + 1, nsp = nx * ny * nzc;
     // real columns i + nx*j of the z axis; the complex output has the same
     // column and point strides
-    if ((st = make_stage(p, 3, 2, IO_R2C, cols_view(2, nx, ny, nz)))) return st;
-    if ((st = make_stage(p, 4, 1, IO_C2C, cols_view(1, nx, ny, nzc)))) return st;
-    if ((st = make_stage(p, 5, 0, IO_C2C, cols_view(0, nx, ny, nzc)))) return st;
-    if ((st = make_stage(p, 6, 2, IO_C2R, cols_view(2, nx, ny, nz)))) return st;
+    if ((st = make_stage(p, 3, 2, IO_R2C, cols_view(2, nx, ny, nz), nph, nsp))) return st;
+    if ((st = make_stage(p, 4, 1, IO_C2C, cols_view(1, nx, ny, nzc), nsp, nsp))) return st;
+    if ((st = make_stage(p, 5, 0, IO_C2C, cols_view(0, nx, ny, nzc), nsp, nsp))) return st;
+    if ((st = make_stage(p, 6, 2, IO_C2R, cols_view(2, nx, ny, nz), nsp, nph))) return st;
   }
   const size_t bytes = sizeof(AxisDesc) * d2dhip_plan_s::kStages;
   if (hipMalloc(&p->d_desc, bytes) != hipSuccess) {
     p->d_desc = nullptr;
     return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(stage descriptors) failed");
   }
-  return hip_check(hipMemcpy(p->d_desc, p->h_desc, bytes, hipMemcpyHostToDevice),
-                   "hipMemcpy(stage descriptors)");
+  return upload_descs(p);
+}
+
+// ---- autotuner -----------------------------------------------------------------
+// Times every kernel that can run a stage (the general kernel and each
+// specialised variant) on scratch buffers of the stage's size and keeps the
+// fastest.  Choices persist in a tab-separated cache file
+// ($D2DHIP_TUNE_CACHE, default $HOME/.cache/d2dhip/tuning.tsv) keyed by the
+// stage geometry, so later plans of the same shape skip the timing.
+
+std::mutex g_tune_mu;
+std::map<std::string, std::string> g_tune;
+bool g_tune_loaded = false;
+
+std::string tune_path() {
+  if (const char *e = std::getenv("D2DHIP_TUNE_CACHE")) return e;
+  const char *h = std::getenv("HOME");
+  return std::string(h ? h : "/tmp") + "/.cache/d2dhip/tuning.tsv";
+}
+
+void tune_load() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  FILE *f = std::fopen(tune_path().c_str(), "r");
+  if (!f) return;
+  char line[512];
+  while (std::fgets(line, sizeof(line), f)) {
+    char *tab = std::strchr(line, '\t');
+    if (!tab) continue;
+    *tab = 0;
+    std::string val(tab + 1);
+    while (!val.empty() && (val.back() == '\n' || val.back() == '\r')) val.pop_back();
+    g_tune[line] = val;
+  }
+  std::fclose(f);
+}
+
+void tune_store(const std::string &key, const std::string &val) {
+  g_tune[key] = val;
+  const std::string path = tune_path();
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos) {
+    std::string dir = path.substr(0, slash);
+    for (size_t i = 1; i <= dir.size(); ++i)
+      if (i == dir.size() || dir[i] == '/') (void)mkdir_one(dir.substr(0, i));
+  }
+  if (FILE *f = std::fopen(path.c_str(), "a")) {
+    std::fprintf(f, "%s\t%s\n", key.c_str(), val.c_str());
+    std::fclose(f);
+  }
+}
+
+std::string tune_key(const d2dhip_plan_s *p, int idx) {
+  const StageGeom &g = p->geom[idx];
+  char b[256];
+  std::snprintf(b, sizeof(b), "v1 rb=%d io=%d ax=%d n=%d C0r=%lld C1=%lld s=%lld,%lld,%lld,%lld,%lld,%lld",
+                p->rb, g.io, g.axis, g.n, g.C0r, g.C1, g.in_s0, g.in_s1, g.in_sj, g.out_s0,
+                g.out_s1, g.out_sj);
+  return b;
+}
+
+std::string choice_name(const d2dhip_plan_s *p, int idx) {
+  const StageGeom &g = p->geom[idx];
+  return g.choice < 0 ? std::string("general") : std::string(g.cands[g.choice]->name);
+}
+
+int time_stage(d2dhip_plan_s *p, int idx, const void *in, void *out, hipEvent_t e0, hipEvent_t e1,
+               float *ms) {
+  int st = hip_check(hipMemcpy(p->d_desc + idx, &p->h_desc[idx], sizeof(AxisDesc),
+                               hipMemcpyHostToDevice),
+                     "hipMemcpy(descriptor)");
+  if (st) return st;
+  const AxisIo io = {in, out, 0};
+  auto launch = [&]() {
+    return p->fast[idx] ? p->fast[idx]->launch(p->d_desc + idx, p->h_desc[idx], io, p->stream)
+                        : launch_axis(p->rb, p->geom[idx].io, p->d_desc + idx, p->h_desc[idx],
+                                      io, p->shape[idx], p->stream);
+  };
+  if ((st = launch())) return st;
+  if ((st = hip_check(hipEventRecord(e0, p->stream), "hipEventRecord"))) return st;
+  const int reps = 3;
+  for (int r = 0; r < reps && !st; ++r) st = launch();
+  if (st) return st;
+  if ((st = hip_check(hipEventRecord(e1, p->stream), "hipEventRecord"))) return st;
+  if ((st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize"))) return st;
+  float t = 0.f;
+  st = hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+  *ms = t / reps;
+  return st;
+}
+
+int tune_stages(d2dhip_plan_s *p) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_load();
+  size_t need = 0;
+  bool any = false;
+  for (int i = 0; i < d2dhip_plan_s::kStages; ++i) {
+    const StageGeom &g = p->geom[i];
+    if (g.cands.empty()) continue;
+    any = true;
+    const size_t bi = (size_t)g.in_elems * (g.io == IO_R2C ? 1 : 2) * p->rb;
+    const size_t bo = (size_t)g.out_elems * (g.io == IO_C2R ? 1 : 2) * p->rb;
+    need = std::max(need, std::max(bi, bo));
+  }
+  if (!any) return D2DHIP_SUCCESS;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = D2DHIP_SUCCESS;
+  for (int i = 0; i < d2dhip_plan_s::kStages && !st; ++i) {
+    StageGeom &g = p->geom[i];
+    if (g.cands.empty()) continue;
+    const std::string key = tune_key(p, i);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      int c = -1;
+      for (size_t k = 0; k < g.cands.size(); ++k)
+        if (it->second == g.cands[k]->name) c = (int)k;
+      if (c >= 0 || it->second == "general") {
+        st = configure(p, i, c);
+        continue;
+      }
+    }
+    if (!a) {
+      if (hipMalloc(&a, need) != hipSuccess || hipMalloc(&b, need) != hipSuccess) {
+        st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(tuning buffers) failed");
+        break;
+      }
+      st = hip_check(hipMemset(a, 0, need), "hipMemset");
+      if (!st) st = hip_check(hipMemset(b, 0, need), "hipMemset");
+      if (!st) st = hip_check(hipEventCreate(&e0), "hipEventCreate");
+      if (!st) st = hip_check(hipEventCreate(&e1), "hipEventCreate");
+      if (st) break;
+    }
+    float best = 1e30f;
+    int best_c = g.choice;
+    for (int c = -1; c < (int)g.cands.size() && !st; ++c) {
+      if (configure(p, i, c)) continue;  // e.g. the general kernel cannot run this length
+      float ms = 0.f;
+      st = time_stage(p, i, a, b, e0, e1, &ms);
+      if (!st && ms < best) {
+        best = ms;
+        best_c = c;
+      }
+    }
+    if (!st) st = configure(p, i, best_c);
+    if (!st) tune_store(key, choice_name(p, i));
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (!st) st = upload_descs(p);
+  return st;
 }
 
 hipEvent_t take_event(d2dhip_plan_s *p) {
@@ -347,8 +584,10 @@ int run(d2dhip_plan_s *p, int idx, const void *in, void *out, int conj) {
     int st = hip_check(hipEventRecord(a, p->stream), "hipEventRecord");
     if (st) return st;
   }
-  int st = launch_axis(p->rb, p->stage_io[idx], p->d_desc + idx, p->h_desc[idx], io,
-                       p->shape[idx], p->stream);
+  int st = p->fast[idx]
+               ? p->fast[idx]->launch(p->d_desc + idx, p->h_desc[idx], io, p->stream)
+               : launch_axis(p->rb, p->geom[idx].io, p->d_desc + idx, p->h_desc[idx], io,
+                             p->shape[idx], p->stream);
   if (p->prof) {
     if (!st) st = hip_check(hipEventRecord(b, p->stream), "hipEventRecord");
     p->recs.push_back({idx, a, b});
@@ -384,9 +623,8 @@ int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, in
   p->flags = flags;
   p->stream = (hipStream_t)stream;
   int st = axis_kernel_prepare(p->rb);
-  const int ns[3] = {nx, ny, nz};
-  for (int a = 0; a < 3 && !st; ++a) st = build_axis(p->ax[a], ns[a], p->rb);
   if (!st) st = make_stages(p);
+  if (!st && ((flags & D2DHIP_FLAG_TUNE) || std::getenv("D2DHIP_TUNE"))) st = tune_stages(p);
   if (!st && !(flags & D2DHIP_FLAG_INPLACE)) {
     const long long nsp = (format == D2DHIP_PHYSICAL_IN_X)
                               ? (long long)(nx / 2 + 1) * ny * nz
@@ -412,8 +650,8 @@ int d2dhip_plan_destroy(d2dhip_plan p) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : p->pool) (void)hipEventDestroy(e);
-  for (auto &a : p->ax)
-    if (a.d_tw) (void)hipFree(a.d_tw);
+  for (auto &kv : p->tw)
+    if (kv.second.d) (void)hipFree(kv.second.d);
   if (p->wk) (void)hipFree(p->wk);
   if (p->d_desc) (void)hipFree(p->d_desc);
   delete p;
@@ -469,10 +707,45 @@ int d2dhip_plan_profile_report(d2dhip_plan p, double ms[7], int launches[7]) {
   return st;
 }
 
+int d2dhip_plan_stage_kernel(d2dhip_plan p, int stage, char *name, int len) {
+  if (!p || !name || len <= 0) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (stage < 0 || stage >= d2dhip_plan_s::kStages)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "stage out of range");
+  const std::string s = choice_name(p, stage);
+  std::snprintf(name, (size_t)len, "%s", s.c_str());
+  return D2DHIP_SUCCESS;
+}
+
+int d2dhip_plan_stage_candidate(d2dhip_plan p, int stage, int idx, char *name, int len) {
+  if (!p || !name || len <= 0) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (stage < 0 || stage >= d2dhip_plan_s::kStages)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "stage out of range");
+  const StageGeom &g = p->geom[stage];
+  if (idx < 0 || idx > (int)g.cands.size())
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "candidate out of range");
+  std::snprintf(name, (size_t)len, "%s", idx == 0 ? "general" : g.cands[idx - 1]->name);
+  return D2DHIP_SUCCESS;
+}
+
+int d2dhip_plan_set_stage_kernel(d2dhip_plan p, int stage, const char *name) {
+  if (!p || !name) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (stage < 0 || stage >= d2dhip_plan_s::kStages)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "stage out of range");
+  const StageGeom &g = p->geom[stage];
+  int c = -2;
+  if (std::strcmp(name, "general") == 0) c = -1;
+  for (size_t k = 0; k < g.cands.size(); ++k)
+    if (std::strcmp(name, g.cands[k]->name) == 0) c = (int)k;
+  if (c == -2) return set_error(D2DHIP_ERR_INVALID_VALUE, std::string("no kernel ") + name);
+  int st = configure(p, stage, c);
+  if (!st) st = upload_descs(p);
+  return st;
+}
+
 long long d2dhip_plan_device_bytes(d2dhip_plan p) {
   if (!p) return 0;
   long long b = (long long)p->wk_bytes;
-  for (auto &a : p->ax) b += (long long)a.tw_bytes;
+  for (auto &kv : p->tw) b += (long long)kv.second.bytes;
   return b;
 }
 

@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-lines", type=int, default=16384)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the plan-time autotuner (use the default kernels)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the 3D FFT path on one GPU only; "
@@ -116,7 +118,9 @@ def main():
     d2d.fill_modal(x, n, n, n, d2d.R8, modes)
     spec = torch.empty(nxc * n * n, dtype=torch.complex128, device="cuda")
     y = torch.empty_like(x)
-    eng = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, prec=d2d.PREC_F64)
+    t0 = time.time()
+    eng = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, prec=d2d.PREC_F64, tune=not args.no_tune)
+    plan_s = time.time() - t0
     ms, rep = time_round_trips(torch, eng, x, spec, y, args.steps, args.warmup, profile=True)
     print("bench: fp64 %.3f ms per fwd+bwd" % ms, file=sys.stderr, flush=True)
     rt_err = float(torch.linalg.vector_norm(y / n ** 3 - x) / torch.linalg.vector_norm(x))
@@ -150,7 +154,8 @@ def main():
     # ---- FP32 VALU variant ---------------------------------------------------
     variants = {}
     del y
-    eng32 = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, prec=d2d.PREC_F32)
+    eng32 = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, prec=d2d.PREC_F32, tune=not args.no_tune)
+    kernels32 = eng32.stage_kernels()
     x32 = x.float()
     spec32 = torch.empty(nxc * n * n, dtype=torch.complex64, device="cuda")
     y32 = torch.empty_like(x32)
@@ -160,7 +165,8 @@ def main():
     maxabs32 = float((spec32.to(torch.complex128) - spec64).abs().max())
     variants["fp32"] = {"ms": round(ms32, 3), "rel_l2_vs_fp64": err32, "max_abs_vs_fp64": maxabs32,
                         "GBps_3pass": round(round_trip_bytes(n, 4) / (ms32 * 1e-3) / 1e9, 1),
-                        "speedup_vs_fp64": round(ms / ms32, 3)}
+                        "speedup_vs_fp64": round(ms / ms32, 3),
+                        "kernels": {k: kernels32[k] for k in ("r2c", "spec_y", "spec_last", "c2r")}}
     variants["split_bf16"] = {"status": "not built yet"}
     del x32, spec32, y32, eng32
 
@@ -184,6 +190,8 @@ def main():
         "copy_GBps": round(copy_gbs, 1),
         "round_trip_rel_l2": rt_err,
         "stages": stages,
+        "kernels": {k: v for k, v in eng.stage_kernels().items() if k in stages},
+        "plan_seconds": round(plan_s, 2),
         "roofline": {"bound": "hbm", "kernel": "axis_fft_kernel (%s stage)" % dom,
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},

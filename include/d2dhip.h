@@ -37,6 +37,10 @@ extern "C" {
 #define D2DHIP_FLAG_SKIP_X 0x2u  /* opt_skip_XYZ_c2c(1) */
 #define D2DHIP_FLAG_SKIP_Y 0x4u  /* opt_skip_XYZ_c2c(2) */
 #define D2DHIP_FLAG_SKIP_Z 0x8u  /* opt_skip_XYZ_c2c(3) */
+#define D2DHIP_FLAG_TUNE 0x10u   /* time the candidate kernels of each stage at plan
+                                    creation (also with env D2DHIP_TUNE=1); choices
+                                    are cached in $D2DHIP_TUNE_CACHE or
+                                    ~/.cache/d2dhip/tuning.tsv */
 
 /* element kinds for transposes and packs */
 #define D2DHIP_R8 0  /* real(real64)    */
@@ -126,6 +130,13 @@ int d2dhip_plan_get_sizes(d2dhip_plan plan, int ph_sz[3], int sp_sz[3]);
  * 5 the other spectral c2c stage (z for PHYSICAL_IN_X, x for _Z); 6 c2r. */
 int d2dhip_plan_profile(d2dhip_plan plan, int enable);
 int d2dhip_plan_profile_report(d2dhip_plan plan, double ms[7], int launches[7]);
+/* Name of the kernel a stage runs ("general" or a specialised variant). */
+int d2dhip_plan_stage_kernel(d2dhip_plan plan, int stage, char *name, int len);
+/* idx-th kernel able to run a stage: 0 = "general", then the specialised
+ * variants compiled for its length; D2DHIP_ERR_INVALID_VALUE past the end. */
+int d2dhip_plan_stage_candidate(d2dhip_plan plan, int stage, int idx, char *name, int len);
+/* Makes a stage run the named candidate (what the autotuner does). */
+int d2dhip_plan_set_stage_kernel(d2dhip_plan plan, int stage, const char *name);
 /* Device bytes the plan holds (twiddles + workspace). */
 long long d2dhip_plan_device_bytes(d2dhip_plan plan);
 

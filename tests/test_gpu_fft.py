@@ -294,3 +294,55 @@ def test_config3a_r2c_z_720_540_432(d2d, torch_gpu):
     y /= nx * ny * nz
     err = float(torch_gpu.linalg.vector_norm(y - x) / torch_gpu.linalg.vector_norm(x))
     assert err < F64_TOL
+
+
+THIN = [(1024, 6, 5), (6, 1024, 5), (6, 5, 1024), (512, 6, 5), (6, 512, 5), (6, 5, 512),
+        (1024, 5, 3), (5, 3, 1024), (512, 3, 5)]
+STAGE_USE = {"c2c_x": "c2c", "c2c_y": "c2c", "c2c_z": "c2c", "r2c": "r2c", "spec_y": "rt",
+             "spec_last": "rt", "c2r": "c2r"}
+
+
+@pytest.mark.parametrize("shape", THIN)
+@pytest.mark.parametrize("fmt", [1, 3])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_every_stage_kernel(d2d, torch_gpu, shape, fmt, prec):
+    """Every kernel the planner can pick for a stage (the general kernel and
+    each specialised variant) gives the pocketfft result."""
+    torch = torch_gpu
+    nx, ny, nz = shape
+    tol = F64_TOL if prec == 64 else F32_TOL
+    rng = np.random.default_rng(sum(shape) + fmt + prec)
+    x = np.asfortranarray(rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape))
+    r = np.asfortranarray(x.real)
+    cdt = torch.complex128 if prec == 64 else torch.complex64
+    rdt = torch.float64 if prec == 64 else torch.float32
+    npc = np.complex128 if prec == 64 else np.complex64
+    npr = np.float64 if prec == 64 else np.float32
+    axes = (2, 1, 0) if fmt == 1 else (0, 1, 2)
+    ss = spec_shape(fmt, nx, ny, nz)
+    want_c = np.fft.fftn(x.astype(npc).astype(np.complex128))
+    want_b = np.fft.ifftn(x.astype(npc).astype(np.complex128)) * x.size
+    want_r = np.fft.rfftn(r.astype(npr).astype(np.float64), axes=axes)
+    e = d2d.FFTEngine(fmt, nx, ny, nz, prec=prec)
+    xin = to_dev(torch, x, npc)
+    rin = to_dev(torch, r, npr)
+    for stage in e.STAGES:
+        for name in e.stage_candidates(stage):
+            e.set_stage_kernel(stage, name)
+            print(shape, fmt, prec, stage, name, flush=True)
+            use = STAGE_USE[stage]
+            if use == "c2c":
+                out = torch.empty_like(xin)
+                e.fft_c2c(xin, out, -1)
+                assert rel_l2(from_dev(out, shape), want_c) < tol, (stage, name)
+                e.fft_c2c(xin, out, 1)
+                assert rel_l2(from_dev(out, shape), want_b) < tol, (stage, name)
+            if use in ("r2c", "rt", "c2r"):
+                sp = torch.empty(int(np.prod(ss)), dtype=cdt, device="cuda")
+                e.fft_r2c(rin, sp)
+                assert rel_l2(from_dev(sp, ss), want_r) < tol, (stage, name)
+                back = torch.empty(nx * ny * nz, dtype=rdt, device="cuda")
+                e.fft_c2r(sp, back)
+                got = from_dev(back, shape) / (nx * ny * nz)
+                assert rel_l2(got, r) < tol, (stage, name)
+        e.set_stage_kernel(stage, e.stage_candidates(stage)[-1])
