@@ -50,10 +50,14 @@ struct FastCfg {
     for (int p = 0; p < P; ++p) e = e > nb(p) * RL::r[p] ? e : nb(p) * RL::r[p];
     return e;
   }
-  static constexpr int LDT = LINES ? ((N % 2 == 0) ? N + 1 : N) : 1;
+  // LDS row of point j: one pad row after every 16, so that the stride-R
+  // writes of the first exchange spread over all banks.
+  static constexpr int row(int j) { return j + (j >> 4); }
+  static constexpr int ROWS = row(N - 1) + 1;
+  static constexpr int LDT = LINES ? (ROWS | 1) : 1;
   static constexpr int LDJ = LINES ? 1 : TX;
   static constexpr size_t lds_bytes() {
-    return (size_t)(LINES ? TX * LDT : TX * N) * sizeof(cpx<T>);
+    return (size_t)(LINES ? TX * LDT : TX * ROWS) * sizeof(cpx<T>);
   }
 };
 
@@ -134,8 +138,8 @@ __global__ void __launch_bounds__(NT)
         A.y = T(0);
         B.y = T(0);
       }
-      lds[k * LDJ + tt * LDT] = cpx<T>{A.x - B.y, -(A.y + B.x)};
-      if (k > 0 && 2 * k < N) lds[(N - k) * LDJ + tt * LDT] = cpx<T>{A.x + B.y, A.y - B.x};
+      lds[C::row(k) * LDJ + tt * LDT] = cpx<T>{A.x - B.y, -(A.y + B.x)};
+      if (k > 0 && 2 * k < N) lds[C::row(N - k) * LDJ + tt * LDT] = cpx<T>{A.x + B.y, A.y - B.x};
     }
     __syncthreads();
     constexpr int R = RL::r[0], NBR = N / R;
@@ -144,7 +148,7 @@ __global__ void __launch_bounds__(NT)
       const int b = r0 + q * RPT;
       if (C::exact(0) || b < NBR) {
 #pragma unroll
-        for (int s = 0; s < R; ++s) v[q * R + s] = lds[(b + s * NBR) * LDJ + t * LDT];
+        for (int s = 0; s < R; ++s) v[q * R + s] = lds[C::row(b + s * NBR) * LDJ + t * LDT];
       }
     }
   } else {
@@ -175,7 +179,7 @@ __global__ void __launch_bounds__(NT)
           const int k = b % Lq, g = b / Lq;
 #pragma unroll
           for (int s = 0; s < Rq; ++s)
-            lds[(g * Lq * Rq + k + s * Lq) * LDJ + t * LDT] = v[q * Rq + s];
+            lds[C::row(g * Lq * Rq + k + s * Lq) * LDJ + t * LDT] = v[q * Rq + s];
         }
       }
       __syncthreads();
@@ -184,7 +188,7 @@ __global__ void __launch_bounds__(NT)
         const int b = r0 + q * RPT;
         if (C::exact(p) || b < NBR) {
 #pragma unroll
-          for (int s = 0; s < R; ++s) v[q * R + s] = lds[(b + s * NBR) * LDJ + t * LDT];
+          for (int s = 0; s < R; ++s) v[q * R + s] = lds[C::row(b + s * NBR) * LDJ + t * LDT];
         }
       }
     }
@@ -213,7 +217,7 @@ __global__ void __launch_bounds__(NT)
       if (C::exact(P - 1) || b < NBZ) {
         const int k = b % LZ, g = b / LZ;
 #pragma unroll
-        for (int s = 0; s < RZ; ++s) lds[(g * LZ * RZ + k + s * LZ) * LDJ + t * LDT] = v[q * RZ + s];
+        for (int s = 0; s < RZ; ++s) lds[C::row(g * LZ * RZ + k + s * LZ) * LDJ + t * LDT] = v[q * RZ + s];
       }
     }
     __syncthreads();
@@ -225,9 +229,9 @@ __global__ void __launch_bounds__(NT)
       const int k = LINES ? w % NH : w / TX;
       const long long cc = 2 * (c0b + tt);
       if (cc >= d->C0r) continue;
-      const cpx<T> zk = lds[k * LDJ + tt * LDT];
+      const cpx<T> zk = lds[C::row(k) * LDJ + tt * LDT];
       const int km = (k == 0) ? 0 : N - k;
-      const cpx<T> zm = lds[km * LDJ + tt * LDT];
+      const cpx<T> zm = lds[C::row(km) * LDJ + tt * LDT];
       const long long base = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
       dst[base] = cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half};
       if (cc + 1 < d->C0r) dst[base + d->out_s0] = cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half};

@@ -229,9 +229,107 @@ __global__ void __launch_bounds__(512)
   }
 }
 
+// Bluestein's chirp-z algorithm (L. Bluestein, 1970) for a line length nb
+// with a prime factor the Stockham codelets do not cover:
+//   y_k = c_k sum_j (x_j c_j) w_{k-j},  c_j = exp(-i pi j^2/nb), w_m = exp(i pi m^2/nb),
+// the convolution done as two power-of-two FFTs of length M >= 2 nb - 1 in
+// LDS with the DFT of w precomputed (bhat).  The IO pairing of real
+// transforms wraps around the complex line transform as in axis_fft_kernel.
+template <typename T, int E, int IO>
+__global__ void __launch_bounds__(512)
+    bluestein_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
+  const AxisIo io = {in, out, conj};
+  const long long tiles0 = d->tiles0;
+  const long long tile = (long long)blockIdx.x + (long long)blockIdx.y * gridDim.x;
+  if (tile >= tiles0 * d->C1) return;  // whole block, before any barrier
+  const long long c1 = tile / tiles0;
+  const long long c0b = (tile - c1 * tiles0) * d->TX;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int M = d->n, n = d->nblue, TX = d->TX, lg = d->log2TX, tmask = TX - 1;
+  const int ldj = d->ldj, ldt = d->ldt, nh = d->nh;
+  const cpx<T> *chirp = reinterpret_cast<const cpx<T> *>(d->chirp);
+  const cpx<T> *bhat = reinterpret_cast<const cpx<T> *>(d->bhat);
+
+  // line values (conjugated for inverse transforms) into LDS[0, n)
+  if constexpr (IO == IO_C2R) {
+    const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in);
+    for (int w = tid; w < TX * nh; w += nthr) {
+      const int t = w & tmask, k = w >> lg;
+      const long long ca = 2 * (c0b + t);
+      const long long base = c1 * d->in_s1 + (long long)k * d->in_sj;
+      cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
+      if (ca < d->C0r) A = src[base + ca * d->in_s0];
+      if (ca + 1 < d->C0r) B = src[base + (ca + 1) * d->in_s0];
+      if (k == 0 || 2 * k == n) {
+        A.y = T(0);
+        B.y = T(0);
+      }
+      lds[k * ldj + t * ldt] = cpx<T>{A.x - B.y, -(A.y + B.x)};
+      if (k > 0 && 2 * k < n) lds[(n - k) * ldj + t * ldt] = cpx<T>{A.x + B.y, A.y - B.x};
+    }
+  } else {
+    for (int w = tid; w < TX * n; w += nthr) {
+      const int t = w & tmask, j = w >> lg;
+      lds[j * ldj + t * ldt] = gload<T, IO>(d, io, c0b + t, c1, j);
+    }
+  }
+  __syncthreads();
+  // a_j = x_j c_j, zero padded to M
+  for (int w = tid; w < TX * M; w += nthr) {
+    const int t = w & tmask, j = w >> lg;
+    cpx<T> &e = lds[j * ldj + t * ldt];
+    e = (j < n) ? cmul(e, chirp[j]) : cpx<T>{T(0), T(0)};
+  }
+  __syncthreads();
+  for (int p = 0; p < d->npass; ++p) dispatch_pass<T, E, IO_C2C>(d, io, p, false, false, lds, c0b, c1);
+  // conj(A * W): the inverse DFT as conj(DFT(conj(.)))
+  for (int w = tid; w < TX * M; w += nthr) {
+    const int t = w & tmask, j = w >> lg;
+    cpx<T> &e = lds[j * ldj + t * ldt];
+    e = cconj(cmul(e, bhat[j]));
+  }
+  __syncthreads();
+  for (int p = 0; p < d->npass; ++p) dispatch_pass<T, E, IO_C2C>(d, io, p, false, false, lds, c0b, c1);
+  // y_k = c_k conj(r_k) / M
+  const T invM = T(1) / T(M);
+  for (int w = tid; w < TX * n; w += nthr) {
+    const int t = w & tmask, k = w >> lg;
+    cpx<T> &e = lds[k * ldj + t * ldt];
+    const cpx<T> y = cmul(cconj(e), chirp[k]);
+    e = cpx<T>{y.x * invM, y.y * invM};
+  }
+  __syncthreads();
+  if constexpr (IO == IO_R2C) {
+    cpx<T> *dst = reinterpret_cast<cpx<T> *>(out);
+    const T half = T(0.5);
+    for (int w = tid; w < TX * nh; w += nthr) {
+      const int t = w & tmask, k = w >> lg;
+      const long long ca = 2 * (c0b + t);
+      if (ca >= d->C0r) continue;
+      const cpx<T> zk = lds[k * ldj + t * ldt];
+      const int km = (k == 0) ? 0 : n - k;
+      const cpx<T> zm = lds[km * ldj + t * ldt];
+      const long long base = c1 * d->out_s1 + (long long)k * d->out_sj;
+      dst[base + ca * d->out_s0] = cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half};
+      if (ca + 1 < d->C0r)
+        dst[base + (ca + 1) * d->out_s0] = cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half};
+    }
+  } else {
+    for (int w = tid; w < TX * n; w += nthr) {
+      const int t = w & tmask, j = w >> lg;
+      gstore<T, IO>(d, io, c0b + t, c1, j, lds[j * ldj + t * ldt]);
+    }
+  }
+}
+
 template <typename T, int IO>
 static hipError_t set_lds_limit() {
-  return hipFuncSetAttribute((const void *)axis_fft_kernel<T, kRegCap, IO>,
+  hipError_t e = hipFuncSetAttribute((const void *)axis_fft_kernel<T, kRegCap, IO>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)bluestein_kernel<T, kRegCap, IO>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
@@ -259,8 +357,12 @@ static int launch_t(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, con
   const long long gx = ntiles < (1LL << 20) ? ntiles : (1LL << 20);
   const long long gy = (ntiles + gx - 1) / gx;
   if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "axis stage has too many tiles");
-  hipLaunchKernelGGL((axis_fft_kernel<T, kRegCap, IO>), dim3((unsigned)gx, (unsigned)gy),
-                     dim3(s.nthreads), s.lds_bytes, st, dd, io.in, io.out, io.conj);
+  if (h.blue)
+    hipLaunchKernelGGL((bluestein_kernel<T, kRegCap, IO>), dim3((unsigned)gx, (unsigned)gy),
+                       dim3(s.nthreads), s.lds_bytes, st, dd, io.in, io.out, io.conj);
+  else
+    hipLaunchKernelGGL((axis_fft_kernel<T, kRegCap, IO>), dim3((unsigned)gx, (unsigned)gy),
+                       dim3(s.nthreads), s.lds_bytes, st, dd, io.in, io.out, io.conj);
   return hip_check(hipGetLastError(), "axis_fft_kernel launch");
 }
 

@@ -47,7 +47,13 @@ struct AxisDesc {
   long long in_s0, in_s1, in_sj;
   long long out_s0, out_s1, out_sj;
   int ldj, ldt;       // LDS strides (complex units) of point j and column t
-  int nh;             // n/2 + 1
+  int nh;             // n/2 + 1 of the line length
+  // Bluestein (chirp-z) stages: the line length is nblue, `n` and the passes
+  // are those of the power-of-two convolution length M = n >= 2*nblue - 1.
+  int blue;           // 1: Bluestein stage
+  int nblue;          // line length of a Bluestein stage
+  const void *chirp;  // exp(-i pi j^2 / nblue), j < nblue
+  const void *bhat;   // DFT_M of the chirp sequence exp(+i pi m^2 / nblue), wrapped
 };
 
 // Per-launch arguments: the descriptor lives in device memory (built once per

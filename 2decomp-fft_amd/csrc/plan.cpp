@@ -45,6 +45,13 @@ struct StageGeom {
   int choice = -1;                         // index into cands, -1 = general kernel
 };
 
+// Chirp tables of a Bluestein length.
+struct BlueTables {
+  int M = 0;
+  void *chirp = nullptr;
+  void *bhat = nullptr;
+};
+
 struct d2dhip_plan_s {
   int format = 1;
   int nx = 0, ny = 0, nz = 0;
@@ -53,6 +60,7 @@ struct d2dhip_plan_s {
   unsigned flags = 0;
   hipStream_t stream = nullptr;
   std::map<std::vector<int>, TwTable> tw;
+  std::map<int, BlueTables> blue;
   // Stage descriptors: 0-2 c2c along x, y, z of the (nx,ny,nz) pencil;
   // 3 the real-to-complex stage; 4 the spectral y stage; 5 the other
   // spectral c2c stage (z for PHYSICAL_IN_X, x for _Z); 6 the c2r stage.
@@ -284,6 +292,86 @@ StageIo cols_view(int axis, long long n1, long long n2, long long n3) {
   return v;
 }
 
+// Bluestein tables of line length n: c_j = exp(-i pi j^2/n) (j < n) and the
+// length-M DFT of w_m = exp(+i pi m^2/n) wrapped as w[m] = w[M-m] = w_m,
+// evaluated in long double with j^2 reduced mod 2n exactly.
+int get_bluestein(d2dhip_plan_s *p, int n, const BlueTables **out) {
+  auto it = p->blue.find(n);
+  if (it != p->blue.end()) {
+    *out = &it->second;
+    return D2DHIP_SUCCESS;
+  }
+  BlueTables b;
+  b.M = 1;
+  while (b.M < 2 * n - 1) b.M *= 2;
+  const int M = b.M;
+  std::vector<long double> cr(n), ci(n), wr(M, 0.0L), wi(M, 0.0L);
+  for (int j = 0; j < n; ++j) {
+    long double c, s;
+    unit_root(((long long)j * j) % (2LL * n), 2LL * n, c, s);
+    cr[j] = c;
+    ci[j] = -s;
+    wr[j] = c;
+    wi[j] = s;
+    if (j > 0) {
+      wr[M - j] = c;
+      wi[M - j] = s;
+    }
+  }
+  // iterative radix-2 DFT of w (sign -1), bit-reversed input order
+  for (int i = 1, j = 0; i < M; ++i) {
+    int bit = M >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      std::swap(wr[i], wr[j]);
+      std::swap(wi[i], wi[j]);
+    }
+  }
+  for (int len = 2; len <= M; len <<= 1) {
+    for (int k = 0; k < len / 2; ++k) {
+      long double c, s;
+      unit_root(k, len, c, s);
+      const long double tr = c, ti = -s;
+      for (int i = k; i < M; i += len) {
+        const int j = i + len / 2;
+        const long double xr = wr[j] * tr - wi[j] * ti, xi = wr[j] * ti + wi[j] * tr;
+        wr[j] = wr[i] - xr;
+        wi[j] = wi[i] - xi;
+        wr[i] += xr;
+        wi[i] += xi;
+      }
+    }
+  }
+  const int rb = p->rb;
+  std::vector<unsigned char> hc((size_t)n * 2 * rb), hw((size_t)M * 2 * rb);
+  for (int j = 0; j < n; ++j) {
+    if (rb == 8) {
+      reinterpret_cast<double *>(hc.data())[2 * j] = (double)cr[j];
+      reinterpret_cast<double *>(hc.data())[2 * j + 1] = (double)ci[j];
+    } else {
+      reinterpret_cast<float *>(hc.data())[2 * j] = (float)cr[j];
+      reinterpret_cast<float *>(hc.data())[2 * j + 1] = (float)ci[j];
+    }
+  }
+  for (int m = 0; m < M; ++m) {
+    if (rb == 8) {
+      reinterpret_cast<double *>(hw.data())[2 * m] = (double)wr[m];
+      reinterpret_cast<double *>(hw.data())[2 * m + 1] = (double)wi[m];
+    } else {
+      reinterpret_cast<float *>(hw.data())[2 * m] = (float)wr[m];
+      reinterpret_cast<float *>(hw.data())[2 * m + 1] = (float)wi[m];
+    }
+  }
+  if (hipMalloc(&b.chirp, hc.size()) != hipSuccess || hipMalloc(&b.bhat, hw.size()) != hipSuccess)
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(Bluestein tables)");
+  int st = hip_check(hipMemcpy(b.chirp, hc.data(), hc.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  if (!st) st = hip_check(hipMemcpy(b.bhat, hw.data(), hw.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  auto res = p->blue.emplace(n, b);
+  *out = &res.first->second;
+  return st;
+}
+
 // Fills the descriptor and launch shape of stage idx for kernel choice c
 // (-1 = general kernel, else the c-th specialised candidate).
 int configure(d2dhip_plan_s *p, int idx, int c) {
@@ -295,21 +383,29 @@ int configure(d2dhip_plan_s *p, int idx, int c) {
   s = AxisShape();
   std::vector<int> radices;
   const FastVariant *fv = (c >= 0) ? g.cands[c] : nullptr;
+  const BlueTables *bt = nullptr;
+  int st;
   if (fv) {
     radices = fv->radices;
   } else {
     bool ok = true;
     radices = factor_radices(g.n, ok);
-    if (!ok)
-      return set_error(D2DHIP_ERR_UNSUPPORTED,
-                       "FFT length " + std::to_string(g.n) +
-                           " has a prime factor above 17 (Bluestein path not available)");
+    if (!ok) {  // a prime factor above 17: Bluestein over a power-of-two length
+      if ((st = get_bluestein(p, g.n, &bt))) return st;
+      radices = factor_radices(bt->M, ok);
+    }
   }
   const TwTable *tt = nullptr;
-  int st = get_twiddles(p, radices, &tt);
+  st = get_twiddles(p, radices, &tt);
   if (st) return st;
   a.tw = tt->d;
-  a.n = g.n;
+  a.n = bt ? bt->M : g.n;
+  if (bt) {
+    a.blue = 1;
+    a.nblue = g.n;
+    a.chirp = bt->chirp;
+    a.bhat = bt->bhat;
+  }
   a.npass = (int)tt->passes.size();
   for (int i = 0; i < a.npass; ++i) a.pass[i] = tt->passes[i];
   a.C0r = g.C0r;
@@ -324,7 +420,7 @@ int configure(d2dhip_plan_s *p, int idx, int c) {
     st = hip_check(fv->prepare(), "hipFuncSetAttribute(fast kernel)");
     if (st) return st;
   } else {
-    st = choose_shape(g.n, tt->passes, g.axis == 0, cb, a.C0, s);
+    st = choose_shape(a.n, tt->passes, g.axis == 0, cb, a.C0, s);
     if (st) return st;
   }
   a.TX = s.TX;
@@ -652,6 +748,10 @@ int d2dhip_plan_destroy(d2dhip_plan p) {
   for (auto e : p->pool) (void)hipEventDestroy(e);
   for (auto &kv : p->tw)
     if (kv.second.d) (void)hipFree(kv.second.d);
+  for (auto &kv : p->blue) {
+    if (kv.second.chirp) (void)hipFree(kv.second.chirp);
+    if (kv.second.bhat) (void)hipFree(kv.second.bhat);
+  }
   if (p->wk) (void)hipFree(p->wk);
   if (p->d_desc) (void)hipFree(p->d_desc);
   delete p;

@@ -346,3 +346,26 @@ def test_every_stage_kernel(d2d, torch_gpu, shape, fmt, prec):
                 got = from_dev(back, shape) / (nx * ny * nz)
                 assert rel_l2(got, r) < tol, (stage, name)
         e.set_stage_kernel(stage, e.stage_candidates(stage)[-1])
+
+
+@pytest.mark.parametrize("shape", [(19, 23, 29), (37, 8, 6), (101, 7, 3), (8, 6, 211), (46, 31, 5),
+                                   (1021, 3, 2), (3, 2, 1021)])
+@pytest.mark.parametrize("fmt", [1, 3])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_bluestein_lengths(d2d, torch_gpu, shape, fmt, prec):
+    """Lengths with a prime factor above 17 run Bluestein's chirp-z path."""
+    tol = F64_TOL if prec == 64 else F32_TOL
+    rng = np.random.default_rng(sum(shape) * 7 + fmt)
+    x = np.asfortranarray(rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape))
+    npc = np.complex128 if prec == 64 else np.complex64
+    npr = np.float64 if prec == 64 else np.float32
+    got, _ = run_c2c(d2d, torch_gpu, x, -1, fmt, prec=prec)
+    assert rel_l2(got, np.fft.fftn(x.astype(npc).astype(np.complex128))) < tol
+    got, _ = run_c2c(d2d, torch_gpu, x, 1, fmt, prec=prec)
+    assert rel_l2(got, np.fft.ifftn(x.astype(npc).astype(np.complex128)) * x.size) < tol
+    r = np.asfortranarray(x.real)
+    axes = (2, 1, 0) if fmt == 1 else (0, 1, 2)
+    s = run_r2c(d2d, torch_gpu, r, fmt, prec=prec)
+    assert rel_l2(s, np.fft.rfftn(r.astype(npr).astype(np.float64), axes=axes)) < tol
+    back, _ = run_c2r(d2d, torch_gpu, s, shape, fmt, prec=prec)
+    assert rel_l2(back / r.size, r) < tol
