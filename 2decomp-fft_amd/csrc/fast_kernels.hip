@@ -106,7 +106,8 @@ __global__ void __launch_bounds__(NT)
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
 
   const long long tiles0 = d->tiles0;
-  const long long tile = (long long)blockIdx.x + (long long)blockIdx.y * gridDim.x;
+  const long long tile =
+      xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles0 * d->C1);
   if (tile >= tiles0 * d->C1) return;  // whole block, before any barrier
   const long long c1 = tile / tiles0;
   const long long c0b = (tile - c1 * tiles0) * TX;
@@ -256,7 +257,7 @@ __global__ void __launch_bounds__(NT)
 template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL>
 static int launch_fast(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, hipStream_t st) {
   using C = FastCfg<T, N, TX, NT, LINES, RL>;
-  const long long ntiles = (long long)h.tiles0 * h.C1;
+  const long long ntiles = xcd_grid((long long)h.tiles0 * h.C1);
   if (ntiles <= 0) return D2DHIP_SUCCESS;
   const long long gx = ntiles < (1LL << 20) ? ntiles : (1LL << 20);
   const long long gy = (ntiles + gx - 1) / gx;
@@ -311,6 +312,9 @@ const std::vector<FastVariant> &fast_registry() {
       // FP64, 1024: strided columns and contiguous lines
       D2D_VARIANTS(double, 1024, 4, 256, false, R16_16_4),
       D2D_VARIANTS(double, 1024, 2, 128, false, R16_16_4),
+      D2D_VARIANTS(double, 1024, 8, 512, false, R16_16_4),
+      D2D_VARIANTS(double, 1024, 8, 256, false, R16_16_4),
+      D2D_VARIANTS(double, 1024, 1, 64, false, R16_16_4),
       D2D_VARIANTS(double, 1024, 4, 256, true, R16_16_4),
       D2D_VARIANTS(double, 1024, 2, 128, true, R16_16_4),
       // FP64, 512

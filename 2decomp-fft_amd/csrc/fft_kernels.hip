@@ -167,7 +167,8 @@ __global__ void __launch_bounds__(512)
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
   const AxisIo io = {in, out, conj};
   const long long tiles0 = d->tiles0;
-  const long long tile = (long long)blockIdx.x + (long long)blockIdx.y * gridDim.x;
+  const long long tile =
+      xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles0 * d->C1);
   if (tile >= tiles0 * d->C1) return;  // whole block, before any barrier
   const long long c1 = tile / tiles0;
   const long long c0b = (tile - c1 * tiles0) * d->TX;
@@ -242,7 +243,8 @@ __global__ void __launch_bounds__(512)
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
   const AxisIo io = {in, out, conj};
   const long long tiles0 = d->tiles0;
-  const long long tile = (long long)blockIdx.x + (long long)blockIdx.y * gridDim.x;
+  const long long tile =
+      xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles0 * d->C1);
   if (tile >= tiles0 * d->C1) return;  // whole block, before any barrier
   const long long c1 = tile / tiles0;
   const long long c0b = (tile - c1 * tiles0) * d->TX;
@@ -352,7 +354,7 @@ int axis_kernel_prepare(int prec_bytes) {
 template <typename T, int IO>
 static int launch_t(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, const AxisShape &s,
                     hipStream_t st) {
-  const long long ntiles = (long long)h.tiles0 * h.C1;
+  const long long ntiles = xcd_grid((long long)h.tiles0 * h.C1);
   if (ntiles <= 0) return D2DHIP_SUCCESS;
   const long long gx = ntiles < (1LL << 20) ? ntiles : (1LL << 20);
   const long long gy = (ntiles + gx - 1) / gx;

@@ -81,6 +81,20 @@ struct AxisPlan {
   size_t tw_bytes = 0;
 };
 
+// Workgroups of a launch are dealt to the 8 XCDs of an MI355X round-robin.
+// Tile t of a stage goes to block xcd_block(t): each XCD then works through
+// a contiguous range of tiles, so tiles that share 128-byte lines (TX * 16 B
+// < 128) meet in one L2.  Launches use xcd_grid(ntiles) blocks; blocks past
+// the last tile exit at once.
+constexpr int kXcds = 8;
+__host__ __device__ inline long long xcd_grid(long long ntiles) {
+  return (ntiles + kXcds - 1) / kXcds * kXcds;
+}
+__device__ inline long long xcd_tile(long long block, long long ntiles) {
+  const long long per = (ntiles + kXcds - 1) / kXcds;
+  return (block % kXcds) * per + block / kXcds;
+}
+
 // status helpers (api.cpp)
 int set_error(int status, const std::string &msg);
 int hip_check(hipError_t e, const char *what);
