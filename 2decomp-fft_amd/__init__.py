@@ -118,6 +118,14 @@ def lib():
             "d2dhip_exec_r2c": ([pl, vp, vp], ci),
             "d2dhip_exec_c2r": ([pl, vp, vp], ci),
             "d2dhip_transpose": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), vp], ci),
+            "d2dhip_batch_create": ([ctypes.POINTER(vp), ci, ctypes.POINTER(vp), vp], ci),
+            "d2dhip_batch_bind_c2c": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ci], ci),
+            "d2dhip_batch_launch": ([vp], ci),
+            "d2dhip_batch_capture_graph": ([vp], ci),
+            "d2dhip_batch_graph_launch": ([vp], ci),
+            "d2dhip_batch_check": ([vp], ci),
+            "d2dhip_batch_set_stream": ([vp, vp], ci),
+            "d2dhip_batch_destroy": ([vp], ci),
             "d2dhip_pack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
             "d2dhip_unpack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
             "d2dhip_pack_buffer_elems": ([ci, ctypes.POINTER(Decomp), ci], cll),
@@ -289,6 +297,56 @@ class FFTEngine:
 
 def decomp_2d_fft_init(pencil, nx, ny, nz, **kw):
     return FFTEngine(pencil, nx, ny, nz, **kw)
+
+
+class Batch:
+    """c2c stages of many engines in one persistent-kernel launch."""
+
+    def __init__(self, engines, stream=None):
+        self.engines = list(engines)
+        n = len(self.engines)
+        arr = (ctypes.c_void_p * n)(*[e._plan.value for e in self.engines])
+        self._b = ctypes.c_void_p()
+        check(lib().d2dhip_batch_create(ctypes.byref(self._b), n, arr,
+                                        ctypes.c_void_p(stream) if stream is not None
+                                        else ctypes.c_void_p()))
+        self._explicit_stream = stream is not None
+
+    def _sync_stream(self):
+        if not self._explicit_stream:
+            check(lib().d2dhip_batch_set_stream(self._b, _stream(None)))
+
+    def bind(self, ins, outs, mode=0):
+        """mode -1 forward, +1 backward, 0 forward then backward (in place on outs)."""
+        n = len(self.engines)
+        a = (ctypes.c_void_p * n)(*[_ptr(t).value for t in ins])
+        b = (ctypes.c_void_p * n)(*[_ptr(t).value for t in outs])
+        check(lib().d2dhip_batch_bind_c2c(self._b, a, b, mode))
+
+    def launch(self):
+        self._sync_stream()
+        check(lib().d2dhip_batch_launch(self._b))
+
+    def capture_graph(self):
+        check(lib().d2dhip_batch_capture_graph(self._b))
+
+    def graph_launch(self):
+        self._sync_stream()
+        check(lib().d2dhip_batch_graph_launch(self._b))
+
+    def check(self):
+        check(lib().d2dhip_batch_check(self._b))
+
+    def destroy(self):
+        if self._b:
+            lib().d2dhip_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 # ---- transposes and packs -------------------------------------------------------

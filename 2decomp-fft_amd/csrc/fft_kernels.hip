@@ -160,16 +160,14 @@ __device__ __forceinline__ void dispatch_pass(const AxisDesc *__restrict__ d, co
   }
 }
 
+// One tile of a stage, by the whole workgroup (any blockDim at least the
+// stage's planned block size).
 template <typename T, int E, int IO>
-__global__ void __launch_bounds__(512)
-    axis_fft_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
-  const AxisIo io = {in, out, conj};
+__device__ __forceinline__ void axis_tile(const AxisDesc *__restrict__ d, const AxisIo &io,
+                                          long long tile, cpx<T> *lds) {
+  const void *in = io.in;
+  void *out = io.out;
   const long long tiles0 = d->tiles0;
-  const long long tile =
-      xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles0 * d->C1);
-  if (tile >= tiles0 * d->C1) return;  // whole block, before any barrier
   const long long c1 = tile / tiles0;
   const long long c0b = (tile - c1 * tiles0) * d->TX;
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -228,6 +226,122 @@ __global__ void __launch_bounds__(512)
       }
     }
   }
+}
+
+template <typename T, int E, int IO>
+__global__ void __launch_bounds__(512)
+    axis_fft_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
+  const AxisIo io = {in, out, conj};
+  const long long ntiles = (long long)d->tiles0 * d->C1;
+  const long long tile = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, ntiles);
+  if (tile >= ntiles) return;  // whole block, before any barrier
+  axis_tile<T, E, IO>(d, io, tile, lds);
+}
+
+// ---- many grids in one launch ---------------------------------------------------
+// A persistent kernel pulls work items from a device queue: item i belongs
+// to the BatchGroup g with groups[g].first <= i < groups[g].first + ntiles
+// (all tiles of one stage of one grid).  A group may depend on an earlier
+// group (the previous stage of the same grid); its items wait until that
+// group's completion counter reaches its tile count.  Items are dequeued in
+// order, so every awaited item is already held by a running block and the
+// wait ends; a spin limit turns a broken schedule into an error flag instead
+// of a hang.
+template <typename T, int E>
+__global__ void __launch_bounds__(512)
+    batch_kernel(const BatchGroup *__restrict__ groups, int ngroups, long long total,
+                 unsigned long long *head, unsigned int *done, unsigned int *err) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
+  __shared__ long long s_item;
+  __shared__ int s_fail;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) {
+      s_item = (long long)atomicAdd(head, 1ull);
+      s_fail = 0;
+    }
+    __syncthreads();
+    const long long it = s_item;
+    if (it >= total) break;  // uniform over the block
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (groups[mid].first <= it) lo = mid; else hi = mid - 1;
+    }
+    const BatchGroup &g = groups[lo];
+    if (g.dep >= 0) {
+      if (tid == 0) {
+        const unsigned int need = (unsigned int)groups[g.dep].ntiles;
+        unsigned int spins = 0;
+        while (__hip_atomic_load(&done[g.dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          __builtin_amdgcn_s_sleep(8);
+          if (++spins > (1u << 24)) {
+            atomicOr(err, 1u);
+            s_fail = 1;
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    if (!s_fail) {
+      const AxisIo io = {g.in, g.out, g.conj};
+      axis_tile<T, E, IO_C2C>(g.d, io, it - g.first, lds);
+    }
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(&done[lo], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <typename T>
+__global__ void batch_reset_kernel(unsigned long long *head, unsigned int *done, int ngroups,
+                                   unsigned int *err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ngroups) done[i] = 0u;
+  if (i == 0) {
+    *head = 0ull;
+    *err = 0u;
+  }
+}
+
+int batch_prepare(int prec_bytes, int nthreads, size_t lds, int *blocks_per_cu) {
+  hipError_t e;
+  if (prec_bytes == 8) {
+    e = hipFuncSetAttribute((const void *)batch_kernel<double, kRegCap>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_kernel<double, kRegCap>,
+                                                       nthreads, lds);
+  } else {
+    e = hipFuncSetAttribute((const void *)batch_kernel<float, kRegCap>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_kernel<float, kRegCap>,
+                                                       nthreads, lds);
+  }
+  return hip_check(e, "batch kernel set-up");
+}
+
+int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long long total,
+                 unsigned long long *head, unsigned int *done, unsigned int *err, int blocks,
+                 int nthreads, size_t lds, hipStream_t st) {
+  const int rb = (ngroups + 255) / 256;
+  if (prec_bytes == 8) {
+    hipLaunchKernelGGL(batch_reset_kernel<double>, dim3(rb > 0 ? rb : 1), dim3(256), 0, st, head,
+                       done, ngroups, err);
+    hipLaunchKernelGGL((batch_kernel<double, kRegCap>), dim3(blocks), dim3(nthreads), lds, st,
+                       groups, ngroups, total, head, done, err);
+  } else {
+    hipLaunchKernelGGL(batch_reset_kernel<float>, dim3(rb > 0 ? rb : 1), dim3(256), 0, st, head,
+                       done, ngroups, err);
+    hipLaunchKernelGGL((batch_kernel<float, kRegCap>), dim3(blocks), dim3(nthreads), lds, st,
+                       groups, ngroups, total, head, done, err);
+  }
+  return hip_check(hipGetLastError(), "batch kernel launch");
 }
 
 // Bluestein's chirp-z algorithm (L. Bluestein, 1970) for a line length nb

@@ -64,6 +64,18 @@ struct AxisIo {
   int conj;  // 1: inverse transform (conjugate on load and store)
 };
 
+// One group of work items of the batched kernel: all tiles of one stage of
+// one grid.
+struct BatchGroup {
+  const AxisDesc *d;  // device descriptor (general kernel)
+  const void *in;
+  void *out;
+  int conj;
+  int dep;            // group this one waits for (-1: none)
+  long long ntiles;
+  long long first;    // index of its first item in the queue
+};
+
 // Launch shape chosen by the planner for one axis stage.
 struct AxisShape {
   int TX = 1;
@@ -103,6 +115,10 @@ int hip_check(hipError_t e, const char *what);
 int launch_axis(int prec_bytes, int io, const AxisDesc *d_desc, const AxisDesc &h_desc,
                 const AxisIo &io_ptrs, const AxisShape &s, hipStream_t st);
 int axis_kernel_prepare(int prec_bytes);  // raise LDS limits once
+int batch_prepare(int prec_bytes, int nthreads, size_t lds, int *blocks_per_cu);
+int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long long total,
+                 unsigned long long *head, unsigned int *done, unsigned int *err, int blocks,
+                 int nthreads, size_t lds, hipStream_t st);
 
 // copy / pack kernels (copy_kernels.hip)
 int launch_pack(int dir, int split, int esize, const void *src, void *dst,

@@ -925,3 +925,216 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
 }
 
 }  // extern "C"
+
+// ---- many grids in one launch -------------------------------------------------
+// The fft_multiple_grids use (examples/fft_multiple_grids/fft_multiple_grids.f90):
+// every stage of every grid, in order, through one persistent kernel.
+
+struct d2dhip_batch_s {
+  int rb = 8;
+  hipStream_t stream = nullptr;
+  std::vector<d2dhip_plan_s *> plans;
+  AxisDesc *d_descs = nullptr;   // 3 general-kernel c2c descriptors per plan (x, y, z)
+  std::vector<AxisDesc> h_descs;
+  std::vector<AxisShape> shapes;
+  BatchGroup *d_groups = nullptr;
+  int ngroups = 0;
+  long long total = 0;
+  unsigned long long *d_head = nullptr;
+  unsigned int *d_done = nullptr, *d_err = nullptr;
+  int nthreads = 64;
+  size_t lds = 0;
+  int blocks = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+};
+
+extern "C" {
+
+int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans, void *stream) {
+  if (!out || !plans) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  *out = nullptr;
+  if (nplans < 1) return set_error(D2DHIP_ERR_INVALID_VALUE, "nplans < 1");
+  d2dhip_batch_s *b = new (std::nothrow) d2dhip_batch_s();
+  if (!b) return set_error(D2DHIP_ERR_ALLOC, "host allocation failed");
+  b->stream = (hipStream_t)stream;
+  int st = D2DHIP_SUCCESS;
+  for (int i = 0; i < nplans && !st; ++i) {
+    d2dhip_plan_s *p = plans[i];
+    if (!p) st = set_error(D2DHIP_ERR_NULL, "NULL plan in batch");
+    else if (i > 0 && p->rb != b->rb) st = set_error(D2DHIP_ERR_INVALID_VALUE, "mixed precisions in batch");
+    if (st) break;
+    b->rb = p->rb;
+    b->plans.push_back(p);
+    for (int a = 0; a < 3 && !st; ++a) {
+      // the batched kernel runs the general kernel: configure it, copy, restore
+      const int keep = p->geom[a].choice;
+      st = configure(p, a, -1);
+      if (!st) {
+        b->h_descs.push_back(p->h_desc[a]);
+        b->shapes.push_back(p->shape[a]);
+        b->nthreads = std::max(b->nthreads, p->shape[a].nthreads);
+        b->lds = std::max(b->lds, p->shape[a].lds_bytes);
+        if (p->h_desc[a].blue) st = set_error(D2DHIP_ERR_UNSUPPORTED, "Bluestein lengths are not batched");
+      }
+      if (!st) st = configure(p, a, keep);
+    }
+  }
+  if (!st) {
+    const size_t bytes = sizeof(AxisDesc) * b->h_descs.size();
+    if (hipMalloc(&b->d_descs, bytes) != hipSuccess) st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch)");
+    if (!st) st = hip_check(hipMemcpy(b->d_descs, b->h_descs.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  if (!st) {
+    int bpc = 0, dev = 0, ncu = 0;
+    st = batch_prepare(b->rb, b->nthreads, b->lds, &bpc);
+    if (!st) st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (!st) st = hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    b->blocks = std::max(1, bpc) * std::max(1, ncu);
+  }
+  if (!st && (hipMalloc(&b->d_head, sizeof(unsigned long long)) != hipSuccess ||
+              hipMalloc(&b->d_err, sizeof(unsigned int)) != hipSuccess))
+    st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch counters)");
+  if (st) {
+    d2dhip_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return D2DHIP_SUCCESS;
+}
+
+// Lays out the work queue for the given buffers.  mode -1 / +1: one
+// direction; 0: forward then backward (out = unnormalised round trip).
+int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int mode) {
+  if (!b || !in || !out) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (mode != -1 && mode != 0 && mode != 1) return set_error(D2DHIP_ERR_INVALID_VALUE, "mode");
+  std::vector<BatchGroup> groups;
+  // phase-major order: stage 0 of every grid, then stage 1, ...
+  const int nph = mode == 0 ? 6 : 3;
+  const int np = (int)b->plans.size();
+  std::vector<std::vector<int>> last(np);
+  long long first = 0;
+  std::vector<int> prev(np, -1);
+  for (int ph = 0; ph < nph; ++ph) {
+    for (int g = 0; g < np; ++g) {
+      d2dhip_plan_s *p = b->plans[g];
+      const int isign = (mode == 0) ? (ph < 3 ? -1 : 1) : mode;
+      const bool x_first = (p->format == D2DHIP_PHYSICAL_IN_X && isign == -1) ||
+                           (p->format == D2DHIP_PHYSICAL_IN_Z && isign == 1);
+      const int k = ph % 3;
+      const int ax = (k == 1) ? 1 : ((k == 0) == x_first ? 0 : 2);
+      const unsigned skipbit[3] = {D2DHIP_FLAG_SKIP_X, D2DHIP_FLAG_SKIP_Y, D2DHIP_FLAG_SKIP_Z};
+      if (p->flags & skipbit[ax]) continue;
+      BatchGroup G;
+      G.d = b->d_descs + 3 * g + ax;
+      G.in = (prev[g] < 0) ? in[g] : out[g];
+      G.out = out[g];
+      G.conj = isign == 1 ? 1 : 0;
+      G.dep = prev[g];
+      const AxisDesc &h = b->h_descs[3 * g + ax];
+      G.ntiles = (long long)h.tiles0 * h.C1;
+      G.first = first;
+      first += G.ntiles;
+      prev[g] = (int)groups.size();
+      groups.push_back(G);
+    }
+  }
+  for (int g = 0; g < np; ++g)
+    if (prev[g] < 0 && in[g] != out[g])
+      return set_error(D2DHIP_ERR_UNSUPPORTED, "every axis skipped: nothing to batch");
+  if (b->d_groups) (void)hipFree(b->d_groups);
+  if (b->d_done) (void)hipFree(b->d_done);
+  b->d_groups = nullptr;
+  b->d_done = nullptr;
+  b->ngroups = (int)groups.size();
+  b->total = first;
+  if (hipMalloc(&b->d_groups, sizeof(BatchGroup) * groups.size()) != hipSuccess ||
+      hipMalloc(&b->d_done, sizeof(unsigned int) * groups.size()) != hipSuccess)
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch queue)");
+  if (b->graph_exec) {
+    (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return hip_check(hipMemcpy(b->d_groups, groups.data(), sizeof(BatchGroup) * groups.size(),
+                             hipMemcpyHostToDevice),
+                   "hipMemcpy(batch queue)");
+}
+
+int d2dhip_batch_launch(d2dhip_batch b) {
+  if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
+  if (!b->d_groups) return set_error(D2DHIP_ERR_INVALID_VALUE, "batch not bound");
+  return launch_batch(b->rb, b->d_groups, b->ngroups, b->total, b->d_head, b->d_done, b->d_err,
+                      b->blocks, b->nthreads, b->lds, b->stream);
+}
+
+// Captures d2dhip_batch_launch (counter reset + persistent kernel, no
+// memset nodes) into a hipGraph owned by the batch.
+int d2dhip_batch_capture_graph(d2dhip_batch b) {
+  if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
+  if (!b->d_groups) return set_error(D2DHIP_ERR_INVALID_VALUE, "batch not bound");
+  hipStream_t cs = nullptr;
+  int st = hip_check(hipStreamCreate(&cs), "hipStreamCreate");
+  if (st) return st;
+  hipStream_t keep = b->stream;
+  b->stream = cs;
+  st = hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (!st) {
+    const int lst = d2dhip_batch_launch(b);
+    hipGraph_t g = nullptr;
+    st = hip_check(hipStreamEndCapture(cs, &g), "hipStreamEndCapture");
+    if (!st && lst) st = lst;
+    if (!st) {
+      if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->graph = g;
+      st = hip_check(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    } else if (g) {
+      (void)hipGraphDestroy(g);
+    }
+  }
+  b->stream = keep;
+  (void)hipStreamDestroy(cs);
+  return st;
+}
+
+int d2dhip_batch_graph_launch(d2dhip_batch b) {
+  if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
+  if (!b->graph_exec) return set_error(D2DHIP_ERR_INVALID_VALUE, "no captured graph");
+  return hip_check(hipGraphLaunch(b->graph_exec, b->stream), "hipGraphLaunch");
+}
+
+// Waits for the batch's stream and reports a dependency-wait timeout.
+int d2dhip_batch_check(d2dhip_batch b) {
+  if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
+  int st = hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+  if (st) return st;
+  unsigned int err = 0;
+  st = hip_check(hipMemcpy(&err, b->d_err, sizeof(err), hipMemcpyDeviceToHost), "hipMemcpy");
+  if (!st && err) st = set_error(D2DHIP_ERR_HIP, "batched stage dependency wait timed out");
+  return st;
+}
+
+int d2dhip_batch_set_stream(d2dhip_batch b, void *stream) {
+  if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
+  b->stream = (hipStream_t)stream;
+  return D2DHIP_SUCCESS;
+}
+
+int d2dhip_batch_destroy(d2dhip_batch b) {
+  if (!b) return D2DHIP_SUCCESS;
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_descs) (void)hipFree(b->d_descs);
+  if (b->d_groups) (void)hipFree(b->d_groups);
+  if (b->d_done) (void)hipFree(b->d_done);
+  if (b->d_head) (void)hipFree(b->d_head);
+  if (b->d_err) (void)hipFree(b->d_err);
+  delete b;
+  return D2DHIP_SUCCESS;
+}
+
+}  // extern "C"

@@ -87,6 +87,72 @@ def cpu_baseline(n, budget_lines):
     }
 
 
+def bench_batch(args, torch, d2d):
+    """64 independent c2c grids, each dimension from {32,...,256} (seed 4),
+    inputs uniform[-1,1): forward+backward of every grid as 64 sequential
+    decomp_2d_fft_3d pairs, as one batched persistent-kernel launch, and as a
+    replay of that launch captured in a hipGraph."""
+    import numpy as np
+    rng = np.random.default_rng(4)
+    sizes = [32, 48, 64, 80, 96, 128, 160, 192, 256]
+    prec = d2d.PREC_F64 if args.batch_prec == 64 else d2d.PREC_F32
+    cdt = torch.complex128 if prec == d2d.PREC_F64 else torch.complex64
+    elem = d2d.C16 if prec == d2d.PREC_F64 else d2d.C8
+    shapes = [tuple(int(v) for v in rng.choice(sizes, 3)) for _ in range(64)]
+    engines = [d2d.FFTEngine(d2d.PHYSICAL_IN_X, *s, prec=prec) for s in shapes]
+    ins, outs = [], []
+    for g, s in enumerate(shapes):
+        a = torch.empty(int(np.prod(s)), dtype=cdt, device="cuda")
+        d2d.fill_uniform(a, a.numel(), elem, 4 + g)
+        ins.append(a)
+        outs.append(torch.empty_like(a))
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    def sequential():
+        for e, a, o in zip(engines, ins, outs):
+            e.fft_c2c(a, o, -1)
+            e.fft_c2c(o, o, 1)
+
+    batch = d2d.Batch(engines)
+    batch.bind(ins, outs, mode=0)
+    ms_seq = timed(sequential)
+    ms_batch = timed(batch.launch)
+    batch.check()
+    batch.capture_graph()
+    ms_graph = timed(batch.graph_launch)
+    batch.check()
+    rt = max(float(torch.linalg.vector_norm(o / float(np.prod(s)) - a) / torch.linalg.vector_norm(a))
+             for a, o, s in zip(ins, outs, shapes))
+    rb = 16 if prec == d2d.PREC_F64 else 8
+    nbytes = sum(2 * 6 * int(np.prod(s)) * rb for s in shapes)  # 3-pass model, both directions
+    line = {
+        "metric": "grids/s, 64 uneven c2c grids fwd+bwd (batched persistent kernel + hipGraph)",
+        "value": round(64 / (ms_graph * 1e-3), 1), "unit": "grids/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_graph, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == d2d.PREC_F64 else "f32", "data": "synthetic",
+        "config": {"workload": "64 c2c grids, dims from {32..256} seed 4, fwd+bwd",
+                   "shapes": ["%dx%dx%d" % s for s in shapes[:4]] + ["..."]},
+        "sequential_ms": round(ms_seq, 4), "batched_ms": round(ms_batch, 4),
+        "graph_ms": round(ms_graph, 4),
+        "speedup_vs_sequential": round(ms_seq / ms_graph, 3),
+        "GBps_3pass": round(nbytes / (ms_graph * 1e-3) / 1e9, 1),
+        "round_trip_rel_l2_max": rt,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +163,9 @@ def main():
     ap.add_argument("--cpu-lines", type=int, default=16384)
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan-time autotuner (use the default kernels)")
+    ap.add_argument("--workload", choices=["r2c1024", "batch64"], default="r2c1024",
+                    help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids")
+    ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the 3D FFT path on one GPU only; "
@@ -107,6 +176,9 @@ def main():
     import importlib
     import torch
     d2d = importlib.import_module("2decomp-fft_amd")
+    if args.workload == "batch64":
+        bench_batch(args, torch, d2d)
+        return
     n = args.n
     nxc = n // 2 + 1
     print("bench: n=%d steps=%d warmup=%d" % (n, args.steps, args.warmup), file=sys.stderr,

@@ -149,6 +149,28 @@ int d2dhip_exec_r2c(d2dhip_plan plan, const void *in_r, void *out_c);
  * in_c is preserved unless the plan has D2DHIP_FLAG_INPLACE. */
 int d2dhip_exec_c2r(d2dhip_plan plan, void *in_c, void *out_r);
 
+/* ---- many grids in one launch ---------------------------------------------
+ * The multiple-grid use of the reference (decomp_2d_fft_set_ngrid /
+ * decomp_2d_fft_use_grid, src/fft_common.f90:381-443, exercised by
+ * examples/fft_multiple_grids/fft_multiple_grids.f90) runs one
+ * decomp_2d_fft_3d per grid.  A batch runs the c2c stages of many plans (same
+ * precision) in one persistent-kernel launch fed by a device work queue: the
+ * queue is ordered stage by stage, and a stage of a grid starts when the
+ * previous stage of that grid has completed.  bind lays out the queue for the
+ * given device buffers (mode -1 forward, +1 backward, 0 forward then backward
+ * in place on out); launch is stream-ordered and capturable; capture_graph
+ * records it into a hipGraph replayed by graph_launch; check synchronises and
+ * reports a dependency-wait timeout as D2DHIP_ERR_HIP. */
+typedef struct d2dhip_batch_s *d2dhip_batch;
+int d2dhip_batch_create(d2dhip_batch *batch, int nplans, const d2dhip_plan *plans, void *stream);
+int d2dhip_batch_bind_c2c(d2dhip_batch batch, void *const *in, void *const *out, int mode);
+int d2dhip_batch_launch(d2dhip_batch batch);
+int d2dhip_batch_capture_graph(d2dhip_batch batch);
+int d2dhip_batch_graph_launch(d2dhip_batch batch);
+int d2dhip_batch_check(d2dhip_batch batch);
+int d2dhip_batch_set_stream(d2dhip_batch batch, void *stream);
+int d2dhip_batch_destroy(d2dhip_batch batch);
+
 /* ---- transposes and alltoallv packing ------------------------------------
  * d2dhip_transpose replaces transpose_x_to_y / y_to_z / z_to_y / y_to_x for
  * one rank (src/transpose_x_to_y.f90:41-50 and the matching branches of the

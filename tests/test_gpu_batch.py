@@ -1,0 +1,110 @@
+"""Many grids in one persistent-kernel launch (the fft_multiple_grids use,
+examples/fft_multiple_grids/fft_multiple_grids.f90), eager and captured in a
+hipGraph.  The batch runs the same general-kernel stages as the per-plan
+path, so its results must equal the sequential decomp_2d_fft_3d calls bit
+for bit, and match pocketfft to the FP64 / FP32 tolerances."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [32, 48, 64, 80, 96, 128, 160, 192, 256]
+
+
+def grids(count, seed, pool):
+    rng = np.random.default_rng(seed)
+    return [tuple(int(v) for v in rng.choice(pool, 3)) for _ in range(count)]
+
+
+def dev(torch, a, dt):
+    return torch.from_numpy(np.ascontiguousarray(a.ravel(order="F")).astype(dt)).to("cuda")
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_batch_matches_sequential_and_pocketfft(d2d, torch_gpu, prec):
+    torch = torch_gpu
+    shapes = grids(6, 1, [8, 12, 16, 20, 24, 30, 32]) + [(17, 13, 11), (9, 8, 7)]
+    npc = np.complex128 if prec == 64 else np.complex64
+    tol = 1e-13 if prec == 64 else 5e-6
+    rng = np.random.default_rng(2)
+    xs = [np.asfortranarray(rng.uniform(-1, 1, s) + 1j * rng.uniform(-1, 1, s)) for s in shapes]
+    engines = [d2d.FFTEngine(1 if i % 2 == 0 else 3, *s, prec=prec) for i, s in enumerate(shapes)]
+    ins = [dev(torch, x, npc) for x in xs]
+    outs = [torch.empty_like(a) for a in ins]
+    seq = [torch.empty_like(a) for a in ins]
+    for e, a, o in zip(engines, ins, seq):
+        e.fft_c2c(a, o, -1)
+    b = d2d.Batch(engines)
+    b.bind(ins, outs, mode=-1)
+    b.launch()
+    b.check()
+    for x, o, q, s in zip(xs, outs, seq, shapes):
+        assert torch.equal(o, q)
+        got = o.cpu().numpy().reshape(s, order="F")
+        assert rel_l2(got, np.fft.fftn(x.astype(npc).astype(np.complex128))) < tol
+    # forward then backward in one launch: out = N * in
+    b.bind(ins, outs, mode=0)
+    b.launch()
+    b.check()
+    for x, o, s in zip(xs, outs, shapes):
+        got = o.cpu().numpy().reshape(s, order="F") / np.prod(s)
+        assert rel_l2(got, x.astype(npc)) < tol
+
+
+def test_batch_graph_replays(d2d, torch_gpu):
+    """A captured batch replayed several times, with eager work and
+    synchronisation in between, gives the eager result every time."""
+    torch = torch_gpu
+    shapes = [(16, 12, 10), (32, 8, 6), (24, 20, 18)]
+    rng = np.random.default_rng(5)
+    xs = [np.asfortranarray(rng.uniform(-1, 1, s) + 1j * rng.uniform(-1, 1, s)) for s in shapes]
+    engines = [d2d.FFTEngine(1, *s) for s in shapes]
+    ins = [dev(torch, x, np.complex128) for x in xs]
+    outs = [torch.empty_like(a) for a in ins]
+    b = d2d.Batch(engines)
+    b.bind(ins, outs, mode=0)
+    b.launch()
+    b.check()
+    eager = [o.clone() for o in outs]
+    b.capture_graph()
+    for rep in range(4):
+        for o in outs:
+            o.zero_()
+        junk = torch.randn(1000, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        b.graph_launch()
+        b.check()
+        for o, e in zip(outs, eager):
+            assert torch.equal(o, e), rep
+        assert float(junk) == float(junk)
+
+
+def test_config5_64_uneven_grids(d2d, torch_gpu):
+    """64 c2c FP64 grids, each dimension from {32,...,256} (seed 4), inputs
+    uniform[-1,1): one launch for all forward+backward pairs, round trip to
+    1/N, and the forward of the first grids against pocketfft."""
+    torch = torch_gpu
+    shapes = grids(64, 4, SIZES)
+    engines = [d2d.FFTEngine(1, *s) for s in shapes]
+    ins, outs = [], []
+    for g, s in enumerate(shapes):
+        a = torch.empty(int(np.prod(s)), dtype=torch.complex128, device="cuda")
+        d2d.fill_uniform(a, a.numel(), d2d.C16, 4 + g)
+        ins.append(a)
+        outs.append(torch.empty_like(a))
+    b = d2d.Batch(engines)
+    b.bind(ins, outs, mode=0)
+    b.launch()
+    b.check()
+    for a, o, s in zip(ins, outs, shapes):
+        err = float(torch.linalg.vector_norm(o / np.prod(s) - a) / torch.linalg.vector_norm(a))
+        assert err < 1e-13, s
+    b.bind(ins, outs, mode=-1)
+    b.launch()
+    b.check()
+    for g in range(3):
+        x = ins[g].cpu().numpy().reshape(shapes[g], order="F")
+        got = outs[g].cpu().numpy().reshape(shapes[g], order="F")
+        assert rel_l2(got, np.fft.fftn(x)) < 1e-13
