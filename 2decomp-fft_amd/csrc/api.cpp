@@ -20,6 +20,7 @@ int set_error(int status, const std::string &msg) {
 
 int hip_check(hipError_t e, const char *what) {
   if (e == hipSuccess) return D2DHIP_SUCCESS;
+  (void)hipGetLastError();  // do not let this failure resurface at the next launch
   return set_error(e == hipErrorOutOfMemory ? D2DHIP_ERR_ALLOC : D2DHIP_ERR_HIP,
                    std::string(what) + ": " + hipGetErrorString(e));
 }

@@ -312,13 +312,13 @@ int batch_prepare(int prec_bytes, int nthreads, size_t lds, int *blocks_per_cu) 
   hipError_t e;
   if (prec_bytes == 8) {
     e = hipFuncSetAttribute((const void *)batch_kernel<double, kRegCap>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess)
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_kernel<double, kRegCap>,
                                                        nthreads, lds);
   } else {
     e = hipFuncSetAttribute((const void *)batch_kernel<float, kRegCap>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess)
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_kernel<float, kRegCap>,
                                                        nthreads, lds);

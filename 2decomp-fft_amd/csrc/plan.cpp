@@ -985,6 +985,8 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
     if (hipMalloc(&b->d_descs, bytes) != hipSuccess) st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch)");
     if (!st) st = hip_check(hipMemcpy(b->d_descs, b->h_descs.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy");
   }
+  if (!st && b->lds > 160 * 1024 - 64)
+    st = set_error(D2DHIP_ERR_UNSUPPORTED, "batched tile does not fit LDS");
   if (!st) {
     int bpc = 0, dev = 0, ncu = 0;
     st = batch_prepare(b->rb, b->nthreads, b->lds, &bpc);
@@ -1012,7 +1014,6 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   // phase-major order: stage 0 of every grid, then stage 1, ...
   const int nph = mode == 0 ? 6 : 3;
   const int np = (int)b->plans.size();
-  std::vector<std::vector<int>> last(np);
   long long first = 0;
   std::vector<int> prev(np, -1);
   for (int ph = 0; ph < nph; ++ph) {
