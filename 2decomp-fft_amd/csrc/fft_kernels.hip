@@ -249,6 +249,16 @@ __global__ void __launch_bounds__(512)
 // order, so every awaited item is already held by a running block and the
 // wait ends; a spin limit turns a broken schedule into an error flag instead
 // of a hang.
+// The loop control (item index, group, wait outcome) is read back from LDS
+// and made wave-uniform with readfirstlane, so the compiler keeps every
+// barrier in uniform control flow; the completion of an item is signalled in
+// the same single-thread block that fetches the next one.
+__device__ __forceinline__ long long uniform_ll(long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(v & 0xffffffffll));
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
 template <typename T, int E>
 __global__ void __launch_bounds__(512)
     batch_kernel(const BatchGroup *__restrict__ groups, int ngroups, long long total,
@@ -258,27 +268,32 @@ __global__ void __launch_bounds__(512)
   __shared__ long long s_item;
   __shared__ int s_fail;
   const int tid = threadIdx.x;
+  int finished = -1;  // group of the item this block completed last
   for (;;) {
     if (tid == 0) {
+      if (finished >= 0)
+        __hip_atomic_fetch_add(&done[finished], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       s_item = (long long)atomicAdd(head, 1ull);
       s_fail = 0;
     }
     __syncthreads();
-    const long long it = s_item;
-    if (it >= total) break;  // uniform over the block
+    const long long it = uniform_ll(s_item);
+    if (it >= total) break;
     int lo = 0, hi = ngroups - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (groups[mid].first <= it) lo = mid; else hi = mid - 1;
     }
+    lo = __builtin_amdgcn_readfirstlane(lo);
     const BatchGroup &g = groups[lo];
+    int fail = 0;
     if (g.dep >= 0) {
       if (tid == 0) {
         const unsigned int need = (unsigned int)groups[g.dep].ntiles;
         unsigned int spins = 0;
         while (__hip_atomic_load(&done[g.dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
           __builtin_amdgcn_s_sleep(8);
-          if (++spins > (1u << 24)) {
+          if (++spins > (1u << 22)) {
             atomicOr(err, 1u);
             s_fail = 1;
             break;
@@ -287,13 +302,14 @@ __global__ void __launch_bounds__(512)
       }
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      fail = __builtin_amdgcn_readfirstlane(s_fail);
     }
-    if (!s_fail) {
+    if (!fail) {
       const AxisIo io = {g.in, g.out, g.conj};
       axis_tile<T, E, IO_C2C>(g.d, io, it - g.first, lds);
     }
     __syncthreads();
-    if (tid == 0) __hip_atomic_fetch_add(&done[lo], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    finished = lo;
   }
 }
 
