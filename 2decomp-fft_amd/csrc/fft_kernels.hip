@@ -243,16 +243,28 @@ __global__ void __launch_bounds__(512)
 // ---- many grids in one launch ---------------------------------------------------
 // A persistent kernel pulls work items from a device queue: item i belongs
 // to the BatchGroup g with groups[g].first <= i < groups[g].first + ntiles
-// (all tiles of one stage of one grid).  A group may depend on an earlier
-// group (the previous stage of the same grid); its items wait until that
-// group's completion counter reaches its tile count.  Items are dequeued in
-// order, so every awaited item is already held by a running block and the
-// wait ends; a spin limit turns a broken schedule into an error flag instead
-// of a hang.
-// The loop control (item index, group, wait outcome) is read back from LDS
-// and made wave-uniform with readfirstlane, so the compiler keeps every
-// barrier in uniform control flow; the completion of an item is signalled in
-// the same single-thread block that fetches the next one.
+// (all tiles of one stage of one grid).  Groups are laid out phase-major
+// (stage 0 of every grid, then stage 1, ...), and phase p starts only when
+// every item of phase p-1 has completed.
+//
+// The synchronisation is per block and per phase, not per item: a block
+// counts the items it completes in its current phase and publishes the count
+// with one agent-scope release (done[phase] += count) when it dequeues an
+// item of a later phase or finds the queue empty; before its first item of
+// phase p it waits, with one agent-scope acquire, until done[p-1] equals the
+// phase's item count.  Each gfx950 XCD has its own L2, so the release writes
+// back that L2 and the acquire invalidates it; doing both once per phase
+// instead of once per tile keeps the L2 warm within a phase.
+//
+// Progress: items are dequeued in order, so when a block holds an item of
+// phase p, every item of phase p-1 has been taken by a block that is
+// running; such a block publishes its count before it waits for anything,
+// so the wait ends.  A limit on the wall clock (s_memrealtime ticks) turns a
+// broken schedule into an error flag instead of a hang.
+//
+// The loop control (item index, phase, wait outcome) is read back from LDS
+// and made wave-uniform with readfirstlane, so every barrier stays in
+// uniform control flow.
 __device__ __forceinline__ long long uniform_ll(long long v) {
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(v & 0xffffffffll));
   const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
@@ -262,62 +274,71 @@ __device__ __forceinline__ long long uniform_ll(long long v) {
 template <typename T, int E>
 __global__ void __launch_bounds__(512)
     batch_kernel(const BatchGroup *__restrict__ groups, int ngroups, long long total,
-                 unsigned long long *head, unsigned int *done, unsigned int *err) {
+                 const unsigned long long *__restrict__ phase_items, int nphase,
+                 unsigned long long *head, unsigned long long *done, unsigned int *err,
+                 long long wait_ticks) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
   __shared__ long long s_item;
   __shared__ int s_fail;
   const int tid = threadIdx.x;
-  int finished = -1;  // group of the item this block completed last
+  int phase = 0;                // phase of the items counted in `held`
+  unsigned long long held = 0;  // items of `phase` completed here, not yet published
+  int fail = 0;
   for (;;) {
-    if (tid == 0) {
-      if (finished >= 0)
-        __hip_atomic_fetch_add(&done[finished], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      s_item = (long long)atomicAdd(head, 1ull);
-      s_fail = 0;
-    }
+    if (tid == 0) s_item = (long long)atomicAdd(head, 1ull);
     __syncthreads();
     const long long it = uniform_ll(s_item);
-    if (it >= total) break;
-    int lo = 0, hi = ngroups - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (groups[mid].first <= it) lo = mid; else hi = mid - 1;
+    int gi = 0, ph = nphase;
+    if (it < total) {
+      int hi = ngroups - 1;
+      while (gi < hi) {
+        const int mid = (gi + hi + 1) >> 1;
+        if (groups[mid].first <= it) gi = mid; else hi = mid - 1;
+      }
+      gi = __builtin_amdgcn_readfirstlane(gi);
+      ph = __builtin_amdgcn_readfirstlane(groups[gi].phase);
     }
-    lo = __builtin_amdgcn_readfirstlane(lo);
-    const BatchGroup &g = groups[lo];
-    int fail = 0;
-    if (g.dep >= 0) {
+    if (ph != phase) {
       if (tid == 0) {
-        const unsigned int need = (unsigned int)groups[g.dep].ntiles;
-        unsigned int spins = 0;
-        while (__hip_atomic_load(&done[g.dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
-          __builtin_amdgcn_s_sleep(8);
-          if (++spins > (1u << 22)) {
-            atomicOr(err, 1u);
-            s_fail = 1;
-            break;
+        s_fail = 0;
+        if (held)
+          __hip_atomic_fetch_add(&done[phase], held, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (ph < nphase && ph > 0) {
+          const unsigned long long need = phase_items[ph - 1];
+          const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+          while (__hip_atomic_load(&done[ph - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+            __builtin_amdgcn_s_sleep(8);
+            if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks) {
+              atomicOr(err, 1u);
+              s_fail = 1;
+              break;
+            }
           }
         }
       }
       __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (ph < nphase) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       fail = __builtin_amdgcn_readfirstlane(s_fail);
+      phase = ph;
+      held = 0;
     }
+    if (it >= total) break;
     if (!fail) {
+      const BatchGroup &g = groups[gi];
       const AxisIo io = {g.in, g.out, g.conj};
       axis_tile<T, E, IO_C2C>(g.d, io, it - g.first, lds);
     }
     __syncthreads();
-    finished = lo;
+    ++held;
   }
 }
 
 template <typename T>
-__global__ void batch_reset_kernel(unsigned long long *head, unsigned int *done, int ngroups,
+__global__ void batch_reset_kernel(unsigned long long *head, unsigned long long *done, int nphase,
                                    unsigned int *err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ngroups) done[i] = 0u;
+  if (i < nphase) done[i] = 0ull;
   if (i == 0) {
     *head = 0ull;
     *err = 0u;
@@ -343,19 +364,20 @@ int batch_prepare(int prec_bytes, int nthreads, size_t lds, int *blocks_per_cu) 
 }
 
 int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long long total,
-                 unsigned long long *head, unsigned int *done, unsigned int *err, int blocks,
+                 const unsigned long long *phase_items, int nphase, unsigned long long *head,
+                 unsigned long long *done, unsigned int *err, long long wait_ticks, int blocks,
                  int nthreads, size_t lds, hipStream_t st) {
-  const int rb = (ngroups + 255) / 256;
+  const int rb = (nphase + 255) / 256;
   if (prec_bytes == 8) {
     hipLaunchKernelGGL(batch_reset_kernel<double>, dim3(rb > 0 ? rb : 1), dim3(256), 0, st, head,
-                       done, ngroups, err);
+                       done, nphase, err);
     hipLaunchKernelGGL((batch_kernel<double, kRegCap>), dim3(blocks), dim3(nthreads), lds, st,
-                       groups, ngroups, total, head, done, err);
+                       groups, ngroups, total, phase_items, nphase, head, done, err, wait_ticks);
   } else {
     hipLaunchKernelGGL(batch_reset_kernel<float>, dim3(rb > 0 ? rb : 1), dim3(256), 0, st, head,
-                       done, ngroups, err);
+                       done, nphase, err);
     hipLaunchKernelGGL((batch_kernel<float, kRegCap>), dim3(blocks), dim3(nthreads), lds, st,
-                       groups, ngroups, total, head, done, err);
+                       groups, ngroups, total, phase_items, nphase, head, done, err, wait_ticks);
   }
   return hip_check(hipGetLastError(), "batch kernel launch");
 }

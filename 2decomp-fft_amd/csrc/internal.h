@@ -71,7 +71,7 @@ struct BatchGroup {
   const void *in;
   void *out;
   int conj;
-  int dep;            // group this one waits for (-1: none)
+  int phase;          // stage phase: every item of phase p-1 completes before phase p starts
   long long ntiles;
   long long first;    // index of its first item in the queue
 };
@@ -117,7 +117,8 @@ int launch_axis(int prec_bytes, int io, const AxisDesc *d_desc, const AxisDesc &
 int axis_kernel_prepare(int prec_bytes);  // raise LDS limits once
 int batch_prepare(int prec_bytes, int nthreads, size_t lds, int *blocks_per_cu);
 int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long long total,
-                 unsigned long long *head, unsigned int *done, unsigned int *err, int blocks,
+                 const unsigned long long *phase_items, int nphase, unsigned long long *head,
+                 unsigned long long *done, unsigned int *err, long long wait_ticks, int blocks,
                  int nthreads, size_t lds, hipStream_t st);
 
 // copy / pack kernels (copy_kernels.hip)

@@ -940,13 +940,18 @@ struct d2dhip_batch_s {
   BatchGroup *d_groups = nullptr;
   int ngroups = 0;
   long long total = 0;
+  unsigned long long *d_phase_items = nullptr;  // items per phase
+  int nphase = 0;
   unsigned long long *d_head = nullptr;
-  unsigned int *d_done = nullptr, *d_err = nullptr;
+  unsigned long long *d_done = nullptr;  // completed items per phase
+  unsigned int *d_err = nullptr;
+  long long wait_ticks = 0;              // phase-wait limit in wall-clock ticks
   int nthreads = 64;
   size_t lds = 0;
   int blocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  bool bound = false;
 };
 
 extern "C" {
@@ -993,6 +998,9 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
     if (!st) st = hip_check(hipGetDevice(&dev), "hipGetDevice");
     if (!st) st = hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
     b->blocks = std::max(1, bpc) * std::max(1, ncu);
+    int khz = 0;  // s_memrealtime rate
+    if (!st) st = hip_check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
+    b->wait_ticks = 5000LL * (long long)std::max(khz, 1000);  // 5 s
   }
   if (!st && (hipMalloc(&b->d_head, sizeof(unsigned long long)) != hipSuccess ||
               hipMalloc(&b->d_err, sizeof(unsigned int)) != hipSuccess))
@@ -1011,12 +1019,15 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   if (!b || !in || !out) return set_error(D2DHIP_ERR_NULL, "NULL argument");
   if (mode != -1 && mode != 0 && mode != 1) return set_error(D2DHIP_ERR_INVALID_VALUE, "mode");
   std::vector<BatchGroup> groups;
-  // phase-major order: stage 0 of every grid, then stage 1, ...
+  std::vector<unsigned long long> phase_items;
+  // phase-major order: stage 0 of every grid, then stage 1, ...; phases in
+  // which every grid skips its axis are left out
   const int nph = mode == 0 ? 6 : 3;
   const int np = (int)b->plans.size();
   long long first = 0;
-  std::vector<int> prev(np, -1);
+  std::vector<char> started(np, 0);
   for (int ph = 0; ph < nph; ++ph) {
+    unsigned long long items = 0;
     for (int g = 0; g < np; ++g) {
       d2dhip_plan_s *p = b->plans[g];
       const int isign = (mode == 0) ? (ph < 3 ? -1 : 1) : mode;
@@ -1028,30 +1039,33 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
       if (p->flags & skipbit[ax]) continue;
       BatchGroup G;
       G.d = b->d_descs + 3 * g + ax;
-      G.in = (prev[g] < 0) ? in[g] : out[g];
+      G.in = started[g] ? out[g] : in[g];
       G.out = out[g];
       G.conj = isign == 1 ? 1 : 0;
-      G.dep = prev[g];
+      G.phase = (int)phase_items.size();
       const AxisDesc &h = b->h_descs[3 * g + ax];
       G.ntiles = (long long)h.tiles0 * h.C1;
       G.first = first;
       first += G.ntiles;
-      prev[g] = (int)groups.size();
+      items += (unsigned long long)G.ntiles;
+      started[g] = 1;
       groups.push_back(G);
     }
+    if (items) phase_items.push_back(items);
   }
   for (int g = 0; g < np; ++g)
-    if (prev[g] < 0 && in[g] != out[g])
+    if (!started[g] && in[g] != out[g])
       return set_error(D2DHIP_ERR_UNSUPPORTED, "every axis skipped: nothing to batch");
   if (b->d_groups) (void)hipFree(b->d_groups);
   if (b->d_done) (void)hipFree(b->d_done);
+  if (b->d_phase_items) (void)hipFree(b->d_phase_items);
   b->d_groups = nullptr;
   b->d_done = nullptr;
+  b->d_phase_items = nullptr;
   b->ngroups = (int)groups.size();
+  b->nphase = (int)phase_items.size();
   b->total = first;
-  if (hipMalloc(&b->d_groups, sizeof(BatchGroup) * groups.size()) != hipSuccess ||
-      hipMalloc(&b->d_done, sizeof(unsigned int) * groups.size()) != hipSuccess)
-    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch queue)");
+  b->bound = false;
   if (b->graph_exec) {
     (void)hipGraphExecDestroy(b->graph_exec);
     b->graph_exec = nullptr;
@@ -1060,23 +1074,39 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
     (void)hipGraphDestroy(b->graph);
     b->graph = nullptr;
   }
-  return hip_check(hipMemcpy(b->d_groups, groups.data(), sizeof(BatchGroup) * groups.size(),
-                             hipMemcpyHostToDevice),
-                   "hipMemcpy(batch queue)");
+  if (groups.empty()) {  // in == out everywhere: nothing to do
+    b->bound = true;
+    return D2DHIP_SUCCESS;
+  }
+  if (hipMalloc(&b->d_groups, sizeof(BatchGroup) * groups.size()) != hipSuccess ||
+      hipMalloc(&b->d_done, sizeof(unsigned long long) * phase_items.size()) != hipSuccess ||
+      hipMalloc(&b->d_phase_items, sizeof(unsigned long long) * phase_items.size()) != hipSuccess)
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch queue)");
+  int st = hip_check(hipMemcpy(b->d_groups, groups.data(), sizeof(BatchGroup) * groups.size(),
+                               hipMemcpyHostToDevice),
+                     "hipMemcpy(batch queue)");
+  if (!st)
+    st = hip_check(hipMemcpy(b->d_phase_items, phase_items.data(),
+                             sizeof(unsigned long long) * phase_items.size(), hipMemcpyHostToDevice),
+                   "hipMemcpy(batch phases)");
+  b->bound = (st == D2DHIP_SUCCESS);
+  return st;
 }
 
 int d2dhip_batch_launch(d2dhip_batch b) {
   if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
-  if (!b->d_groups) return set_error(D2DHIP_ERR_INVALID_VALUE, "batch not bound");
-  return launch_batch(b->rb, b->d_groups, b->ngroups, b->total, b->d_head, b->d_done, b->d_err,
-                      b->blocks, b->nthreads, b->lds, b->stream);
+  if (!b->bound) return set_error(D2DHIP_ERR_INVALID_VALUE, "batch not bound");
+  if (b->ngroups == 0) return D2DHIP_SUCCESS;
+  return launch_batch(b->rb, b->d_groups, b->ngroups, b->total, b->d_phase_items, b->nphase,
+                      b->d_head, b->d_done, b->d_err, b->wait_ticks, b->blocks, b->nthreads,
+                      b->lds, b->stream);
 }
 
 // Captures d2dhip_batch_launch (counter reset + persistent kernel, no
 // memset nodes) into a hipGraph owned by the batch.
 int d2dhip_batch_capture_graph(d2dhip_batch b) {
   if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
-  if (!b->d_groups) return set_error(D2DHIP_ERR_INVALID_VALUE, "batch not bound");
+  if (!b->bound) return set_error(D2DHIP_ERR_INVALID_VALUE, "batch not bound");
   hipStream_t cs = nullptr;
   int st = hip_check(hipStreamCreate(&cs), "hipStreamCreate");
   if (st) return st;
@@ -1115,7 +1145,7 @@ int d2dhip_batch_check(d2dhip_batch b) {
   if (st) return st;
   unsigned int err = 0;
   st = hip_check(hipMemcpy(&err, b->d_err, sizeof(err), hipMemcpyDeviceToHost), "hipMemcpy");
-  if (!st && err) st = set_error(D2DHIP_ERR_HIP, "batched stage dependency wait timed out");
+  if (!st && err) st = set_error(D2DHIP_ERR_HIP, "batched phase wait timed out (5 s)");
   return st;
 }
 
@@ -1132,6 +1162,7 @@ int d2dhip_batch_destroy(d2dhip_batch b) {
   if (b->d_descs) (void)hipFree(b->d_descs);
   if (b->d_groups) (void)hipFree(b->d_groups);
   if (b->d_done) (void)hipFree(b->d_done);
+  if (b->d_phase_items) (void)hipFree(b->d_phase_items);
   if (b->d_head) (void)hipFree(b->d_head);
   if (b->d_err) (void)hipFree(b->d_err);
   delete b;

@@ -25,6 +25,18 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))  # oracle/oracle.py (cpu_baseline only)
 
 PEAK_HBM_GBS = 8000.0
+
+# HBM bytes per launch of the 1024^3 FP64 stage kernels, from two rocprofv3
+# counter passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, factors calibrated on a
+# device-to-device copy of known size in the same run); the summary with the
+# commands is profiles/hbm_traffic_r2c_c2r_1024cube.md.  Keyed by
+# (n, stage, kernel label); a stage whose tuned kernel differs reports null.
+MEASURED_TRAFFIC = {
+    (1024, "r2c", "double n=1024 tx=4 nt=256 r2c"): 17197260846,
+    (1024, "spec_y", "double n=1024 tx=4 nt=256 c2c"): 17491144854,
+    (1024, "spec_last", "double n=1024 tx=8 nt=512 c2c"): 17222730955,
+    (1024, "c2r", "double n=1024 tx=2 nt=128 c2r"): 17203597149,
+}
 METRIC = "fwd+bwd 3D FFT ms & HBM-roofline % at 1024³ r2c, FP64 vs split-BF16 MFMA"
 
 
@@ -209,6 +221,8 @@ def main():
     dom = max(rep.items(), key=lambda kv: kv[1][0])[0]
     dom_avg = rep[dom][0] / rep[dom][1]
     achieved = sb[dom] / (dom_avg * 1e-3) / 1e9
+    kernels64 = eng.stage_kernels()
+    traffic = MEASURED_TRAFFIC.get((n, dom, kernels64.get(dom)))
 
     # device-to-device copy bandwidth of the same bytes, for context
     cpy = torch.empty_like(spec)
@@ -262,11 +276,12 @@ def main():
         "copy_GBps": round(copy_gbs, 1),
         "round_trip_rel_l2": rt_err,
         "stages": stages,
-        "kernels": {k: v for k, v in eng.stage_kernels().items() if k in stages},
+        "kernels": {k: v for k, v in kernels64.items() if k in stages},
         "plan_seconds": round(plan_s, 2),
-        "roofline": {"bound": "hbm", "kernel": "axis_fft_kernel (%s stage)" % dom,
+        "roofline": {"bound": "hbm", "kernel": "%s (%s stage)" % (kernels64.get(dom), dom),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": sb[dom]},
         "variants": variants,
     }
     if not args.no_cpu:

@@ -155,12 +155,15 @@ int d2dhip_exec_c2r(d2dhip_plan plan, void *in_c, void *out_r);
  * examples/fft_multiple_grids/fft_multiple_grids.f90) runs one
  * decomp_2d_fft_3d per grid.  A batch runs the c2c stages of many plans (same
  * precision) in one persistent-kernel launch fed by a device work queue: the
- * queue is ordered stage by stage, and a stage of a grid starts when the
- * previous stage of that grid has completed.  bind lays out the queue for the
+ * queue is ordered stage by stage (stage 0 of every grid, then stage 1, ...),
+ * and a stage phase starts when every tile of the previous phase has
+ * completed (one release/acquire per workgroup and phase).  Calls that
+ * overlap on one batch object race: the queue counters belong to the batch.
+ * bind lays out the queue for the
  * given device buffers (mode -1 forward, +1 backward, 0 forward then backward
  * in place on out); launch is stream-ordered and capturable; capture_graph
  * records it into a hipGraph replayed by graph_launch; check synchronises and
- * reports a dependency-wait timeout as D2DHIP_ERR_HIP. */
+ * reports a phase-wait timeout (5 s) as D2DHIP_ERR_HIP. */
 typedef struct d2dhip_batch_s *d2dhip_batch;
 int d2dhip_batch_create(d2dhip_batch *batch, int nplans, const d2dhip_plan *plans, void *stream);
 int d2dhip_batch_bind_c2c(d2dhip_batch batch, void *const *in, void *const *out, int mode);
