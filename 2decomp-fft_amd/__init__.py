@@ -111,6 +111,8 @@ def lib():
             "d2dhip_plan_stage_kernel": ([pl, ci, ctypes.c_char_p, ci], ci),
             "d2dhip_plan_stage_candidate": ([pl, ci, ci, ctypes.c_char_p, ci], ci),
             "d2dhip_plan_set_stage_kernel": ([pl, ci, ctypes.c_char_p], ci),
+            "d2dhip_plan_set_chunking": ([pl, ci, ci], ci),
+            "d2dhip_plan_get_chunking": ([pl, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
             "d2dhip_plan_profile": ([pl, ci], ci),
             "d2dhip_plan_profile_report": ([pl, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ci)], ci),
@@ -257,6 +259,16 @@ class FFTEngine:
 
     def device_bytes(self):
         return lib().d2dhip_plan_device_bytes(self._plan)
+
+    def set_chunking(self, fwd_planes, bwd_planes):
+        """Alternate the (r2c x, y) and (y, c2r x) stage pairs over chunks of
+        this many z-planes (PHYSICAL_IN_X; 0 = whole stages)."""
+        check(lib().d2dhip_plan_set_chunking(self._plan, int(fwd_planes), int(bwd_planes)))
+
+    def chunking(self):
+        f, b = ctypes.c_int(), ctypes.c_int()
+        check(lib().d2dhip_plan_get_chunking(self._plan, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
 
     STAGES = ("c2c_x", "c2c_y", "c2c_z", "r2c", "spec_y", "spec_last", "c2r")
 

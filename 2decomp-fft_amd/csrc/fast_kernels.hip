@@ -37,7 +37,11 @@ struct RList {
   static constexpr int prod() { return L(P); }
 };
 
-template <typename T, int N, int TX, int NT, bool LINES, class RL>
+// XM = 0: the LDS exchange between passes moves complex values; XM = 1: it
+// moves the real parts, then the imaginary parts, through a buffer of half
+// the size (two more barriers per exchange), so that twice as many
+// workgroups fit a CU's LDS.
+template <typename T, int N, int TX, int NT, bool LINES, class RL, int XM = 0>
 struct FastCfg {
   static constexpr int P = RL::P;
   static constexpr int RPT = NT / TX;  // butterfly rows per tile column
@@ -57,7 +61,7 @@ struct FastCfg {
   static constexpr int LDT = LINES ? (ROWS | 1) : 1;
   static constexpr int LDJ = LINES ? 1 : TX;
   static constexpr size_t lds_bytes() {
-    return (size_t)(LINES ? TX * LDT : TX * ROWS) * sizeof(cpx<T>);
+    return (size_t)(LINES ? TX * LDT : TX * ROWS) * (XM ? sizeof(T) : sizeof(cpx<T>));
   }
 };
 
@@ -97,10 +101,13 @@ __device__ __forceinline__ void fstore(const AxisDesc *__restrict__ d, void *out
   }
 }
 
-template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL>
-__global__ void __launch_bounds__(NT)
+// WPE: minimum waves per SIMD the register allocation must allow (1 = no
+// constraint beyond the block size).
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
     fast_axis_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
-  using C = FastCfg<T, N, TX, NT, LINES, RL>;
+  using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
+  static_assert(XM == 0 || IO == IO_C2C, "split exchange is for c2c stages");
   constexpr int P = C::P, RPT = C::RPT, LDT = C::LDT, LDJ = C::LDJ, E = C::emax();
   extern __shared__ __align__(16) unsigned char lds_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
@@ -173,23 +180,56 @@ __global__ void __launch_bounds__(NT)
       // outputs of pass p-1 -> LDS -> inputs of pass p
       constexpr int Rq = RL::r[p - 1], NBq = N / Rq, Lq = RL::L(p - 1);
       if constexpr (p > 1 || IO == IO_C2R) __syncthreads();
+      if constexpr (XM == 0) {
 #pragma unroll
-      for (int q = 0; q < C::nb(p - 1); ++q) {
-        const int b = r0 + q * RPT;
-        if (C::exact(p - 1) || b < NBq) {
-          const int k = b % Lq, g = b / Lq;
+        for (int q = 0; q < C::nb(p - 1); ++q) {
+          const int b = r0 + q * RPT;
+          if (C::exact(p - 1) || b < NBq) {
+            const int k = b % Lq, g = b / Lq;
 #pragma unroll
-          for (int s = 0; s < Rq; ++s)
-            lds[C::row(g * Lq * Rq + k + s * Lq) * LDJ + t * LDT] = v[q * Rq + s];
+            for (int s = 0; s < Rq; ++s)
+              lds[C::row(g * Lq * Rq + k + s * Lq) * LDJ + t * LDT] = v[q * Rq + s];
+          }
         }
-      }
-      __syncthreads();
+        __syncthreads();
 #pragma unroll
-      for (int q = 0; q < C::nb(p); ++q) {
-        const int b = r0 + q * RPT;
-        if (C::exact(p) || b < NBR) {
+        for (int q = 0; q < C::nb(p); ++q) {
+          const int b = r0 + q * RPT;
+          if (C::exact(p) || b < NBR) {
 #pragma unroll
-          for (int s = 0; s < R; ++s) v[q * R + s] = lds[C::row(b + s * NBR) * LDJ + t * LDT];
+            for (int s = 0; s < R; ++s) v[q * R + s] = lds[C::row(b + s * NBR) * LDJ + t * LDT];
+          }
+        }
+      } else {
+        // real parts, then imaginary parts; .x of the new layout is filled
+        // before .y of the old layout is written, and the two never alias
+        T *ldr = reinterpret_cast<T *>(lds_raw);
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          if (half) __syncthreads();
+#pragma unroll
+          for (int q = 0; q < C::nb(p - 1); ++q) {
+            const int b = r0 + q * RPT;
+            if (C::exact(p - 1) || b < NBq) {
+              const int k = b % Lq, g = b / Lq;
+#pragma unroll
+              for (int s = 0; s < Rq; ++s)
+                ldr[C::row(g * Lq * Rq + k + s * Lq) * LDJ + t * LDT] =
+                    half ? v[q * Rq + s].y : v[q * Rq + s].x;
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < C::nb(p); ++q) {
+            const int b = r0 + q * RPT;
+            if (C::exact(p) || b < NBR) {
+#pragma unroll
+              for (int s = 0; s < R; ++s) {
+                const T w = ldr[C::row(b + s * NBR) * LDJ + t * LDT];
+                if (half) v[q * R + s].y = w; else v[q * R + s].x = w;
+              }
+            }
+          }
         }
       }
     }
@@ -254,23 +294,23 @@ __global__ void __launch_bounds__(NT)
 
 // ---- registry -----------------------------------------------------------------
 
-template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL>
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
 static int launch_fast(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, hipStream_t st) {
-  using C = FastCfg<T, N, TX, NT, LINES, RL>;
+  using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
   const long long ntiles = xcd_grid((long long)h.tiles0 * h.C1);
   if (ntiles <= 0) return D2DHIP_SUCCESS;
   const long long gx = ntiles < (1LL << 20) ? ntiles : (1LL << 20);
   const long long gy = (ntiles + gx - 1) / gx;
   if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "axis stage has too many tiles");
-  hipLaunchKernelGGL((fast_axis_kernel<T, N, TX, NT, IO, LINES, RL>), dim3((unsigned)gx, (unsigned)gy),
+  hipLaunchKernelGGL((fast_axis_kernel<T, N, TX, NT, IO, LINES, RL, XM, WPE>), dim3((unsigned)gx, (unsigned)gy),
                      dim3(NT), C::lds_bytes(), st, dd, io.in, io.out, io.conj);
   return hip_check(hipGetLastError(), "fast_axis_kernel launch");
 }
 
-template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL>
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
 static hipError_t prepare_fast() {
-  using C = FastCfg<T, N, TX, NT, LINES, RL>;
-  return hipFuncSetAttribute((const void *)fast_axis_kernel<T, N, TX, NT, IO, LINES, RL>,
+  using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
+  return hipFuncSetAttribute((const void *)fast_axis_kernel<T, N, TX, NT, IO, LINES, RL, XM, WPE>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
 }
 
@@ -279,9 +319,9 @@ static std::vector<int> rvec(RList<Rs...>) {
   return {Rs...};
 }
 
-template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL>
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM = 0, int WPE = 1>
 static FastVariant make_variant(const char *name) {
-  using C = FastCfg<T, N, TX, NT, LINES, RL>;
+  using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
   FastVariant v;
   v.rb = (int)sizeof(T);
   v.io = IO;
@@ -293,8 +333,8 @@ static FastVariant make_variant(const char *name) {
   v.ldj = C::LDJ;
   v.ldt = C::LDT;
   v.radices = rvec(RL{});
-  v.launch = &launch_fast<T, N, TX, NT, IO, LINES, RL>;
-  v.prepare = &prepare_fast<T, N, TX, NT, IO, LINES, RL>;
+  v.launch = &launch_fast<T, N, TX, NT, IO, LINES, RL, XM, WPE>;
+  v.prepare = &prepare_fast<T, N, TX, NT, IO, LINES, RL, XM, WPE>;
   v.name = name;
   return v;
 }
@@ -303,6 +343,11 @@ static FastVariant make_variant(const char *name) {
   make_variant<T, N, TX, NT, IO_C2C, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " c2c"), \
       make_variant<T, N, TX, NT, IO_R2C, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " r2c"), \
       make_variant<T, N, TX, NT, IO_C2R, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " c2r")
+
+// c2c only, split exchange, at least WPE waves per SIMD
+#define D2D_SPLIT_C2C(T, N, TX, NT, LINES, RL, WPE)                                      \
+  make_variant<T, N, TX, NT, IO_C2C, LINES, RL, 1, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT \
+                                                        " c2c xs w" #WPE)
 
 using R16_16_4 = RList<16, 16, 4>;
 using R8_8_8 = RList<8, 8, 8>;
@@ -317,6 +362,12 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(double, 1024, 1, 64, false, R16_16_4),
       D2D_VARIANTS(double, 1024, 4, 256, true, R16_16_4),
       D2D_VARIANTS(double, 1024, 2, 128, true, R16_16_4),
+      D2D_SPLIT_C2C(double, 1024, 2, 128, false, R16_16_4, 4),
+      D2D_SPLIT_C2C(double, 1024, 4, 256, false, R16_16_4, 4),
+      D2D_SPLIT_C2C(double, 1024, 8, 512, false, R16_16_4, 4),
+      D2D_SPLIT_C2C(double, 1024, 16, 1024, false, R16_16_4, 4),
+      D2D_SPLIT_C2C(double, 1024, 4, 256, false, R16_16_4, 3),
+      D2D_SPLIT_C2C(double, 1024, 8, 512, false, R16_16_4, 3),
       // FP64, 512
       D2D_VARIANTS(double, 512, 8, 512, false, R8_8_8),
       D2D_VARIANTS(double, 512, 4, 256, false, R8_8_8),

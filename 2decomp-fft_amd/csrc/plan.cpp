@@ -70,6 +70,16 @@ struct d2dhip_plan_s {
   AxisShape shape[kStages];
   const FastVariant *fast[kStages] = {};
   AxisDesc *d_desc = nullptr;
+  // Plane chunking (PHYSICAL_IN_X): the forward r2c x stage and y stage, and
+  // the inverse y stage and c2r x stage, alternate over chunks of `chunk`
+  // z-planes, so that the second stage of each pair reads a chunk the first
+  // has just written while it still sits in the memory-side cache (256 MiB
+  // of Infinity Cache on MI355X).  0 = whole stages.  Chunk descriptors:
+  // slot 2*q + t, q = 0 stage 3 forward, 1 stage 4 forward, 2 stage 4
+  // inverse, 3 stage 6 inverse; t = 0 full chunk, 1 last (shorter) chunk.
+  int chunk_fwd = 0, chunk_bwd = 0;
+  AxisDesc h_cdesc[8];
+  AxisDesc *d_cdesc = nullptr;
   void *wk = nullptr;  // c2r workspace (spectral pencil)
   size_t wk_bytes = 0;
   // stage timing: an event pair around every launch while enabled
@@ -77,6 +87,7 @@ struct d2dhip_plan_s {
   struct Rec {
     int stage;
     hipEvent_t a, b;
+    double w;  // fraction of the stage's work (chunks of a chunked stage)
   };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
@@ -467,10 +478,14 @@ int make_stage(d2dhip_plan_s *p, int idx, int axis, int io, const StageIo &sio, 
   return configure(p, idx, g.cands.empty() ? -1 : 0);
 }
 
+int upload_chunk_descs(d2dhip_plan_s *p);
+
 int upload_descs(d2dhip_plan_s *p) {
-  return hip_check(hipMemcpy(p->d_desc, p->h_desc, sizeof(AxisDesc) * d2dhip_plan_s::kStages,
-                             hipMemcpyHostToDevice),
-                   "hipMemcpy(stage descriptors)");
+  int st = hip_check(hipMemcpy(p->d_desc, p->h_desc, sizeof(AxisDesc) * d2dhip_plan_s::kStages,
+                               hipMemcpyHostToDevice),
+                     "hipMemcpy(stage descriptors)");
+  if (!st) st = upload_chunk_descs(p);
+  return st;
 }
 
 // Builds the seven stage descriptors of a plan and uploads them.
@@ -686,9 +701,176 @@ int run(d2dhip_plan_s *p, int idx, const void *in, void *out, int conj) {
                              p->shape[idx], p->stream);
   if (p->prof) {
     if (!st) st = hip_check(hipEventRecord(b, p->stream), "hipEventRecord");
-    p->recs.push_back({idx, a, b});
+    p->recs.push_back({idx, a, b, 1.0});
   }
   return st;
+}
+
+// ---- plane chunking ------------------------------------------------------------
+
+// Stage descriptor `idx` restricted to `planes` z-planes of a PHYSICAL_IN_X
+// pencil: the x stages own ny lines per plane, the y stage one column set
+// per plane.
+AxisDesc chunk_desc(const d2dhip_plan_s *p, int idx, long long planes) {
+  AxisDesc a = p->h_desc[idx];
+  const StageGeom &g = p->geom[idx];
+  if (g.axis == 0) {
+    a.C0r = planes * p->ny;
+    a.C0 = (g.io == IO_C2C) ? a.C0r : (a.C0r + 1) / 2;
+    a.tiles0 = (int)((a.C0 + a.TX - 1) / a.TX);
+  } else {
+    a.C1 = planes;
+  }
+  return a;
+}
+
+// Element offsets of z-plane k0 in a stage's input and output, and the
+// element sizes in bytes.
+void chunk_offsets(const d2dhip_plan_s *p, int idx, long long k0, size_t *in_b, size_t *out_b) {
+  const StageGeom &g = p->geom[idx];
+  const size_t ie = (size_t)p->rb * (g.io == IO_R2C ? 1 : 2);
+  const size_t oe = (size_t)p->rb * (g.io == IO_C2R ? 1 : 2);
+  const long long in_pl = (g.axis == 0) ? (long long)p->ny * g.in_s0 : g.in_s1;
+  const long long out_pl = (g.axis == 0) ? (long long)p->ny * g.out_s0 : g.out_s1;
+  *in_b = (size_t)(k0 * in_pl) * ie;
+  *out_b = (size_t)(k0 * out_pl) * oe;
+}
+
+// Rebuilds and uploads the eight chunk descriptors for the current kernel
+// choices and chunk sizes.
+int upload_chunk_descs(d2dhip_plan_s *p) {
+  if (p->format != D2DHIP_PHYSICAL_IN_X) return D2DHIP_SUCCESS;
+  const int stage_of[4] = {3, 4, 4, 6};
+  for (int q = 0; q < 4; ++q) {
+    const long long K = (q < 2) ? p->chunk_fwd : p->chunk_bwd;
+    const long long tail = K > 0 ? p->nz - (p->nz - 1) / K * K : 0;
+    p->h_cdesc[2 * q] = chunk_desc(p, stage_of[q], K > 0 ? K : 1);
+    p->h_cdesc[2 * q + 1] = chunk_desc(p, stage_of[q], tail > 0 ? tail : 1);
+  }
+  if (!p->d_cdesc && hipMalloc(&p->d_cdesc, sizeof(p->h_cdesc)) != hipSuccess) {
+    p->d_cdesc = nullptr;
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(chunk descriptors) failed");
+  }
+  return hip_check(hipMemcpy(p->d_cdesc, p->h_cdesc, sizeof(p->h_cdesc), hipMemcpyHostToDevice),
+                   "hipMemcpy(chunk descriptors)");
+}
+
+// One stage launch over z-planes [k0, k0 + planes) through chunk slot q.
+int run_chunk(d2dhip_plan_s *p, int q, int idx, long long k0, long long planes, long long K,
+              const void *in, void *out, int conj) {
+  const int slot = 2 * q + (planes == K ? 0 : 1);
+  size_t ib = 0, ob = 0;
+  chunk_offsets(p, idx, k0, &ib, &ob);
+  const AxisIo io = {static_cast<const unsigned char *>(in) + ib, static_cast<unsigned char *>(out) + ob,
+                     conj};
+  hipEvent_t a = nullptr, b = nullptr;
+  if (p->prof) {
+    a = take_event(p);
+    b = take_event(p);
+    if (!a || !b) return set_error(D2DHIP_ERR_HIP, "hipEventCreate failed");
+    int st = hip_check(hipEventRecord(a, p->stream), "hipEventRecord");
+    if (st) return st;
+  }
+  const AxisDesc *dd = p->d_cdesc + slot;
+  int st = p->fast[idx] ? p->fast[idx]->launch(dd, p->h_cdesc[slot], io, p->stream)
+                        : launch_axis(p->rb, p->geom[idx].io, dd, p->h_cdesc[slot], io,
+                                      p->shape[idx], p->stream);
+  if (p->prof) {
+    if (!st) st = hip_check(hipEventRecord(b, p->stream), "hipEventRecord");
+    p->recs.push_back({idx, a, b, (double)planes / (double)p->nz});
+  }
+  return st;
+}
+
+// The stage pair (i1 then i2) over chunks of K z-planes: q1/q2 are their
+// chunk slots; i1 reads in1 and writes mid, i2 reads mid and writes out2.
+int run_pair_chunked(d2dhip_plan_s *p, long long K, int q1, int i1, int q2, int i2,
+                     const void *in1, void *mid, void *out2, int conj) {
+  for (long long k0 = 0; k0 < p->nz; k0 += K) {
+    const long long pl = std::min<long long>(K, p->nz - k0);
+    int st = run_chunk(p, q1, i1, k0, pl, K, in1, mid, conj);
+    if (!st) st = run_chunk(p, q2, i2, k0, pl, K, mid, out2, conj);
+    if (st) return st;
+  }
+  return D2DHIP_SUCCESS;
+}
+
+// Times the forward pair (r2c x, y) and the inverse pair (y, c2r x) for
+// chunk sizes K in {0, 2, 4, ..., 64} (K < nz) on scratch buffers and keeps
+// the fastest of each; cached like the kernel choices, keyed by the grid and
+// the chosen kernels.
+int tune_chunks(d2dhip_plan_s *p) {
+  if (p->format != D2DHIP_PHYSICAL_IN_X || p->nz < 2) return D2DHIP_SUCCESS;
+  char kb[512];
+  std::snprintf(kb, sizeof(kb), "v1 chunk rb=%d n=%d,%d,%d k=%s|%s|%s", p->rb, p->nx, p->ny, p->nz,
+                choice_name(p, 3).c_str(), choice_name(p, 4).c_str(), choice_name(p, 6).c_str());
+  const std::string key(kb);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end() && std::sscanf(it->second.c_str(), "%d,%d", &p->chunk_fwd, &p->chunk_bwd) == 2)
+    return upload_chunk_descs(p);
+  const long long nph = (long long)p->nx * p->ny * p->nz;
+  const long long nsp = (long long)(p->nx / 2 + 1) * p->ny * p->nz;
+  const size_t need = (size_t)std::max(nph * p->rb, nsp * 2 * p->rb);
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = D2DHIP_SUCCESS;
+  if (hipMalloc(&a, need) != hipSuccess || hipMalloc(&b, need) != hipSuccess)
+    st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(tuning buffers) failed");
+  if (!st) st = hip_check(hipMemset(a, 0, need), "hipMemset");
+  if (!st) st = hip_check(hipMemset(b, 0, need), "hipMemset");
+  if (!st) st = hip_check(hipEventCreate(&e0), "hipEventCreate");
+  if (!st) st = hip_check(hipEventCreate(&e1), "hipEventCreate");
+  const int prof = p->prof;
+  p->prof = 0;
+  int best[2] = {0, 0};
+  for (int dir = 0; dir < 2 && !st; ++dir) {
+    float best_ms = 1e30f;
+    for (int K = 0; K <= 64 && K < p->nz && !st; K = (K == 0 ? 2 : 2 * K)) {
+      (dir == 0 ? p->chunk_fwd : p->chunk_bwd) = K;
+      if ((st = upload_chunk_descs(p))) break;
+      auto once = [&]() {
+        if (dir == 0) {
+          if (K == 0) {
+            int r = run(p, 3, a, b, 0);
+            return r ? r : run(p, 4, b, b, 0);
+          }
+          return run_pair_chunked(p, K, 0, 3, 1, 4, a, b, b, 0);
+        }
+        if (K == 0) {
+          int r = run(p, 4, a, a, 1);
+          return r ? r : run(p, 6, a, b, 1);
+        }
+        return run_pair_chunked(p, K, 2, 4, 3, 6, a, a, b, 1);
+      };
+      if ((st = once())) break;
+      if ((st = hip_check(hipEventRecord(e0, p->stream), "hipEventRecord"))) break;
+      const int reps = 3;
+      for (int r = 0; r < reps && !st; ++r) st = once();
+      if (st) break;
+      if ((st = hip_check(hipEventRecord(e1, p->stream), "hipEventRecord"))) break;
+      if ((st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize"))) break;
+      float t = 0.f;
+      if ((st = hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))) break;
+      if (t / reps < best_ms) {
+        best_ms = t / reps;
+        best[dir] = K;
+      }
+    }
+  }
+  p->prof = prof;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  p->chunk_fwd = best[0];
+  p->chunk_bwd = best[1];
+  if (!st) {
+    char v[64];
+    std::snprintf(v, sizeof(v), "%d,%d", best[0], best[1]);
+    tune_store(key, v);
+  }
+  int st2 = upload_chunk_descs(p);
+  return st ? st : st2;
 }
 
 }  // namespace
@@ -720,7 +902,20 @@ int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, in
   p->stream = (hipStream_t)stream;
   int st = axis_kernel_prepare(p->rb);
   if (!st) st = make_stages(p);
-  if (!st && ((flags & D2DHIP_FLAG_TUNE) || std::getenv("D2DHIP_TUNE"))) st = tune_stages(p);
+  const bool tune = (flags & D2DHIP_FLAG_TUNE) || std::getenv("D2DHIP_TUNE");
+  if (!st && tune) st = tune_stages(p);
+  if (!st) {
+    // D2DHIP_CHUNK_PLANES="f,b" fixes the plane chunks (0 = whole stages)
+    const char *e = std::getenv("D2DHIP_CHUNK_PLANES");
+    int f = 0, b = 0;
+    if (e && std::sscanf(e, "%d,%d", &f, &b) == 2) {
+      st = d2dhip_plan_set_chunking(p, f, b);
+    } else if (tune) {
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      tune_load();
+      st = tune_chunks(p);
+    }
+  }
   if (!st && !(flags & D2DHIP_FLAG_INPLACE)) {
     const long long nsp = (format == D2DHIP_PHYSICAL_IN_X)
                               ? (long long)(nx / 2 + 1) * ny * nz
@@ -754,6 +949,7 @@ int d2dhip_plan_destroy(d2dhip_plan p) {
   }
   if (p->wk) (void)hipFree(p->wk);
   if (p->d_desc) (void)hipFree(p->d_desc);
+  if (p->d_cdesc) (void)hipFree(p->d_cdesc);
   delete p;
   return D2DHIP_SUCCESS;
 }
@@ -792,18 +988,21 @@ int d2dhip_plan_profile_report(d2dhip_plan p, double ms[7], int launches[7]) {
     if (launches) launches[i] = 0;
   }
   int st = D2DHIP_SUCCESS;
+  double wsum[d2dhip_plan_s::kStages] = {};
   for (auto &r : p->recs) {
     if (!st) st = hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
     float t = 0.f;
     if (!st) st = hip_check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
     if (!st) {
       if (ms) ms[r.stage] += t;
-      if (launches) launches[r.stage] += 1;
+      if (launches) wsum[r.stage] += r.w;
     }
     p->pool.push_back(r.a);
     p->pool.push_back(r.b);
   }
   p->recs.clear();
+  if (launches)
+    for (int i = 0; i < d2dhip_plan_s::kStages; ++i) launches[i] = (int)std::lround(wsum[i]);
   return st;
 }
 
@@ -840,6 +1039,24 @@ int d2dhip_plan_set_stage_kernel(d2dhip_plan p, int stage, const char *name) {
   int st = configure(p, stage, c);
   if (!st) st = upload_descs(p);
   return st;
+}
+
+int d2dhip_plan_set_chunking(d2dhip_plan p, int fwd_planes, int bwd_planes) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (fwd_planes < 0 || bwd_planes < 0)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "chunk size must be >= 0");
+  if (p->format != D2DHIP_PHYSICAL_IN_X && (fwd_planes || bwd_planes))
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "plane chunking is for PHYSICAL_IN_X plans");
+  p->chunk_fwd = fwd_planes >= p->nz ? 0 : fwd_planes;
+  p->chunk_bwd = bwd_planes >= p->nz ? 0 : bwd_planes;
+  return upload_chunk_descs(p);
+}
+
+int d2dhip_plan_get_chunking(d2dhip_plan p, int *fwd_planes, int *bwd_planes) {
+  if (!p || !fwd_planes || !bwd_planes) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  *fwd_planes = p->chunk_fwd;
+  *bwd_planes = p->chunk_bwd;
+  return D2DHIP_SUCCESS;
 }
 
 long long d2dhip_plan_device_bytes(d2dhip_plan p) {
@@ -890,8 +1107,12 @@ int d2dhip_exec_r2c(d2dhip_plan p, const void *in_r, void *out_c) {
   const int warn = (p->flags & (fx ? D2DHIP_FLAG_SKIP_X : D2DHIP_FLAG_SKIP_Z)) ? 1 : 0;
   const unsigned skip_last = fx ? D2DHIP_FLAG_SKIP_Z : D2DHIP_FLAG_SKIP_X;
   int st;
-  if ((st = run(p, 3, in_r, out_c, 0))) return st;
-  if (!(p->flags & D2DHIP_FLAG_SKIP_Y) && (st = run(p, 4, out_c, out_c, 0))) return st;
+  if (fx && p->chunk_fwd > 0 && !(p->flags & D2DHIP_FLAG_SKIP_Y)) {
+    if ((st = run_pair_chunked(p, p->chunk_fwd, 0, 3, 1, 4, in_r, out_c, out_c, 0))) return st;
+  } else {
+    if ((st = run(p, 3, in_r, out_c, 0))) return st;
+    if (!(p->flags & D2DHIP_FLAG_SKIP_Y) && (st = run(p, 4, out_c, out_c, 0))) return st;
+  }
   if (!(p->flags & skip_last) && (st = run(p, 5, out_c, out_c, 0))) return st;
   if (warn)
     return set_error(D2DHIP_WARN_SKIP_IGNORED, "r2c / c2r transform can not be skipped");
@@ -914,11 +1135,15 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
     if ((st = run(p, 5, src, wk, 1))) return st;
     src = wk;
   }
-  if (!(p->flags & D2DHIP_FLAG_SKIP_Y)) {
-    if ((st = run(p, 4, src, wk, 1))) return st;
-    src = wk;
+  if (fx && p->chunk_bwd > 0 && !(p->flags & D2DHIP_FLAG_SKIP_Y)) {
+    if ((st = run_pair_chunked(p, p->chunk_bwd, 2, 4, 3, 6, src, wk, out_r, 1))) return st;
+  } else {
+    if (!(p->flags & D2DHIP_FLAG_SKIP_Y)) {
+      if ((st = run(p, 4, src, wk, 1))) return st;
+      src = wk;
+    }
+    if ((st = run(p, 6, src, out_r, 1))) return st;
   }
-  if ((st = run(p, 6, src, out_r, 1))) return st;
   if (warn)
     return set_error(D2DHIP_WARN_SKIP_IGNORED, "r2c / c2r transform can not be skipped");
   return D2DHIP_SUCCESS;

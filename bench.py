@@ -56,13 +56,13 @@ def round_trip_bytes(n, rb):
 
 
 def time_round_trips(torch, eng, x, spec, y, steps, warmup, profile):
+    """ms per r2c+c2r over `steps` timed steps (no per-launch events inside
+    the timed region); with `profile`, the same number of further steps run
+    with a hipEvent pair around every kernel launch for per-stage times."""
     for _ in range(warmup):
         eng.fft_r2c(x, spec)
         eng.fft_c2r(spec, y)
     torch.cuda.synchronize()
-    if profile:
-        eng.profile(True)
-        eng.profile_report()  # drop anything recorded before the timed region
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record()
@@ -72,8 +72,15 @@ def time_round_trips(torch, eng, x, spec, y, steps, warmup, profile):
     stop.record()
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / steps
-    rep = eng.profile_report() if profile else {}
+    rep = {}
     if profile:
+        eng.profile(True)
+        eng.profile_report()  # drop anything recorded before
+        for _ in range(steps):
+            eng.fft_r2c(x, spec)
+            eng.fft_c2r(spec, y)
+        torch.cuda.synchronize()
+        rep = eng.profile_report()
         eng.profile(False)
     return ms, rep
 
@@ -252,7 +259,8 @@ def main():
     variants["fp32"] = {"ms": round(ms32, 3), "rel_l2_vs_fp64": err32, "max_abs_vs_fp64": maxabs32,
                         "GBps_3pass": round(round_trip_bytes(n, 4) / (ms32 * 1e-3) / 1e9, 1),
                         "speedup_vs_fp64": round(ms / ms32, 3),
-                        "kernels": {k: kernels32[k] for k in ("r2c", "spec_y", "spec_last", "c2r")}}
+                        "kernels": {k: kernels32[k] for k in ("r2c", "spec_y", "spec_last", "c2r")},
+                        "chunk_planes": {"fwd": eng32.chunking()[0], "bwd": eng32.chunking()[1]}}
     variants["split_bf16"] = {"status": "not built yet"}
     del x32, spec32, y32, eng32
 
@@ -277,6 +285,7 @@ def main():
         "round_trip_rel_l2": rt_err,
         "stages": stages,
         "kernels": {k: v for k, v in kernels64.items() if k in stages},
+        "chunk_planes": {"fwd": eng.chunking()[0], "bwd": eng.chunking()[1]},
         "plan_seconds": round(plan_s, 2),
         "roofline": {"bound": "hbm", "kernel": "%s (%s stage)" % (kernels64.get(dom), dom),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

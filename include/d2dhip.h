@@ -137,6 +137,18 @@ int d2dhip_plan_stage_kernel(d2dhip_plan plan, int stage, char *name, int len);
 int d2dhip_plan_stage_candidate(d2dhip_plan plan, int stage, int idx, char *name, int len);
 /* Makes a stage run the named candidate (what the autotuner does). */
 int d2dhip_plan_set_stage_kernel(d2dhip_plan plan, int stage, const char *name);
+/* Plane chunking of a PHYSICAL_IN_X plan (no counterpart in the reference,
+ * whose stages always cover the whole pencil): the forward r2c-x and y
+ * stages, and the inverse y and c2r-x stages, alternate over chunks of
+ * fwd_planes / bwd_planes z-planes so that the second stage of each pair
+ * re-reads a chunk from the memory-side cache instead of HBM; 0 = whole
+ * stages (the default).  Plans created with D2DHIP_FLAG_TUNE pick both
+ * sizes by timing; the environment variable D2DHIP_CHUNK_PLANES="f,b"
+ * fixes them at plan creation.  Results agree with unchunked execution to
+ * rounding (real pairs in the x stages can differ at a chunk edge when ny
+ * times the chunk size is odd). */
+int d2dhip_plan_set_chunking(d2dhip_plan plan, int fwd_planes, int bwd_planes);
+int d2dhip_plan_get_chunking(d2dhip_plan plan, int *fwd_planes, int *bwd_planes);
 /* Device bytes the plan holds (twiddles + workspace). */
 long long d2dhip_plan_device_bytes(d2dhip_plan plan);
 
