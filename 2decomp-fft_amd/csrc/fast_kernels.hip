@@ -376,6 +376,10 @@ const std::vector<FastVariant> &fast_registry() {
       // FP32, 1024 and 512
       D2D_VARIANTS(float, 1024, 8, 512, false, R16_16_4),
       D2D_VARIANTS(float, 1024, 4, 256, false, R16_16_4),
+      D2D_VARIANTS(float, 1024, 16, 1024, false, R16_16_4),
+      D2D_SPLIT_C2C(float, 1024, 8, 512, false, R16_16_4, 6),
+      D2D_SPLIT_C2C(float, 1024, 4, 256, false, R16_16_4, 6),
+      D2D_SPLIT_C2C(float, 1024, 16, 1024, false, R16_16_4, 8),
       D2D_VARIANTS(float, 1024, 8, 512, true, R16_16_4),
       D2D_VARIANTS(float, 1024, 4, 256, true, R16_16_4),
       D2D_VARIANTS(float, 512, 16, 512, false, R8_8_8),
