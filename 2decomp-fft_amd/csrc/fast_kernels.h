@@ -11,6 +11,7 @@ namespace d2dhip {
 
 struct FastVariant {
   int rb = 8;      // bytes per real (8 FP64, 4 FP32)
+  int mf = 0;      // 1: split-BF16 radix-16 passes on the matrix cores
   int io = 0;      // IO_C2C / IO_R2C / IO_C2R
   int lines = 0;   // 1: contiguous lines (x axis), 0: strided columns
   int n = 0;       // FFT length
@@ -25,5 +26,7 @@ struct FastVariant {
 
 // All specialised variants compiled into the library.
 const std::vector<FastVariant> &fast_registry();
+// The split-BF16 matrix-core variants (mfma_kernels.hip).
+std::vector<FastVariant> mf_variants();
 
 }  // namespace d2dhip

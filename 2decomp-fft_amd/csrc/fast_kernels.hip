@@ -15,91 +15,11 @@
 #include <hip/hip_runtime.h>
 
 #include "codelets.h"
+#include "fast_common.h"
 #include "fast_kernels.h"
 #include "internal.h"
 
 namespace d2dhip {
-
-template <int... Rs>
-struct RList {
-  static constexpr int P = sizeof...(Rs);
-  static constexpr int r[P] = {Rs...};
-  static constexpr int L(int p) {
-    int l = 1;
-    for (int i = 0; i < p; ++i) l *= r[i];
-    return l;
-  }
-  static constexpr int tw_off(int p) {
-    int o = 0;
-    for (int i = 0; i < p; ++i) o += L(i) * (r[i] - 1);
-    return o;
-  }
-  static constexpr int prod() { return L(P); }
-};
-
-// XM = 0: the LDS exchange between passes moves complex values; XM = 1: it
-// moves the real parts, then the imaginary parts, through a buffer of half
-// the size (two more barriers per exchange), so that twice as many
-// workgroups fit a CU's LDS.
-template <typename T, int N, int TX, int NT, bool LINES, class RL, int XM = 0>
-struct FastCfg {
-  static constexpr int P = RL::P;
-  static constexpr int RPT = NT / TX;  // butterfly rows per tile column
-  static_assert(NT % TX == 0, "NT must be a multiple of TX");
-  static_assert(RL::prod() == N, "radices must multiply to N");
-  static constexpr int nb(int p) { return (N / RL::r[p] + RPT - 1) / RPT; }
-  static constexpr bool exact(int p) { return (N / RL::r[p]) % RPT == 0; }
-  static constexpr int emax() {
-    int e = 0;
-    for (int p = 0; p < P; ++p) e = e > nb(p) * RL::r[p] ? e : nb(p) * RL::r[p];
-    return e;
-  }
-  // LDS row of point j: one pad row after every 16, so that the stride-R
-  // writes of the first exchange spread over all banks.
-  static constexpr int row(int j) { return j + (j >> 4); }
-  static constexpr int ROWS = row(N - 1) + 1;
-  static constexpr int LDT = LINES ? (ROWS | 1) : 1;
-  static constexpr int LDJ = LINES ? 1 : TX;
-  static constexpr size_t lds_bytes() {
-    return (size_t)(LINES ? TX * LDT : TX * ROWS) * (XM ? sizeof(T) : sizeof(cpx<T>));
-  }
-};
-
-template <typename T, int IO>
-__device__ __forceinline__ cpx<T> fload(const AxisDesc *__restrict__ d, const void *in,
-                                        long long base, bool cvalid, bool bvalid, long long ca,
-                                        int j, int conj) {
-  if constexpr (IO == IO_R2C) {
-    const T *p = reinterpret_cast<const T *>(in);
-    const long long o = base + (long long)j * d->in_sj;
-    cpx<T> v = {T(0), T(0)};
-    if (ca < d->C0r) v.x = p[o];
-    if (ca + 1 < d->C0r) v.y = p[o + d->in_s0];
-    return v;
-  } else {
-    cpx<T> v = {T(0), T(0)};
-    if (cvalid) v = reinterpret_cast<const cpx<T> *>(in)[base + (long long)j * d->in_sj];
-    if (conj) v.y = -v.y;
-    return v;
-  }
-  (void)bvalid;
-}
-
-template <typename T, int IO>
-__device__ __forceinline__ void fstore(const AxisDesc *__restrict__ d, void *out, long long base,
-                                       bool cvalid, long long ca, int j, cpx<T> v, int conj) {
-  if constexpr (IO == IO_C2R) {
-    T *p = reinterpret_cast<T *>(out);
-    const long long o = base + (long long)j * d->out_sj;
-    if (ca < d->C0r) p[o] = v.x;
-    if (ca + 1 < d->C0r) p[o + d->out_s0] = -v.y;
-  } else {
-    if (cvalid) {
-      if (conj) v.y = -v.y;
-      reinterpret_cast<cpx<T> *>(out)[base + (long long)j * d->out_sj] = v;
-    }
-  }
-}
 
 // WPE: minimum waves per SIMD the register allocation must allow (1 = no
 // constraint beyond the block size).
@@ -134,20 +54,38 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
     // conj(Z) of the merged half spectra, all N points, into LDS
     constexpr int NH = N / 2 + 1;
     const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in);
-    for (int w = tid; w < TX * NH; w += NT) {
-      const int tt = LINES ? w / NH : w % TX;
-      const int k = LINES ? w % NH : w / TX;
-      const long long cc = 2 * (c0b + tt);
-      const long long base = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
-      cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
-      if (cc < d->C0r) A = src[base];
-      if (cc + 1 < d->C0r) B = src[base + d->in_s0];
-      if (k == 0 || 2 * k == N) {
-        A.y = T(0);
-        B.y = T(0);
+    // all loads first (compile-time trip count), then the LDS writes, so a
+    // thread has every load of the phase in flight at once
+    constexpr int ITER = (TX * NH + NT - 1) / NT;
+    cpx<T> A[ITER], B[ITER];
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+      const int w = tid + it * NT;
+      A[it] = cpx<T>{T(0), T(0)};
+      B[it] = cpx<T>{T(0), T(0)};
+      if (w < TX * NH) {
+        const int tt = LINES ? w / NH : w % TX;
+        const int k = LINES ? w % NH : w / TX;
+        const long long cc = 2 * (c0b + tt);
+        const long long base = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
+        if (cc < d->C0r) A[it] = src[base];
+        if (cc + 1 < d->C0r) B[it] = src[base + d->in_s0];
       }
-      lds[C::row(k) * LDJ + tt * LDT] = cpx<T>{A.x - B.y, -(A.y + B.x)};
-      if (k > 0 && 2 * k < N) lds[C::row(N - k) * LDJ + tt * LDT] = cpx<T>{A.x + B.y, A.y - B.x};
+    }
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+      const int w = tid + it * NT;
+      if (w < TX * NH) {
+        const int tt = LINES ? w / NH : w % TX;
+        const int k = LINES ? w % NH : w / TX;
+        cpx<T> a = A[it], b = B[it];
+        if (k == 0 || 2 * k == N) {
+          a.y = T(0);
+          b.y = T(0);
+        }
+        lds[C::row(k) * LDJ + tt * LDT] = cpx<T>{a.x - b.y, -(a.y + b.x)};
+        if (k > 0 && 2 * k < N) lds[C::row(N - k) * LDJ + tt * LDT] = cpx<T>{a.x + b.y, a.y - b.x};
+      }
     }
     __syncthreads();
     constexpr int R = RL::r[0], NBR = N / R;
@@ -353,7 +291,8 @@ using R16_16_4 = RList<16, 16, 4>;
 using R8_8_8 = RList<8, 8, 8>;
 
 const std::vector<FastVariant> &fast_registry() {
-  static const std::vector<FastVariant> reg = {
+  static const std::vector<FastVariant> reg = [] {
+    std::vector<FastVariant> r = {
       // FP64, 1024: strided columns and contiguous lines
       D2D_VARIANTS(double, 1024, 4, 256, false, R16_16_4),
       D2D_VARIANTS(double, 1024, 2, 128, false, R16_16_4),
@@ -384,7 +323,10 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(float, 1024, 4, 256, true, R16_16_4),
       D2D_VARIANTS(float, 512, 16, 512, false, R8_8_8),
       D2D_VARIANTS(float, 512, 8, 256, true, R8_8_8),
-  };
+    };
+    for (const FastVariant &v : mf_variants()) r.push_back(v);
+    return r;
+  }();
   return reg;
 }
 

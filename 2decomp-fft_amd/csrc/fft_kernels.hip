@@ -242,8 +242,10 @@ __global__ void __launch_bounds__(512)
 
 // ---- many grids in one launch ---------------------------------------------------
 // A persistent kernel pulls work items from a device queue: item i belongs
-// to the BatchGroup g with groups[g].first <= i < groups[g].first + ntiles
-// (all tiles of one stage of one grid).  Groups are laid out phase-major
+// to the BatchGroup g (one stage of one grid) with groups[g].first <= i <
+// groups[g].first + ceil(ntiles / per_item), and covers per_item consecutive
+// tiles of it, so that the queue, barrier and descriptor costs of an item
+// are spread over enough work.  Groups are laid out phase-major
 // (stage 0 of every grid, then stage 1, ...), and phase p starts only when
 // every item of phase p-1 has completed.
 //
@@ -327,7 +329,12 @@ __global__ void __launch_bounds__(512)
     if (!fail) {
       const BatchGroup &g = groups[gi];
       const AxisIo io = {g.in, g.out, g.conj};
-      axis_tile<T, E, IO_C2C>(g.d, io, it - g.first, lds);
+      const long long t0 = uniform_ll((it - g.first) * g.per_item);
+      const long long t1 = uniform_ll(min(t0 + (long long)g.per_item, g.ntiles));
+      for (long long tile = t0; tile < t1; ++tile) {
+        if (tile > t0) __syncthreads();  // LDS reuse across tiles
+        axis_tile<T, E, IO_C2C>(g.d, io, tile, lds);
+      }
     }
     __syncthreads();
     ++held;

@@ -72,8 +72,9 @@ struct BatchGroup {
   void *out;
   int conj;
   int phase;          // stage phase: every item of phase p-1 completes before phase p starts
+  int per_item;       // consecutive tiles one queue item covers
   long long ntiles;
-  long long first;    // index of its first item in the queue
+  long long first;    // index of its first queue item (items: ceil(ntiles / per_item))
 };
 
 // Launch shape chosen by the planner for one axis stage.

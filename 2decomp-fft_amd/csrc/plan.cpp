@@ -470,10 +470,18 @@ int make_stage(d2dhip_plan_s *p, int idx, int axis, int io, const StageIo &sio, 
   g.in_elems = in_elems;
   g.out_elems = out_elems;
   g.cands.clear();
+  // Split-BF16 plans take the matrix-core variants of a length; a length
+  // without one runs the FP32 VALU kernels (same storage, FP32 arithmetic).
+  const bool split = p->prec == D2DHIP_PREC_SPLIT_BF16;
   if (!std::getenv("D2DHIP_NO_FAST")) {
-    for (const FastVariant &v : fast_registry())
-      if (v.rb == p->rb && v.io == io && v.lines == (axis == 0 ? 1 : 0) && v.n == g.n)
-        g.cands.push_back(&v);
+    for (int pass = 0; pass < 2 && g.cands.empty(); ++pass) {
+      const int want_mf = (split && pass == 0) ? 1 : 0;
+      if (!split && pass == 1) break;
+      for (const FastVariant &v : fast_registry())
+        if (v.rb == p->rb && v.mf == want_mf && v.io == io && v.lines == (axis == 0 ? 1 : 0) &&
+            v.n == g.n)
+          g.cands.push_back(&v);
+    }
   }
   return configure(p, idx, g.cands.empty() ? -1 : 0);
 }
@@ -574,8 +582,8 @@ void tune_store(const std::string &key, const std::string &val) {
 std::string tune_key(const d2dhip_plan_s *p, int idx) {
   const StageGeom &g = p->geom[idx];
   char b[256];
-  std::snprintf(b, sizeof(b), "v1 rb=%d io=%d ax=%d n=%d C0r=%lld C1=%lld s=%lld,%lld,%lld,%lld,%lld,%lld",
-                p->rb, g.io, g.axis, g.n, g.C0r, g.C1, g.in_s0, g.in_s1, g.in_sj, g.out_s0,
+  std::snprintf(b, sizeof(b), "v2 pr=%d io=%d ax=%d n=%d C0r=%lld C1=%lld s=%lld,%lld,%lld,%lld,%lld,%lld",
+                p->prec, g.io, g.axis, g.n, g.C0r, g.C1, g.in_s0, g.in_s1, g.in_sj, g.out_s0,
                 g.out_s1, g.out_sj);
   return b;
 }
@@ -802,7 +810,7 @@ int run_pair_chunked(d2dhip_plan_s *p, long long K, int q1, int i1, int q2, int 
 int tune_chunks(d2dhip_plan_s *p) {
   if (p->format != D2DHIP_PHYSICAL_IN_X || p->nz < 2) return D2DHIP_SUCCESS;
   char kb[512];
-  std::snprintf(kb, sizeof(kb), "v1 chunk rb=%d n=%d,%d,%d k=%s|%s|%s", p->rb, p->nx, p->ny, p->nz,
+  std::snprintf(kb, sizeof(kb), "v2 chunk pr=%d n=%d,%d,%d k=%s|%s|%s", p->prec, p->nx, p->ny, p->nz,
                 choice_name(p, 3).c_str(), choice_name(p, 4).c_str(), choice_name(p, 6).c_str());
   const std::string key(kb);
   auto it = g_tune.find(key);
@@ -888,8 +896,6 @@ int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, in
     return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid value for pencil");
   if (prec != D2DHIP_PREC_F64 && prec != D2DHIP_PREC_F32 && prec != D2DHIP_PREC_SPLIT_BF16)
     return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid precision");
-  if (prec == D2DHIP_PREC_SPLIT_BF16)
-    return set_error(D2DHIP_ERR_UNSUPPORTED, "split-BF16 plans are not available yet");
   d2dhip_plan_s *p = new (std::nothrow) d2dhip_plan_s();
   if (!p) return set_error(D2DHIP_ERR_ALLOC, "host allocation failed");
   p->format = format;
@@ -1155,6 +1161,10 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
 // The fft_multiple_grids use (examples/fft_multiple_grids/fft_multiple_grids.f90):
 // every stage of every grid, in order, through one persistent kernel.
 
+// Work per batched queue item (complex points), enough to hide the
+// dequeue atomic and the per-item barriers.
+constexpr long long kBatchItemPoints = 16384;
+
 struct d2dhip_batch_s {
   int rb = 8;
   hipStream_t stream = nullptr;
@@ -1270,9 +1280,13 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
       G.phase = (int)phase_items.size();
       const AxisDesc &h = b->h_descs[3 * g + ax];
       G.ntiles = (long long)h.tiles0 * h.C1;
+      // about kBatchItemPoints complex points per queue item
+      const long long tile_pts = (long long)h.TX * h.n;
+      G.per_item = (int)std::max<long long>(1, std::min<long long>(64, kBatchItemPoints / tile_pts));
+      const long long nitems = (G.ntiles + G.per_item - 1) / G.per_item;
       G.first = first;
-      first += G.ntiles;
-      items += (unsigned long long)G.ntiles;
+      first += nitems;
+      items += (unsigned long long)nitems;
       started[g] = 1;
       groups.push_back(G);
     }

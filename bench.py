@@ -25,17 +25,20 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))  # oracle/oracle.py (cpu_baseline only)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_FLOPS = 2.5e15  # dense bf16 MFMA peak of MI355X
 
 # HBM bytes per launch of the 1024^3 FP64 stage kernels, from two rocprofv3
 # counter passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, factors calibrated on a
 # device-to-device copy of known size in the same run); the summary with the
 # commands is profiles/hbm_traffic_r2c_c2r_1024cube.md.  Keyed by
 # (n, stage, kernel label); a stage whose tuned kernel differs reports null.
+# The spectral kernel serves both the y and the z stage; its figure is the
+# mean over the launches of both.
 MEASURED_TRAFFIC = {
-    (1024, "r2c", "double n=1024 tx=4 nt=256 r2c"): 17197260846,
-    (1024, "spec_y", "double n=1024 tx=4 nt=256 c2c"): 17491144854,
-    (1024, "spec_last", "double n=1024 tx=8 nt=512 c2c"): 17222730955,
-    (1024, "c2r", "double n=1024 tx=2 nt=128 c2r"): 17203597149,
+    (1024, "r2c", "double n=1024 tx=4 nt=256 r2c"): 17197275332,
+    (1024, "spec_y", "double n=1024 tx=16 nt=1024 c2c xs w4"): 17451485416,
+    (1024, "spec_last", "double n=1024 tx=16 nt=1024 c2c xs w4"): 17451485416,
+    (1024, "c2r", "double n=1024 tx=2 nt=128 c2r"): 17204005649,
 }
 METRIC = "fwd+bwd 3D FFT ms & HBM-roofline % at 1024³ r2c, FP64 vs split-BF16 MFMA"
 
@@ -179,7 +182,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-lines", type=int, default=16384)
+    ap.add_argument("--cpu-lines", type=int, default=524288,
+                    help="lines of each stage in the CPU baseline sample (about 13 s on 16 cores)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan-time autotuner (use the default kernels)")
     ap.add_argument("--workload", choices=["r2c1024", "batch64"], default="r2c1024",
@@ -261,8 +265,35 @@ def main():
                         "speedup_vs_fp64": round(ms / ms32, 3),
                         "kernels": {k: kernels32[k] for k in ("r2c", "spec_y", "spec_last", "c2r")},
                         "chunk_planes": {"fwd": eng32.chunking()[0], "bwd": eng32.chunking()[1]}}
-    variants["split_bf16"] = {"status": "not built yet"}
-    del x32, spec32, y32, eng32
+    del spec32, y32, eng32
+
+    # ---- split-BF16 matrix-core variant (FP32 storage) -------------------------
+    engs = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, prec=d2d.PREC_SPLIT_BF16,
+                         tune=not args.no_tune)
+    kernels_s = engs.stage_kernels()
+    specs = torch.empty(nxc * n * n, dtype=torch.complex64, device="cuda")
+    ys = torch.empty_like(x32)
+    mss, _ = time_round_trips(torch, engs, x32, specs, ys, args.steps, args.warmup, False)
+    errs = float(torch.linalg.vector_norm(specs.to(torch.complex128) - spec64) /
+                 torch.linalg.vector_norm(spec64))
+    maxabs_s = float((specs.to(torch.complex128) - spec64).abs().max())
+    rt_s = float(torch.linalg.vector_norm(ys.double() / n ** 3 - x) / torch.linalg.vector_norm(x))
+    # matrix-core work: two radix-16 passes per 1024-point line, 12 MFMAs
+    # (16x16x32, 16384 flops each) per 16 butterflies = 768 flops per point
+    # and pass; r2c/c2r stages transform n^3/2 complex points, the spectral
+    # stages nxc*n*n each
+    pts = {"r2c": n ** 3 // 2, "spec_y": nxc * n * n, "spec_last": nxc * n * n, "c2r": n ** 3 // 2}
+    mf_flops = sum(2 * 768 * pts[k] * (2 if k in ("spec_y", "spec_last") else 1)
+                   for k in pts if kernels_s[k].startswith("mfma"))
+    variants["split_bf16"] = {
+        "ms": round(mss, 3), "rel_l2_vs_fp64": errs, "max_abs_vs_fp64": maxabs_s,
+        "round_trip_rel_l2": rt_s, "speedup_vs_fp64": round(ms / mss, 3),
+        "GBps_3pass": round(round_trip_bytes(n, 4) / (mss * 1e-3) / 1e9, 1),
+        "mfma_tflops": round(mf_flops / (mss * 1e-3) / 1e12, 1),
+        "mfma_util": round(mf_flops / (mss * 1e-3) / PEAK_BF16_FLOPS, 4),
+        "kernels": {k: kernels_s[k] for k in ("r2c", "spec_y", "spec_last", "c2r")},
+    }
+    del x32, specs, ys, engs
 
     line = {
         "metric": METRIC,

@@ -30,7 +30,10 @@ extern "C" {
 /* arithmetic/storage precision of a plan */
 #define D2DHIP_PREC_F64 64        /* real64 storage and arithmetic (drop-in) */
 #define D2DHIP_PREC_F32 32        /* real32 storage and arithmetic */
-#define D2DHIP_PREC_SPLIT_BF16 16 /* real32 storage, bf16 hi+lo MFMA stages */
+#define D2DHIP_PREC_SPLIT_BF16 16 /* real32 storage; radix-16 passes on the matrix cores
+                                     with operands split into three bf16 parts (axes of
+                                     length 512 and 1024; other lengths run the FP32
+                                     VALU kernels) */
 
 /* plan flags (optional arguments of decomp_2d_fft_init, src/fft_common.f90:141-193) */
 #define D2DHIP_FLAG_INPLACE 0x1u /* opt_inplace: c2c/c2r may overwrite input */

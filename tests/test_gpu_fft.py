@@ -5,6 +5,8 @@ Tolerances (relative L2 over the whole output):
   FP64 against the Glassman restatement of the reference's generic backend:
       2e-13 (the generic backend's own error, see tests/test_oracle.py)
   FP32 against FP64 references: 5e-6
+  split-BF16 (FP32 storage, matrix-core radix-16 passes) against FP64
+      references of the FP32-rounded input: 1e-6
 """
 import numpy as np
 import pytest
@@ -16,6 +18,7 @@ pytestmark = pytest.mark.gpu
 F64_TOL = 1e-13
 GLASSMAN_TOL = 2e-13
 F32_TOL = 5e-6
+SPLIT_TOL = 1e-6  # split-BF16 plans against FP64 references of the FP32-rounded input
 
 
 def to_dev(torch, a, dtype=None):
@@ -304,13 +307,13 @@ STAGE_USE = {"c2c_x": "c2c", "c2c_y": "c2c", "c2c_z": "c2c", "r2c": "r2c", "spec
 
 @pytest.mark.parametrize("shape", THIN)
 @pytest.mark.parametrize("fmt", [1, 3])
-@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("prec", [64, 32, 16])
 def test_every_stage_kernel(d2d, torch_gpu, shape, fmt, prec):
     """Every kernel the planner can pick for a stage (the general kernel and
     each specialised variant) gives the pocketfft result."""
     torch = torch_gpu
     nx, ny, nz = shape
-    tol = F64_TOL if prec == 64 else F32_TOL
+    tol = {64: F64_TOL, 32: F32_TOL, 16: SPLIT_TOL}[prec]
     rng = np.random.default_rng(sum(shape) + fmt + prec)
     x = np.asfortranarray(rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape))
     r = np.asfortranarray(x.real)
@@ -412,3 +415,38 @@ def test_plane_chunking(d2d, torch_gpu, shape, chunks, prec):
             assert rel_l2(from_dev(y1, shape) / x.size, x) <= F64_TOL
     e.set_chunking(0, 0)
     assert e.chunking() == (0, 0)
+
+
+@pytest.mark.parametrize("shape", [(1024, 16, 8), (16, 1024, 8), (16, 8, 1024), (1024, 6, 1024),
+                                   (512, 12, 10), (10, 512, 12), (12, 10, 512)])
+@pytest.mark.parametrize("fmt", [1, 3])
+def test_split_bf16_plan(d2d, torch_gpu, shape, fmt):
+    """Split-BF16 plans run their length-512/1024 axes on the matrix-core kernels
+    and stay within 1e-6 (relative L2) of FP64 pocketfft on the FP32-rounded
+    input, for c2c both ways, r2c and the c2r round trip."""
+    torch = torch_gpu
+    nx, ny, nz = shape
+    rng = np.random.default_rng(sum(shape) + fmt)
+    x = np.asfortranarray(rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(np.complex64)
+    r = np.asfortranarray(x.real)
+    e = d2d.FFTEngine(fmt, nx, ny, nz, prec=d2d.PREC_SPLIT_BF16)
+    kern = e.stage_kernels()
+    axes_len = {"c2c_x": nx, "c2c_y": ny, "c2c_z": nz}
+    for st, n in axes_len.items():
+        assert kern[st].startswith("mfma") == (n in (512, 1024)), (st, kern[st])
+    xin = to_dev(torch, x)
+    out = torch.empty_like(xin)
+    e.fft_c2c(xin, out, -1)
+    want = np.fft.fftn(x.astype(np.complex128))
+    assert rel_l2(from_dev(out, shape), want) <= SPLIT_TOL
+    e.fft_c2c(xin, out, 1)
+    assert rel_l2(from_dev(out, shape), np.fft.ifftn(x.astype(np.complex128)) * x.size) <= SPLIT_TOL
+    axes = (2, 1, 0) if fmt == 1 else (0, 1, 2)
+    ss = spec_shape(fmt, nx, ny, nz)
+    rin = to_dev(torch, r)
+    sp = torch.empty(int(np.prod(ss)), dtype=torch.complex64, device="cuda")
+    e.fft_r2c(rin, sp)
+    assert rel_l2(from_dev(sp, ss), np.fft.rfftn(r.astype(np.float64), axes=axes)) <= SPLIT_TOL
+    back = torch.empty_like(rin)
+    e.fft_c2r(sp, back)
+    assert rel_l2(from_dev(back, shape) / r.size, r) <= SPLIT_TOL
