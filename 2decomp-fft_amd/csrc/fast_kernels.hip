@@ -27,7 +27,6 @@ template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int X
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
     fast_axis_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
   using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
-  static_assert(XM == 0 || IO == IO_C2C, "split exchange is for c2c stages");
   constexpr int P = C::P, RPT = C::RPT, LDT = C::LDT, LDJ = C::LDJ, E = C::emax();
   extern __shared__ __align__(16) unsigned char lds_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
@@ -72,29 +71,51 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
         if (cc + 1 < d->C0r) B[it] = src[base + d->in_s0];
       }
     }
-#pragma unroll
-    for (int it = 0; it < ITER; ++it) {
-      const int w = tid + it * NT;
-      if (w < TX * NH) {
-        const int tt = LINES ? w / NH : w % TX;
-        const int k = LINES ? w % NH : w / TX;
-        cpx<T> a = A[it], b = B[it];
-        if (k == 0 || 2 * k == N) {
-          a.y = T(0);
-          b.y = T(0);
-        }
-        lds[C::row(k) * LDJ + tt * LDT] = cpx<T>{a.x - b.y, -(a.y + b.x)};
-        if (k > 0 && 2 * k < N) lds[C::row(N - k) * LDJ + tt * LDT] = cpx<T>{a.x + b.y, a.y - b.x};
-      }
-    }
-    __syncthreads();
+    // conj(Z) of the merged half spectra into LDS, then the inputs of pass 0
+    // (both halves in turn for the split exchange)
     constexpr int R = RL::r[0], NBR = N / R;
 #pragma unroll
-    for (int q = 0; q < C::nb(0); ++q) {
-      const int b = r0 + q * RPT;
-      if (C::exact(0) || b < NBR) {
+    for (int half = 0; half < (XM ? 2 : 1); ++half) {
+      if (half) __syncthreads();
 #pragma unroll
-        for (int s = 0; s < R; ++s) v[q * R + s] = lds[C::row(b + s * NBR) * LDJ + t * LDT];
+      for (int it = 0; it < ITER; ++it) {
+        const int w = tid + it * NT;
+        if (w < TX * NH) {
+          const int tt = LINES ? w / NH : w % TX;
+          const int k = LINES ? w % NH : w / TX;
+          cpx<T> a = A[it], b = B[it];
+          if (k == 0 || 2 * k == N) {
+            a.y = T(0);
+            b.y = T(0);
+          }
+          const cpx<T> zk = {a.x - b.y, -(a.y + b.x)}, zm = {a.x + b.y, a.y - b.x};
+          const bool mirror = k > 0 && 2 * k < N;
+          if constexpr (XM == 0) {
+            lds[C::row(k) * LDJ + tt * LDT] = zk;
+            if (mirror) lds[C::row(N - k) * LDJ + tt * LDT] = zm;
+          } else {
+            T *ldr = reinterpret_cast<T *>(lds_raw);
+            ldr[C::row(k) * LDJ + tt * LDT] = half ? zk.y : zk.x;
+            if (mirror) ldr[C::row(N - k) * LDJ + tt * LDT] = half ? zm.y : zm.x;
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < C::nb(0); ++q) {
+        const int b = r0 + q * RPT;
+        if (C::exact(0) || b < NBR) {
+#pragma unroll
+          for (int s = 0; s < R; ++s) {
+            const int a = C::row(b + s * NBR) * LDJ + t * LDT;
+            if constexpr (XM == 0) {
+              v[q * R + s] = lds[a];
+            } else {
+              const T w = reinterpret_cast<const T *>(lds_raw)[a];
+              if (half) v[q * R + s].y = w; else v[q * R + s].x = w;
+            }
+          }
+        }
       }
     }
   } else {
@@ -189,31 +210,73 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
   // ---- outputs of the last pass ---------------------------------------------
   constexpr int RZ = RL::r[P - 1], NBZ = N / RZ, LZ = RL::L(P - 1);
   if constexpr (IO == IO_R2C) {
-    if constexpr (P > 1) __syncthreads();
+    // the last pass's outputs Z -> LDS; A[k] = (Z[k] + conj Z[N-k]) / 2 and
+    // B[k] = (Z[k] - conj Z[N-k]) / (2i) for k <= N/2.  Re A and Im B need
+    // only real parts of Z, Im A and Re B only imaginary parts, so the split
+    // exchange builds them in two rounds.
+    constexpr int NH = N / 2 + 1;
+    constexpr int ITER = (TX * NH + NT - 1) / NT;
+    cpx<T> oa[ITER], ob[ITER];
+    const T half_ = T(0.5);
 #pragma unroll
-    for (int q = 0; q < C::nb(P - 1); ++q) {
-      const int b = r0 + q * RPT;
-      if (C::exact(P - 1) || b < NBZ) {
-        const int k = b % LZ, g = b / LZ;
+    for (int half = 0; half < (XM ? 2 : 1); ++half) {
+      if (P > 1 || half) __syncthreads();
 #pragma unroll
-        for (int s = 0; s < RZ; ++s) lds[C::row(g * LZ * RZ + k + s * LZ) * LDJ + t * LDT] = v[q * RZ + s];
+      for (int q = 0; q < C::nb(P - 1); ++q) {
+        const int b = r0 + q * RPT;
+        if (C::exact(P - 1) || b < NBZ) {
+          const int k = b % LZ, g = b / LZ;
+#pragma unroll
+          for (int s = 0; s < RZ; ++s) {
+            const int a = C::row(g * LZ * RZ + k + s * LZ) * LDJ + t * LDT;
+            if constexpr (XM == 0)
+              lds[a] = v[q * RZ + s];
+            else
+              reinterpret_cast<T *>(lds_raw)[a] = half ? v[q * RZ + s].y : v[q * RZ + s].x;
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < ITER; ++it) {
+        const int w = tid + it * NT;
+        if (w < TX * NH) {
+          const int tt = LINES ? w / NH : w % TX;
+          const int k = LINES ? w % NH : w / TX;
+          const int km = (k == 0) ? 0 : N - k;
+          const int a1 = C::row(k) * LDJ + tt * LDT, a2 = C::row(km) * LDJ + tt * LDT;
+          if constexpr (XM == 0) {
+            const cpx<T> zk = lds[a1], zm = lds[a2];
+            oa[it] = cpx<T>{(zk.x + zm.x) * half_, (zk.y - zm.y) * half_};
+            ob[it] = cpx<T>{(zk.y + zm.y) * half_, (zm.x - zk.x) * half_};
+          } else {
+            const T *ldr = reinterpret_cast<const T *>(lds_raw);
+            const T zk = ldr[a1], zm = ldr[a2];
+            if (half == 0) {
+              oa[it].x = (zk + zm) * half_;
+              ob[it].y = (zm - zk) * half_;
+            } else {
+              oa[it].y = (zk - zm) * half_;
+              ob[it].x = (zk + zm) * half_;
+            }
+          }
+        }
       }
     }
-    __syncthreads();
-    constexpr int NH = N / 2 + 1;
     cpx<T> *dst = reinterpret_cast<cpx<T> *>(out);
-    const T half = T(0.5);
-    for (int w = tid; w < TX * NH; w += NT) {
-      const int tt = LINES ? w / NH : w % TX;
-      const int k = LINES ? w % NH : w / TX;
-      const long long cc = 2 * (c0b + tt);
-      if (cc >= d->C0r) continue;
-      const cpx<T> zk = lds[C::row(k) * LDJ + tt * LDT];
-      const int km = (k == 0) ? 0 : N - k;
-      const cpx<T> zm = lds[C::row(km) * LDJ + tt * LDT];
-      const long long base = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
-      dst[base] = cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half};
-      if (cc + 1 < d->C0r) dst[base + d->out_s0] = cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half};
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+      const int w = tid + it * NT;
+      if (w < TX * NH) {
+        const int tt = LINES ? w / NH : w % TX;
+        const int k = LINES ? w % NH : w / TX;
+        const long long cc = 2 * (c0b + tt);
+        if (cc < d->C0r) {
+          const long long base = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
+          dst[base] = oa[it];
+          if (cc + 1 < d->C0r) dst[base + d->out_s0] = ob[it];
+        }
+      }
     }
   } else {
     static_assert(P >= 2, "single-pass lengths go through the general kernel");
@@ -287,6 +350,14 @@ static FastVariant make_variant(const char *name) {
   make_variant<T, N, TX, NT, IO_C2C, LINES, RL, 1, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT \
                                                         " c2c xs w" #WPE)
 
+// c2c, r2c and c2r, split exchange, at least WPE waves per SIMD
+#define D2D_SPLIT_ALL(T, N, TX, NT, LINES, RL, WPE)                                           \
+  D2D_SPLIT_C2C(T, N, TX, NT, LINES, RL, WPE),                                                \
+      make_variant<T, N, TX, NT, IO_R2C, LINES, RL, 1, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT \
+                                                            " r2c xs w" #WPE),               \
+      make_variant<T, N, TX, NT, IO_C2R, LINES, RL, 1, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT \
+                                                            " c2r xs w" #WPE)
+
 using R16_16_4 = RList<16, 16, 4>;
 using R8_8_8 = RList<8, 8, 8>;
 
@@ -307,6 +378,9 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_SPLIT_C2C(double, 1024, 16, 1024, false, R16_16_4, 4),
       D2D_SPLIT_C2C(double, 1024, 4, 256, false, R16_16_4, 3),
       D2D_SPLIT_C2C(double, 1024, 8, 512, false, R16_16_4, 3),
+      D2D_SPLIT_ALL(double, 1024, 2, 128, true, R16_16_4, 3),
+      D2D_SPLIT_ALL(double, 1024, 4, 256, true, R16_16_4, 3),
+      D2D_SPLIT_ALL(double, 1024, 8, 512, true, R16_16_4, 3),
       // FP64, 512
       D2D_VARIANTS(double, 512, 8, 512, false, R8_8_8),
       D2D_VARIANTS(double, 512, 4, 256, false, R8_8_8),
@@ -319,6 +393,7 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_SPLIT_C2C(float, 1024, 8, 512, false, R16_16_4, 6),
       D2D_SPLIT_C2C(float, 1024, 4, 256, false, R16_16_4, 6),
       D2D_SPLIT_C2C(float, 1024, 16, 1024, false, R16_16_4, 8),
+      D2D_SPLIT_ALL(float, 1024, 4, 256, true, R16_16_4, 5),
       D2D_VARIANTS(float, 1024, 8, 512, true, R16_16_4),
       D2D_VARIANTS(float, 1024, 4, 256, true, R16_16_4),
       D2D_VARIANTS(float, 512, 16, 512, false, R8_8_8),

@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(512)
 // ---- many grids in one launch ---------------------------------------------------
 // A persistent kernel pulls work items from a device queue: item i belongs
 // to the BatchGroup g (one stage of one grid) with groups[g].first <= i <
-// groups[g].first + ceil(ntiles / per_item), and covers per_item consecutive
+// groups[g].first + ceil(ntiles / per_item), and covers up to per_item
 // tiles of it, so that the queue, barrier and descriptor costs of an item
 // are spread over enough work.  Groups are laid out phase-major
 // (stage 0 of every grid, then stage 1, ...), and phase p starts only when
@@ -329,10 +329,13 @@ __global__ void __launch_bounds__(512)
     if (!fail) {
       const BatchGroup &g = groups[gi];
       const AxisIo io = {g.in, g.out, g.conj};
-      const long long t0 = uniform_ll((it - g.first) * g.per_item);
-      const long long t1 = uniform_ll(min(t0 + (long long)g.per_item, g.ntiles));
-      for (long long tile = t0; tile < t1; ++tile) {
-        if (tile > t0) __syncthreads();  // LDS reuse across tiles
+      // tiles i, i + ni, i + 2 ni, ... of the group (ni items in it), so
+      // that the items running at one time cover neighbouring tiles
+      const long long nt = uniform_ll(g.ntiles);
+      const long long ni = uniform_ll((nt + g.per_item - 1) / g.per_item);
+      const long long i = uniform_ll(it - g.first);
+      for (long long tile = i; tile < nt; tile += ni) {
+        if (tile > i) __syncthreads();  // LDS reuse across tiles
         axis_tile<T, E, IO_C2C>(g.d, io, tile, lds);
       }
     }

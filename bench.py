@@ -88,6 +88,11 @@ def time_round_trips(torch, eng, x, spec, y, steps, warmup, profile):
     return ms, rep
 
 
+def stage_ms(rep):
+    """{stage: average ms per launch} from a profile report."""
+    return {k: round(tot / cnt, 4) for k, (tot, cnt) in rep.items() if cnt}
+
+
 def cpu_baseline(n, budget_lines):
     import oracle as O
     ncores = min(os.cpu_count() or 1, 16)
@@ -256,13 +261,14 @@ def main():
     x32 = x.float()
     spec32 = torch.empty(nxc * n * n, dtype=torch.complex64, device="cuda")
     y32 = torch.empty_like(x32)
-    ms32, _ = time_round_trips(torch, eng32, x32, spec32, y32, args.steps, args.warmup, False)
+    ms32, rep32 = time_round_trips(torch, eng32, x32, spec32, y32, args.steps, args.warmup, True)
     err32 = float(torch.linalg.vector_norm(spec32.to(torch.complex128) - spec64) /
                   torch.linalg.vector_norm(spec64))
     maxabs32 = float((spec32.to(torch.complex128) - spec64).abs().max())
     variants["fp32"] = {"ms": round(ms32, 3), "rel_l2_vs_fp64": err32, "max_abs_vs_fp64": maxabs32,
                         "GBps_3pass": round(round_trip_bytes(n, 4) / (ms32 * 1e-3) / 1e9, 1),
                         "speedup_vs_fp64": round(ms / ms32, 3),
+                        "stage_ms": stage_ms(rep32),
                         "kernels": {k: kernels32[k] for k in ("r2c", "spec_y", "spec_last", "c2r")},
                         "chunk_planes": {"fwd": eng32.chunking()[0], "bwd": eng32.chunking()[1]}}
     del spec32, y32, eng32
@@ -273,7 +279,7 @@ def main():
     kernels_s = engs.stage_kernels()
     specs = torch.empty(nxc * n * n, dtype=torch.complex64, device="cuda")
     ys = torch.empty_like(x32)
-    mss, _ = time_round_trips(torch, engs, x32, specs, ys, args.steps, args.warmup, False)
+    mss, reps = time_round_trips(torch, engs, x32, specs, ys, args.steps, args.warmup, True)
     errs = float(torch.linalg.vector_norm(specs.to(torch.complex128) - spec64) /
                  torch.linalg.vector_norm(spec64))
     maxabs_s = float((specs.to(torch.complex128) - spec64).abs().max())
@@ -291,6 +297,7 @@ def main():
         "GBps_3pass": round(round_trip_bytes(n, 4) / (mss * 1e-3) / 1e9, 1),
         "mfma_tflops": round(mf_flops / (mss * 1e-3) / 1e12, 1),
         "mfma_util": round(mf_flops / (mss * 1e-3) / PEAK_BF16_FLOPS, 4),
+        "stage_ms": stage_ms(reps),
         "kernels": {k: kernels_s[k] for k in ("r2c", "spec_y", "spec_last", "c2r")},
     }
     del x32, specs, ys, engs
