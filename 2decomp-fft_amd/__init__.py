@@ -26,6 +26,8 @@ BACKWARD = 1
 PREC_F64 = 64
 PREC_F32 = 32
 PREC_SPLIT_BF16 = 16
+BATCH_PERSISTENT = 0
+BATCH_GROUPED = 1
 FLAG_INPLACE = 0x1
 FLAG_SKIP_X = 0x2
 FLAG_SKIP_Y = 0x4
@@ -127,6 +129,7 @@ def lib():
             "d2dhip_batch_graph_launch": ([vp], ci),
             "d2dhip_batch_check": ([vp], ci),
             "d2dhip_batch_set_stream": ([vp, vp], ci),
+            "d2dhip_batch_set_mode": ([vp, ci], ci),
             "d2dhip_batch_destroy": ([vp], ci),
             "d2dhip_pack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
             "d2dhip_unpack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
@@ -338,6 +341,11 @@ class Batch:
     def launch(self):
         self._sync_stream()
         check(lib().d2dhip_batch_launch(self._b))
+
+    def set_mode(self, mode):
+        """BATCH_PERSISTENT (one persistent launch) or BATCH_GROUPED (one launch
+        per stage phase)."""
+        check(lib().d2dhip_batch_set_mode(self._b, int(mode)))
 
     def capture_graph(self):
         check(lib().d2dhip_batch_capture_graph(self._b))

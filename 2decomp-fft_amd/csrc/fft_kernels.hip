@@ -253,10 +253,11 @@ __global__ void __launch_bounds__(512)
 // counts the items it completes in its current phase and publishes the count
 // with one agent-scope release (done[phase] += count) when it dequeues an
 // item of a later phase or finds the queue empty; before its first item of
-// phase p it waits, with one agent-scope acquire, until done[p-1] equals the
-// phase's item count.  Each gfx950 XCD has its own L2, so the release writes
-// back that L2 and the acquire invalidates it; doing both once per phase
-// instead of once per tile keeps the L2 warm within a phase.
+// phase p it polls done[p-1] with relaxed agent-scope loads until it equals
+// the phase's item count, then takes one agent-scope acquire.  Each gfx950
+// XCD has its own L2, so the release writes back that L2 and the acquire
+// invalidates it; doing both once per phase instead of once per tile (and
+// never inside the poll loop) keeps the L2 warm for the blocks still working.
 //
 // Progress: items are dequeued in order, so when a block holds an item of
 // phase p, every item of phase p-1 has been taken by a block that is
@@ -309,7 +310,10 @@ __global__ void __launch_bounds__(512)
         if (ph < nphase && ph > 0) {
           const unsigned long long need = phase_items[ph - 1];
           const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
-          while (__hip_atomic_load(&done[ph - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          // relaxed agent-scope loads see the other XCDs' released counts
+          // without invalidating this XCD's L2 on every poll; the acquire
+          // fence after the barrier below orders the phase's data
+          while (__hip_atomic_load(&done[ph - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
             __builtin_amdgcn_s_sleep(8);
             if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks) {
               atomicOr(err, 1u);
@@ -390,6 +394,56 @@ int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long lon
                        groups, ngroups, total, phase_items, nphase, head, done, err, wait_ticks);
   }
   return hip_check(hipGetLastError(), "batch kernel launch");
+}
+
+// ---- many grids, one launch per phase -------------------------------------------
+// The alternative executor of a batch: per phase, one launch whose
+// workgroups cover every tile of every group of that phase (XCD-aware
+// order); the group of a tile is found by binary search over tile0, and the
+// kernel boundary between phases orders the stages.
+template <typename T, int E>
+__global__ void __launch_bounds__(512)
+    grouped_kernel(const BatchGroup *__restrict__ groups, int g0, int g1, long long tiles) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
+  const long long it = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles);
+  if (it >= tiles) return;  // whole block, before any barrier
+  int lo = g0, hi = g1 - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (groups[mid].tile0 <= it) lo = mid; else hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const BatchGroup &g = groups[lo];
+  const AxisIo io = {g.in, g.out, g.conj};
+  axis_tile<T, E, IO_C2C>(g.d, io, it - g.tile0, lds);
+}
+
+int grouped_prepare(int prec_bytes, size_t lds) {
+  hipError_t e;
+  if (prec_bytes == 8)
+    e = hipFuncSetAttribute((const void *)grouped_kernel<double, kRegCap>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  else
+    e = hipFuncSetAttribute((const void *)grouped_kernel<float, kRegCap>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return hip_check(e, "grouped kernel set-up");
+}
+
+int launch_grouped(int prec_bytes, const BatchGroup *groups, int g0, int g1, long long tiles,
+                   int nthreads, size_t lds, hipStream_t st) {
+  const long long nb = xcd_grid(tiles);
+  if (nb <= 0) return D2DHIP_SUCCESS;
+  const long long gx = nb < (1LL << 20) ? nb : (1LL << 20);
+  const long long gy = (nb + gx - 1) / gx;
+  if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "batch phase has too many tiles");
+  if (prec_bytes == 8)
+    hipLaunchKernelGGL((grouped_kernel<double, kRegCap>), dim3((unsigned)gx, (unsigned)gy),
+                       dim3(nthreads), lds, st, groups, g0, g1, tiles);
+  else
+    hipLaunchKernelGGL((grouped_kernel<float, kRegCap>), dim3((unsigned)gx, (unsigned)gy),
+                       dim3(nthreads), lds, st, groups, g0, g1, tiles);
+  return hip_check(hipGetLastError(), "grouped kernel launch");
 }
 
 // Bluestein's chirp-z algorithm (L. Bluestein, 1970) for a line length nb

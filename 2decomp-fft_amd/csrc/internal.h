@@ -75,6 +75,7 @@ struct BatchGroup {
   int per_item;       // consecutive tiles one queue item covers
   long long ntiles;
   long long first;    // index of its first queue item (items: ceil(ntiles / per_item))
+  long long tile0;    // index of its first tile among all tiles of its phase
 };
 
 // Launch shape chosen by the planner for one axis stage.
@@ -121,6 +122,12 @@ int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long lon
                  const unsigned long long *phase_items, int nphase, unsigned long long *head,
                  unsigned long long *done, unsigned int *err, long long wait_ticks, int blocks,
                  int nthreads, size_t lds, hipStream_t st);
+
+// One ordinary launch over all tiles of batch groups [g0, g1) (one phase):
+// workgroup b runs tile b - tile0 of the group holding it.
+int launch_grouped(int prec_bytes, const BatchGroup *groups, int g0, int g1, long long tiles,
+                   int nthreads, size_t lds, hipStream_t st);
+int grouped_prepare(int prec_bytes, size_t lds);
 
 // copy / pack kernels (copy_kernels.hip)
 int launch_pack(int dir, int split, int esize, const void *src, void *dst,

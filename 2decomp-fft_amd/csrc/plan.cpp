@@ -1174,7 +1174,6 @@ struct d2dhip_batch_s {
   std::vector<d2dhip_plan_s *> plans;
   AxisDesc *d_descs = nullptr;   // 3 general-kernel c2c descriptors per plan (x, y, z)
   std::vector<AxisDesc> h_descs;
-  std::vector<AxisShape> shapes;
   BatchGroup *d_groups = nullptr;
   int ngroups = 0;
   long long total = 0;
@@ -1190,7 +1189,46 @@ struct d2dhip_batch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool bound = false;
+  int mode = D2DHIP_BATCH_PERSISTENT;
+  std::vector<int> ph_g0, ph_g1;       // group range of each phase
+  std::vector<long long> ph_tiles;     // tiles of each phase
 };
+
+namespace {
+
+// Widest power-of-two tile (TX columns) of a general-kernel stage that needs
+// at most nt threads and lds bytes of LDS, capped at the stage's column
+// count: in a batch every workgroup has the batch's block size and LDS, so
+// stages with short lines take more columns per tile instead of idling.
+void reshape_for_batch(AxisDesc &a, bool lines, int cb, int nt, size_t lds) {
+  const int ldt_lines = (a.n % 2 == 0) ? a.n + 1 : a.n;
+  long long c0p2 = 1;
+  while (c0p2 < a.C0 && c0p2 < 1024) c0p2 <<= 1;
+  int best = 0;
+  for (int tx = 1; tx <= c0p2; tx <<= 1) {
+    int need = 64;
+    for (int i = 0; i < a.npass; ++i) {
+      const int r = a.pass[i].r, per = kRegCap / r;
+      need = std::max<long long>(need, ((long long)tx * (a.n / r) + per - 1) / per);
+    }
+    const size_t l = lines ? (size_t)tx * ldt_lines * cb : (size_t)tx * a.n * cb;
+    if (need > nt || l > lds) break;
+    best = tx;
+  }
+  if (best == 0) return;  // not even one column fits: keep the planned tile
+  a.TX = best;
+  a.log2TX = ilog2(best);
+  a.tiles0 = (int)((a.C0 + best - 1) / best);
+  if (lines) {
+    a.ldj = 1;
+    a.ldt = ldt_lines;
+  } else {
+    a.ldj = best;
+    a.ldt = 1;
+  }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1215,15 +1253,38 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
       st = configure(p, a, -1);
       if (!st) {
         b->h_descs.push_back(p->h_desc[a]);
-        b->shapes.push_back(p->shape[a]);
-        b->nthreads = std::max(b->nthreads, p->shape[a].nthreads);
-        b->lds = std::max(b->lds, p->shape[a].lds_bytes);
         if (p->h_desc[a].blue) st = set_error(D2DHIP_ERR_UNSUPPORTED, "Bluestein lengths are not batched");
       }
       if (!st) st = configure(p, a, keep);
     }
   }
   if (!st) {
+    // Block shape of the batch: the general kernel runs at most 8 waves per
+    // CU (its register use), so 2-wave blocks with a quarter of the LDS keep
+    // 4 independent tiles per CU; stages re-tile to fill that block.  A stage
+    // whose single column needs more keeps its planned tile, and the batch
+    // grows to fit it.
+    int nt = 128;
+    size_t lds = 40 * 1024;
+    for (size_t i = 0; i < b->h_descs.size(); ++i) {
+      AxisDesc &a = b->h_descs[i];
+      reshape_for_batch(a, i % 3 == 0, 2 * b->rb, nt, lds);
+    }
+    b->nthreads = nt;
+    b->lds = 0;
+    for (size_t i = 0; i < b->h_descs.size(); ++i) {
+      const AxisDesc &a = b->h_descs[i];
+      const bool lines = (i % 3 == 0);
+      const int ldt_lines = (a.n % 2 == 0) ? a.n + 1 : a.n;
+      const size_t l = lines ? (size_t)a.TX * ldt_lines * 2 * b->rb : (size_t)a.TX * a.n * 2 * b->rb;
+      int need = 64;
+      for (int k = 0; k < a.npass; ++k) {
+        const int r = a.pass[k].r, per = kRegCap / r;
+        need = std::max<long long>(need, ((long long)a.TX * (a.n / r) + per - 1) / per);
+      }
+      b->nthreads = std::max(b->nthreads, (need + 63) / 64 * 64);
+      b->lds = std::max(b->lds, l);
+    }
     const size_t bytes = sizeof(AxisDesc) * b->h_descs.size();
     if (hipMalloc(&b->d_descs, bytes) != hipSuccess) st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch)");
     if (!st) st = hip_check(hipMemcpy(b->d_descs, b->h_descs.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy");
@@ -1233,6 +1294,7 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
   if (!st) {
     int bpc = 0, dev = 0, ncu = 0;
     st = batch_prepare(b->rb, b->nthreads, b->lds, &bpc);
+    if (!st) st = grouped_prepare(b->rb, b->lds);
     if (!st) st = hip_check(hipGetDevice(&dev), "hipGetDevice");
     if (!st) st = hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
     b->blocks = std::max(1, bpc) * std::max(1, ncu);
@@ -1264,8 +1326,12 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   const int np = (int)b->plans.size();
   long long first = 0;
   std::vector<char> started(np, 0);
+  std::vector<int> ph_g0, ph_g1;
+  std::vector<long long> ph_tiles;
   for (int ph = 0; ph < nph; ++ph) {
     unsigned long long items = 0;
+    long long tiles = 0;
+    const int g_begin = (int)groups.size();
     for (int g = 0; g < np; ++g) {
       d2dhip_plan_s *p = b->plans[g];
       const int isign = (mode == 0) ? (ph < 3 ? -1 : 1) : mode;
@@ -1288,12 +1354,19 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
       G.per_item = (int)std::max<long long>(1, std::min<long long>(64, kBatchItemPoints / tile_pts));
       const long long nitems = (G.ntiles + G.per_item - 1) / G.per_item;
       G.first = first;
+      G.tile0 = tiles;
       first += nitems;
+      tiles += G.ntiles;
       items += (unsigned long long)nitems;
       started[g] = 1;
       groups.push_back(G);
     }
-    if (items) phase_items.push_back(items);
+    if (items) {
+      phase_items.push_back(items);
+      ph_g0.push_back(g_begin);
+      ph_g1.push_back((int)groups.size());
+      ph_tiles.push_back(tiles);
+    }
   }
   for (int g = 0; g < np; ++g)
     if (!started[g] && in[g] != out[g])
@@ -1307,6 +1380,9 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   b->ngroups = (int)groups.size();
   b->nphase = (int)phase_items.size();
   b->total = first;
+  b->ph_g0 = ph_g0;
+  b->ph_g1 = ph_g1;
+  b->ph_tiles = ph_tiles;
   b->bound = false;
   if (b->graph_exec) {
     (void)hipGraphExecDestroy(b->graph_exec);
@@ -1339,6 +1415,14 @@ int d2dhip_batch_launch(d2dhip_batch b) {
   if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
   if (!b->bound) return set_error(D2DHIP_ERR_INVALID_VALUE, "batch not bound");
   if (b->ngroups == 0) return D2DHIP_SUCCESS;
+  if (b->mode == D2DHIP_BATCH_GROUPED) {
+    for (int ph = 0; ph < b->nphase; ++ph) {
+      const int st = launch_grouped(b->rb, b->d_groups, b->ph_g0[ph], b->ph_g1[ph], b->ph_tiles[ph],
+                                    b->nthreads, b->lds, b->stream);
+      if (st) return st;
+    }
+    return D2DHIP_SUCCESS;
+  }
   return launch_batch(b->rb, b->d_groups, b->ngroups, b->total, b->d_phase_items, b->nphase,
                       b->d_head, b->d_done, b->d_err, b->wait_ticks, b->blocks, b->nthreads,
                       b->lds, b->stream);
@@ -1389,6 +1473,20 @@ int d2dhip_batch_check(d2dhip_batch b) {
   st = hip_check(hipMemcpy(&err, b->d_err, sizeof(err), hipMemcpyDeviceToHost), "hipMemcpy");
   if (!st && err) st = set_error(D2DHIP_ERR_HIP, "batched phase wait timed out (5 s)");
   return st;
+}
+
+int d2dhip_batch_set_mode(d2dhip_batch b, int mode) {
+  if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
+  if (mode != D2DHIP_BATCH_PERSISTENT && mode != D2DHIP_BATCH_GROUPED)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "batch mode");
+  if (mode != b->mode) {
+    b->mode = mode;
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+  }
+  return D2DHIP_SUCCESS;
 }
 
 int d2dhip_batch_set_stream(d2dhip_batch b, void *stream) {

@@ -159,6 +159,12 @@ def bench_batch(args, torch, d2d):
     batch.capture_graph()
     ms_graph = timed(batch.graph_launch)
     batch.check()
+    # the same batch as one launch per stage phase, captured the same way
+    batch.set_mode(d2d.BATCH_GROUPED)
+    ms_grouped = timed(batch.launch)
+    batch.capture_graph()
+    ms_grouped_graph = timed(batch.graph_launch)
+    batch.check()
     rt = max(float(torch.linalg.vector_norm(o / float(np.prod(s)) - a) / torch.linalg.vector_norm(a))
              for a, o, s in zip(ins, outs, shapes))
     rb = 16 if prec == d2d.PREC_F64 else 8
@@ -174,6 +180,8 @@ def bench_batch(args, torch, d2d):
         "sequential_ms": round(ms_seq, 4), "batched_ms": round(ms_batch, 4),
         "graph_ms": round(ms_graph, 4),
         "speedup_vs_sequential": round(ms_seq / ms_graph, 3),
+        "grouped_ms": round(ms_grouped, 4), "grouped_graph_ms": round(ms_grouped_graph, 4),
+        "grouped_speedup_vs_sequential": round(ms_seq / ms_grouped_graph, 3),
         "GBps_3pass": round(nbytes / (ms_graph * 1e-3) / 1e9, 1),
         "round_trip_rel_l2_max": rt,
     }

@@ -187,6 +187,14 @@ int d2dhip_batch_capture_graph(d2dhip_batch batch);
 int d2dhip_batch_graph_launch(d2dhip_batch batch);
 int d2dhip_batch_check(d2dhip_batch batch);
 int d2dhip_batch_set_stream(d2dhip_batch batch, void *stream);
+/* Executor of a batch: D2DHIP_BATCH_PERSISTENT (default) is the single
+ * persistent launch above; D2DHIP_BATCH_GROUPED runs one ordinary launch per
+ * stage phase covering the tiles of every grid in that phase (the kernel
+ * boundaries order the phases; no device queue).  Results are identical.
+ * Changing the mode drops a captured graph. */
+#define D2DHIP_BATCH_PERSISTENT 0
+#define D2DHIP_BATCH_GROUPED 1
+int d2dhip_batch_set_mode(d2dhip_batch batch, int mode);
 int d2dhip_batch_destroy(d2dhip_batch batch);
 
 /* ---- transposes and alltoallv packing ------------------------------------

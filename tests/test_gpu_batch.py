@@ -1,8 +1,9 @@
 """Many grids in one persistent-kernel launch (the fft_multiple_grids use,
 examples/fft_multiple_grids/fft_multiple_grids.f90), eager and captured in a
-hipGraph.  The batch runs the same general-kernel stages as the per-plan
-path, so its results must equal the sequential decomp_2d_fft_3d calls bit
-for bit, and match pocketfft to the FP64 / FP32 tolerances."""
+hipGraph, and the same batch run as one launch per stage phase.  Both
+executors run the same general-kernel stages as the per-plan path, so their
+results must equal the sequential decomp_2d_fft_3d calls bit for bit, and
+match pocketfft to the FP64 / FP32 tolerances."""
 import numpy as np
 import pytest
 
@@ -22,8 +23,12 @@ def dev(torch, a, dt):
     return torch.from_numpy(np.ascontiguousarray(a.ravel(order="F")).astype(dt)).to("cuda")
 
 
+MODES = [0, 1]  # BATCH_PERSISTENT, BATCH_GROUPED
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("prec", [64, 32])
-def test_batch_matches_sequential_and_pocketfft(d2d, torch_gpu, prec):
+def test_batch_matches_sequential_and_pocketfft(d2d, torch_gpu, prec, mode):
     torch = torch_gpu
     shapes = grids(6, 1, [8, 12, 16, 20, 24, 30, 32]) + [(17, 13, 11), (9, 8, 7)]
     npc = np.complex128 if prec == 64 else np.complex64
@@ -37,6 +42,7 @@ def test_batch_matches_sequential_and_pocketfft(d2d, torch_gpu, prec):
     for e, a, o in zip(engines, ins, seq):
         e.fft_c2c(a, o, -1)
     b = d2d.Batch(engines)
+    b.set_mode(mode)
     b.bind(ins, outs, mode=-1)
     b.launch()
     b.check()
@@ -53,7 +59,8 @@ def test_batch_matches_sequential_and_pocketfft(d2d, torch_gpu, prec):
         assert rel_l2(got, x.astype(npc)) < tol
 
 
-def test_batch_graph_replays(d2d, torch_gpu):
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_graph_replays(d2d, torch_gpu, mode):
     """A captured batch replayed several times, with eager work and
     synchronisation in between, gives the eager result every time."""
     torch = torch_gpu
@@ -64,6 +71,7 @@ def test_batch_graph_replays(d2d, torch_gpu):
     ins = [dev(torch, x, np.complex128) for x in xs]
     outs = [torch.empty_like(a) for a in ins]
     b = d2d.Batch(engines)
+    b.set_mode(mode)
     b.bind(ins, outs, mode=0)
     b.launch()
     b.check()
@@ -108,3 +116,10 @@ def test_config5_64_uneven_grids(d2d, torch_gpu):
         x = ins[g].cpu().numpy().reshape(shapes[g], order="F")
         got = outs[g].cpu().numpy().reshape(shapes[g], order="F")
         assert rel_l2(got, np.fft.fftn(x)) < 1e-13
+    # the per-phase executor gives the same bits
+    persistent = [o.clone() for o in outs]
+    b.set_mode(1)
+    b.launch()
+    b.check()
+    for o, q in zip(outs, persistent):
+        assert torch.equal(o, q)
