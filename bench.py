@@ -35,10 +35,10 @@ PEAK_BF16_FLOPS = 2.5e15  # dense bf16 MFMA peak of MI355X
 # The spectral kernel serves both the y and the z stage; its figure is the
 # mean over the launches of both.
 MEASURED_TRAFFIC = {
-    (1024, "r2c", "double n=1024 tx=4 nt=256 r2c"): 17197275332,
-    (1024, "spec_y", "double n=1024 tx=16 nt=1024 c2c xs w4"): 17451485416,
-    (1024, "spec_last", "double n=1024 tx=16 nt=1024 c2c xs w4"): 17451485416,
-    (1024, "c2r", "double n=1024 tx=2 nt=128 c2r"): 17204005649,
+    (1024, "r2c", "double n=1024 tx=4 nt=256 r2c xs w3"): 17197090128,
+    (1024, "spec_y", "double n=1024 tx=16 nt=1024 c2c xs w4"): 17447469336,
+    (1024, "spec_last", "double n=1024 tx=16 nt=1024 c2c xs w4"): 17447469336,
+    (1024, "c2r", "double n=1024 tx=4 nt=256 c2r xs w3"): 17196989264,
 }
 METRIC = "fwd+bwd 3D FFT ms & HBM-roofline % at 1024³ r2c, FP64 vs split-BF16 MFMA"
 
