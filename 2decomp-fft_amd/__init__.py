@@ -136,6 +136,7 @@ def lib():
             "d2dhip_pack_buffer_elems": ([ci, ctypes.POINTER(Decomp), ci], cll),
             "d2dhip_fill_uniform": ([vp, cll, ci, ctypes.c_ulonglong, vp], ci),
             "d2dhip_host_fill_uniform": ([vp, cll, ci, ctypes.c_ulonglong], None),
+            "d2dhip_column_copy": ([vp, vp, ci, cll, cll, cll, cll, ci, vp], ci),
             "d2dhip_modes_make": ([ctypes.POINTER(Modes), ci, ci, ci, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong], ci),
             "d2dhip_fill_modal": ([vp, ci, ci, ci, ci, ctypes.POINTER(Modes), vp], ci),
@@ -407,6 +408,11 @@ def pack_buffer_elems(direction, decomp, even=False):
 
 
 # ---- synthetic inputs ------------------------------------------------------------
+
+def column_copy(src, dst, elem, C0, C1, s1, sj, n, stream=None):
+    """Copy complex columns with an axis stage's access pattern (bandwidth reference)."""
+    check(lib().d2dhip_column_copy(_ptr(src), _ptr(dst), elem, C0, C1, s1, sj, n, _stream(stream)))
+
 
 def fill_uniform(dst, n, elem, seed, stream=None):
     return check(lib().d2dhip_fill_uniform(_ptr(dst), n, elem, seed, _stream(stream)))

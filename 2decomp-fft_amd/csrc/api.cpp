@@ -236,6 +236,14 @@ int d2dhip_fill_uniform(void *dst, long long n, int elem, unsigned long long see
   return launch_fill_uniform(dst, n, elem, seed, (hipStream_t)stream);
 }
 
+int d2dhip_column_copy(const void *src, void *dst, int elem, long long C0, long long C1,
+                       long long s1, long long sj, int n, void *stream) {
+  if (!src || !dst) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if ((elem != D2DHIP_C16 && elem != D2DHIP_C8) || C0 < 0 || C1 < 0 || n < 0 || s1 < 0 || sj < 0)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "bad column-copy arguments");
+  return launch_column_copy(elem, src, dst, C0, C1, s1, sj, n, (hipStream_t)stream);
+}
+
 void d2dhip_host_fill_uniform(void *dst, long long n, int elem, unsigned long long seed) {
   const bool cplx = (elem == D2DHIP_C16 || elem == D2DHIP_C8);
   const long long nv = cplx ? 2 * n : n;

@@ -15,6 +15,7 @@
 // read and written in consecutive addresses.
 #include <hip/hip_runtime.h>
 
+#include "codelets.h"
 #include "gen.h"
 #include "internal.h"
 
@@ -159,6 +160,52 @@ __global__ void __launch_bounds__(256) fill_uniform_kernel(T *dst, long long nv,
                                                            unsigned long long seed) {
   for (long long i = blockIdx.x * 256LL + threadIdx.x; i < nv; i += (long long)gridDim.x * 256)
     dst[i] = (T)gen_uniform_pm1(seed, (unsigned long long)i);
+}
+
+// Copy of complex columns with the access pattern of an axis stage and no
+// arithmetic: tiles of 16 columns (c0, c1) of n points sj apart, 1024-thread
+// blocks in XCD-aware order; each thread loads its n/64 points, then stores
+// them.  It measures the bandwidth that pattern allows.
+template <typename T>
+__global__ void __launch_bounds__(1024)
+    column_copy_kernel(const cpx<T> *__restrict__ src, cpx<T> *__restrict__ dst, long long C0,
+                       long long C1, long long s1, long long sj, int n) {
+  constexpr int TX = 16, RPT = 64, NP = 16;
+  const long long tiles0 = (C0 + TX - 1) / TX, ntiles = tiles0 * C1;
+  const long long tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  const long long c1 = tile / tiles0;
+  const long long c0 = (tile - c1 * tiles0) * TX + threadIdx.x % TX;
+  if (c0 >= C0) return;
+  const long long base = c0 + c1 * s1;
+  for (int j0 = (int)(threadIdx.x / TX); j0 < n; j0 += RPT * NP) {
+    cpx<T> v[NP];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {  // all loads in flight before the stores
+      const int j = j0 + m * RPT;
+      v[m] = (j < n) ? src[base + (long long)j * sj] : cpx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+      const int j = j0 + m * RPT;
+      if (j < n) dst[base + (long long)j * sj] = v[m];
+    }
+  }
+}
+
+int launch_column_copy(int elem, const void *src, void *dst, long long C0, long long C1,
+                       long long s1, long long sj, int n, hipStream_t st) {
+  const long long ntiles = (C0 + 15) / 16 * C1;
+  const long long nb = xcd_grid(ntiles);
+  if (nb <= 0) return D2DHIP_SUCCESS;
+  if (nb > 0x7fffffffLL) return set_error(D2DHIP_ERR_UNSUPPORTED, "too many columns");
+  if (elem == D2DHIP_C16)
+    hipLaunchKernelGGL(column_copy_kernel<double>, dim3((unsigned)nb), dim3(1024), 0, st,
+                       (const cpx<double> *)src, (cpx<double> *)dst, C0, C1, s1, sj, n);
+  else
+    hipLaunchKernelGGL(column_copy_kernel<float>, dim3((unsigned)nb), dim3(1024), 0, st,
+                       (const cpx<float> *)src, (cpx<float> *)dst, C0, C1, s1, sj, n);
+  return hip_check(hipGetLastError(), "column_copy_kernel launch");
 }
 
 int launch_fill_uniform(void *dst, long long n, int elem, unsigned long long seed,

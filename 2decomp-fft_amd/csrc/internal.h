@@ -134,6 +134,8 @@ int launch_pack(int dir, int split, int esize, const void *src, void *dst,
                 const d2dhip_decomp *d, int even, hipStream_t st);
 int launch_fill_uniform(void *dst, long long n, int elem, unsigned long long seed,
                         hipStream_t st);
+int launch_column_copy(int elem, const void *src, void *dst, long long C0, long long C1,
+                       long long s1, long long sj, int n, hipStream_t st);
 int launch_fill_modal(void *dst, int nx, int ny, int nz, int elem,
                       const d2dhip_modes *m, hipStream_t st);
 

@@ -259,6 +259,18 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     copy_gbs = 2 * spec.numel() * 16 / (e0.elapsed_time(e1) / 5 * 1e-3) / 1e9
+    # the same bytes moved with the spectral stages' access pattern and no
+    # arithmetic (16-column tiles of 1024 points at the z / y strides): the
+    # bandwidth a strided stage can reach on this device
+    pattern_gbs = {}
+    for name, geo in (("spec_last", (nxc * n, 1, 0, nxc * n)), ("spec_y", (nxc, n, nxc * n, nxc))):
+        d2d.column_copy(spec, cpy, d2d.C16, *geo, n)
+        e0.record()
+        for _ in range(5):
+            d2d.column_copy(spec, cpy, d2d.C16, *geo, n)
+        e1.record()
+        torch.cuda.synchronize()
+        pattern_gbs[name] = round(2 * spec.numel() * 16 / (e0.elapsed_time(e1) / 5 * 1e-3) / 1e9, 1)
     del cpy
 
     # ---- FP32 VALU variant ---------------------------------------------------
@@ -328,6 +340,7 @@ def main():
         "GBps_3pass": round(gbs, 1),
         "hbm_frac_3pass": round(gbs / PEAK_HBM_GBS, 4),
         "copy_GBps": round(copy_gbs, 1),
+        "pattern_copy_GBps": pattern_gbs,
         "round_trip_rel_l2": rt_err,
         "stages": stages,
         "kernels": {k: v for k, v in kernels64.items() if k in stages},
@@ -336,6 +349,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "%s (%s stage)" % (kernels64.get(dom), dom),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "frac_of_pattern_copy": (round(achieved / pattern_gbs[dom], 4)
+                                              if dom in pattern_gbs else None),
                      "algorithmic_bytes": sb[dom]},
         "variants": variants,
     }

@@ -227,6 +227,13 @@ int d2dhip_fill_uniform(void *dst, long long n, int elem,
                         unsigned long long seed, void *stream);
 void d2dhip_host_fill_uniform(void *dst, long long n, int elem,
                               unsigned long long seed);
+/* Bandwidth reference (no reference counterpart; used by the benchmark):
+ * copies the n points (sj elements apart) of the complex columns
+ * c0 + c1*s1, c0 < C0, c1 < C1, from src to the same positions of dst, in
+ * the tile order and access pattern of a strided axis stage (16 columns per
+ * 1024-thread block) but without arithmetic.  elem D2DHIP_C16 or _C8. */
+int d2dhip_column_copy(const void *src, void *dst, int elem, long long C0, long long C1,
+                       long long s1, long long sj, int n, void *stream);
 /* Modal field sum_m a_m cos(2 pi (kx i/nx + ky j/ny + kz k/nz)) + sigma*xi,
  * i,j,k 0-based, with nmodes integer wavevectors uniform in [-kmax,kmax]^3;
  * amp_kind 0: a_m ~ N(0,1); amp_kind 1: |a_m| = 10^U(amp_lo,amp_hi) with a

@@ -105,3 +105,25 @@ def test_config3b_pack_2048_cube_4x4(d2d, torch_gpu, oracle):
     torch_gpu.cuda.synchronize()
     yh = x.cpu().numpy().reshape((512, 2048, 512), order="F")
     assert np.array_equal(send2.cpu().numpy(), oracle.mem_split(oracle.YZ, yh, do))
+
+
+@pytest.mark.parametrize("geo", [(37, 3, 37 * 5, 37, 5), (100, 1, 0, 100, 1030), (16, 4, 16 * 70, 16, 70)])
+@pytest.mark.parametrize("elem", ["C16", "C8"])
+def test_column_copy_pattern(d2d, torch_gpu, geo, elem):
+    """The bandwidth reference copies exactly the columns it is given (bit
+    for bit) and touches nothing else, for partial 16-column tiles and
+    columns shorter and longer than one thread's share."""
+    torch = torch_gpu
+    C0, C1, s1, sj, n = geo
+    total = C0 - 1 + (C1 - 1) * s1 + (n - 1) * sj + 1
+    cdt = torch.complex128 if elem == "C16" else torch.complex64
+    src = torch.randn(total + 7, dtype=cdt, device="cuda")
+    dst = torch.full_like(src, complex(-7.0, 3.0))
+    d2d.column_copy(src, dst, getattr(d2d, elem), C0, C1, s1, sj, n)
+    torch.cuda.synchronize()
+    idx = np.array([c0 + c1 * s1 + j * sj for c0 in range(C0) for c1 in range(C1) for j in range(n)])
+    mask = np.zeros(total + 7, dtype=bool)
+    mask[idx] = True
+    s, d = src.cpu().numpy(), dst.cpu().numpy()
+    assert np.array_equal(d[mask], s[mask])
+    assert np.all(d[~mask] == complex(-7.0, 3.0))
