@@ -360,6 +360,9 @@ static FastVariant make_variant(const char *name) {
 
 using R16_16_4 = RList<16, 16, 4>;
 using R8_8_8 = RList<8, 8, 8>;
+using R16_9_5 = RList<16, 9, 5>;     // 720
+using R4_9_3_5 = RList<4, 9, 3, 5>;  // 540
+using R16_9_3 = RList<16, 9, 3>;     // 432
 
 const std::vector<FastVariant> &fast_registry() {
   static const std::vector<FastVariant> reg = [] {
@@ -398,6 +401,16 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(float, 1024, 4, 256, true, R16_16_4),
       D2D_VARIANTS(float, 512, 16, 512, false, R8_8_8),
       D2D_VARIANTS(float, 512, 8, 256, true, R8_8_8),
+      // FP64, the 720 x 540 x 432 grid's lengths (lines of 720, columns of 540 and 432)
+      D2D_VARIANTS(double, 720, 4, 256, true, R16_9_5),
+      D2D_VARIANTS(double, 720, 2, 128, true, R16_9_5),
+      D2D_VARIANTS(double, 720, 8, 256, false, R16_9_5),
+      D2D_VARIANTS(double, 540, 8, 512, false, R4_9_3_5),
+      D2D_VARIANTS(double, 540, 4, 256, false, R4_9_3_5),
+      D2D_VARIANTS(double, 540, 4, 256, true, R4_9_3_5),
+      D2D_VARIANTS(double, 432, 8, 512, false, R16_9_3),
+      D2D_VARIANTS(double, 432, 4, 256, false, R16_9_3),
+      D2D_VARIANTS(double, 432, 4, 256, true, R16_9_3),
     };
     for (const FastVariant &v : mf_variants()) r.push_back(v);
     return r;

@@ -188,6 +188,89 @@ def bench_batch(args, torch, d2d):
     print(json.dumps(line), flush=True)
 
 
+def bench_config(args, torch, d2d):
+    """One of the other configurations, timed like the headline (FP64, tuned
+    plans, inputs in HBM): c2c 128^3 forward+backward (PHYSICAL_IN_X, uniform
+    inputs seed 0); r2c+c2r 512^3 PHYSICAL_IN_X (modal field seed 1); r2c+c2r
+    720x540x432 PHYSICAL_IN_Z (uniform seed 2); the mem_split_xy pack of one
+    rank of 2048^3 on a 4x4 grid (x-pencil 2048x512x512 real8, seed 2)."""
+    w = args.workload
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    extra = {}
+    if w == "c2c128":
+        n = 128
+        x = torch.empty(n ** 3, dtype=torch.complex128, device="cuda")
+        d2d.fill_uniform(x, x.numel(), d2d.C16, 0)
+        y = torch.empty_like(x)
+        e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, tune=not args.no_tune)
+
+        def step():
+            e.fft_c2c(x, y, -1)
+            e.fft_c2c(y, y, 1)
+        ms = timed(step)
+        nbytes = 2 * 6 * n ** 3 * 16
+        err = float(torch.linalg.vector_norm(y / n ** 3 - x) / torch.linalg.vector_norm(x))
+        cfg = {"workload": "c2c FP64 128^3 fwd+bwd PHYSICAL_IN_X, uniform seed 0"}
+        extra["kernels"] = e.stage_kernels()
+    elif w in ("r2c512", "r2cz720"):
+        fmt = d2d.PHYSICAL_IN_X if w == "r2c512" else d2d.PHYSICAL_IN_Z
+        nx, ny, nz = (512, 512, 512) if w == "r2c512" else (720, 540, 432)
+        x = torch.empty(nx * ny * nz, dtype=torch.float64, device="cuda")
+        if w == "r2c512":
+            d2d.fill_modal(x, nx, ny, nz, d2d.R8, d2d.modes_make(32, 64, 0, 0.0, 0.0, 1e-3, 1))
+        else:
+            d2d.fill_uniform(x, x.numel(), d2d.R8, 2)
+        ns = (nx // 2 + 1) * ny * nz if fmt == d2d.PHYSICAL_IN_X else nx * ny * (nz // 2 + 1)
+        sp = torch.empty(ns, dtype=torch.complex128, device="cuda")
+        y = torch.empty_like(x)
+        e = d2d.FFTEngine(fmt, nx, ny, nz, tune=not args.no_tune)
+
+        def step():
+            e.fft_r2c(x, sp)
+            e.fft_c2r(sp, y)
+        ms = timed(step)
+        s_real, s_spec = nx * ny * nz * 8, ns * 16
+        nbytes = 2 * (s_real + 5 * s_spec)
+        err = float(torch.linalg.vector_norm(y / (nx * ny * nz) - x) / torch.linalg.vector_norm(x))
+        cfg = {"workload": "r2c+c2r FP64 %dx%dx%d %s" % (nx, ny, nz, "PHYSICAL_IN_X" if
+                                                         fmt == d2d.PHYSICAL_IN_X else "PHYSICAL_IN_Z")}
+        extra["kernels"] = e.stage_kernels()
+    else:  # pack2048
+        dec = d2d.decomp_info_init(2048, 2048, 2048, 4, 4, 0, 0)
+        xs = dec.xsz
+        x = torch.empty(xs[0] * xs[1] * xs[2], dtype=torch.float64, device="cuda")
+        d2d.fill_uniform(x, x.numel(), d2d.R8, 2)
+        buf = torch.empty(d2d.pack_buffer_elems(0, dec, 0), dtype=torch.float64, device="cuda")
+
+        def step():
+            d2d.pack(0, d2d.R8, x, buf, dec, 0)
+        ms = timed(step)
+        nbytes = 2 * x.numel() * 8
+        err = 0.0
+        cfg = {"workload": "mem_split_xy pack, one rank of 2048^3 on 4x4 (2048x512x512 real8)"}
+    line = {"metric": "ms per step (%s)" % w, "value": round(ms, 4), "unit": "ms", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": cfg,
+            "GBps_model": round(nbytes / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac_model": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "round_trip_rel_l2": err}
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,8 +282,11 @@ def main():
                     help="lines of each stage in the CPU baseline sample (about 13 s on 16 cores)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan-time autotuner (use the default kernels)")
-    ap.add_argument("--workload", choices=["r2c1024", "batch64"], default="r2c1024",
-                    help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids")
+    ap.add_argument("--workload", choices=["r2c1024", "batch64", "c2c128", "r2c512", "r2cz720",
+                                           "pack2048"], default="r2c1024",
+                    help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids; "
+                         "c2c128 / r2c512 / r2cz720 / pack2048: the other parity configurations, "
+                         "timed")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -214,6 +300,9 @@ def main():
     d2d = importlib.import_module("2decomp-fft_amd")
     if args.workload == "batch64":
         bench_batch(args, torch, d2d)
+        return
+    if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048"):
+        bench_config(args, torch, d2d)
         return
     n = args.n
     nxc = n // 2 + 1

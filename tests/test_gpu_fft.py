@@ -300,7 +300,8 @@ def test_config3a_r2c_z_720_540_432(d2d, torch_gpu):
 
 
 THIN = [(1024, 6, 5), (6, 1024, 5), (6, 5, 1024), (512, 6, 5), (6, 512, 5), (6, 5, 512),
-        (1024, 5, 3), (5, 3, 1024), (512, 3, 5)]
+        (1024, 5, 3), (5, 3, 1024), (512, 3, 5),
+        (720, 5, 3), (3, 720, 5), (540, 3, 5), (5, 540, 3), (432, 5, 3), (5, 3, 432)]
 STAGE_USE = {"c2c_x": "c2c", "c2c_y": "c2c", "c2c_z": "c2c", "r2c": "r2c", "spec_y": "rt",
              "spec_last": "rt", "c2r": "c2r"}
 
