@@ -37,34 +37,68 @@ struct alignas(16) E16 {
   unsigned long long a, b;
 };
 
-template <typename E>
+// One unit = a chunk of up to 1024 consecutive i of one pencil row (j, k);
+// each thread moves 4 elements of it, 256 apart, loads first.  AX is the
+// block axis: for j or k the destination block is fixed per row, for i it is
+// found per element.
+template <typename E, int AX>
 __global__ void __launch_bounds__(256) pack_kernel(PackArgs a, long long units, int cpr) {
+  constexpr int V = 4, CH = 256 * V;
   const E *src = reinterpret_cast<const E *>(a.src);
   E *dst = reinterpret_cast<E *>(a.dst);
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
     const long long row = u / cpr;
     const int chunk = (int)(u - row * cpr);
-    const int i = chunk * 256 + (int)threadIdx.x;
-    if (i >= a.n1) continue;
     const int j = (int)(row % a.n2);
     const int k = (int)(row / a.n2);
-    const int key = a.ax == 0 ? i : (a.ax == 1 ? j : k);
-    int m = 0;
-    while (m + 1 < a.np && key >= a.b1[m + 1]) ++m;
-    const int b1 = a.b1[m], w = a.w[m];
-    long long pos = a.init[m];
-    if (a.ax == 0)
-      pos += (long long)(i - b1) + (long long)j * w + (long long)k * a.n2 * w;
-    else if (a.ax == 1)
-      pos += (long long)i + (long long)(j - b1) * a.n1 + (long long)k * w * a.n1;
-    else
-      pos += (long long)i + (long long)j * a.n1 + (long long)(k - b1) * a.n1 * a.n2;
-    const long long lin = (long long)i + (long long)a.n1 * row;
-    if (a.split)
-      dst[pos] = src[lin];
-    else
-      dst[lin] = src[pos];
+    long long rbase = 0;  // row part of the buffer position (blocks along j or k)
+    if constexpr (AX != 0) {
+      const int key = AX == 1 ? j : k;
+      int m = 0;
+      while (m + 1 < a.np && key >= a.b1[m + 1]) ++m;
+      rbase = a.init[m] + (AX == 1 ? (long long)(j - a.b1[m]) * a.n1 + (long long)k * a.w[m] * a.n1
+                                   : (long long)j * a.n1 + (long long)(k - a.b1[m]) * a.n1 * a.n2);
+    }
+    const long long lin0 = (long long)a.n1 * row;
+    long long pos[V];
+    E v[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const int i = chunk * CH + e * 256 + (int)threadIdx.x;
+      pos[e] = -1;
+      if (i < a.n1) {
+        if constexpr (AX == 0) {
+          int m = 0;
+          while (m + 1 < a.np && i >= a.b1[m + 1]) ++m;
+          pos[e] = a.init[m] + (long long)(i - a.b1[m]) + (long long)j * a.w[m] +
+                   (long long)k * a.n2 * a.w[m];
+        } else {
+          pos[e] = rbase + i;
+        }
+        v[e] = a.split ? src[lin0 + i] : src[pos[e]];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const int i = chunk * CH + e * 256 + (int)threadIdx.x;
+      if (pos[e] >= 0) {
+        if (a.split)
+          dst[pos[e]] = v[e];
+        else
+          dst[lin0 + i] = v[e];
+      }
+    }
   }
+}
+
+template <typename E>
+static void launch_pack_e(const PackArgs &a, long long units, int cpr, long long grid, hipStream_t st) {
+  if (a.ax == 0)
+    hipLaunchKernelGGL((pack_kernel<E, 0>), dim3((unsigned)grid), dim3(256), 0, st, a, units, cpr);
+  else if (a.ax == 1)
+    hipLaunchKernelGGL((pack_kernel<E, 1>), dim3((unsigned)grid), dim3(256), 0, st, a, units, cpr);
+  else
+    hipLaunchKernelGGL((pack_kernel<E, 2>), dim3((unsigned)grid), dim3(256), 0, st, a, units, cpr);
 }
 
 int launch_pack(int dir, int split, int esize, const void *src, void *dst, const d2dhip_decomp *d,
@@ -134,21 +168,13 @@ int launch_pack(int dir, int split, int esize, const void *src, void *dst, const
   if (wsum != axlen)
     return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition does not match the pencil");
   if ((long long)a.n1 * a.n2 * a.n3 == 0) return D2DHIP_SUCCESS;
-  const int cpr = (a.n1 + 255) / 256;
+  const int cpr = (a.n1 + 1023) / 1024;
   const long long units = (long long)a.n2 * a.n3 * cpr;
   const long long grid = units < (1LL << 20) ? units : (1LL << 20);
   switch (esize) {
-    case 4:
-      hipLaunchKernelGGL(pack_kernel<unsigned int>, dim3((unsigned)grid), dim3(256), 0, st, a,
-                         units, cpr);
-      break;
-    case 8:
-      hipLaunchKernelGGL(pack_kernel<unsigned long long>, dim3((unsigned)grid), dim3(256), 0, st,
-                         a, units, cpr);
-      break;
-    default:
-      hipLaunchKernelGGL(pack_kernel<E16>, dim3((unsigned)grid), dim3(256), 0, st, a, units, cpr);
-      break;
+    case 4: launch_pack_e<unsigned int>(a, units, cpr, grid, st); break;
+    case 8: launch_pack_e<unsigned long long>(a, units, cpr, grid, st); break;
+    default: launch_pack_e<E16>(a, units, cpr, grid, st); break;
   }
   return hip_check(hipGetLastError(), "pack_kernel launch");
 }
