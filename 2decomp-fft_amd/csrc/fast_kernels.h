@@ -21,6 +21,9 @@ struct FastVariant {
   std::vector<int> radices;
   int (*launch)(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, hipStream_t st) = nullptr;
   hipError_t (*prepare)() = nullptr;
+  // the same kernel over the tiles of several grids (MultiGroup list)
+  int (*launch_multi)(const MultiGroup *g, int ng, long long tiles, hipStream_t st) = nullptr;
+  hipError_t (*prepare_multi)() = nullptr;
   const char *name = "";
 };
 

@@ -23,18 +23,16 @@ namespace d2dhip {
 
 // WPE: minimum waves per SIMD the register allocation must allow (1 = no
 // constraint beyond the block size).
-template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
-    fast_axis_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
+// One tile of a stage (tile < tiles0 * C1), by the whole workgroup.
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM>
+__device__ __forceinline__ void fast_tile(const AxisDesc *__restrict__ d, const void *in, void *out,
+                                          int conj, long long tile) {
   using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
   constexpr int P = C::P, RPT = C::RPT, LDT = C::LDT, LDJ = C::LDJ, E = C::emax();
   extern __shared__ __align__(16) unsigned char lds_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
 
   const long long tiles0 = d->tiles0;
-  const long long tile =
-      xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles0 * d->C1);
-  if (tile >= tiles0 * d->C1) return;  // whole block, before any barrier
   const long long c1 = tile / tiles0;
   const long long c0b = (tile - c1 * tiles0) * TX;
   const int tid = threadIdx.x;
@@ -293,6 +291,26 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
   }
 }
 
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+    fast_axis_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
+  const long long ntiles = (long long)d->tiles0 * d->C1;
+  const long long tile = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, ntiles);
+  if (tile >= ntiles) return;  // whole block, before any barrier
+  fast_tile<T, N, TX, NT, IO, LINES, RL, XM>(d, in, out, conj, tile);
+}
+
+// The same stage kernel over the tiles of several grids.
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+    fast_multi_kernel(const MultiGroup *__restrict__ g, int ng, long long tiles) {
+  const long long it = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles);
+  if (it >= tiles) return;  // whole block, before any barrier
+  const int gi = find_multi_group(g, ng, it);
+  fast_tile<T, N, TX, NT, IO, LINES, RL, XM>(g[gi].d, g[gi].in, g[gi].out, g[gi].conj,
+                                             it - g[gi].tile0);
+}
+
 // ---- registry -----------------------------------------------------------------
 
 template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
@@ -312,6 +330,26 @@ template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int X
 static hipError_t prepare_fast() {
   using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
   return hipFuncSetAttribute((const void *)fast_axis_kernel<T, N, TX, NT, IO, LINES, RL, XM, WPE>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
+}
+
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+static int launch_fast_multi(const MultiGroup *g, int ng, long long tiles, hipStream_t st) {
+  using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
+  const long long nb = xcd_grid(tiles);
+  if (nb <= 0) return D2DHIP_SUCCESS;
+  const long long gx = nb < (1LL << 20) ? nb : (1LL << 20);
+  const long long gy = (nb + gx - 1) / gx;
+  if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "launch has too many tiles");
+  hipLaunchKernelGGL((fast_multi_kernel<T, N, TX, NT, IO, LINES, RL, XM, WPE>),
+                     dim3((unsigned)gx, (unsigned)gy), dim3(NT), C::lds_bytes(), st, g, ng, tiles);
+  return hip_check(hipGetLastError(), "fast_multi_kernel launch");
+}
+
+template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+static hipError_t prepare_fast_multi() {
+  using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
+  return hipFuncSetAttribute((const void *)fast_multi_kernel<T, N, TX, NT, IO, LINES, RL, XM, WPE>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
 }
 
@@ -336,6 +374,8 @@ static FastVariant make_variant(const char *name) {
   v.radices = rvec(RL{});
   v.launch = &launch_fast<T, N, TX, NT, IO, LINES, RL, XM, WPE>;
   v.prepare = &prepare_fast<T, N, TX, NT, IO, LINES, RL, XM, WPE>;
+  v.launch_multi = &launch_fast_multi<T, N, TX, NT, IO, LINES, RL, XM, WPE>;
+  v.prepare_multi = &prepare_fast_multi<T, N, TX, NT, IO, LINES, RL, XM, WPE>;
   v.name = name;
   return v;
 }
@@ -363,6 +403,18 @@ using R8_8_8 = RList<8, 8, 8>;
 using R16_9_5 = RList<16, 9, 5>;     // 720
 using R4_9_3_5 = RList<4, 9, 3, 5>;  // 540
 using R16_9_3 = RList<16, 9, 3>;     // 432
+using R16_16 = RList<16, 16>;         // 256
+using R16_8 = RList<16, 8>;           // 128
+using R8_8 = RList<8, 8>;             // 64
+using R8_4 = RList<8, 4>;             // 32
+// Lengths 2^a 3^b 5^c of the uneven-grid batches: radices chosen so that
+// every pass has a whole number of butterflies per row of the tile
+// (N / r divisible by NT / TX), i.e. no idle lanes, at 12-24 values per lane.
+using R12_4 = RList<12, 4>;           // 48, 4 rows per column
+using R4_4_5 = RList<4, 4, 5>;        // 80, 4 rows
+using R12_8 = RList<12, 8>;           // 96, 4 rows
+using R4_4_10 = RList<4, 4, 10>;      // 160, 8 rows
+using R4_4_12 = RList<4, 4, 12>;      // 192, 16 rows (or 8)
 
 const std::vector<FastVariant> &fast_registry() {
   static const std::vector<FastVariant> reg = [] {
@@ -411,6 +463,48 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(double, 432, 8, 512, false, R16_9_3),
       D2D_VARIANTS(double, 432, 4, 256, false, R16_9_3),
       D2D_VARIANTS(double, 432, 4, 256, true, R16_9_3),
+      // short power-of-two lengths: 128-byte rows on strided axes
+      D2D_VARIANTS(double, 256, 8, 128, false, R16_16),
+      D2D_VARIANTS(double, 256, 4, 64, true, R16_16),
+      D2D_VARIANTS(double, 256, 8, 128, true, R16_16),
+      D2D_VARIANTS(double, 128, 8, 64, false, R16_8),
+      D2D_VARIANTS(double, 128, 16, 128, false, R16_8),
+      D2D_VARIANTS(double, 128, 8, 64, true, R16_8),
+      D2D_VARIANTS(double, 64, 8, 64, false, R8_8),
+      D2D_VARIANTS(double, 64, 16, 128, true, R8_8),
+      D2D_VARIANTS(double, 32, 16, 64, false, R8_4),
+      D2D_VARIANTS(double, 32, 32, 128, true, R8_4),
+      D2D_VARIANTS(double, 48, 16, 64, false, R12_4),
+      D2D_VARIANTS(double, 48, 16, 64, true, R12_4),
+      D2D_VARIANTS(double, 80, 16, 64, false, R4_4_5),
+      D2D_VARIANTS(double, 80, 16, 64, true, R4_4_5),
+      D2D_VARIANTS(double, 96, 16, 64, false, R12_8),
+      D2D_VARIANTS(double, 96, 16, 64, true, R12_8),
+      D2D_VARIANTS(double, 160, 8, 64, false, R4_4_10),
+      D2D_VARIANTS(double, 160, 16, 128, false, R4_4_10),
+      D2D_VARIANTS(double, 160, 8, 64, true, R4_4_10),
+      D2D_VARIANTS(double, 192, 8, 128, false, R4_4_12),
+      D2D_VARIANTS(double, 192, 8, 64, false, R4_4_12),
+      D2D_VARIANTS(double, 192, 4, 64, true, R4_4_12),
+      D2D_VARIANTS(double, 192, 8, 64, true, R4_4_12),
+      D2D_VARIANTS(float, 256, 16, 256, false, R16_16),
+      D2D_VARIANTS(float, 256, 4, 64, true, R16_16),
+      D2D_VARIANTS(float, 128, 16, 128, false, R16_8),
+      D2D_VARIANTS(float, 128, 8, 64, true, R16_8),
+      D2D_VARIANTS(float, 64, 16, 128, false, R8_8),
+      D2D_VARIANTS(float, 64, 16, 128, true, R8_8),
+      D2D_VARIANTS(float, 32, 16, 64, false, R8_4),
+      D2D_VARIANTS(float, 32, 32, 128, true, R8_4),
+      D2D_VARIANTS(float, 48, 16, 64, false, R12_4),
+      D2D_VARIANTS(float, 48, 16, 64, true, R12_4),
+      D2D_VARIANTS(float, 80, 16, 64, false, R4_4_5),
+      D2D_VARIANTS(float, 80, 16, 64, true, R4_4_5),
+      D2D_VARIANTS(float, 96, 16, 64, false, R12_8),
+      D2D_VARIANTS(float, 96, 16, 64, true, R12_8),
+      D2D_VARIANTS(float, 160, 16, 128, false, R4_4_10),
+      D2D_VARIANTS(float, 160, 8, 64, true, R4_4_10),
+      D2D_VARIANTS(float, 192, 8, 128, false, R4_4_12),
+      D2D_VARIANTS(float, 192, 4, 64, true, R4_4_12),
     };
     for (const FastVariant &v : mf_variants()) r.push_back(v);
     return r;

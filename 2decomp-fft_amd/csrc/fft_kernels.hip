@@ -396,54 +396,46 @@ int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long lon
   return hip_check(hipGetLastError(), "batch kernel launch");
 }
 
-// ---- many grids, one launch per phase -------------------------------------------
-// The alternative executor of a batch: per phase, one launch whose
-// workgroups cover every tile of every group of that phase (XCD-aware
-// order); the group of a tile is found by binary search over tile0, and the
-// kernel boundary between phases orders the stages.
+// ---- many grids, one launch per phase and kernel -------------------------------
+// The alternative executor of a batch: per phase, the grids that run a stage
+// on the same kernel share one launch whose workgroups cover all their tiles
+// (XCD-aware order); a tile's grid is found by binary search over tile0, and
+// the kernel boundary between phases orders the stages.
+// The general kernel over the tiles of several grids, each with its own
+// descriptor (block size and LDS at least every grid's planned shape).
 template <typename T, int E>
 __global__ void __launch_bounds__(512)
-    grouped_kernel(const BatchGroup *__restrict__ groups, int g0, int g1, long long tiles) {
+    general_multi_kernel(const MultiGroup *__restrict__ g, int ng, long long tiles) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
   const long long it = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles);
   if (it >= tiles) return;  // whole block, before any barrier
-  int lo = g0, hi = g1 - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (groups[mid].tile0 <= it) lo = mid; else hi = mid - 1;
-  }
-  lo = __builtin_amdgcn_readfirstlane(lo);
-  const BatchGroup &g = groups[lo];
-  const AxisIo io = {g.in, g.out, g.conj};
-  axis_tile<T, E, IO_C2C>(g.d, io, it - g.tile0, lds);
+  const int gi = find_multi_group(g, ng, it);
+  const AxisIo io = {g[gi].in, g[gi].out, g[gi].conj};
+  axis_tile<T, E, IO_C2C>(g[gi].d, io, it - g[gi].tile0, lds);
 }
 
-int grouped_prepare(int prec_bytes, size_t lds) {
-  hipError_t e;
-  if (prec_bytes == 8)
-    e = hipFuncSetAttribute((const void *)grouped_kernel<double, kRegCap>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  else
-    e = hipFuncSetAttribute((const void *)grouped_kernel<float, kRegCap>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  return hip_check(e, "grouped kernel set-up");
-}
-
-int launch_grouped(int prec_bytes, const BatchGroup *groups, int g0, int g1, long long tiles,
-                   int nthreads, size_t lds, hipStream_t st) {
+int launch_general_multi(int prec_bytes, const MultiGroup *g, int ng, long long tiles,
+                         int nthreads, size_t lds, hipStream_t st) {
   const long long nb = xcd_grid(tiles);
   if (nb <= 0) return D2DHIP_SUCCESS;
   const long long gx = nb < (1LL << 20) ? nb : (1LL << 20);
   const long long gy = (nb + gx - 1) / gx;
-  if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "batch phase has too many tiles");
+  if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "launch has too many tiles");
   if (prec_bytes == 8)
-    hipLaunchKernelGGL((grouped_kernel<double, kRegCap>), dim3((unsigned)gx, (unsigned)gy),
-                       dim3(nthreads), lds, st, groups, g0, g1, tiles);
+    hipLaunchKernelGGL((general_multi_kernel<double, kRegCap>), dim3((unsigned)gx, (unsigned)gy),
+                       dim3(nthreads), lds, st, g, ng, tiles);
   else
-    hipLaunchKernelGGL((grouped_kernel<float, kRegCap>), dim3((unsigned)gx, (unsigned)gy),
-                       dim3(nthreads), lds, st, groups, g0, g1, tiles);
-  return hip_check(hipGetLastError(), "grouped kernel launch");
+    hipLaunchKernelGGL((general_multi_kernel<float, kRegCap>), dim3((unsigned)gx, (unsigned)gy),
+                       dim3(nthreads), lds, st, g, ng, tiles);
+  return hip_check(hipGetLastError(), "general multi kernel launch");
+}
+
+int general_multi_prepare(int prec_bytes, size_t lds) {
+  const void *f = prec_bytes == 8 ? (const void *)general_multi_kernel<double, kRegCap>
+                                  : (const void *)general_multi_kernel<float, kRegCap>;
+  return hip_check(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                   "hipFuncSetAttribute(general multi kernel)");
 }
 
 // Bluestein's chirp-z algorithm (L. Bluestein, 1970) for a line length nb

@@ -123,11 +123,30 @@ int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long lon
                  unsigned long long *done, unsigned int *err, long long wait_ticks, int blocks,
                  int nthreads, size_t lds, hipStream_t st);
 
-// One ordinary launch over all tiles of batch groups [g0, g1) (one phase):
-// workgroup b runs tile b - tile0 of the group holding it.
-int launch_grouped(int prec_bytes, const BatchGroup *groups, int g0, int g1, long long tiles,
-                   int nthreads, size_t lds, hipStream_t st);
-int grouped_prepare(int prec_bytes, size_t lds);
+// One grid's stage in a launch that covers several grids: its descriptor
+// and buffers, and the index of its first tile among the launch's tiles.
+struct MultiGroup {
+  const AxisDesc *d;
+  const void *in;
+  void *out;
+  int conj;
+  long long tile0;
+};
+
+// Group of launch tile `it` (groups sorted by tile0), wave-uniform.
+__device__ inline int find_multi_group(const MultiGroup *__restrict__ g, int n, long long it) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].tile0 <= it) lo = mid; else hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// The general kernel over the tiles of several grids (their own descriptors).
+int launch_general_multi(int prec_bytes, const MultiGroup *g, int ng, long long tiles,
+                         int nthreads, size_t lds, hipStream_t st);
+int general_multi_prepare(int prec_bytes, size_t lds);
 
 // copy / pack kernels (copy_kernels.hip)
 int launch_pack(int dir, int split, int esize, const void *src, void *dst,

@@ -207,9 +207,10 @@ constexpr int mf_regs() {
   }
 }
 
-template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
-    mf_axis_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
+// One tile of a stage (tile < tiles0 * C1), by the whole workgroup.
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM>
+__device__ __forceinline__ void mf_tile(const AxisDesc *__restrict__ d, const void *in, void *out,
+                                        int conj, long long tile) {
   using T = float;
   using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
   constexpr int P = RL::P, LDT = C::LDT, LDJ = C::LDJ;
@@ -221,9 +222,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
   T *ldr = reinterpret_cast<T *>(lds_raw);
 
   const long long tiles0 = d->tiles0;
-  const long long tile =
-      xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles0 * d->C1);
-  if (tile >= tiles0 * d->C1) return;  // whole block, before any barrier
   const long long c1 = tile / tiles0;
   const long long c0b = (tile - c1 * tiles0) * TX;
   const int tid = threadIdx.x;
@@ -473,6 +471,25 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
   }
 }
 
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+    mf_axis_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
+  const long long ntiles = (long long)d->tiles0 * d->C1;
+  const long long tile = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, ntiles);
+  if (tile >= ntiles) return;  // whole block, before any barrier
+  mf_tile<N, TX, NT, IO, LINES, RL, XM>(d, in, out, conj, tile);
+}
+
+// The same stage kernel over the tiles of several grids.
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+    mf_multi_kernel(const MultiGroup *__restrict__ g, int ng, long long tiles) {
+  const long long it = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles);
+  if (it >= tiles) return;  // whole block, before any barrier
+  const int gi = find_multi_group(g, ng, it);
+  mf_tile<N, TX, NT, IO, LINES, RL, XM>(g[gi].d, g[gi].in, g[gi].out, g[gi].conj, it - g[gi].tile0);
+}
+
 // ---- registry -----------------------------------------------------------------
 
 template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
@@ -492,6 +509,26 @@ template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
 static hipError_t prepare_mf() {
   using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
   return hipFuncSetAttribute((const void *)mf_axis_kernel<N, TX, NT, IO, LINES, RL, XM, WPE>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
+}
+
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+static int launch_mf_multi(const MultiGroup *g, int ng, long long tiles, hipStream_t st) {
+  using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
+  const long long nb = xcd_grid(tiles);
+  if (nb <= 0) return D2DHIP_SUCCESS;
+  const long long gx = nb < (1LL << 20) ? nb : (1LL << 20);
+  const long long gy = (nb + gx - 1) / gx;
+  if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "launch has too many tiles");
+  hipLaunchKernelGGL((mf_multi_kernel<N, TX, NT, IO, LINES, RL, XM, WPE>), dim3((unsigned)gx, (unsigned)gy),
+                     dim3(NT), C::lds_bytes(), st, g, ng, tiles);
+  return hip_check(hipGetLastError(), "mf_multi_kernel launch");
+}
+
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+static hipError_t prepare_mf_multi() {
+  using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
+  return hipFuncSetAttribute((const void *)mf_multi_kernel<N, TX, NT, IO, LINES, RL, XM, WPE>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
 }
 
@@ -517,6 +554,8 @@ static FastVariant make_mf(const char *name) {
   v.radices = mf_rvec(RL{});
   v.launch = &launch_mf<N, TX, NT, IO, LINES, RL, XM, WPE>;
   v.prepare = &prepare_mf<N, TX, NT, IO, LINES, RL, XM, WPE>;
+  v.launch_multi = &launch_mf_multi<N, TX, NT, IO, LINES, RL, XM, WPE>;
+  v.prepare_multi = &prepare_mf_multi<N, TX, NT, IO, LINES, RL, XM, WPE>;
   v.name = name;
   return v;
 }

@@ -579,12 +579,21 @@ void tune_store(const std::string &key, const std::string &val) {
   }
 }
 
+// The key also carries a hash of the candidate kernels' names, so that a
+// library with other specialised kernels for a stage tunes it again instead
+// of reusing a choice made among fewer candidates.
 std::string tune_key(const d2dhip_plan_s *p, int idx) {
   const StageGeom &g = p->geom[idx];
-  char b[256];
-  std::snprintf(b, sizeof(b), "v2 pr=%d io=%d ax=%d n=%d C0r=%lld C1=%lld s=%lld,%lld,%lld,%lld,%lld,%lld",
+  unsigned h = 2166136261u;  // FNV-1a
+  for (const FastVariant *v : g.cands) {
+    for (const char *c = v->name; *c; ++c) h = (h ^ (unsigned char)*c) * 16777619u;
+    h = (h ^ 0xffu) * 16777619u;
+  }
+  char b[288];
+  std::snprintf(b, sizeof(b),
+                "v2 pr=%d io=%d ax=%d n=%d C0r=%lld C1=%lld s=%lld,%lld,%lld,%lld,%lld,%lld c=%08x",
                 p->prec, g.io, g.axis, g.n, g.C0r, g.C1, g.in_s0, g.in_s1, g.in_sj, g.out_s0,
-                g.out_s1, g.out_sj);
+                g.out_s1, g.out_sj, h);
   return b;
 }
 
@@ -1192,6 +1201,20 @@ struct d2dhip_batch_s {
   int mode = D2DHIP_BATCH_PERSISTENT;
   std::vector<int> ph_g0, ph_g1;       // group range of each phase
   std::vector<long long> ph_tiles;     // tiles of each phase
+  // Grouped executor: per phase, the grids partitioned by the stage kernel
+  // their own plans run (as of bind); each part is one launch of that kernel
+  // over the tiles of its grids, with the plans' own descriptors.
+  struct Part {
+    const FastVariant *fv = nullptr;  // nullptr: the general kernel
+    int nthreads = 64;
+    size_t lds = 0;
+    long long m0 = 0;                 // first entry in d_multi
+    int ng = 0;
+    long long tiles = 0;
+  };
+  std::vector<std::vector<Part>> parts;  // per phase
+  MultiGroup *d_multi = nullptr;
+  std::vector<const FastVariant *> bound_fast;  // the plans' c2c kernels at bind (3 per plan)
 };
 
 namespace {
@@ -1294,7 +1317,6 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
   if (!st) {
     int bpc = 0, dev = 0, ncu = 0;
     st = batch_prepare(b->rb, b->nthreads, b->lds, &bpc);
-    if (!st) st = grouped_prepare(b->rb, b->lds);
     if (!st) st = hip_check(hipGetDevice(&dev), "hipGetDevice");
     if (!st) st = hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
     b->blocks = std::max(1, bpc) * std::max(1, ncu);
@@ -1328,10 +1350,17 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   std::vector<char> started(np, 0);
   std::vector<int> ph_g0, ph_g1;
   std::vector<long long> ph_tiles;
+  std::vector<std::vector<d2dhip_batch_s::Part>> parts;
+  std::vector<MultiGroup> multi;
+  size_t general_lds = 0;
   for (int ph = 0; ph < nph; ++ph) {
     unsigned long long items = 0;
     long long tiles = 0;
     const int g_begin = (int)groups.size();
+    // grouped executor: grids of this phase by their plan's stage kernel
+    std::vector<const FastVariant *> keys;
+    std::vector<std::vector<MultiGroup>> members;
+    std::vector<d2dhip_batch_s::Part> pp;
     for (int g = 0; g < np; ++g) {
       d2dhip_plan_s *p = b->plans[g];
       const int isign = (mode == 0) ? (ph < 3 ? -1 : 1) : mode;
@@ -1358,14 +1387,42 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
       first += nitems;
       tiles += G.ntiles;
       items += (unsigned long long)nitems;
-      started[g] = 1;
       groups.push_back(G);
+      // the same stage as the plan itself runs it
+      const FastVariant *fv = p->fast[ax];
+      size_t q = 0;
+      while (q < keys.size() && keys[q] != fv) ++q;
+      if (q == keys.size()) {
+        keys.push_back(fv);
+        members.emplace_back();
+        pp.emplace_back();
+        pp[q].fv = fv;
+      }
+      const AxisDesc &ph_d = p->h_desc[ax];
+      MultiGroup M;
+      M.d = p->d_desc + ax;
+      M.in = started[g] ? out[g] : in[g];
+      M.out = out[g];
+      M.conj = G.conj;
+      M.tile0 = pp[q].tiles;
+      members[q].push_back(M);
+      pp[q].tiles += (long long)ph_d.tiles0 * ph_d.C1;
+      pp[q].nthreads = std::max(pp[q].nthreads, p->shape[ax].nthreads);
+      pp[q].lds = std::max(pp[q].lds, p->shape[ax].lds_bytes);
+      started[g] = 1;
     }
     if (items) {
       phase_items.push_back(items);
       ph_g0.push_back(g_begin);
       ph_g1.push_back((int)groups.size());
       ph_tiles.push_back(tiles);
+      for (size_t k = 0; k < pp.size(); ++k) {
+        pp[k].m0 = (long long)multi.size();
+        pp[k].ng = (int)members[k].size();
+        multi.insert(multi.end(), members[k].begin(), members[k].end());
+        if (!pp[k].fv) general_lds = std::max(general_lds, pp[k].lds);
+      }
+      parts.push_back(pp);
     }
   }
   for (int g = 0; g < np; ++g)
@@ -1374,9 +1431,15 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   if (b->d_groups) (void)hipFree(b->d_groups);
   if (b->d_done) (void)hipFree(b->d_done);
   if (b->d_phase_items) (void)hipFree(b->d_phase_items);
+  if (b->d_multi) (void)hipFree(b->d_multi);
   b->d_groups = nullptr;
   b->d_done = nullptr;
   b->d_phase_items = nullptr;
+  b->d_multi = nullptr;
+  b->parts = parts;
+  b->bound_fast.assign(3 * np, nullptr);
+  for (int g = 0; g < np; ++g)
+    for (int a = 0; a < 3; ++a) b->bound_fast[3 * g + a] = b->plans[g]->fast[a];
   b->ngroups = (int)groups.size();
   b->nphase = (int)phase_items.size();
   b->total = first;
@@ -1398,11 +1461,21 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   }
   if (hipMalloc(&b->d_groups, sizeof(BatchGroup) * groups.size()) != hipSuccess ||
       hipMalloc(&b->d_done, sizeof(unsigned long long) * phase_items.size()) != hipSuccess ||
-      hipMalloc(&b->d_phase_items, sizeof(unsigned long long) * phase_items.size()) != hipSuccess)
+      hipMalloc(&b->d_phase_items, sizeof(unsigned long long) * phase_items.size()) != hipSuccess ||
+      hipMalloc(&b->d_multi, sizeof(MultiGroup) * multi.size()) != hipSuccess)
     return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch queue)");
   int st = hip_check(hipMemcpy(b->d_groups, groups.data(), sizeof(BatchGroup) * groups.size(),
                                hipMemcpyHostToDevice),
                      "hipMemcpy(batch queue)");
+  if (!st)
+    st = hip_check(hipMemcpy(b->d_multi, multi.data(), sizeof(MultiGroup) * multi.size(),
+                             hipMemcpyHostToDevice),
+                   "hipMemcpy(batch groups)");
+  // kernel attributes now, so that a captured launch makes no such call
+  if (!st && general_lds) st = general_multi_prepare(b->rb, general_lds);
+  for (size_t ph = 0; ph < parts.size() && !st; ++ph)
+    for (const auto &q : parts[ph])
+      if (!st && q.fv) st = hip_check(q.fv->prepare_multi(), "hipFuncSetAttribute(multi kernel)");
   if (!st)
     st = hip_check(hipMemcpy(b->d_phase_items, phase_items.data(),
                              sizeof(unsigned long long) * phase_items.size(), hipMemcpyHostToDevice),
@@ -1416,11 +1489,18 @@ int d2dhip_batch_launch(d2dhip_batch b) {
   if (!b->bound) return set_error(D2DHIP_ERR_INVALID_VALUE, "batch not bound");
   if (b->ngroups == 0) return D2DHIP_SUCCESS;
   if (b->mode == D2DHIP_BATCH_GROUPED) {
-    for (int ph = 0; ph < b->nphase; ++ph) {
-      const int st = launch_grouped(b->rb, b->d_groups, b->ph_g0[ph], b->ph_g1[ph], b->ph_tiles[ph],
-                                    b->nthreads, b->lds, b->stream);
-      if (st) return st;
-    }
+    for (size_t g = 0; g < b->plans.size(); ++g)
+      for (int a = 0; a < 3; ++a)
+        if (b->plans[g]->fast[a] != b->bound_fast[3 * g + a])
+          return set_error(D2DHIP_ERR_INVALID_VALUE, "a plan's stage kernel changed since bind: bind again");
+    for (const auto &pp : b->parts)
+      for (const auto &q : pp) {
+        const MultiGroup *g = b->d_multi + q.m0;
+        const int st = q.fv ? q.fv->launch_multi(g, q.ng, q.tiles, b->stream)
+                            : launch_general_multi(b->rb, g, q.ng, q.tiles, q.nthreads, q.lds,
+                                                   b->stream);
+        if (st) return st;
+      }
     return D2DHIP_SUCCESS;
   }
   return launch_batch(b->rb, b->d_groups, b->ngroups, b->total, b->d_phase_items, b->nphase,
@@ -1503,6 +1583,7 @@ int d2dhip_batch_destroy(d2dhip_batch b) {
   if (b->d_groups) (void)hipFree(b->d_groups);
   if (b->d_done) (void)hipFree(b->d_done);
   if (b->d_phase_items) (void)hipFree(b->d_phase_items);
+  if (b->d_multi) (void)hipFree(b->d_multi);
   if (b->d_head) (void)hipFree(b->d_head);
   if (b->d_err) (void)hipFree(b->d_err);
   delete b;

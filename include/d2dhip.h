@@ -188,10 +188,14 @@ int d2dhip_batch_graph_launch(d2dhip_batch batch);
 int d2dhip_batch_check(d2dhip_batch batch);
 int d2dhip_batch_set_stream(d2dhip_batch batch, void *stream);
 /* Executor of a batch: D2DHIP_BATCH_PERSISTENT (default) is the single
- * persistent launch above; D2DHIP_BATCH_GROUPED runs one ordinary launch per
- * stage phase covering the tiles of every grid in that phase (the kernel
- * boundaries order the phases; no device queue).  Results are identical.
- * Changing the mode drops a captured graph. */
+ * persistent launch above (general kernel, batch-wide block shape);
+ * D2DHIP_BATCH_GROUPED runs, per stage phase, one ordinary launch per distinct
+ * stage kernel of the grids' own plans, covering the tiles of every grid that
+ * runs that kernel in that phase (the kernel boundaries order the phases; no
+ * device queue).  Grouped results are bit-identical to the plans' own
+ * d2dhip_fft_3d_c2c calls; it uses the plans' stage kernels as of bind, and
+ * launch fails with D2DHIP_ERR_INVALID_VALUE if one has changed since (bind
+ * again).  Changing the mode drops a captured graph. */
 #define D2DHIP_BATCH_PERSISTENT 0
 #define D2DHIP_BATCH_GROUPED 1
 int d2dhip_batch_set_mode(d2dhip_batch batch, int mode);

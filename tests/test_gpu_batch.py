@@ -1,9 +1,11 @@
 """Many grids in one persistent-kernel launch (the fft_multiple_grids use,
 examples/fft_multiple_grids/fft_multiple_grids.f90), eager and captured in a
-hipGraph, and the same batch run as one launch per stage phase.  Both
-executors run the same general-kernel stages as the per-plan path, so their
-results must equal the sequential decomp_2d_fft_3d calls bit for bit, and
-match pocketfft to the FP64 / FP32 tolerances."""
+hipGraph, and the same batch run as one launch per stage phase and kernel.
+The persistent executor runs the general kernel, so its results must equal
+the sequential decomp_2d_fft_3d calls of plans set to the general kernel bit
+for bit; the grouped executor runs each plan's own stage kernels, so its
+results must equal the plans' own sequential calls bit for bit.  Both match
+pocketfft to the FP64 / FP32 tolerances."""
 import numpy as np
 import pytest
 
@@ -40,6 +42,10 @@ def test_batch_matches_sequential_and_pocketfft(d2d, torch_gpu, prec, mode):
     outs = [torch.empty_like(a) for a in ins]
     seq = [torch.empty_like(a) for a in ins]
     for e, a, o in zip(engines, ins, seq):
+        if mode == 0:
+            # the persistent kernel runs the general kernel: compare with the plans on it
+            for st in ("c2c_x", "c2c_y", "c2c_z"):
+                e.set_stage_kernel(st, "general")
         e.fft_c2c(a, o, -1)
     b = d2d.Batch(engines)
     b.set_mode(mode)
@@ -116,10 +122,45 @@ def test_config5_64_uneven_grids(d2d, torch_gpu):
         x = ins[g].cpu().numpy().reshape(shapes[g], order="F")
         got = outs[g].cpu().numpy().reshape(shapes[g], order="F")
         assert rel_l2(got, np.fft.fftn(x)) < 1e-13
-    # the per-phase executor gives the same bits
-    persistent = [o.clone() for o in outs]
+    # the per-phase executor runs the plans' own kernels: their bits
+    seq = [torch.empty_like(a) for a in ins]
+    for e, a, q in zip(engines, ins, seq):
+        e.fft_c2c(a, q, -1)
     b.set_mode(1)
     b.launch()
     b.check()
-    for o, q in zip(outs, persistent):
+    for o, q in zip(outs, seq):
         assert torch.equal(o, q)
+    for o in outs:
+        o.zero_()
+    b.bind(ins, outs, mode=0)
+    b.launch()
+    b.check()
+    for a, o, s in zip(ins, outs, shapes):
+        err = float(torch.linalg.vector_norm(o / np.prod(s) - a) / torch.linalg.vector_norm(a))
+        assert err < 1e-13, s
+
+
+def test_grouped_rejects_changed_plan(d2d, torch_gpu):
+    """A plan whose stage kernel changes after bind must be bound again."""
+    torch = torch_gpu
+    shapes = [(64, 32, 16), (32, 32, 32)]
+    engines = [d2d.FFTEngine(1, *s) for s in shapes]
+    ins = [torch.zeros(int(np.prod(s)), dtype=torch.complex128, device="cuda") for s in shapes]
+    outs = [torch.empty_like(a) for a in ins]
+    b = d2d.Batch(engines)
+    b.set_mode(1)
+    b.bind(ins, outs, mode=-1)
+    b.launch()
+    b.check()
+    kinds = engines[0].stage_kernels()
+    names = [kinds[st] for st in ("c2c_x", "c2c_y", "c2c_z")]
+    if all(n == "general" for n in names):
+        pytest.skip("every stage already runs the general kernel")
+    for st in ("c2c_x", "c2c_y", "c2c_z"):
+        engines[0].set_stage_kernel(st, "general")
+    with pytest.raises(d2d.D2dError):
+        b.launch()
+    b.bind(ins, outs, mode=-1)
+    b.launch()
+    b.check()

@@ -301,7 +301,12 @@ def test_config3a_r2c_z_720_540_432(d2d, torch_gpu):
 
 THIN = [(1024, 6, 5), (6, 1024, 5), (6, 5, 1024), (512, 6, 5), (6, 512, 5), (6, 5, 512),
         (1024, 5, 3), (5, 3, 1024), (512, 3, 5),
-        (720, 5, 3), (3, 720, 5), (540, 3, 5), (5, 540, 3), (432, 5, 3), (5, 3, 432)]
+        (720, 5, 3), (3, 720, 5), (540, 3, 5), (5, 540, 3), (432, 5, 3), (5, 3, 432),
+        (256, 5, 3), (5, 256, 3), (5, 3, 256), (128, 5, 3), (5, 128, 3), (3, 5, 128),
+        (64, 5, 3), (5, 64, 3), (5, 3, 64), (32, 5, 3), (3, 32, 5), (5, 3, 32),
+        (48, 5, 3), (5, 48, 3), (3, 5, 48), (80, 5, 3), (5, 80, 3), (3, 5, 80),
+        (96, 5, 3), (5, 96, 3), (3, 5, 96), (160, 5, 3), (5, 160, 3), (3, 5, 160),
+        (192, 5, 3), (5, 192, 3), (3, 5, 192), (160, 48, 20), (96, 80, 36)]
 STAGE_USE = {"c2c_x": "c2c", "c2c_y": "c2c", "c2c_z": "c2c", "r2c": "r2c", "spec_y": "rt",
              "spec_last": "rt", "c2r": "c2r"}
 
