@@ -130,6 +130,8 @@ def lib():
             "d2dhip_batch_check": ([vp], ci),
             "d2dhip_batch_set_stream": ([vp, vp], ci),
             "d2dhip_batch_set_mode": ([vp, ci], ci),
+            "d2dhip_batch_set_cluster_bytes": ([vp, cll], ci),
+            "d2dhip_batch_get_clusters": ([vp, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
             "d2dhip_batch_destroy": ([vp], ci),
             "d2dhip_pack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
             "d2dhip_unpack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
@@ -316,7 +318,9 @@ def decomp_2d_fft_init(pencil, nx, ny, nz, **kw):
 
 
 class Batch:
-    """c2c stages of many engines in one persistent-kernel launch."""
+    """c2c stages of many engines run together: one launch per stage phase
+    and stage kernel (BATCH_GROUPED, default) or one persistent-kernel launch
+    (BATCH_PERSISTENT)."""
 
     def __init__(self, engines, stream=None):
         self.engines = list(engines)
@@ -344,9 +348,21 @@ class Batch:
         check(lib().d2dhip_batch_launch(self._b))
 
     def set_mode(self, mode):
-        """BATCH_PERSISTENT (one persistent launch) or BATCH_GROUPED (one launch
-        per stage phase)."""
+        """BATCH_GROUPED (default: one launch per stage phase and stage kernel,
+        the plans' own kernels) or BATCH_PERSISTENT (one persistent launch)."""
         check(lib().d2dhip_batch_set_mode(self._b, int(mode)))
+
+    def set_cluster_bytes(self, nbytes):
+        """Byte budget of a cluster of consecutive grids that runs all its
+        stages before the next (0 = the whole batch phase by phase); takes
+        effect at the next bind."""
+        check(lib().d2dhip_batch_set_cluster_bytes(self._b, int(nbytes)))
+
+    def clusters(self):
+        """(clusters, stage phases) of the current binding."""
+        c, p = ctypes.c_int(), ctypes.c_int()
+        check(lib().d2dhip_batch_get_clusters(self._b, ctypes.byref(c), ctypes.byref(p)))
+        return c.value, p.value
 
     def capture_graph(self):
         check(lib().d2dhip_batch_capture_graph(self._b))

@@ -20,6 +20,37 @@ struct alignas(2 * sizeof(T)) cpx {
   T x, y;
 };
 
+// Device-memory (HBM) accesses through address-space-1 pointers: the
+// compiler emits global_load / global_store for them.  A pointer that a
+// kernel loads from memory (a descriptor's twiddle table, a grid's buffers
+// in a multi-grid launch) is otherwise a generic pointer, and generic
+// accesses become flat instructions, which also count against the LDS wait
+// counter and so hold up the LDS exchanges between passes.
+template <typename T>
+using gvec2 = T __attribute__((ext_vector_type(2)));
+#define D2D_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ cpx<T> gload(const void *base, long long i) {
+  const gvec2<T> v = ((const D2D_GLOBAL gvec2<T> *)base)[i];
+  return cpx<T>{v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void gstore(void *base, long long i, cpx<T> c) {
+  gvec2<T> v;
+  v.x = c.x;
+  v.y = c.y;
+  ((D2D_GLOBAL gvec2<T> *)base)[i] = v;
+}
+template <typename T>
+__device__ __forceinline__ T gload_r(const void *base, long long i) {
+  return ((const D2D_GLOBAL T *)base)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gstore_r(void *base, long long i, T v) {
+  ((D2D_GLOBAL T *)base)[i] = v;
+}
+
 template <typename T>
 __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
   return {a.x + b.x, a.y + b.y};

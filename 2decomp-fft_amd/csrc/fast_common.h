@@ -56,19 +56,18 @@ struct FastCfg {
 };
 
 template <typename T, int IO>
-__device__ __forceinline__ cpx<T> fload(const AxisDesc *__restrict__ d, const void *in,
+__device__ __forceinline__ cpx<T> fload(DescPtr d, const void *in,
                                         long long base, bool cvalid, bool bvalid, long long ca,
                                         int j, int conj) {
   if constexpr (IO == IO_R2C) {
-    const T *p = reinterpret_cast<const T *>(in);
     const long long o = base + (long long)j * d->in_sj;
     cpx<T> v = {T(0), T(0)};
-    if (ca < d->C0r) v.x = p[o];
-    if (ca + 1 < d->C0r) v.y = p[o + d->in_s0];
+    if (ca < d->C0r) v.x = gload_r<T>(in, o);
+    if (ca + 1 < d->C0r) v.y = gload_r<T>(in, o + d->in_s0);
     return v;
   } else {
     cpx<T> v = {T(0), T(0)};
-    if (cvalid) v = reinterpret_cast<const cpx<T> *>(in)[base + (long long)j * d->in_sj];
+    if (cvalid) v = gload<T>(in, base + (long long)j * d->in_sj);
     if (conj) v.y = -v.y;
     return v;
   }
@@ -76,17 +75,16 @@ __device__ __forceinline__ cpx<T> fload(const AxisDesc *__restrict__ d, const vo
 }
 
 template <typename T, int IO>
-__device__ __forceinline__ void fstore(const AxisDesc *__restrict__ d, void *out, long long base,
+__device__ __forceinline__ void fstore(DescPtr d, void *out, long long base,
                                        bool cvalid, long long ca, int j, cpx<T> v, int conj) {
   if constexpr (IO == IO_C2R) {
-    T *p = reinterpret_cast<T *>(out);
     const long long o = base + (long long)j * d->out_sj;
-    if (ca < d->C0r) p[o] = v.x;
-    if (ca + 1 < d->C0r) p[o + d->out_s0] = -v.y;
+    if (ca < d->C0r) gstore_r<T>(out, o, v.x);
+    if (ca + 1 < d->C0r) gstore_r<T>(out, o + d->out_s0, -v.y);
   } else {
     if (cvalid) {
       if (conj) v.y = -v.y;
-      reinterpret_cast<cpx<T> *>(out)[base + (long long)j * d->out_sj] = v;
+      gstore<T>(out, base + (long long)j * d->out_sj, v);
     }
   }
 }

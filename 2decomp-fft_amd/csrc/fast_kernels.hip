@@ -25,7 +25,7 @@ namespace d2dhip {
 // constraint beyond the block size).
 // One tile of a stage (tile < tiles0 * C1), by the whole workgroup.
 template <typename T, int N, int TX, int NT, int IO, bool LINES, class RL, int XM>
-__device__ __forceinline__ void fast_tile(const AxisDesc *__restrict__ d, const void *in, void *out,
+__device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
                                           int conj, long long tile) {
   using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
   constexpr int P = C::P, RPT = C::RPT, LDT = C::LDT, LDJ = C::LDJ, E = C::emax();
@@ -43,14 +43,13 @@ __device__ __forceinline__ void fast_tile(const AxisDesc *__restrict__ d, const 
   const bool cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
   const long long in_base = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1;
   const long long out_base = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1;
-  const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(d->tw);
+  const void *tw = d->tw;
   cpx<T> v[E];
 
   // ---- inputs of pass 0 -------------------------------------------------
   if constexpr (IO == IO_C2R) {
     // conj(Z) of the merged half spectra, all N points, into LDS
     constexpr int NH = N / 2 + 1;
-    const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in);
     // all loads first (compile-time trip count), then the LDS writes, so a
     // thread has every load of the phase in flight at once
     constexpr int ITER = (TX * NH + NT - 1) / NT;
@@ -65,8 +64,8 @@ __device__ __forceinline__ void fast_tile(const AxisDesc *__restrict__ d, const 
         const int k = LINES ? w % NH : w / TX;
         const long long cc = 2 * (c0b + tt);
         const long long base = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
-        if (cc < d->C0r) A[it] = src[base];
-        if (cc + 1 < d->C0r) B[it] = src[base + d->in_s0];
+        if (cc < d->C0r) A[it] = gload<T>(in, base);
+        if (cc + 1 < d->C0r) B[it] = gload<T>(in, base + d->in_s0);
       }
     }
     // conj(Z) of the merged half spectra into LDS, then the inputs of pass 0
@@ -196,9 +195,9 @@ __device__ __forceinline__ void fast_tile(const AxisDesc *__restrict__ d, const 
       if (C::exact(p) || b < NBR) {
         if constexpr (L > 1) {
           const int k = b % L;
-          const cpx<T> *tp = tw + RL::tw_off(p) + k * (R - 1);
+          const int tp = RL::tw_off(p) + k * (R - 1);
 #pragma unroll
-          for (int s = 1; s < R; ++s) v[q * R + s] = cmul(v[q * R + s], tp[s - 1]);
+          for (int s = 1; s < R; ++s) v[q * R + s] = cmul(v[q * R + s], gload<T>(tw, tp + s - 1));
         }
         dft<T, R>(&v[q * R]);
       }
@@ -261,7 +260,6 @@ __device__ __forceinline__ void fast_tile(const AxisDesc *__restrict__ d, const 
         }
       }
     }
-    cpx<T> *dst = reinterpret_cast<cpx<T> *>(out);
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
       const int w = tid + it * NT;
@@ -271,8 +269,8 @@ __device__ __forceinline__ void fast_tile(const AxisDesc *__restrict__ d, const 
         const long long cc = 2 * (c0b + tt);
         if (cc < d->C0r) {
           const long long base = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
-          dst[base] = oa[it];
-          if (cc + 1 < d->C0r) dst[base + d->out_s0] = ob[it];
+          gstore<T>(out, base, oa[it]);
+          if (cc + 1 < d->C0r) gstore<T>(out, base + d->out_s0, ob[it]);
         }
       }
     }
@@ -297,7 +295,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
   const long long ntiles = (long long)d->tiles0 * d->C1;
   const long long tile = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, ntiles);
   if (tile >= ntiles) return;  // whole block, before any barrier
-  fast_tile<T, N, TX, NT, IO, LINES, RL, XM>(d, in, out, conj, tile);
+  fast_tile<T, N, TX, NT, IO, LINES, RL, XM>(desc_ptr(d), in, out, conj, tile);
 }
 
 // The same stage kernel over the tiles of several grids.
@@ -307,7 +305,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
   const long long it = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles);
   if (it >= tiles) return;  // whole block, before any barrier
   const int gi = find_multi_group(g, ng, it);
-  fast_tile<T, N, TX, NT, IO, LINES, RL, XM>(g[gi].d, g[gi].in, g[gi].out, g[gi].conj,
+  fast_tile<T, N, TX, NT, IO, LINES, RL, XM>(desc_ptr(g[gi].d), g[gi].in, g[gi].out, g[gi].conj,
                                              it - g[gi].tile0);
 }
 

@@ -31,21 +31,18 @@ namespace d2dhip {
 
 // Input value of tile column c0 (global), outer index c1, point j.
 template <typename T, int IO>
-__device__ __forceinline__ cpx<T> gload(const AxisDesc *__restrict__ d, const AxisIo &io,
-                                        long long c0, long long c1, int j) {
+__device__ __forceinline__ cpx<T> pload(DescPtr d, const AxisIo &io, long long c0, long long c1,
+                                        int j) {
   if constexpr (IO == IO_R2C) {
-    const T *in = reinterpret_cast<const T *>(io.in);
     const long long base = c1 * d->in_s1 + (long long)j * d->in_sj;
     const long long ca = 2 * c0;
     cpx<T> v = {T(0), T(0)};
-    if (ca < d->C0r) v.x = in[base + ca * d->in_s0];
-    if (ca + 1 < d->C0r) v.y = in[base + (ca + 1) * d->in_s0];
+    if (ca < d->C0r) v.x = gload_r<T>(io.in, base + ca * d->in_s0);
+    if (ca + 1 < d->C0r) v.y = gload_r<T>(io.in, base + (ca + 1) * d->in_s0);
     return v;
   } else {
     cpx<T> v = {T(0), T(0)};
-    if (c0 < d->C0)
-      v = reinterpret_cast<const cpx<T> *>(io.in)[c0 * d->in_s0 + c1 * d->in_s1 +
-                                                  (long long)j * d->in_sj];
+    if (c0 < d->C0) v = gload<T>(io.in, c0 * d->in_s0 + c1 * d->in_s1 + (long long)j * d->in_sj);
     if (io.conj) v.y = -v.y;
     return v;
   }
@@ -53,30 +50,38 @@ __device__ __forceinline__ cpx<T> gload(const AxisDesc *__restrict__ d, const Ax
 
 // Store of an output point of the (conjugated, for inverses) transform.
 template <typename T, int IO>
-__device__ __forceinline__ void gstore(const AxisDesc *__restrict__ d, const AxisIo &io,
-                                       long long c0, long long c1, int j, cpx<T> v) {
+__device__ __forceinline__ void pstore(DescPtr d, const AxisIo &io, long long c0, long long c1,
+                                       int j, cpx<T> v) {
   if constexpr (IO == IO_C2R) {
     // the pre-phase stored conj(Z): z = conj(result) = a + i b
-    T *out = reinterpret_cast<T *>(io.out);
     const long long base = c1 * d->out_s1 + (long long)j * d->out_sj;
     const long long ca = 2 * c0;
-    if (ca < d->C0r) out[base + ca * d->out_s0] = v.x;
-    if (ca + 1 < d->C0r) out[base + (ca + 1) * d->out_s0] = -v.y;
+    if (ca < d->C0r) gstore_r<T>(io.out, base + ca * d->out_s0, v.x);
+    if (ca + 1 < d->C0r) gstore_r<T>(io.out, base + (ca + 1) * d->out_s0, -v.y);
   } else {
     if (c0 < d->C0) {
       if (io.conj) v.y = -v.y;
-      reinterpret_cast<cpx<T> *>(io.out)[c0 * d->out_s0 + c1 * d->out_s1 +
-                                         (long long)j * d->out_sj] = v;
+      gstore<T>(io.out, c0 * d->out_s0 + c1 * d->out_s1 + (long long)j * d->out_sj, v);
     }
   }
+}
+
+// The stage's LDS tile: the dynamic LDS of the launch.  Taken from the
+// array itself rather than passed as a pointer, so that the non-inlined pass
+// function addresses it with ds_read / ds_write.
+template <typename T>
+__device__ __forceinline__ cpx<T> *lds_tile() {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  return reinterpret_cast<cpx<T> *>(lds_raw);
 }
 
 // One Stockham pass of radix R.  `sg` reads the pass inputs from HBM (first
 // pass), `dg` writes the outputs to HBM (last pass); otherwise LDS.  Ends
 // with a barrier whenever it wrote LDS.
 template <typename T, int E, int IO, int R>
-__device__ __noinline__ void run_pass(const AxisDesc *__restrict__ d, AxisIo io, int p, bool sg,
-                                      bool dg, cpx<T> *lds, long long c0b, long long c1) {
+__device__ __noinline__ void run_pass(DescPtr d, AxisIo io, int p, bool sg, bool dg, long long c0b,
+                                      long long c1) {
+  cpx<T> *lds = lds_tile<T>();
   constexpr int NB = E / R;
   static_assert(NB >= 1, "radix larger than the register capacity");
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -95,13 +100,14 @@ __device__ __noinline__ void run_pass(const AxisDesc *__restrict__ d, AxisIo io,
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         const int j = b + s * nb;
-        v[q][s] = sg ? gload<T, IO>(d, io, c0b + t, c1, j) : lds[j * ldj + t * ldt];
+        v[q][s] = sg ? pload<T, IO>(d, io, c0b + t, c1, j) : lds[j * ldj + t * ldt];
       }
     }
   }
   if (!sg && !dg) __syncthreads();  // all LDS reads before LDS writes
 
-  const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(d->tw) + d->pass[p].tw_off;
+  const void *tw = d->tw;
+  const int tw_off = d->pass[p].tw_off;
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     const int w = tid + q * nthr;
@@ -118,7 +124,7 @@ __device__ __noinline__ void run_pass(const AxisDesc *__restrict__ d, AxisIo io,
       }
       if (L > 1) {
 #pragma unroll
-        for (int s = 1; s < R; ++s) v[q][s] = cmul(v[q][s], tw[k * (R - 1) + (s - 1)]);
+        for (int s = 1; s < R; ++s) v[q][s] = cmul(v[q][s], gload<T>(tw, tw_off + k * (R - 1) + (s - 1)));
       }
       dft<T, R>(v[q]);
       const int jb = g * L * R + k;
@@ -126,7 +132,7 @@ __device__ __noinline__ void run_pass(const AxisDesc *__restrict__ d, AxisIo io,
       for (int s = 0; s < R; ++s) {
         const int j = jb + s * L;
         if (dg)
-          gstore<T, IO>(d, io, c0b + t, c1, j, v[q][s]);
+          pstore<T, IO>(d, io, c0b + t, c1, j, v[q][s]);
         else
           lds[j * ldj + t * ldt] = v[q][s];
       }
@@ -136,26 +142,25 @@ __device__ __noinline__ void run_pass(const AxisDesc *__restrict__ d, AxisIo io,
 }
 
 template <typename T, int E, int IO>
-__device__ __forceinline__ void dispatch_pass(const AxisDesc *__restrict__ d, const AxisIo &io,
-                                              int p, bool sg, bool dg, cpx<T> *lds,
+__device__ __forceinline__ void dispatch_pass(DescPtr d, const AxisIo &io, int p, bool sg, bool dg,
                                               long long c0b, long long c1) {
   switch (d->pass[p].r) {
-    case 1: run_pass<T, E, IO, 1>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 2: run_pass<T, E, IO, 2>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 3: run_pass<T, E, IO, 3>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 4: run_pass<T, E, IO, 4>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 5: run_pass<T, E, IO, 5>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 6: run_pass<T, E, IO, 6>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 7: run_pass<T, E, IO, 7>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 8: run_pass<T, E, IO, 8>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 9: run_pass<T, E, IO, 9>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 10: run_pass<T, E, IO, 10>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 11: run_pass<T, E, IO, 11>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 12: run_pass<T, E, IO, 12>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 13: run_pass<T, E, IO, 13>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 15: run_pass<T, E, IO, 15>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 16: run_pass<T, E, IO, 16>(d, io, p, sg, dg, lds, c0b, c1); break;
-    case 17: run_pass<T, E, IO, 17>(d, io, p, sg, dg, lds, c0b, c1); break;
+    case 1: run_pass<T, E, IO, 1>(d, io, p, sg, dg, c0b, c1); break;
+    case 2: run_pass<T, E, IO, 2>(d, io, p, sg, dg, c0b, c1); break;
+    case 3: run_pass<T, E, IO, 3>(d, io, p, sg, dg, c0b, c1); break;
+    case 4: run_pass<T, E, IO, 4>(d, io, p, sg, dg, c0b, c1); break;
+    case 5: run_pass<T, E, IO, 5>(d, io, p, sg, dg, c0b, c1); break;
+    case 6: run_pass<T, E, IO, 6>(d, io, p, sg, dg, c0b, c1); break;
+    case 7: run_pass<T, E, IO, 7>(d, io, p, sg, dg, c0b, c1); break;
+    case 8: run_pass<T, E, IO, 8>(d, io, p, sg, dg, c0b, c1); break;
+    case 9: run_pass<T, E, IO, 9>(d, io, p, sg, dg, c0b, c1); break;
+    case 10: run_pass<T, E, IO, 10>(d, io, p, sg, dg, c0b, c1); break;
+    case 11: run_pass<T, E, IO, 11>(d, io, p, sg, dg, c0b, c1); break;
+    case 12: run_pass<T, E, IO, 12>(d, io, p, sg, dg, c0b, c1); break;
+    case 13: run_pass<T, E, IO, 13>(d, io, p, sg, dg, c0b, c1); break;
+    case 15: run_pass<T, E, IO, 15>(d, io, p, sg, dg, c0b, c1); break;
+    case 16: run_pass<T, E, IO, 16>(d, io, p, sg, dg, c0b, c1); break;
+    case 17: run_pass<T, E, IO, 17>(d, io, p, sg, dg, c0b, c1); break;
     default: break;  // the planner only emits the radices above
   }
 }
@@ -163,8 +168,8 @@ __device__ __forceinline__ void dispatch_pass(const AxisDesc *__restrict__ d, co
 // One tile of a stage, by the whole workgroup (any blockDim at least the
 // stage's planned block size).
 template <typename T, int E, int IO>
-__device__ __forceinline__ void axis_tile(const AxisDesc *__restrict__ d, const AxisIo &io,
-                                          long long tile, cpx<T> *lds) {
+__device__ __forceinline__ void axis_tile(DescPtr d, const AxisIo &io, long long tile) {
+  cpx<T> *lds = lds_tile<T>();
   const void *in = io.in;
   void *out = io.out;
   const long long tiles0 = d->tiles0;
@@ -178,14 +183,13 @@ __device__ __forceinline__ void axis_tile(const AxisDesc *__restrict__ d, const 
     // Merge the two half-spectra of columns (2c0, 2c0+1) into conj(Z) over
     // all n points; the imaginary parts of the zero and Nyquist modes are
     // ignored, as a complex-to-real transform does.
-    const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in);
     for (int w = tid; w < TX * nh; w += nthr) {
       const int t = w & tmask, k = w >> lg;
       const long long ca = 2 * (c0b + t);
       const long long base = c1 * d->in_s1 + (long long)k * d->in_sj;
       cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
-      if (ca < d->C0r) A = src[base + ca * d->in_s0];
-      if (ca + 1 < d->C0r) B = src[base + (ca + 1) * d->in_s0];
+      if (ca < d->C0r) A = gload<T>(in, base + ca * d->in_s0);
+      if (ca + 1 < d->C0r) B = gload<T>(in, base + (ca + 1) * d->in_s0);
       if (k == 0 || 2 * k == n) {
         A.y = T(0);
         B.y = T(0);
@@ -200,11 +204,10 @@ __device__ __forceinline__ void axis_tile(const AxisDesc *__restrict__ d, const 
   for (int p = 0; p < np; ++p) {
     const bool sg = (p == 0) && (IO != IO_C2R);
     const bool dg = (p == np - 1) && (IO != IO_R2C) && (np >= 2);
-    dispatch_pass<T, E, IO>(d, io, p, sg, dg, lds, c0b, c1);
+    dispatch_pass<T, E, IO>(d, io, p, sg, dg, c0b, c1);
   }
 
   if constexpr (IO == IO_R2C) {
-    cpx<T> *dst = reinterpret_cast<cpx<T> *>(out);
     const T half = T(0.5);
     for (int w = tid; w < TX * nh; w += nthr) {
       const int t = w & tmask, k = w >> lg;
@@ -214,15 +217,15 @@ __device__ __forceinline__ void axis_tile(const AxisDesc *__restrict__ d, const 
       const int km = (k == 0) ? 0 : n - k;
       const cpx<T> zm = lds[km * ldj + t * ldt];
       const long long base = c1 * d->out_s1 + (long long)k * d->out_sj;
-      dst[base + ca * d->out_s0] = cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half};
+      gstore<T>(out, base + ca * d->out_s0, cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half});
       if (ca + 1 < d->C0r)
-        dst[base + (ca + 1) * d->out_s0] = cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half};
+        gstore<T>(out, base + (ca + 1) * d->out_s0, cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half});
     }
   } else {
     if (np < 2) {
       for (int w = tid; w < TX * n; w += nthr) {
         const int t = w & tmask, j = w >> lg;
-        gstore<T, IO>(d, io, c0b + t, c1, j, lds[j * ldj + t * ldt]);
+        pstore<T, IO>(d, io, c0b + t, c1, j, lds[j * ldj + t * ldt]);
       }
     }
   }
@@ -231,13 +234,11 @@ __device__ __forceinline__ void axis_tile(const AxisDesc *__restrict__ d, const 
 template <typename T, int E, int IO>
 __global__ void __launch_bounds__(512)
     axis_fft_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
   const AxisIo io = {in, out, conj};
   const long long ntiles = (long long)d->tiles0 * d->C1;
   const long long tile = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, ntiles);
   if (tile >= ntiles) return;  // whole block, before any barrier
-  axis_tile<T, E, IO>(d, io, tile, lds);
+  axis_tile<T, E, IO>(desc_ptr(d), io, tile);
 }
 
 // ---- many grids in one launch ---------------------------------------------------
@@ -280,8 +281,6 @@ __global__ void __launch_bounds__(512)
                  const unsigned long long *__restrict__ phase_items, int nphase,
                  unsigned long long *head, unsigned long long *done, unsigned int *err,
                  long long wait_ticks) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
   __shared__ long long s_item;
   __shared__ int s_fail;
   const int tid = threadIdx.x;
@@ -340,7 +339,7 @@ __global__ void __launch_bounds__(512)
       const long long i = uniform_ll(it - g.first);
       for (long long tile = i; tile < nt; tile += ni) {
         if (tile > i) __syncthreads();  // LDS reuse across tiles
-        axis_tile<T, E, IO_C2C>(g.d, io, tile, lds);
+        axis_tile<T, E, IO_C2C>(desc_ptr(g.d), io, tile);
       }
     }
     __syncthreads();
@@ -406,13 +405,11 @@ int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long lon
 template <typename T, int E>
 __global__ void __launch_bounds__(512)
     general_multi_kernel(const MultiGroup *__restrict__ g, int ng, long long tiles) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
   const long long it = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles);
   if (it >= tiles) return;  // whole block, before any barrier
   const int gi = find_multi_group(g, ng, it);
   const AxisIo io = {g[gi].in, g[gi].out, g[gi].conj};
-  axis_tile<T, E, IO_C2C>(g[gi].d, io, it - g[gi].tile0, lds);
+  axis_tile<T, E, IO_C2C>(desc_ptr(g[gi].d), io, it - g[gi].tile0);
 }
 
 int launch_general_multi(int prec_bytes, const MultiGroup *g, int ng, long long tiles,
@@ -459,19 +456,18 @@ __global__ void __launch_bounds__(512)
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int M = d->n, n = d->nblue, TX = d->TX, lg = d->log2TX, tmask = TX - 1;
   const int ldj = d->ldj, ldt = d->ldt, nh = d->nh;
-  const cpx<T> *chirp = reinterpret_cast<const cpx<T> *>(d->chirp);
-  const cpx<T> *bhat = reinterpret_cast<const cpx<T> *>(d->bhat);
+  const void *chirp = d->chirp;
+  const void *bhat = d->bhat;
 
   // line values (conjugated for inverse transforms) into LDS[0, n)
   if constexpr (IO == IO_C2R) {
-    const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in);
     for (int w = tid; w < TX * nh; w += nthr) {
       const int t = w & tmask, k = w >> lg;
       const long long ca = 2 * (c0b + t);
       const long long base = c1 * d->in_s1 + (long long)k * d->in_sj;
       cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
-      if (ca < d->C0r) A = src[base + ca * d->in_s0];
-      if (ca + 1 < d->C0r) B = src[base + (ca + 1) * d->in_s0];
+      if (ca < d->C0r) A = gload<T>(in, base + ca * d->in_s0);
+      if (ca + 1 < d->C0r) B = gload<T>(in, base + (ca + 1) * d->in_s0);
       if (k == 0 || 2 * k == n) {
         A.y = T(0);
         B.y = T(0);
@@ -482,7 +478,7 @@ __global__ void __launch_bounds__(512)
   } else {
     for (int w = tid; w < TX * n; w += nthr) {
       const int t = w & tmask, j = w >> lg;
-      lds[j * ldj + t * ldt] = gload<T, IO>(d, io, c0b + t, c1, j);
+      lds[j * ldj + t * ldt] = pload<T, IO>(desc_ptr(d), io, c0b + t, c1, j);
     }
   }
   __syncthreads();
@@ -490,29 +486,28 @@ __global__ void __launch_bounds__(512)
   for (int w = tid; w < TX * M; w += nthr) {
     const int t = w & tmask, j = w >> lg;
     cpx<T> &e = lds[j * ldj + t * ldt];
-    e = (j < n) ? cmul(e, chirp[j]) : cpx<T>{T(0), T(0)};
+    e = (j < n) ? cmul(e, gload<T>(chirp, j)) : cpx<T>{T(0), T(0)};
   }
   __syncthreads();
-  for (int p = 0; p < d->npass; ++p) dispatch_pass<T, E, IO_C2C>(d, io, p, false, false, lds, c0b, c1);
+  for (int p = 0; p < d->npass; ++p) dispatch_pass<T, E, IO_C2C>(desc_ptr(d), io, p, false, false, c0b, c1);
   // conj(A * W): the inverse DFT as conj(DFT(conj(.)))
   for (int w = tid; w < TX * M; w += nthr) {
     const int t = w & tmask, j = w >> lg;
     cpx<T> &e = lds[j * ldj + t * ldt];
-    e = cconj(cmul(e, bhat[j]));
+    e = cconj(cmul(e, gload<T>(bhat, j)));
   }
   __syncthreads();
-  for (int p = 0; p < d->npass; ++p) dispatch_pass<T, E, IO_C2C>(d, io, p, false, false, lds, c0b, c1);
+  for (int p = 0; p < d->npass; ++p) dispatch_pass<T, E, IO_C2C>(desc_ptr(d), io, p, false, false, c0b, c1);
   // y_k = c_k conj(r_k) / M
   const T invM = T(1) / T(M);
   for (int w = tid; w < TX * n; w += nthr) {
     const int t = w & tmask, k = w >> lg;
     cpx<T> &e = lds[k * ldj + t * ldt];
-    const cpx<T> y = cmul(cconj(e), chirp[k]);
+    const cpx<T> y = cmul(cconj(e), gload<T>(chirp, k));
     e = cpx<T>{y.x * invM, y.y * invM};
   }
   __syncthreads();
   if constexpr (IO == IO_R2C) {
-    cpx<T> *dst = reinterpret_cast<cpx<T> *>(out);
     const T half = T(0.5);
     for (int w = tid; w < TX * nh; w += nthr) {
       const int t = w & tmask, k = w >> lg;
@@ -522,14 +517,14 @@ __global__ void __launch_bounds__(512)
       const int km = (k == 0) ? 0 : n - k;
       const cpx<T> zm = lds[km * ldj + t * ldt];
       const long long base = c1 * d->out_s1 + (long long)k * d->out_sj;
-      dst[base + ca * d->out_s0] = cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half};
+      gstore<T>(out, base + ca * d->out_s0, cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half});
       if (ca + 1 < d->C0r)
-        dst[base + (ca + 1) * d->out_s0] = cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half};
+        gstore<T>(out, base + (ca + 1) * d->out_s0, cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half});
     }
   } else {
     for (int w = tid; w < TX * n; w += nthr) {
       const int t = w & tmask, j = w >> lg;
-      gstore<T, IO>(d, io, c0b + t, c1, j, lds[j * ldj + t * ldt]);
+      pstore<T, IO>(desc_ptr(d), io, c0b + t, c1, j, lds[j * ldj + t * ldt]);
     }
   }
 }

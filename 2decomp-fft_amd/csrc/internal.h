@@ -56,6 +56,13 @@ struct AxisDesc {
   const void *bhat;   // DFT_M of the chirp sequence exp(+i pi m^2 / nblue), wrapped
 };
 
+// A stage descriptor as the kernels read it: constant address space (it is
+// not written while a kernel runs), so its fields are scalar loads even when
+// the pointer itself was loaded from memory (batch and multi-grid launches)
+// or passed to a non-inlined function.
+using DescPtr = const __attribute__((address_space(4))) AxisDesc *;
+__device__ __forceinline__ DescPtr desc_ptr(const AxisDesc *d) { return (DescPtr)d; }
+
 // Per-launch arguments: the descriptor lives in device memory (built once per
 // plan), the data pointers and the direction come with each launch.
 struct AxisIo {

@@ -209,7 +209,7 @@ constexpr int mf_regs() {
 
 // One tile of a stage (tile < tiles0 * C1), by the whole workgroup.
 template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM>
-__device__ __forceinline__ void mf_tile(const AxisDesc *__restrict__ d, const void *in, void *out,
+__device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out,
                                         int conj, long long tile) {
   using T = float;
   using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
@@ -225,7 +225,7 @@ __device__ __forceinline__ void mf_tile(const AxisDesc *__restrict__ d, const vo
   const long long c1 = tile / tiles0;
   const long long c0b = (tile - c1 * tiles0) * TX;
   const int tid = threadIdx.x;
-  const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(d->tw);
+  const void *tw = d->tw;
 
   // column t of the tile: first pencil column, validity and element bases
   auto col = [&](int t, long long &ca, bool &cvalid, long long &ib, long long &ob)
@@ -247,7 +247,6 @@ __device__ __forceinline__ void mf_tile(const AxisDesc *__restrict__ d, const vo
     // conj(Z) of the merged half spectra into LDS, then the inputs of pass 0
     constexpr int NH = N / 2 + 1;
     constexpr int ITER = (TX * NH + NT - 1) / NT;
-    const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in);
     cpx<T> A[ITER], B[ITER];
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
@@ -259,8 +258,8 @@ __device__ __forceinline__ void mf_tile(const AxisDesc *__restrict__ d, const vo
         const int k = LINES ? w % NH : w / TX;
         const long long cc = 2 * (c0b + tt);
         const long long base = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
-        if (cc < d->C0r) A[it] = src[base];
-        if (cc + 1 < d->C0r) B[it] = src[base + d->in_s0];
+        if (cc < d->C0r) A[it] = gload<T>(in, base);
+        if (cc + 1 < d->C0r) B[it] = gload<T>(in, base + d->in_s0);
       }
     }
 #pragma unroll
@@ -369,7 +368,7 @@ __device__ __forceinline__ void mf_tile(const AxisDesc *__restrict__ d, const vo
         int t, b, sp;
         bool ok;
         M::elem(tid, i, t, b, sp, ok);
-        if (ok && sp > 0) v[i] = cmul(v[i], tw[RL::tw_off(p) + (b % L) * (R - 1) + sp - 1]);
+        if (ok && sp > 0) v[i] = cmul(v[i], gload<T>(tw, RL::tw_off(p) + (b % L) * (R - 1) + sp - 1));
       }
     }
     if constexpr (M::MF) {
@@ -440,7 +439,6 @@ __device__ __forceinline__ void mf_tile(const AxisDesc *__restrict__ d, const vo
         }
       }
     }
-    cpx<T> *dst = reinterpret_cast<cpx<T> *>(out);
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
       const int w = tid + it * NT;
@@ -450,8 +448,8 @@ __device__ __forceinline__ void mf_tile(const AxisDesc *__restrict__ d, const vo
         const long long cc = 2 * (c0b + tt);
         if (cc < d->C0r) {
           const long long base = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
-          dst[base] = oa[it];
-          if (cc + 1 < d->C0r) dst[base + d->out_s0] = ob[it];
+          gstore<T>(out, base, oa[it]);
+          if (cc + 1 < d->C0r) gstore<T>(out, base + d->out_s0, ob[it]);
         }
       }
     }
@@ -477,7 +475,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
   const long long ntiles = (long long)d->tiles0 * d->C1;
   const long long tile = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, ntiles);
   if (tile >= ntiles) return;  // whole block, before any barrier
-  mf_tile<N, TX, NT, IO, LINES, RL, XM>(d, in, out, conj, tile);
+  mf_tile<N, TX, NT, IO, LINES, RL, XM>(desc_ptr(d), in, out, conj, tile);
 }
 
 // The same stage kernel over the tiles of several grids.
@@ -487,7 +485,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
   const long long it = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles);
   if (it >= tiles) return;  // whole block, before any barrier
   const int gi = find_multi_group(g, ng, it);
-  mf_tile<N, TX, NT, IO, LINES, RL, XM>(g[gi].d, g[gi].in, g[gi].out, g[gi].conj, it - g[gi].tile0);
+  mf_tile<N, TX, NT, IO, LINES, RL, XM>(desc_ptr(g[gi].d), g[gi].in, g[gi].out, g[gi].conj,
+                                        it - g[gi].tile0);
 }
 
 // ---- registry -----------------------------------------------------------------

@@ -169,16 +169,17 @@ int d2dhip_exec_c2r(d2dhip_plan plan, void *in_c, void *out_r);
  * decomp_2d_fft_use_grid, src/fft_common.f90:381-443, exercised by
  * examples/fft_multiple_grids/fft_multiple_grids.f90) runs one
  * decomp_2d_fft_3d per grid.  A batch runs the c2c stages of many plans (same
- * precision) in one persistent-kernel launch fed by a device work queue: the
- * queue is ordered stage by stage (stage 0 of every grid, then stage 1, ...),
- * and a stage phase starts when every tile of the previous phase has
- * completed (one release/acquire per workgroup and phase).  Calls that
- * overlap on one batch object race: the queue counters belong to the batch.
- * bind lays out the queue for the
- * given device buffers (mode -1 forward, +1 backward, 0 forward then backward
- * in place on out); launch is stream-ordered and capturable; capture_graph
- * records it into a hipGraph replayed by graph_launch; check synchronises and
- * reports a phase-wait timeout (5 s) as D2DHIP_ERR_HIP. */
+ * precision) as stage phases (stage 0 of every grid, then stage 1, ...) with
+ * one of two executors (d2dhip_batch_set_mode below): one launch per phase
+ * and stage kernel (default), or one persistent-kernel launch fed by a
+ * device work queue in which a phase starts when every tile of the previous
+ * phase has completed (one release/acquire per workgroup and phase).  Calls
+ * that overlap on one batch object race: the queue counters belong to the
+ * batch.  bind lays out the phases for the given device buffers (mode -1
+ * forward, +1 backward, 0 forward then backward in place on out); launch is
+ * stream-ordered and capturable; capture_graph records it into a hipGraph
+ * replayed by graph_launch; check synchronises and reports a persistent
+ * phase-wait timeout (5 s) as D2DHIP_ERR_HIP. */
 typedef struct d2dhip_batch_s *d2dhip_batch;
 int d2dhip_batch_create(d2dhip_batch *batch, int nplans, const d2dhip_plan *plans, void *stream);
 int d2dhip_batch_bind_c2c(d2dhip_batch batch, void *const *in, void *const *out, int mode);
@@ -187,18 +188,28 @@ int d2dhip_batch_capture_graph(d2dhip_batch batch);
 int d2dhip_batch_graph_launch(d2dhip_batch batch);
 int d2dhip_batch_check(d2dhip_batch batch);
 int d2dhip_batch_set_stream(d2dhip_batch batch, void *stream);
-/* Executor of a batch: D2DHIP_BATCH_PERSISTENT (default) is the single
- * persistent launch above (general kernel, batch-wide block shape);
- * D2DHIP_BATCH_GROUPED runs, per stage phase, one ordinary launch per distinct
- * stage kernel of the grids' own plans, covering the tiles of every grid that
- * runs that kernel in that phase (the kernel boundaries order the phases; no
- * device queue).  Grouped results are bit-identical to the plans' own
- * d2dhip_fft_3d_c2c calls; it uses the plans' stage kernels as of bind, and
- * launch fails with D2DHIP_ERR_INVALID_VALUE if one has changed since (bind
- * again).  Changing the mode drops a captured graph. */
+/* Executor of a batch: D2DHIP_BATCH_GROUPED (default) runs, per stage phase,
+ * one ordinary launch per distinct stage kernel of the grids' own plans,
+ * covering the tiles of every grid that runs that kernel in that phase (the
+ * kernel boundaries order the phases; no device queue).  Its results are
+ * bit-identical to the plans' own d2dhip_exec_c2c calls; it uses the plans'
+ * stage kernels as of bind, and launch fails with D2DHIP_ERR_INVALID_VALUE if
+ * one has changed since (bind again).  D2DHIP_BATCH_PERSISTENT is the single
+ * persistent launch (general kernel, batch-wide block shape); its results are
+ * bit-identical to the plans' calls with every stage on the general kernel.
+ * Changing the mode drops a captured graph. */
 #define D2DHIP_BATCH_PERSISTENT 0
 #define D2DHIP_BATCH_GROUPED 1
 int d2dhip_batch_set_mode(d2dhip_batch batch, int mode);
+/* Queue order of both executors: consecutive grids whose input and output
+ * buffers total at most `bytes` form a cluster, and a cluster runs all of its
+ * stage phases before the next cluster starts, so that its later stages read
+ * data the earlier ones left in the memory-side cache (256 MiB on MI355X).
+ * 0 = one cluster, every grid's stage k before any grid's stage k+1.
+ * Default 0.  Takes effect at the next bind (the batch is unbound). */
+int d2dhip_batch_set_cluster_bytes(d2dhip_batch batch, long long bytes);
+/* Clusters and stage phases of the current binding. */
+int d2dhip_batch_get_clusters(d2dhip_batch batch, int *nclusters, int *nphases);
 int d2dhip_batch_destroy(d2dhip_batch batch);
 
 /* ---- transposes and alltoallv packing ------------------------------------

@@ -164,3 +164,42 @@ def test_grouped_rejects_changed_plan(d2d, torch_gpu):
     b.bind(ins, outs, mode=-1)
     b.launch()
     b.check()
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("budget", [0, 1, 8 << 20])
+def test_cluster_order(d2d, torch_gpu, mode, budget):
+    """Cluster-major queue order (one grid per cluster, an 8 MiB budget, and
+    the whole batch phase by phase) gives the same bits as the plans' own
+    sequential calls, forward then backward in one launch."""
+    torch = torch_gpu
+    shapes = grids(12, 7, [16, 32, 48, 64, 80, 96])
+    engines = [d2d.FFTEngine(1, *s) for s in shapes]
+    ins, outs, seq = [], [], []
+    for g, s in enumerate(shapes):
+        a = torch.empty(int(np.prod(s)), dtype=torch.complex128, device="cuda")
+        d2d.fill_uniform(a, a.numel(), d2d.C16, 40 + g)
+        ins.append(a)
+        outs.append(torch.empty_like(a))
+        seq.append(torch.empty_like(a))
+    for e, a, q in zip(engines, ins, seq):
+        if mode == 0:
+            for st in ("c2c_x", "c2c_y", "c2c_z"):
+                e.set_stage_kernel(st, "general")
+        e.fft_c2c(a, q, -1)
+        e.fft_c2c(q, q, 1)
+    b = d2d.Batch(engines)
+    b.set_mode(mode)
+    b.set_cluster_bytes(budget)
+    b.bind(ins, outs, mode=0)
+    ncl, nph = b.clusters()
+    if budget == 1:
+        assert ncl == len(shapes) and nph == 6 * len(shapes)
+    elif budget == 0:
+        assert ncl == 1 and nph == 6
+    else:
+        assert 1 < ncl < len(shapes)
+    b.launch()
+    b.check()
+    for o, q in zip(outs, seq):
+        assert torch.equal(o, q)
