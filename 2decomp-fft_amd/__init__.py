@@ -28,6 +28,7 @@ PREC_F32 = 32
 PREC_SPLIT_BF16 = 16
 BATCH_PERSISTENT = 0
 BATCH_GROUPED = 1
+BATCH_PERSISTENT_GENERAL = 2
 FLAG_INPLACE = 0x1
 FLAG_SKIP_X = 0x2
 FLAG_SKIP_Y = 0x4
@@ -132,6 +133,8 @@ def lib():
             "d2dhip_batch_set_mode": ([vp, ci], ci),
             "d2dhip_batch_set_cluster_bytes": ([vp, cll], ci),
             "d2dhip_batch_get_clusters": ([vp, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
+            "d2dhip_batch_persistent_tiles": ([vp, ctypes.POINTER(ci)], ci),
+            "d2dhip_batch_set_item_points": ([vp, cll], ci),
             "d2dhip_batch_destroy": ([vp], ci),
             "d2dhip_pack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
             "d2dhip_unpack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
@@ -349,7 +352,10 @@ class Batch:
 
     def set_mode(self, mode):
         """BATCH_GROUPED (default: one launch per stage phase and stage kernel,
-        the plans' own kernels) or BATCH_PERSISTENT (one persistent launch)."""
+        the plans' own kernels), BATCH_PERSISTENT (one persistent launch, on
+        specialised tiles where every stage has one) or
+        BATCH_PERSISTENT_GENERAL (one persistent launch on the general
+        kernel's tiles)."""
         check(lib().d2dhip_batch_set_mode(self._b, int(mode)))
 
     def set_cluster_bytes(self, nbytes):
@@ -357,6 +363,17 @@ class Batch:
         stages before the next (0 = the whole batch phase by phase); takes
         effect at the next bind."""
         check(lib().d2dhip_batch_set_cluster_bytes(self._b, int(nbytes)))
+
+    def set_item_points(self, points):
+        """Complex points per queue item of the persistent executors (whole
+        tiles); takes effect at the next bind."""
+        check(lib().d2dhip_batch_set_item_points(self._b, int(points)))
+
+    def persistent_tiles(self):
+        """True if BATCH_PERSISTENT runs specialised tiles for this batch."""
+        v = ctypes.c_int()
+        check(lib().d2dhip_batch_persistent_tiles(self._b, ctypes.byref(v)))
+        return bool(v.value)
 
     def clusters(self):
         """(clusters, stage phases) of the current binding."""

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -29,6 +30,19 @@ struct FastVariant {
 
 // All specialised variants compiled into the library.
 const std::vector<FastVariant> &fast_registry();
+
+// The persistent batch kernel on specialised tiles: one-wave c2c tile
+// functions, one per (length, contiguous lines or strided columns); a queue
+// group's kind is its index here.  name: the registered FastVariant with
+// the same kernel body (its plan descriptor fits the persistent kernel).
+struct PersistKind {
+  int n = 0, TX = 1, lines = 0;
+  size_t lds_bytes = 0;
+  std::string name;
+};
+const std::vector<PersistKind> &persist_kinds(int rb);
+int batch_fast_prepare(int rb, size_t lds, int *blocks_per_cu);
+int launch_batch_fast(int rb, const BatchQueue &q, int blocks, size_t lds, hipStream_t st);
 // The split-BF16 matrix-core variants (mfma_kernels.hip).
 std::vector<FastVariant> mf_variants();
 

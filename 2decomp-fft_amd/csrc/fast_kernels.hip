@@ -14,6 +14,10 @@
 // one line (b fastest: 64 consecutive points per wave).
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <string>
+
+#include "batch_loop.h"
 #include "codelets.h"
 #include "fast_common.h"
 #include "fast_kernels.h"
@@ -414,6 +418,181 @@ using R12_8 = RList<12, 8>;           // 96, 4 rows
 using R4_4_10 = RList<4, 4, 10>;      // 160, 8 rows
 using R4_4_12 = RList<4, 4, 12>;      // 192, 16 rows (or 8)
 
+// c2c only, plain exchange (the one-wave tiles of the persistent batch kernel)
+#define D2D_C2C(T, N, TX, NT, LINES, RL) \
+  make_variant<T, N, TX, NT, IO_C2C, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " c2c")
+
+// ---- persistent batch kernel on specialised tiles ---------------------------------
+// c2c tiles for the lengths of uneven-grid batches, on strided columns and
+// contiguous lines, all of one block size per precision.  Every kind has
+// the radix list of the registered variants of its length, so its results
+// equal the plans' own executions on any specialised kernel bit for bit.
+//   FP64: 2-wave blocks, tiles of up to 35 KB (4 blocks, 8 waves per CU), at
+//         least 8 columns (128-byte rows) on strided axes;
+//   FP32: 1-wave blocks, tiles of up to 17 KB (9 blocks per CU), at least 8
+//         columns (64-byte rows, 128 for most lengths).
+// Some kinds leave lanes idle in a pass (N / r not a multiple of the rows
+// per column).
+template <int N_, int TX_, bool LINES_, class RL_>
+struct PV {
+  static constexpr int N = N_, TX = TX_;
+  static constexpr bool LINES = LINES_;
+  using RL = RL_;
+};
+template <int NT_, int WPE_, class... Vs>
+struct PSet {
+  static constexpr int NT = NT_, WPE = WPE_;
+  static constexpr int size = sizeof...(Vs);
+};
+using PersistSetD =
+    PSet<128, 2, PV<256, 8, false, R16_16>, PV<256, 8, true, R16_16>, PV<192, 8, false, R4_4_12>,
+         PV<192, 8, true, R4_4_12>, PV<160, 8, false, R4_4_10>, PV<160, 8, true, R4_4_10>,
+         PV<128, 16, false, R16_8>, PV<128, 16, true, R16_8>, PV<96, 16, false, R12_8>,
+         PV<96, 16, true, R12_8>, PV<80, 16, false, R4_4_5>, PV<80, 16, true, R4_4_5>,
+         PV<64, 16, false, R8_8>, PV<64, 16, true, R8_8>, PV<48, 32, false, R12_4>,
+         PV<48, 32, true, R12_4>, PV<32, 32, false, R8_4>, PV<32, 32, true, R8_4>>;
+using PersistSetF =
+    PSet<64, 3, PV<256, 8, false, R16_16>, PV<256, 8, true, R16_16>, PV<192, 8, false, R4_4_12>,
+         PV<192, 8, true, R4_4_12>, PV<160, 8, false, R4_4_10>, PV<160, 8, true, R4_4_10>,
+         PV<128, 16, false, R16_8>, PV<128, 16, true, R16_8>, PV<96, 16, false, R12_8>,
+         PV<96, 16, true, R12_8>, PV<80, 16, false, R4_4_5>, PV<80, 16, true, R4_4_5>,
+         PV<64, 16, false, R8_8>, PV<64, 16, true, R8_8>, PV<48, 32, false, R12_4>,
+         PV<48, 32, true, R12_4>, PV<32, 32, false, R8_4>, PV<32, 32, true, R8_4>>;
+template <typename T>
+struct PersistSetOf {
+  using type = PersistSetD;
+};
+template <>
+struct PersistSetOf<float> {
+  using type = PersistSetF;
+};
+
+template <int K, class V, class... Vs>
+struct PAt {
+  using type = typename PAt<K - 1, Vs...>::type;
+};
+template <class V, class... Vs>
+struct PAt<0, V, Vs...> {
+  using type = V;
+};
+
+// One kind's tile as a function of its own, so that its registers are
+// allocated for its body alone (inlined side by side, the bodies of all
+// kinds share one allocation and spill).  Function arguments arrive in
+// vector registers; they are wave-uniform, and re-deriving them with
+// readfirstlane lets the body keep the descriptor and its index arithmetic
+// in scalar registers, as a kernel does with its arguments.
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// When head is not null, thread 0 also takes the next position of that
+// queue counter (after the call's entry wait, so that the atomic's latency
+// overlaps the tile's first loads) and returns it.
+template <typename T, int NT, class V>
+__device__ __noinline__ unsigned long long persist_kind_tile(DescPtr d, const void *in, void *out,
+                                                             int conj, long long tile,
+                                                             unsigned long long *head) {
+  unsigned long long *uhead = (unsigned long long *)uniform_u64((unsigned long long)head);
+  unsigned long long r = 0;
+  if (uhead && threadIdx.x == 0) r = atomicAdd(uhead, 1ull);
+  const DescPtr ud = (DescPtr)uniform_u64((unsigned long long)d);
+  const void *uin = (const void *)uniform_u64((unsigned long long)in);
+  void *uout = (void *)uniform_u64((unsigned long long)out);
+  const int uconj = __builtin_amdgcn_readfirstlane(conj);
+  const long long utile = (long long)uniform_u64((unsigned long long)tile);
+  fast_tile<T, V::N, V::TX, NT, IO_C2C, V::LINES, typename V::RL, 0>(ud, uin, uout, uconj, utile);
+  return r;
+}
+
+// Tile of kind `kind` (wave-uniform): a chain of uniform branches.
+template <typename T, int NT, int K, class... Vs>
+__device__ __forceinline__ unsigned long long persist_tile(int kind, DescPtr d, const void *in,
+                                                           void *out, int conj, long long tile,
+                                                           unsigned long long *head) {
+  if constexpr (K < (int)sizeof...(Vs)) {
+    using V = typename PAt<K, Vs...>::type;
+    if (kind == K) return persist_kind_tile<T, NT, V>(d, in, out, conj, tile, head);
+    return persist_tile<T, NT, K + 1, Vs...>(kind, d, in, out, conj, tile, head);
+  }
+  return 0;
+}
+
+template <typename T, int NT, int WPE, class... Vs>
+__device__ __forceinline__ unsigned long long persist_run(const BatchGroup &g, long long tile,
+                                                          unsigned long long *head,
+                                                          PSet<NT, WPE, Vs...>) {
+  const int kind = __builtin_amdgcn_readfirstlane(g.kind);
+  return persist_tile<T, NT, 0, Vs...>(kind, desc_ptr(g.d), g.in, g.out, g.conj, tile, head);
+}
+
+// The waves-per-SIMD bound also holds for the per-kind functions it calls,
+// which would otherwise take up to 256 registers each.
+template <typename T>
+__global__ void __launch_bounds__(PersistSetOf<T>::type::NT)
+    __attribute__((amdgpu_waves_per_eu(PersistSetOf<T>::type::WPE)))
+    batch_fast_kernel(const BatchQueue q) {
+  batch_loop(q, [&](const BatchGroup &g, long long tile, unsigned long long *head)
+                    __attribute__((always_inline)) {
+    return persist_run<T>(g, tile, head, typename PersistSetOf<T>::type());
+  });
+}
+
+template <typename T, int NT, int WPE, class... Vs>
+static std::vector<PersistKind> persist_table(PSet<NT, WPE, Vs...>) {
+  const char *tname = sizeof(T) == 8 ? "double" : "float";
+  std::vector<PersistKind> v;
+  auto add = [&](int n, int tx, bool lines, size_t lds) {
+    char b[96];
+    std::snprintf(b, sizeof(b), "%s n=%d tx=%d nt=%d c2c", tname, n, tx, NT);
+    PersistKind k;
+    k.name = b;
+    k.n = n;
+    k.TX = tx;
+    k.lines = lines ? 1 : 0;
+    k.lds_bytes = lds;
+    v.push_back(k);
+  };
+  (add(Vs::N, Vs::TX, Vs::LINES, FastCfg<T, Vs::N, Vs::TX, NT, Vs::LINES, typename Vs::RL>::lds_bytes()), ...);
+  return v;
+}
+
+const std::vector<PersistKind> &persist_kinds(int rb) {
+  static const std::vector<PersistKind> d = persist_table<double>(PersistSetD());
+  static const std::vector<PersistKind> f = persist_table<float>(PersistSetF());
+  return rb == 8 ? d : f;
+}
+
+int batch_fast_prepare(int rb, size_t lds, int *blocks_per_cu) {
+  hipError_t e;
+  if (rb == 8) {
+    e = hipFuncSetAttribute((const void *)batch_fast_kernel<double>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_fast_kernel<double>,
+                                                       PersistSetD::NT, lds);
+  } else {
+    e = hipFuncSetAttribute((const void *)batch_fast_kernel<float>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_fast_kernel<float>,
+                                                       PersistSetF::NT, lds);
+  }
+  return hip_check(e, "persistent batch kernel set-up");
+}
+
+int launch_batch_fast(int rb, const BatchQueue &q, int blocks, size_t lds, hipStream_t st) {
+  const int rst = launch_batch_reset(q, st);
+  if (rst) return rst;
+  if (rb == 8)
+    hipLaunchKernelGGL(batch_fast_kernel<double>, dim3(blocks), dim3(PersistSetD::NT), lds, st, q);
+  else
+    hipLaunchKernelGGL(batch_fast_kernel<float>, dim3(blocks), dim3(PersistSetF::NT), lds, st, q);
+  return hip_check(hipGetLastError(), "persistent batch kernel launch");
+}
+
 const std::vector<FastVariant> &fast_registry() {
   static const std::vector<FastVariant> reg = [] {
     std::vector<FastVariant> r = {
@@ -485,6 +664,19 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(double, 192, 8, 64, false, R4_4_12),
       D2D_VARIANTS(double, 192, 4, 64, true, R4_4_12),
       D2D_VARIANTS(double, 192, 8, 64, true, R4_4_12),
+      // tiles of the persistent batch kernel not registered above
+      D2D_C2C(double, 192, 8, 128, true, R4_4_12),
+      D2D_C2C(double, 160, 8, 128, false, R4_4_10),
+      D2D_C2C(double, 160, 8, 128, true, R4_4_10),
+      D2D_C2C(double, 128, 16, 128, true, R16_8),
+      D2D_C2C(double, 96, 16, 128, false, R12_8),
+      D2D_C2C(double, 96, 16, 128, true, R12_8),
+      D2D_C2C(double, 80, 16, 128, false, R4_4_5),
+      D2D_C2C(double, 80, 16, 128, true, R4_4_5),
+      D2D_C2C(double, 64, 16, 128, false, R8_8),
+      D2D_C2C(double, 48, 32, 128, false, R12_4),
+      D2D_C2C(double, 48, 32, 128, true, R12_4),
+      D2D_C2C(double, 32, 32, 128, false, R8_4),
       D2D_VARIANTS(float, 256, 16, 256, false, R16_16),
       D2D_VARIANTS(float, 256, 4, 64, true, R16_16),
       D2D_VARIANTS(float, 128, 16, 128, false, R16_8),
@@ -503,6 +695,19 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(float, 160, 8, 64, true, R4_4_10),
       D2D_VARIANTS(float, 192, 8, 128, false, R4_4_12),
       D2D_VARIANTS(float, 192, 4, 64, true, R4_4_12),
+      D2D_C2C(float, 256, 8, 64, false, R16_16),
+      D2D_C2C(float, 256, 8, 64, true, R16_16),
+      D2D_C2C(float, 192, 8, 64, false, R4_4_12),
+      D2D_C2C(float, 192, 8, 64, true, R4_4_12),
+      D2D_C2C(float, 160, 8, 64, false, R4_4_10),
+      D2D_C2C(float, 128, 16, 64, false, R16_8),
+      D2D_C2C(float, 128, 16, 64, true, R16_8),
+      D2D_C2C(float, 64, 16, 64, false, R8_8),
+      D2D_C2C(float, 64, 16, 64, true, R8_8),
+      D2D_C2C(float, 48, 32, 64, false, R12_4),
+      D2D_C2C(float, 48, 32, 64, true, R12_4),
+      D2D_C2C(float, 32, 32, 64, false, R8_4),
+      D2D_C2C(float, 32, 32, 64, true, R8_4),
     };
     for (const FastVariant &v : mf_variants()) r.push_back(v);
     return r;

@@ -24,6 +24,7 @@
 // register arrays are allocated on their own.
 #include <hip/hip_runtime.h>
 
+#include "batch_loop.h"
 #include "codelets.h"
 #include "internal.h"
 
@@ -242,120 +243,25 @@ __global__ void __launch_bounds__(512)
 }
 
 // ---- many grids in one launch ---------------------------------------------------
-// A persistent kernel pulls work items from a device queue: item i belongs
-// to the BatchGroup g (one stage of one grid) with groups[g].first <= i <
-// groups[g].first + ceil(ntiles / per_item), and covers up to per_item
-// tiles of it, so that the queue, barrier and descriptor costs of an item
-// are spread over enough work.  Groups are laid out phase-major
-// (stage 0 of every grid, then stage 1, ...), and phase p starts only when
-// every item of phase p-1 has completed.
-//
-// The synchronisation is per block and per phase, not per item: a block
-// counts the items it completes in its current phase and publishes the count
-// with one agent-scope release (done[phase] += count) when it dequeues an
-// item of a later phase or finds the queue empty; before its first item of
-// phase p it polls done[p-1] with relaxed agent-scope loads until it equals
-// the phase's item count, then takes one agent-scope acquire.  Each gfx950
-// XCD has its own L2, so the release writes back that L2 and the acquire
-// invalidates it; doing both once per phase instead of once per tile (and
-// never inside the poll loop) keeps the L2 warm for the blocks still working.
-//
-// Progress: items are dequeued in order, so when a block holds an item of
-// phase p, every item of phase p-1 has been taken by a block that is
-// running; such a block publishes its count before it waits for anything,
-// so the wait ends.  A limit on the wall clock (s_memrealtime ticks) turns a
-// broken schedule into an error flag instead of a hang.
-//
-// The loop control (item index, phase, wait outcome) is read back from LDS
-// and made wave-uniform with readfirstlane, so every barrier stays in
-// uniform control flow.
-__device__ __forceinline__ long long uniform_ll(long long v) {
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(v & 0xffffffffll));
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
-  return (long long)(((unsigned long long)hi << 32) | lo);
-}
-
+// The persistent batch kernel on the general kernel's tile function (the
+// queue and phase protocol: batch_loop.h).
 template <typename T, int E>
-__global__ void __launch_bounds__(512)
-    batch_kernel(const BatchGroup *__restrict__ groups, int ngroups, long long total,
-                 const unsigned long long *__restrict__ phase_items, int nphase,
-                 unsigned long long *head, unsigned long long *done, unsigned int *err,
-                 long long wait_ticks) {
-  __shared__ long long s_item;
-  __shared__ int s_fail;
-  const int tid = threadIdx.x;
-  int phase = 0;                // phase of the items counted in `held`
-  unsigned long long held = 0;  // items of `phase` completed here, not yet published
-  int fail = 0;
-  for (;;) {
-    if (tid == 0) s_item = (long long)atomicAdd(head, 1ull);
-    __syncthreads();
-    const long long it = uniform_ll(s_item);
-    int gi = 0, ph = nphase;
-    if (it < total) {
-      int hi = ngroups - 1;
-      while (gi < hi) {
-        const int mid = (gi + hi + 1) >> 1;
-        if (groups[mid].first <= it) gi = mid; else hi = mid - 1;
-      }
-      gi = __builtin_amdgcn_readfirstlane(gi);
-      ph = __builtin_amdgcn_readfirstlane(groups[gi].phase);
-    }
-    if (ph != phase) {
-      if (tid == 0) {
-        s_fail = 0;
-        if (held)
-          __hip_atomic_fetch_add(&done[phase], held, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (ph < nphase && ph > 0) {
-          const unsigned long long need = phase_items[ph - 1];
-          const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
-          // relaxed agent-scope loads see the other XCDs' released counts
-          // without invalidating this XCD's L2 on every poll; the acquire
-          // fence after the barrier below orders the phase's data
-          while (__hip_atomic_load(&done[ph - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-            __builtin_amdgcn_s_sleep(8);
-            if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks) {
-              atomicOr(err, 1u);
-              s_fail = 1;
-              break;
-            }
-          }
-        }
-      }
-      __syncthreads();
-      if (ph < nphase) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      fail = __builtin_amdgcn_readfirstlane(s_fail);
-      phase = ph;
-      held = 0;
-    }
-    if (it >= total) break;
-    if (!fail) {
-      const BatchGroup &g = groups[gi];
-      const AxisIo io = {g.in, g.out, g.conj};
-      // tiles i, i + ni, i + 2 ni, ... of the group (ni items in it), so
-      // that the items running at one time cover neighbouring tiles
-      const long long nt = uniform_ll(g.ntiles);
-      const long long ni = uniform_ll((nt + g.per_item - 1) / g.per_item);
-      const long long i = uniform_ll(it - g.first);
-      for (long long tile = i; tile < nt; tile += ni) {
-        if (tile > i) __syncthreads();  // LDS reuse across tiles
-        axis_tile<T, E, IO_C2C>(desc_ptr(g.d), io, tile);
-      }
-    }
-    __syncthreads();
-    ++held;
-  }
+__global__ void __launch_bounds__(512) batch_kernel(const BatchQueue q) {
+  batch_loop(q, [&](const BatchGroup &g, long long tile, unsigned long long *head)
+                    __attribute__((always_inline)) {
+    unsigned long long r = 0;
+    if (head && threadIdx.x == 0) r = atomicAdd(head, 1ull);
+    const AxisIo io = {g.in, g.out, g.conj};
+    axis_tile<T, E, IO_C2C>(desc_ptr(g.d), io, tile);
+    return r;
+  });
 }
 
-template <typename T>
-__global__ void batch_reset_kernel(unsigned long long *head, unsigned long long *done, int nphase,
-                                   unsigned int *err) {
+__global__ void batch_reset_kernel(const BatchQueue q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nphase) done[i] = 0ull;
-  if (i == 0) {
-    *head = 0ull;
-    *err = 0u;
-  }
+  if (i < q.nphase * kBatchQueues) q.done[(long long)i * kHeadStride] = 0ull;
+  if (i < kBatchQueues) q.heads[i * kHeadStride] = 0ull;
+  if (i == 0) *q.err = 0u;
 }
 
 int batch_prepare(int prec_bytes, int nthreads, size_t lds, int *blocks_per_cu) {
@@ -376,22 +282,20 @@ int batch_prepare(int prec_bytes, int nthreads, size_t lds, int *blocks_per_cu) 
   return hip_check(e, "batch kernel set-up");
 }
 
-int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long long total,
-                 const unsigned long long *phase_items, int nphase, unsigned long long *head,
-                 unsigned long long *done, unsigned int *err, long long wait_ticks, int blocks,
-                 int nthreads, size_t lds, hipStream_t st) {
-  const int rb = (nphase + 255) / 256;
-  if (prec_bytes == 8) {
-    hipLaunchKernelGGL(batch_reset_kernel<double>, dim3(rb > 0 ? rb : 1), dim3(256), 0, st, head,
-                       done, nphase, err);
-    hipLaunchKernelGGL((batch_kernel<double, kRegCap>), dim3(blocks), dim3(nthreads), lds, st,
-                       groups, ngroups, total, phase_items, nphase, head, done, err, wait_ticks);
-  } else {
-    hipLaunchKernelGGL(batch_reset_kernel<float>, dim3(rb > 0 ? rb : 1), dim3(256), 0, st, head,
-                       done, nphase, err);
-    hipLaunchKernelGGL((batch_kernel<float, kRegCap>), dim3(blocks), dim3(nthreads), lds, st,
-                       groups, ngroups, total, phase_items, nphase, head, done, err, wait_ticks);
-  }
+int launch_batch_reset(const BatchQueue &q, hipStream_t st) {
+  const int n = q.nphase * kBatchQueues > kBatchQueues ? q.nphase * kBatchQueues : kBatchQueues;
+  hipLaunchKernelGGL(batch_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, st, q);
+  return hip_check(hipGetLastError(), "batch reset launch");
+}
+
+int launch_batch(int prec_bytes, const BatchQueue &q, int blocks, int nthreads, size_t lds,
+                 hipStream_t st) {
+  const int rst = launch_batch_reset(q, st);
+  if (rst) return rst;
+  if (prec_bytes == 8)
+    hipLaunchKernelGGL((batch_kernel<double, kRegCap>), dim3(blocks), dim3(nthreads), lds, st, q);
+  else
+    hipLaunchKernelGGL((batch_kernel<float, kRegCap>), dim3(blocks), dim3(nthreads), lds, st, q);
   return hip_check(hipGetLastError(), "batch kernel launch");
 }
 

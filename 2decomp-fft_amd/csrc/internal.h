@@ -83,6 +83,34 @@ struct BatchGroup {
   long long ntiles;
   long long first;    // index of its first queue item (items: ceil(ntiles / per_item))
   long long tile0;    // index of its first tile among all tiles of its phase
+  int kind;           // specialised tile function (persistent kernel on specialised tiles)
+};
+
+// The device queues of a persistent batch launch.  Each phase's items are
+// split into kBatchQueues contiguous shares, one per queue; block b takes
+// items from queue b % kBatchQueues (blocks are placed on the 8 XCDs round
+// robin, so each XCD mostly works its own queue and its own counter).
+constexpr int kBatchQueues = 8;
+constexpr int kHeadStride = 32;  // u64 between queue counters (256 bytes)
+struct QueueEntry {
+  int group;  // BatchGroup index
+  int local;  // item within the group: tiles [local * per_item, ...)
+  int phase;
+  int pad;
+};
+struct BatchQueue {
+  const BatchGroup *groups;
+  const QueueEntry *entries;  // queue q: entries[off[q] .. off[q] + len[q])
+  long long off[kBatchQueues];
+  long long len[kBatchQueues];
+  // items of phase p in queue q: phase_items[p * kBatchQueues + q]
+  const unsigned long long *phase_items;
+  int nphase;
+  unsigned long long *heads;  // kBatchQueues counters, kHeadStride apart
+  // completed items of phase p in queue q: done[(p * kBatchQueues + q) * kHeadStride]
+  unsigned long long *done;
+  unsigned int *err;
+  long long wait_ticks;
 };
 
 // Launch shape chosen by the planner for one axis stage.
@@ -125,10 +153,11 @@ int launch_axis(int prec_bytes, int io, const AxisDesc *d_desc, const AxisDesc &
                 const AxisIo &io_ptrs, const AxisShape &s, hipStream_t st);
 int axis_kernel_prepare(int prec_bytes);  // raise LDS limits once
 int batch_prepare(int prec_bytes, int nthreads, size_t lds, int *blocks_per_cu);
-int launch_batch(int prec_bytes, const BatchGroup *groups, int ngroups, long long total,
-                 const unsigned long long *phase_items, int nphase, unsigned long long *head,
-                 unsigned long long *done, unsigned int *err, long long wait_ticks, int blocks,
-                 int nthreads, size_t lds, hipStream_t st);
+// Clears the queue counters, the per-phase counters and the error flag (a
+// kernel, not a memset, so that a captured graph replays it).
+int launch_batch_reset(const BatchQueue &q, hipStream_t st);
+int launch_batch(int prec_bytes, const BatchQueue &q, int blocks, int nthreads, size_t lds,
+                 hipStream_t st);
 
 // One grid's stage in a launch that covers several grids: its descriptor
 // and buffers, and the index of its first tile among the launch's tiles.

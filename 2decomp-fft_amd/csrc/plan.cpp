@@ -1182,6 +1182,9 @@ constexpr long long kBatchItemPoints = 2048;
 // 5.09 ms, 192 MiB clusters 5.28 ms, 64 MiB 5.41 ms, one grid per cluster
 // 5.56 ms -- fewer, larger launches gain more than the cache reuse.
 constexpr long long kBatchClusterBytes = 0;
+// Consecutive items a persistent queue takes at a time (neighbouring tiles
+// of strided columns share cache lines, and one queue serves one XCD).
+constexpr size_t kBatchQueueChunk = 4;
 
 struct d2dhip_batch_s {
   int rb = 8;
@@ -1189,6 +1192,23 @@ struct d2dhip_batch_s {
   std::vector<d2dhip_plan_s *> plans;
   AxisDesc *d_descs = nullptr;   // 3 general-kernel c2c descriptors per plan (x, y, z)
   std::vector<AxisDesc> h_descs;
+  // The persistent kernel on specialised one-wave tiles (when every c2c
+  // stage of every plan has one): 3 descriptors and tile kinds per plan.
+  bool fast = false;
+  AxisDesc *d_fdescs = nullptr;
+  std::vector<AxisDesc> h_fdescs;
+  std::vector<int> kinds;
+  size_t flds = 0;
+  int fblocks = 0;
+  BatchGroup *d_fgroups = nullptr;
+  unsigned long long *d_fphase_items = nullptr;
+  int nfgroups = 0;
+  long long ftotal = 0;
+  // the queues (general / specialised groups): entries and each queue's share
+  QueueEntry *d_entries = nullptr, *d_fentries = nullptr;
+  long long qoff[kBatchQueues] = {}, qlen[kBatchQueues] = {};
+  long long fqoff[kBatchQueues] = {}, fqlen[kBatchQueues] = {};
+  long long item_points = kBatchItemPoints;  // complex points per queue item
   BatchGroup *d_groups = nullptr;
   int ngroups = 0;
   long long total = 0;
@@ -1283,7 +1303,8 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
     b->rb = p->rb;
     b->plans.push_back(p);
     for (int a = 0; a < 3 && !st; ++a) {
-      // the batched kernel runs the general kernel: configure it, copy, restore
+      // the general persistent kernel: configure the plan's stage for the
+      // general kernel, copy the descriptor, restore the plan's choice
       const int keep = p->geom[a].choice;
       st = configure(p, a, -1);
       if (!st) {
@@ -1292,6 +1313,42 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
       }
       if (!st) st = configure(p, a, keep);
     }
+  }
+  // the persistent kernel on specialised tiles: the plan's candidate with
+  // the kind's kernel body, configured the same way
+  b->fast = !st;
+  const std::vector<PersistKind> &pk = persist_kinds(b->rb);
+  for (size_t i = 0; i < b->plans.size() && !st && b->fast; ++i) {
+    d2dhip_plan_s *p = b->plans[i];
+    for (int a = 0; a < 3 && !st && b->fast; ++a) {
+      const StageGeom &g = p->geom[a];
+      int kind = -1, cand = -1;
+      for (size_t k = 0; k < pk.size() && kind < 0; ++k)
+        if (pk[k].n == g.n && pk[k].lines == (a == 0 ? 1 : 0)) kind = (int)k;
+      if (kind >= 0)
+        for (size_t c = 0; c < g.cands.size() && cand < 0; ++c)
+          if (pk[kind].name == g.cands[c]->name && g.cands[c]->lines == pk[kind].lines) cand = (int)c;
+      if (cand < 0) {
+        b->fast = false;
+        break;
+      }
+      const int keep = g.choice;
+      st = configure(p, a, cand);
+      if (!st) b->h_fdescs.push_back(p->h_desc[a]);
+      if (!st) st = configure(p, a, keep);
+      b->kinds.push_back(kind);
+      b->flds = std::max(b->flds, pk[kind].lds_bytes);
+    }
+  }
+  if (!st && b->fast) {
+    const size_t bytes = sizeof(AxisDesc) * b->h_fdescs.size();
+    if (hipMalloc(&b->d_fdescs, bytes) != hipSuccess) st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch)");
+    if (!st) st = hip_check(hipMemcpy(b->d_fdescs, b->h_fdescs.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    int bpc = 0, dev = 0, ncu = 0;
+    if (!st) st = batch_fast_prepare(b->rb, b->flds, &bpc);
+    if (!st) st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (!st) st = hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    b->fblocks = std::max(1, bpc) * std::max(1, ncu);
   }
   if (!st) {
     // Block shape of the batch: the general kernel runs at most 8 waves per
@@ -1336,7 +1393,7 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
     if (!st) st = hip_check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
     b->wait_ticks = 5000LL * (long long)std::max(khz, 1000);  // 5 s
   }
-  if (!st && (hipMalloc(&b->d_head, sizeof(unsigned long long)) != hipSuccess ||
+  if (!st && (hipMalloc(&b->d_head, sizeof(unsigned long long) * kBatchQueues * kHeadStride) != hipSuccess ||
               hipMalloc(&b->d_err, sizeof(unsigned int)) != hipSuccess))
     st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch counters)");
   if (st) {
@@ -1377,6 +1434,9 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   const int ncl = (int)cl_begin.size() - 1;
   long long first = 0;
   std::vector<char> started(np, 0);
+  // the same queue for the persistent kernel on specialised tiles
+  std::vector<BatchGroup> fgroups;
+  long long ffirst = 0;
   std::vector<int> ph_g0, ph_g1;
   std::vector<long long> ph_tiles;
   std::vector<std::vector<d2dhip_batch_s::Part>> parts;
@@ -1384,8 +1444,8 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   size_t general_lds = 0;
   for (int cph = 0; cph < ncl * nph; ++cph) {
     const int cl = cph / nph, ph = cph % nph;
-    unsigned long long items = 0;
-    long long tiles = 0;
+    unsigned long long items = 0, fitems = 0;
+    long long tiles = 0, ftiles = 0;
     const int g_begin = (int)groups.size();
     // grouped executor: grids of this phase by their plan's stage kernel
     std::vector<const FastVariant *> keys;
@@ -1400,24 +1460,39 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
       const int ax = (k == 1) ? 1 : ((k == 0) == x_first ? 0 : 2);
       const unsigned skipbit[3] = {D2DHIP_FLAG_SKIP_X, D2DHIP_FLAG_SKIP_Y, D2DHIP_FLAG_SKIP_Z};
       if (p->flags & skipbit[ax]) continue;
-      BatchGroup G;
-      G.d = b->d_descs + 3 * g + ax;
-      G.in = started[g] ? out[g] : in[g];
-      G.out = out[g];
-      G.conj = isign == 1 ? 1 : 0;
-      G.phase = (int)phase_items.size();
-      const AxisDesc &h = b->h_descs[3 * g + ax];
-      G.ntiles = (long long)h.tiles0 * h.C1;
-      // about kBatchItemPoints complex points per queue item
-      const long long tile_pts = (long long)h.TX * h.n;
-      G.per_item = (int)std::max<long long>(1, std::min<long long>(64, kBatchItemPoints / tile_pts));
-      const long long nitems = (G.ntiles + G.per_item - 1) / G.per_item;
-      G.first = first;
-      G.tile0 = tiles;
-      first += nitems;
-      tiles += G.ntiles;
-      items += (unsigned long long)nitems;
+      // a queue group: all tiles of this stage of grid g, about
+      // kBatchItemPoints complex points per queue item
+      auto make_group = [&](const AxisDesc *dd, const AxisDesc &h, int kind, long long &fst,
+                            long long &tl, unsigned long long &its) {
+        BatchGroup G;
+        G.d = dd;
+        G.in = started[g] ? out[g] : in[g];
+        G.out = out[g];
+        G.conj = isign == 1 ? 1 : 0;
+        G.phase = (int)phase_items.size();
+        G.kind = kind;
+        G.ntiles = (long long)h.tiles0 * h.C1;
+        const long long tile_pts = (long long)h.TX * h.n;
+        G.per_item = (int)std::max<long long>(1, std::min<long long>(64, b->item_points / tile_pts));
+        // strided columns: at least one 128-byte line of each row per item
+        if (h.ldt == 1) {
+          const long long row_bytes = (long long)h.TX * 2 * b->rb;
+          G.per_item = (int)std::max<long long>(G.per_item, (128 + row_bytes - 1) / row_bytes);
+        }
+        const long long nitems = (G.ntiles + G.per_item - 1) / G.per_item;
+        G.first = fst;
+        G.tile0 = tl;
+        fst += nitems;
+        tl += G.ntiles;
+        its += (unsigned long long)nitems;
+        return G;
+      };
+      const BatchGroup G = make_group(b->d_descs + 3 * g + ax, b->h_descs[3 * g + ax], 0, first,
+                                      tiles, items);
       groups.push_back(G);
+      if (b->fast)
+        fgroups.push_back(make_group(b->d_fdescs + 3 * g + ax, b->h_fdescs[3 * g + ax],
+                                     b->kinds[3 * g + ax], ffirst, ftiles, fitems));
       // the same stage as the plan itself runs it
       const FastVariant *fv = p->fast[ax];
       size_t q = 0;
@@ -1462,10 +1537,20 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   if (b->d_done) (void)hipFree(b->d_done);
   if (b->d_phase_items) (void)hipFree(b->d_phase_items);
   if (b->d_multi) (void)hipFree(b->d_multi);
+  if (b->d_fgroups) (void)hipFree(b->d_fgroups);
+  if (b->d_fphase_items) (void)hipFree(b->d_fphase_items);
+  if (b->d_entries) (void)hipFree(b->d_entries);
+  if (b->d_fentries) (void)hipFree(b->d_fentries);
+  b->d_entries = nullptr;
+  b->d_fentries = nullptr;
   b->d_groups = nullptr;
   b->d_done = nullptr;
   b->d_phase_items = nullptr;
   b->d_multi = nullptr;
+  b->d_fgroups = nullptr;
+  b->d_fphase_items = nullptr;
+  b->nfgroups = (int)fgroups.size();
+  b->ftotal = ffirst;
   b->parts = parts;
   b->nclusters = ncl;
   b->bound_fast.assign(3 * np, nullptr);
@@ -1491,8 +1576,8 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
     return D2DHIP_SUCCESS;
   }
   if (hipMalloc(&b->d_groups, sizeof(BatchGroup) * groups.size()) != hipSuccess ||
-      hipMalloc(&b->d_done, sizeof(unsigned long long) * phase_items.size()) != hipSuccess ||
-      hipMalloc(&b->d_phase_items, sizeof(unsigned long long) * phase_items.size()) != hipSuccess ||
+      hipMalloc(&b->d_done, sizeof(unsigned long long) * phase_items.size() * kBatchQueues *
+                                kHeadStride) != hipSuccess ||
       hipMalloc(&b->d_multi, sizeof(MultiGroup) * multi.size()) != hipSuccess)
     return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch queue)");
   int st = hip_check(hipMemcpy(b->d_groups, groups.data(), sizeof(BatchGroup) * groups.size(),
@@ -1502,15 +1587,60 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
     st = hip_check(hipMemcpy(b->d_multi, multi.data(), sizeof(MultiGroup) * multi.size(),
                              hipMemcpyHostToDevice),
                    "hipMemcpy(batch groups)");
+  // the queues: each phase's items (contiguous, phase-major) dealt to the
+  // kBatchQueues queues in chunks of kBatchQueueChunk, round robin
+  auto build_queues = [&](const std::vector<BatchGroup> &gs, QueueEntry **dst, long long *off,
+                          long long *len, unsigned long long **dphase) {
+    std::vector<std::vector<QueueEntry>> qs(kBatchQueues);
+    std::vector<unsigned long long> qphase;  // items of phase p in queue q
+    size_t k = 0;
+    while (k < gs.size()) {
+      const int ph = gs[k].phase;
+      std::vector<QueueEntry> items;
+      for (; k < gs.size() && gs[k].phase == ph; ++k) {
+        const long long n = (gs[k].ntiles + gs[k].per_item - 1) / gs[k].per_item;
+        for (long long i = 0; i < n; ++i) items.push_back(QueueEntry{(int)k, (int)i, ph, 0});
+      }
+      const size_t qp0 = qphase.size();
+      qphase.resize(qp0 + kBatchQueues, 0ull);
+      for (size_t i = 0; i < items.size(); ++i) {
+        const size_t q = (i / kBatchQueueChunk) % kBatchQueues;
+        qs[q].push_back(items[i]);
+        ++qphase[qp0 + q];
+      }
+    }
+    std::vector<QueueEntry> all;
+    for (int q = 0; q < kBatchQueues; ++q) {
+      off[q] = (long long)all.size();
+      len[q] = (long long)qs[q].size();
+      all.insert(all.end(), qs[q].begin(), qs[q].end());
+    }
+    if (hipMalloc(dst, sizeof(QueueEntry) * std::max<size_t>(1, all.size())) != hipSuccess ||
+        hipMalloc(dphase, sizeof(unsigned long long) * qphase.size()) != hipSuccess)
+      return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch queue)");
+    int e = hip_check(hipMemcpy(*dst, all.data(), sizeof(QueueEntry) * all.size(), hipMemcpyHostToDevice),
+                      "hipMemcpy(batch queue)");
+    if (!e)
+      e = hip_check(hipMemcpy(*dphase, qphase.data(), sizeof(unsigned long long) * qphase.size(),
+                              hipMemcpyHostToDevice),
+                    "hipMemcpy(batch phases)");
+    return e;
+  };
+  if (!st) st = build_queues(groups, &b->d_entries, b->qoff, b->qlen, &b->d_phase_items);
+  if (!st && b->fast)
+    st = build_queues(fgroups, &b->d_fentries, b->fqoff, b->fqlen, &b->d_fphase_items);
+  if (!st && b->fast) {
+    if (hipMalloc(&b->d_fgroups, sizeof(BatchGroup) * fgroups.size()) != hipSuccess)
+      return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch queue)");
+    st = hip_check(hipMemcpy(b->d_fgroups, fgroups.data(), sizeof(BatchGroup) * fgroups.size(),
+                             hipMemcpyHostToDevice),
+                   "hipMemcpy(batch queue)");
+  }
   // kernel attributes now, so that a captured launch makes no such call
   if (!st && general_lds) st = general_multi_prepare(b->rb, general_lds);
   for (size_t ph = 0; ph < parts.size() && !st; ++ph)
     for (const auto &q : parts[ph])
       if (!st && q.fv) st = hip_check(q.fv->prepare_multi(), "hipFuncSetAttribute(multi kernel)");
-  if (!st)
-    st = hip_check(hipMemcpy(b->d_phase_items, phase_items.data(),
-                             sizeof(unsigned long long) * phase_items.size(), hipMemcpyHostToDevice),
-                   "hipMemcpy(batch phases)");
   b->bound = (st == D2DHIP_SUCCESS);
   return st;
 }
@@ -1534,9 +1664,22 @@ int d2dhip_batch_launch(d2dhip_batch b) {
       }
     return D2DHIP_SUCCESS;
   }
-  return launch_batch(b->rb, b->d_groups, b->ngroups, b->total, b->d_phase_items, b->nphase,
-                      b->d_head, b->d_done, b->d_err, b->wait_ticks, b->blocks, b->nthreads,
-                      b->lds, b->stream);
+  const bool fast = b->mode == D2DHIP_BATCH_PERSISTENT && b->fast;
+  BatchQueue q;
+  q.groups = fast ? b->d_fgroups : b->d_groups;
+  q.entries = fast ? b->d_fentries : b->d_entries;
+  for (int k = 0; k < kBatchQueues; ++k) {
+    q.off[k] = fast ? b->fqoff[k] : b->qoff[k];
+    q.len[k] = fast ? b->fqlen[k] : b->qlen[k];
+  }
+  q.phase_items = fast ? b->d_fphase_items : b->d_phase_items;
+  q.nphase = b->nphase;
+  q.heads = b->d_head;
+  q.done = b->d_done;
+  q.err = b->d_err;
+  q.wait_ticks = b->wait_ticks;
+  if (fast) return launch_batch_fast(b->rb, q, b->fblocks, b->flds, b->stream);
+  return launch_batch(b->rb, q, b->blocks, b->nthreads, b->lds, b->stream);
 }
 
 // Captures d2dhip_batch_launch (counter reset + persistent kernel, no
@@ -1594,6 +1737,20 @@ int d2dhip_batch_set_cluster_bytes(d2dhip_batch b, long long bytes) {
   return D2DHIP_SUCCESS;
 }
 
+int d2dhip_batch_set_item_points(d2dhip_batch b, long long points) {
+  if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
+  if (points < 1) return set_error(D2DHIP_ERR_INVALID_VALUE, "item points < 1");
+  b->item_points = points;
+  b->bound = false;  // the queue changes: bind again
+  return D2DHIP_SUCCESS;
+}
+
+int d2dhip_batch_persistent_tiles(d2dhip_batch b, int *specialised) {
+  if (!b || !specialised) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  *specialised = b->fast ? 1 : 0;
+  return D2DHIP_SUCCESS;
+}
+
 int d2dhip_batch_get_clusters(d2dhip_batch b, int *nclusters, int *nphases) {
   if (!b || !nclusters || !nphases) return set_error(D2DHIP_ERR_NULL, "NULL argument");
   *nclusters = b->nclusters;
@@ -1603,7 +1760,8 @@ int d2dhip_batch_get_clusters(d2dhip_batch b, int *nclusters, int *nphases) {
 
 int d2dhip_batch_set_mode(d2dhip_batch b, int mode) {
   if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
-  if (mode != D2DHIP_BATCH_PERSISTENT && mode != D2DHIP_BATCH_GROUPED)
+  if (mode != D2DHIP_BATCH_PERSISTENT && mode != D2DHIP_BATCH_GROUPED &&
+      mode != D2DHIP_BATCH_PERSISTENT_GENERAL)
     return set_error(D2DHIP_ERR_INVALID_VALUE, "batch mode");
   if (mode != b->mode) {
     b->mode = mode;
@@ -1626,6 +1784,11 @@ int d2dhip_batch_destroy(d2dhip_batch b) {
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->d_descs) (void)hipFree(b->d_descs);
+  if (b->d_fdescs) (void)hipFree(b->d_fdescs);
+  if (b->d_entries) (void)hipFree(b->d_entries);
+  if (b->d_fentries) (void)hipFree(b->d_fentries);
+  if (b->d_fgroups) (void)hipFree(b->d_fgroups);
+  if (b->d_fphase_items) (void)hipFree(b->d_fphase_items);
   if (b->d_groups) (void)hipFree(b->d_groups);
   if (b->d_done) (void)hipFree(b->d_done);
   if (b->d_phase_items) (void)hipFree(b->d_phase_items);

@@ -194,13 +194,24 @@ int d2dhip_batch_set_stream(d2dhip_batch batch, void *stream);
  * kernel boundaries order the phases; no device queue).  Its results are
  * bit-identical to the plans' own d2dhip_exec_c2c calls; it uses the plans'
  * stage kernels as of bind, and launch fails with D2DHIP_ERR_INVALID_VALUE if
- * one has changed since (bind again).  D2DHIP_BATCH_PERSISTENT is the single
- * persistent launch (general kernel, batch-wide block shape); its results are
- * bit-identical to the plans' calls with every stage on the general kernel.
- * Changing the mode drops a captured graph. */
+ * one has changed since (bind again).
+ * D2DHIP_BATCH_PERSISTENT is the single persistent launch.  When every c2c
+ * stage of every plan has a length with a one-wave specialised tile (32, 48,
+ * 64, 80, 96, 128, 160, 192, 256), its workgroups run those tiles
+ * (d2dhip_batch_persistent_tiles reports 1), and results are bit-identical to
+ * the plans' calls on any specialised kernel of each length; otherwise, and
+ * always with D2DHIP_BATCH_PERSISTENT_GENERAL, they run the general kernel's
+ * tiles, bit-identical to the plans' calls with every stage on the general
+ * kernel.  Changing the mode drops a captured graph. */
 #define D2DHIP_BATCH_PERSISTENT 0
 #define D2DHIP_BATCH_GROUPED 1
+#define D2DHIP_BATCH_PERSISTENT_GENERAL 2
 int d2dhip_batch_set_mode(d2dhip_batch batch, int mode);
+/* Work per queue item of the persistent executors, in complex points
+ * (whole tiles, at least one; default 2048).  Takes effect at the next bind. */
+int d2dhip_batch_set_item_points(d2dhip_batch batch, long long points);
+/* 1 if D2DHIP_BATCH_PERSISTENT runs specialised tiles for this batch. */
+int d2dhip_batch_persistent_tiles(d2dhip_batch batch, int *specialised);
 /* Queue order of both executors: consecutive grids whose input and output
  * buffers total at most `bytes` form a cluster, and a cluster runs all of its
  * stage phases before the next cluster starts, so that its later stages read

@@ -1,11 +1,11 @@
-"""Many grids in one persistent-kernel launch (the fft_multiple_grids use,
+"""Many grids in one launch (the fft_multiple_grids use,
 examples/fft_multiple_grids/fft_multiple_grids.f90), eager and captured in a
-hipGraph, and the same batch run as one launch per stage phase and kernel.
-The persistent executor runs the general kernel, so its results must equal
-the sequential decomp_2d_fft_3d calls of plans set to the general kernel bit
-for bit; the grouped executor runs each plan's own stage kernels, so its
-results must equal the plans' own sequential calls bit for bit.  Both match
-pocketfft to the FP64 / FP32 tolerances."""
+hipGraph, with each executor: one launch per stage phase and kernel (the
+plans' own kernels: bit-identical to the plans' sequential decomp_2d_fft_3d
+calls), one persistent launch on specialised one-wave tiles (bit-identical
+to the plans on any specialised kernel of each length), and one persistent
+launch on the general kernel's tiles (bit-identical to the plans on the
+general kernel).  All match pocketfft to the FP64 / FP32 tolerances."""
 import numpy as np
 import pytest
 
@@ -25,7 +25,24 @@ def dev(torch, a, dt):
     return torch.from_numpy(np.ascontiguousarray(a.ravel(order="F")).astype(dt)).to("cuda")
 
 
-MODES = [0, 1]  # BATCH_PERSISTENT, BATCH_GROUPED
+MODES = [0, 1, 2]  # BATCH_PERSISTENT, BATCH_GROUPED, BATCH_PERSISTENT_GENERAL
+C2C = ("c2c_x", "c2c_y", "c2c_z")
+
+
+def match_plans(engines, b, mode):
+    """Sets the plans' c2c stage kernels to what executor `mode` of batch b
+    runs, so that their sequential calls are its bit-exact reference."""
+    if mode == 1:
+        return "plans"
+    if mode == 0 and b.persistent_tiles():
+        for e in engines:
+            for st in C2C:
+                e.set_stage_kernel(st, [n for n in e.stage_candidates(st) if n != "general"][0])
+        return "specialised"
+    for e in engines:
+        for st in C2C:
+            e.set_stage_kernel(st, "general")
+    return "general"
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -41,13 +58,11 @@ def test_batch_matches_sequential_and_pocketfft(d2d, torch_gpu, prec, mode):
     ins = [dev(torch, x, npc) for x in xs]
     outs = [torch.empty_like(a) for a in ins]
     seq = [torch.empty_like(a) for a in ins]
-    for e, a, o in zip(engines, ins, seq):
-        if mode == 0:
-            # the persistent kernel runs the general kernel: compare with the plans on it
-            for st in ("c2c_x", "c2c_y", "c2c_z"):
-                e.set_stage_kernel(st, "general")
-        e.fft_c2c(a, o, -1)
     b = d2d.Batch(engines)
+    assert not b.persistent_tiles()  # lengths 8..30 have no one-wave tiles
+    match_plans(engines, b, mode)
+    for e, a, o in zip(engines, ins, seq):
+        e.fft_c2c(a, o, -1)
     b.set_mode(mode)
     b.bind(ins, outs, mode=-1)
     b.launch()
@@ -139,6 +154,17 @@ def test_config5_64_uneven_grids(d2d, torch_gpu):
     for a, o, s in zip(ins, outs, shapes):
         err = float(torch.linalg.vector_norm(o / np.prod(s) - a) / torch.linalg.vector_norm(a))
         assert err < 1e-13, s
+    # the single persistent launch runs specialised tiles for all 64 grids
+    assert b.persistent_tiles()
+    match_plans(engines, b, 0)
+    for e, a, q in zip(engines, ins, seq):
+        e.fft_c2c(a, q, -1)
+    b.set_mode(0)
+    b.bind(ins, outs, mode=-1)
+    b.launch()
+    b.check()
+    for o, q in zip(outs, seq):
+        assert torch.equal(o, q)
 
 
 def test_grouped_rejects_changed_plan(d2d, torch_gpu):
@@ -182,13 +208,11 @@ def test_cluster_order(d2d, torch_gpu, mode, budget):
         ins.append(a)
         outs.append(torch.empty_like(a))
         seq.append(torch.empty_like(a))
+    b = d2d.Batch(engines)
+    match_plans(engines, b, mode)
     for e, a, q in zip(engines, ins, seq):
-        if mode == 0:
-            for st in ("c2c_x", "c2c_y", "c2c_z"):
-                e.set_stage_kernel(st, "general")
         e.fft_c2c(a, q, -1)
         e.fft_c2c(q, q, 1)
-    b = d2d.Batch(engines)
     b.set_mode(mode)
     b.set_cluster_bytes(budget)
     b.bind(ins, outs, mode=0)
@@ -203,3 +227,52 @@ def test_cluster_order(d2d, torch_gpu, mode, budget):
     b.check()
     for o, q in zip(outs, seq):
         assert torch.equal(o, q)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_persistent_specialised_tiles(d2d, torch_gpu, prec):
+    """Grids whose every length has a one-wave specialised tile run the
+    persistent kernel on those tiles: bit-identical to the plans on their
+    specialised kernels, forward then backward in one launch and as hipGraph
+    replays, with the round trip and the forward against pocketfft."""
+    torch = torch_gpu
+    shapes = grids(10, 11, [32, 48, 64, 80, 96, 128, 160, 192]) + [(256, 32, 48)]
+    npc = np.complex128 if prec == 64 else np.complex64
+    tol = 1e-13 if prec == 64 else 5e-6
+    rng = np.random.default_rng(12)
+    xs = [np.asfortranarray(rng.uniform(-1, 1, s) + 1j * rng.uniform(-1, 1, s)) for s in shapes]
+    engines = [d2d.FFTEngine(1 if i % 3 else 3, *s, prec=prec) for i, s in enumerate(shapes)]
+    ins = [dev(torch, x, npc) for x in xs]
+    outs = [torch.empty_like(a) for a in ins]
+    b = d2d.Batch(engines)
+    assert b.persistent_tiles()
+    assert match_plans(engines, b, 0) == "specialised"
+    seq = [torch.empty_like(a) for a in ins]
+    for e, a, q in zip(engines, ins, seq):
+        e.fft_c2c(a, q, -1)
+    b.set_mode(0)
+    b.bind(ins, outs, mode=-1)
+    b.launch()
+    b.check()
+    for x, o, q, s in zip(xs, outs, seq, shapes):
+        assert torch.equal(o, q), s
+        got = o.cpu().numpy().reshape(s, order="F")
+        assert rel_l2(got, np.fft.fftn(x.astype(npc).astype(np.complex128))) < tol
+    for e, q in zip(engines, seq):
+        e.fft_c2c(q, q, 1)
+    b.bind(ins, outs, mode=0)
+    b.capture_graph()
+    for rep in range(3):
+        for o in outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        b.graph_launch()
+        b.check()
+        for o, q, s in zip(outs, seq, shapes):
+            assert torch.equal(o, q), (rep, s)
+    for x, o, s in zip(xs, outs, shapes):
+        got = o.cpu().numpy().reshape(s, order="F") / np.prod(s)
+        assert rel_l2(got, x.astype(npc)) < tol
+    # one length without a one-wave tile: the general kernel's tiles
+    b2 = d2d.Batch(engines[:2] + [d2d.FFTEngine(1, 40, 32, 32, prec=prec)])
+    assert not b2.persistent_tiles()
