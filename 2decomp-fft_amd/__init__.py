@@ -321,9 +321,9 @@ def decomp_2d_fft_init(pencil, nx, ny, nz, **kw):
 
 
 class Batch:
-    """c2c stages of many engines run together: one launch per stage phase
-    and stage kernel (BATCH_GROUPED, default) or one persistent-kernel launch
-    (BATCH_PERSISTENT)."""
+    """c2c stages of many engines run together: one persistent-kernel launch
+    fed by device work queues (BATCH_PERSISTENT, default) or one launch per
+    stage phase and stage kernel (BATCH_GROUPED)."""
 
     def __init__(self, engines, stream=None):
         self.engines = list(engines)
@@ -351,9 +351,9 @@ class Batch:
         check(lib().d2dhip_batch_launch(self._b))
 
     def set_mode(self, mode):
-        """BATCH_GROUPED (default: one launch per stage phase and stage kernel,
-        the plans' own kernels), BATCH_PERSISTENT (one persistent launch, on
-        specialised tiles where every stage has one) or
+        """BATCH_PERSISTENT (default: one persistent launch, on specialised
+        tiles where every stage has one), BATCH_GROUPED (one launch per stage
+        phase and stage kernel, the plans' own kernels) or
         BATCH_PERSISTENT_GENERAL (one persistent launch on the general
         kernel's tiles)."""
         check(lib().d2dhip_batch_set_mode(self._b, int(mode)))

@@ -1224,7 +1224,7 @@ struct d2dhip_batch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool bound = false;
-  int mode = D2DHIP_BATCH_GROUPED;
+  int mode = D2DHIP_BATCH_PERSISTENT;
   std::vector<int> ph_g0, ph_g1;       // group range of each phase
   std::vector<long long> ph_tiles;     // tiles of each phase
   // Grouped executor: per phase, the grids partitioned by the stage kernel

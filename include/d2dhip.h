@@ -170,16 +170,19 @@ int d2dhip_exec_c2r(d2dhip_plan plan, void *in_c, void *out_r);
  * examples/fft_multiple_grids/fft_multiple_grids.f90) runs one
  * decomp_2d_fft_3d per grid.  A batch runs the c2c stages of many plans (same
  * precision) as stage phases (stage 0 of every grid, then stage 1, ...) with
- * one of two executors (d2dhip_batch_set_mode below): one launch per phase
- * and stage kernel (default), or one persistent-kernel launch fed by a
- * device work queue in which a phase starts when every tile of the previous
- * phase has completed (one release/acquire per workgroup and phase).  Calls
- * that overlap on one batch object race: the queue counters belong to the
- * batch.  bind lays out the phases for the given device buffers (mode -1
- * forward, +1 backward, 0 forward then backward in place on out); launch is
- * stream-ordered and capturable; capture_graph records it into a hipGraph
- * replayed by graph_launch; check synchronises and reports a persistent
- * phase-wait timeout (5 s) as D2DHIP_ERR_HIP. */
+ * one of two executors (d2dhip_batch_set_mode below): by default one
+ * persistent-kernel launch fed by device work queues, in which a phase
+ * starts when every tile of the previous phase has completed (one
+ * release/acquire per workgroup and phase); or one ordinary launch per phase
+ * and stage kernel.  Calls that overlap on one batch object race: the queue
+ * counters belong to the batch.  The persistent launch assumes that at least
+ * 8 of its workgroups can be resident at once (one per queue); otherwise a
+ * phase wait times out after 5 s and check reports it.  bind lays out the
+ * phases for the given device buffers (mode -1 forward, +1 backward, 0
+ * forward then backward in place on out); launch is stream-ordered and
+ * capturable; capture_graph records it into a hipGraph replayed by
+ * graph_launch; check synchronises and reports a phase-wait timeout as
+ * D2DHIP_ERR_HIP. */
 typedef struct d2dhip_batch_s *d2dhip_batch;
 int d2dhip_batch_create(d2dhip_batch *batch, int nplans, const d2dhip_plan *plans, void *stream);
 int d2dhip_batch_bind_c2c(d2dhip_batch batch, void *const *in, void *const *out, int mode);
@@ -188,21 +191,21 @@ int d2dhip_batch_capture_graph(d2dhip_batch batch);
 int d2dhip_batch_graph_launch(d2dhip_batch batch);
 int d2dhip_batch_check(d2dhip_batch batch);
 int d2dhip_batch_set_stream(d2dhip_batch batch, void *stream);
-/* Executor of a batch: D2DHIP_BATCH_GROUPED (default) runs, per stage phase,
- * one ordinary launch per distinct stage kernel of the grids' own plans,
- * covering the tiles of every grid that runs that kernel in that phase (the
- * kernel boundaries order the phases; no device queue).  Its results are
- * bit-identical to the plans' own d2dhip_exec_c2c calls; it uses the plans'
- * stage kernels as of bind, and launch fails with D2DHIP_ERR_INVALID_VALUE if
- * one has changed since (bind again).
- * D2DHIP_BATCH_PERSISTENT is the single persistent launch.  When every c2c
- * stage of every plan has a length with a one-wave specialised tile (32, 48,
- * 64, 80, 96, 128, 160, 192, 256), its workgroups run those tiles
- * (d2dhip_batch_persistent_tiles reports 1), and results are bit-identical to
- * the plans' calls on any specialised kernel of each length; otherwise, and
- * always with D2DHIP_BATCH_PERSISTENT_GENERAL, they run the general kernel's
- * tiles, bit-identical to the plans' calls with every stage on the general
- * kernel.  Changing the mode drops a captured graph. */
+/* Executor of a batch: D2DHIP_BATCH_PERSISTENT (default) is the single
+ * persistent launch.  When every c2c stage of every plan has a length with a
+ * persistent tile (32, 48, 64, 80, 96, 128, 160, 192, 256), its workgroups
+ * run those specialised tiles (d2dhip_batch_persistent_tiles reports 1), and
+ * results are bit-identical to the plans' calls on any specialised kernel of
+ * each length; otherwise, and always with D2DHIP_BATCH_PERSISTENT_GENERAL,
+ * they run the general kernel's tiles, bit-identical to the plans' calls
+ * with every stage on the general kernel.
+ * D2DHIP_BATCH_GROUPED runs, per stage phase, one ordinary launch per
+ * distinct stage kernel of the grids' own plans, covering the tiles of every
+ * grid that runs that kernel in that phase (the kernel boundaries order the
+ * phases; no device queue).  Its results are bit-identical to the plans' own
+ * d2dhip_exec_c2c calls; it uses the plans' stage kernels as of bind, and
+ * launch fails with D2DHIP_ERR_INVALID_VALUE if one has changed since (bind
+ * again).  Changing the mode drops a captured graph. */
 #define D2DHIP_BATCH_PERSISTENT 0
 #define D2DHIP_BATCH_GROUPED 1
 #define D2DHIP_BATCH_PERSISTENT_GENERAL 2
