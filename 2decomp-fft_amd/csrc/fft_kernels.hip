@@ -79,9 +79,27 @@ __device__ __forceinline__ cpx<T> *lds_tile() {
 // One Stockham pass of radix R.  `sg` reads the pass inputs from HBM (first
 // pass), `dg` writes the outputs to HBM (last pass); otherwise LDS.  Ends
 // with a barrier whenever it wrote LDS.
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Function arguments arrive in vector registers; they are wave-uniform, and
+// re-deriving them with readfirstlane keeps the descriptor reads and the
+// index arithmetic of the pass on the scalar unit.
 template <typename T, int E, int IO, int R>
-__device__ __noinline__ void run_pass(DescPtr d, AxisIo io, int p, bool sg, bool dg, long long c0b,
-                                      long long c1) {
+__device__ __noinline__ void run_pass(DescPtr d_, AxisIo io_, int p_, bool sg_, bool dg_,
+                                      long long c0b_, long long c1_) {
+  const DescPtr d = (DescPtr)uniform_u64((unsigned long long)d_);
+  const AxisIo io = {(const void *)uniform_u64((unsigned long long)io_.in),
+                     (void *)uniform_u64((unsigned long long)io_.out),
+                     __builtin_amdgcn_readfirstlane(io_.conj)};
+  const int p = __builtin_amdgcn_readfirstlane(p_);
+  const bool sg = __builtin_amdgcn_readfirstlane((int)sg_) != 0;
+  const bool dg = __builtin_amdgcn_readfirstlane((int)dg_) != 0;
+  const long long c0b = (long long)uniform_u64((unsigned long long)c0b_);
+  const long long c1 = (long long)uniform_u64((unsigned long long)c1_);
   cpx<T> *lds = lds_tile<T>();
   constexpr int NB = E / R;
   static_assert(NB >= 1, "radix larger than the register capacity");
@@ -166,6 +184,15 @@ __device__ __forceinline__ void dispatch_pass(DescPtr d, const AxisIo &io, int p
   }
 }
 
+// Waves per SIMD the general kernels' register allocation must leave room
+// for; the bound also holds for the non-inlined pass functions.  FP32 passes
+// fit 4 waves without spilling; FP64 passes would spill, so they are left
+// unbounded.
+template <typename T>
+constexpr int general_wpe() {
+  return sizeof(T) == 8 ? 1 : 4;
+}
+
 // One tile of a stage, by the whole workgroup (any blockDim at least the
 // stage's planned block size).
 template <typename T, int E, int IO>
@@ -233,7 +260,7 @@ __device__ __forceinline__ void axis_tile(DescPtr d, const AxisIo &io, long long
 }
 
 template <typename T, int E, int IO>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(general_wpe<T>())))
     axis_fft_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
   const AxisIo io = {in, out, conj};
   const long long ntiles = (long long)d->tiles0 * d->C1;
@@ -246,7 +273,8 @@ __global__ void __launch_bounds__(512)
 // The persistent batch kernel on the general kernel's tile function (the
 // queue and phase protocol: batch_loop.h).
 template <typename T, int E>
-__global__ void __launch_bounds__(512) batch_kernel(const BatchQueue q) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(general_wpe<T>())))
+    batch_kernel(const BatchQueue q) {
   batch_loop(q, [&](const BatchGroup &g, long long tile, unsigned long long *head)
                     __attribute__((always_inline)) {
     unsigned long long r = 0;
@@ -307,7 +335,7 @@ int launch_batch(int prec_bytes, const BatchQueue &q, int blocks, int nthreads, 
 // The general kernel over the tiles of several grids, each with its own
 // descriptor (block size and LDS at least every grid's planned shape).
 template <typename T, int E>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(general_wpe<T>())))
     general_multi_kernel(const MultiGroup *__restrict__ g, int ng, long long tiles) {
   const long long it = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles);
   if (it >= tiles) return;  // whole block, before any barrier
@@ -346,7 +374,7 @@ int general_multi_prepare(int prec_bytes, size_t lds) {
 // LDS with the DFT of w precomputed (bhat).  The IO pairing of real
 // transforms wraps around the complex line transform as in axis_fft_kernel.
 template <typename T, int E, int IO>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(general_wpe<T>())))
     bluestein_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
