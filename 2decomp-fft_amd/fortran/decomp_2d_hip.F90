@@ -109,6 +109,34 @@ module d2dhip_c
       type(c_ptr) function d2dhip_last_message() bind(C, name="d2dhip_last_message")
          import :: c_ptr
       end function
+      ! many grids in one persistent launch
+      integer(c_int) function d2dhip_batch_create(batch, nplans, plans, stream) &
+         bind(C, name="d2dhip_batch_create")
+         import :: c_int, c_ptr
+         type(c_ptr), intent(out) :: batch
+         integer(c_int), value :: nplans
+         type(c_ptr), dimension(*), intent(in) :: plans
+         type(c_ptr), value :: stream
+      end function
+      integer(c_int) function d2dhip_batch_bind_c2c(batch, din, dout, mode) &
+         bind(C, name="d2dhip_batch_bind_c2c")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+         type(c_ptr), dimension(*), intent(in) :: din, dout
+         integer(c_int), value :: mode
+      end function
+      integer(c_int) function d2dhip_batch_launch(batch) bind(C, name="d2dhip_batch_launch")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+      end function
+      integer(c_int) function d2dhip_batch_check(batch) bind(C, name="d2dhip_batch_check")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+      end function
+      integer(c_int) function d2dhip_batch_destroy(batch) bind(C, name="d2dhip_batch_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+      end function
       ! HIP runtime
       integer(c_int) function hipMalloc(p, bytes) bind(C, name="hipMalloc")
          import :: c_int, c_ptr, c_size_t
@@ -457,6 +485,9 @@ module decomp_2d_fft
 
    integer, save :: n_grid = 0
    integer, save :: cur_slot = 0
+   ! the batch of all grids' plans (decomp_2d_fft_3d_batch_dev), made on first
+   ! use and dropped whenever the set of grids changes
+   type(c_ptr), save :: grid_batch = c_null_ptr
    type(decomp_2d_fft_engine), allocatable, target, save :: fft_engines(:)
    ! the engine decomp_2d_fft_3d works with (use_it switches it, as the
    ! module-level pointers of src/fft_common.f90:446-479 do)
@@ -467,7 +498,8 @@ module decomp_2d_fft
              decomp_2d_fft_get_ngrid, decomp_2d_fft_set_ngrid, decomp_2d_fft_use_grid, &
              decomp_2d_fft_get_engine, decomp_2d_fft_get_format, decomp_2d_fft_get_inplace, &
              decomp_2d_fft_get_inplace_r2c, decomp_2d_fft_get_inplace_c2r, &
-             decomp_2d_fft_3d_dev, decomp_2d_fft_3d_r2c_dev, decomp_2d_fft_3d_c2r_dev
+             decomp_2d_fft_3d_dev, decomp_2d_fft_3d_r2c_dev, decomp_2d_fft_3d_c2r_dev, &
+             decomp_2d_fft_3d_batch_dev
 
    interface decomp_2d_fft_init
       module procedure fft_init_noarg
@@ -528,6 +560,7 @@ contains
       logical, dimension(3), intent(in), optional :: opt_skip_XYZ_c2c
       if (igrid < 1 .or. igrid > n_grid) &
          call decomp_2d_abort(__FILE__, __LINE__, igrid, "Invalid value for igrid")
+      call drop_grid_batch()
       call fft_engines(igrid)%init(pencil, nx, ny, nz, opt_inplace, opt_inplace_r2c, &
                                    opt_inplace_c2r, opt_skip_XYZ_c2c)
       fft_engines(igrid)%slot = igrid
@@ -619,8 +652,15 @@ contains
       e => cur_e
    end function current
 
+   subroutine drop_grid_batch
+      integer(c_int) :: st
+      if (c_associated(grid_batch)) st = d2dhip_batch_destroy(grid_batch)
+      grid_batch = c_null_ptr
+   end subroutine drop_grid_batch
+
    subroutine decomp_2d_fft_finalize
       integer :: i
+      call drop_grid_batch()
       do i = 1, n_grid
          call fft_engines(i)%fin()
       end do
@@ -634,6 +674,7 @@ contains
       integer, intent(in) :: ngrd
       type(decomp_2d_fft_engine), allocatable :: tmp(:)
       if (ngrd < 1) call decomp_2d_abort(__FILE__, __LINE__, ngrd, "Invalid value for n_grid")
+      call drop_grid_batch()
       if (allocated(fft_engines)) then
          call move_alloc(fft_engines, tmp)
          allocate (fft_engines(ngrd))
@@ -792,6 +833,37 @@ contains
       call check(d2dhip_exec_c2c(e%plan, d_in, d_out, int(isign, c_int)), __FILE__, __LINE__)
       call check(hipDeviceSynchronize(), __FILE__, __LINE__)
    end subroutine
+
+   ! Extension (no counterpart in the reference, whose multi-grid use runs
+   ! one decomp_2d_fft_3d per grid after decomp_2d_fft_use_grid): the c2c
+   ! transforms of all n_grid grids, d_in(g) -> d_out(g) on grid g's device
+   ! buffers, in one persistent-kernel launch.  isign = DECOMP_2D_FFT_FORWARD
+   ! or DECOMP_2D_FFT_BACKWARD, or 0 for forward then backward in place on
+   ! d_out (d_out(g) = N_g d_in(g)).  Results equal the grids' own
+   ! decomp_2d_fft_3d_dev calls.
+   subroutine decomp_2d_fft_3d_batch_dev(d_in, d_out, isign)
+      type(c_ptr), dimension(:), intent(in) :: d_in, d_out
+      integer, intent(in) :: isign
+      type(c_ptr), allocatable :: plans(:)
+      integer :: g
+      if (size(d_in) /= n_grid .or. size(d_out) /= n_grid) &
+         call decomp_2d_abort(__FILE__, __LINE__, size(d_in), "one buffer pair per grid expected")
+      if (isign /= DECOMP_2D_FFT_FORWARD .and. isign /= DECOMP_2D_FFT_BACKWARD .and. isign /= 0) &
+         call decomp_2d_abort(__FILE__, __LINE__, isign, "Invalid value for isign")
+      if (.not. c_associated(grid_batch)) then
+         allocate (plans(n_grid))
+         do g = 1, n_grid
+            if (.not. fft_engines(g)%initialised) &
+               call decomp_2d_abort(__FILE__, __LINE__, g, "FFT engine is not ready")
+            plans(g) = fft_engines(g)%plan
+         end do
+         call check(d2dhip_batch_create(grid_batch, int(n_grid, c_int), plans, c_null_ptr), &
+                    __FILE__, __LINE__)
+      end if
+      call check(d2dhip_batch_bind_c2c(grid_batch, d_in, d_out, int(isign, c_int)), __FILE__, __LINE__)
+      call check(d2dhip_batch_launch(grid_batch), __FILE__, __LINE__)
+      call check(d2dhip_batch_check(grid_batch), __FILE__, __LINE__)
+   end subroutine decomp_2d_fft_3d_batch_dev
 
    subroutine decomp_2d_fft_3d_r2c_dev(d_in_r, d_out_c)
       type(c_ptr), intent(in) :: d_in_r, d_out_c

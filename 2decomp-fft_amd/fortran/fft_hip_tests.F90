@@ -47,6 +47,7 @@ program fft_hip_tests
       call decomp_2d_fft_use_grid(igrid)
       call grid_tests(ntest, igrid)
    end do
+   call check_batch(3)
    call local_engine%use_it()
    call ntest_c2c(ntest, nx + 16, ny + 2, nz, 'c2c local engine')
    call ntest_r2c(ntest, nx + 16, ny + 2, nz, 'r2c local engine')
@@ -208,5 +209,61 @@ contains
       st = hipFree(d_r)
       st = hipFree(d_c)
    end subroutine check_device_path
+
+   ! all grids' c2c transforms in one batched launch against per-grid calls
+   subroutine check_batch(ng)
+      integer, intent(in) :: ng
+      type(c_ptr) :: d_in(ng), d_out(ng), d_ref(ng)
+      integer(c_size_t) :: nb(ng)
+      complex(mytype), allocatable, target :: a(:), b(:), c(:)
+      real(mytype), allocatable :: re(:), im(:)
+      type(decomp_info), pointer :: ph
+      integer :: g, sgn
+      integer(c_int) :: st
+      real(mytype) :: err
+      do g = 1, ng
+         call decomp_2d_fft_use_grid(g)
+         ph => decomp_2d_fft_get_ph()
+         nb(g) = 16_c_size_t * ph%xsz(1) * ph%xsz(2) * ph%xsz(3)
+         allocate (re(product(ph%xsz)), im(product(ph%xsz)))
+         call random_number(re)
+         call random_number(im)
+         a = cmplx(re, im, mytype)
+         deallocate (re, im)
+         st = hipMalloc(d_in(g), nb(g))
+         st = ior(st, hipMalloc(d_out(g), nb(g)))
+         st = ior(st, hipMalloc(d_ref(g), nb(g)))
+         st = ior(st, hipMemcpy(d_in(g), c_loc(a), nb(g), HIP_H2D))
+         if (st /= 0) call decomp_2d_abort(__FILE__, __LINE__, int(st), "hip")
+      end do
+      do sgn = -1, 1, 2
+         call decomp_2d_fft_3d_batch_dev(d_in, d_out, sgn)
+         do g = 1, ng
+            call decomp_2d_fft_use_grid(g)
+            call decomp_2d_fft_3d_dev(d_in(g), d_ref(g), sgn)
+            allocate (b(nb(g) / 16), c(nb(g) / 16))
+            st = hipMemcpy(c_loc(b), d_out(g), nb(g), HIP_D2H)
+            st = ior(st, hipMemcpy(c_loc(c), d_ref(g), nb(g), HIP_D2H))
+            if (st /= 0) call decomp_2d_abort(__FILE__, __LINE__, int(st), "hip")
+            err = maxval(abs(b - c)) / maxval(abs(c))
+            if (err > 1e-13_mytype) call decomp_2d_abort(__FILE__, __LINE__, 12, "batch vs grid")
+            deallocate (b, c)
+         end do
+      end do
+      ! forward then backward in one launch: out = N * in
+      call decomp_2d_fft_3d_batch_dev(d_in, d_out, 0)
+      do g = 1, ng
+         allocate (b(nb(g) / 16), c(nb(g) / 16))
+         st = hipMemcpy(c_loc(b), d_out(g), nb(g), HIP_D2H)
+         st = ior(st, hipMemcpy(c_loc(c), d_in(g), nb(g), HIP_D2H))
+         err = maxval(abs(b / real(nb(g) / 16, mytype) - c))
+         write (*, *) 'batch grid ', g, ' round trip max error: ', err
+         if (err > 1e-13_mytype) call decomp_2d_abort(__FILE__, __LINE__, 13, "batch round trip")
+         deallocate (b, c)
+         st = hipFree(d_in(g))
+         st = hipFree(d_out(g))
+         st = hipFree(d_ref(g))
+      end do
+   end subroutine check_batch
 
 end program fft_hip_tests
