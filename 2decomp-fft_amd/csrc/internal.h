@@ -71,25 +71,23 @@ struct AxisIo {
   int conj;  // 1: inverse transform (conjugate on load and store)
 };
 
-// One group of work items of the batched kernel: all tiles of one stage of
-// one grid.
+// One group of work items of a persistent batch launch: all tiles of one
+// stage of one grid, ceil(ntiles / per_item) queue items.
 struct BatchGroup {
-  const AxisDesc *d;  // device descriptor (general kernel)
+  const AxisDesc *d;  // device descriptor of the stage for the tile function
   const void *in;
   void *out;
   int conj;
   int phase;          // stage phase: every item of phase p-1 completes before phase p starts
   int per_item;       // consecutive tiles one queue item covers
   long long ntiles;
-  long long first;    // index of its first queue item (items: ceil(ntiles / per_item))
-  long long tile0;    // index of its first tile among all tiles of its phase
   int kind;           // specialised tile function (persistent kernel on specialised tiles)
 };
 
 // The device queues of a persistent batch launch.  Each phase's items are
-// split into kBatchQueues contiguous shares, one per queue; block b takes
-// items from queue b % kBatchQueues (blocks are placed on the 8 XCDs round
-// robin, so each XCD mostly works its own queue and its own counter).
+// dealt to kBatchQueues queues in chunks of a few consecutive items; block b
+// takes items from queue b % kBatchQueues (blocks are placed on the 8 XCDs
+// round robin, so each XCD mostly works its own queue and its own counter).
 constexpr int kBatchQueues = 8;
 constexpr int kHeadStride = 32;  // u64 between queue counters (256 bytes)
 struct QueueEntry {

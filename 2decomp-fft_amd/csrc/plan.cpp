@@ -1168,13 +1168,14 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
 
 // ---- many grids in one launch -------------------------------------------------
 // The fft_multiple_grids use (examples/fft_multiple_grids/fft_multiple_grids.f90):
-// every stage of every grid, in order, through one persistent kernel.
+// the c2c stages of many grids as stage phases, through one persistent
+// kernel fed by device queues or through one launch per phase and kernel.
 
-// Work per batched queue item (complex points): small tiles are merged up
-// to this size; larger items lengthen the wait at each phase boundary (the
-// last item of a phase runs on one workgroup while the others wait), which
-// costs more than the dequeue they save (measured on 64 uneven grids: 1 or
-// 2048 points per item 24 ms, 16384 points 32 ms, 65536 points 42 ms).
+// Work per persistent queue item (complex points, whole tiles): larger items
+// lengthen the wait at each phase boundary (the last items of a phase run
+// while other workgroups wait), smaller ones add queue traffic.  Measured on
+// 64 uneven FP64 grids with specialised tiles: 1024 to 4096 points within
+// 2 %, 16384 points 5 % slower.
 constexpr long long kBatchItemPoints = 2048;
 // Default cluster budget (bytes of a cluster's input and output buffers):
 // none, every grid's stage k before any grid's stage k+1.  Measured on 64
@@ -1192,8 +1193,8 @@ struct d2dhip_batch_s {
   std::vector<d2dhip_plan_s *> plans;
   AxisDesc *d_descs = nullptr;   // 3 general-kernel c2c descriptors per plan (x, y, z)
   std::vector<AxisDesc> h_descs;
-  // The persistent kernel on specialised one-wave tiles (when every c2c
-  // stage of every plan has one): 3 descriptors and tile kinds per plan.
+  // The persistent kernel on specialised tiles (when every c2c stage of
+  // every plan has one): 3 descriptors and tile kinds per plan.
   bool fast = false;
   AxisDesc *d_fdescs = nullptr;
   std::vector<AxisDesc> h_fdescs;
@@ -1201,9 +1202,7 @@ struct d2dhip_batch_s {
   size_t flds = 0;
   int fblocks = 0;
   BatchGroup *d_fgroups = nullptr;
-  unsigned long long *d_fphase_items = nullptr;
-  int nfgroups = 0;
-  long long ftotal = 0;
+  unsigned long long *d_fphase_items = nullptr;  // items of each phase and queue
   // the queues (general / specialised groups): entries and each queue's share
   QueueEntry *d_entries = nullptr, *d_fentries = nullptr;
   long long qoff[kBatchQueues] = {}, qlen[kBatchQueues] = {};
@@ -1211,11 +1210,10 @@ struct d2dhip_batch_s {
   long long item_points = kBatchItemPoints;  // complex points per queue item
   BatchGroup *d_groups = nullptr;
   int ngroups = 0;
-  long long total = 0;
-  unsigned long long *d_phase_items = nullptr;  // items per phase
+  unsigned long long *d_phase_items = nullptr;  // items of each phase and queue
   int nphase = 0;
-  unsigned long long *d_head = nullptr;
-  unsigned long long *d_done = nullptr;  // completed items per phase
+  unsigned long long *d_head = nullptr;  // the queue counters
+  unsigned long long *d_done = nullptr;  // completed items of each phase and queue
   unsigned int *d_err = nullptr;
   long long wait_ticks = 0;              // phase-wait limit in wall-clock ticks
   int nthreads = 64;
@@ -1225,8 +1223,6 @@ struct d2dhip_batch_s {
   hipGraphExec_t graph_exec = nullptr;
   bool bound = false;
   int mode = D2DHIP_BATCH_PERSISTENT;
-  std::vector<int> ph_g0, ph_g1;       // group range of each phase
-  std::vector<long long> ph_tiles;     // tiles of each phase
   // Grouped executor: per phase, the grids partitioned by the stage kernel
   // their own plans run (as of bind); each part is one launch of that kernel
   // over the tiles of its grids, with the plans' own descriptors.
@@ -1432,21 +1428,15 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
     cl_begin.push_back(np);
   }
   const int ncl = (int)cl_begin.size() - 1;
-  long long first = 0;
   std::vector<char> started(np, 0);
-  // the same queue for the persistent kernel on specialised tiles
+  // the same groups for the persistent kernel on specialised tiles
   std::vector<BatchGroup> fgroups;
-  long long ffirst = 0;
-  std::vector<int> ph_g0, ph_g1;
-  std::vector<long long> ph_tiles;
   std::vector<std::vector<d2dhip_batch_s::Part>> parts;
   std::vector<MultiGroup> multi;
   size_t general_lds = 0;
   for (int cph = 0; cph < ncl * nph; ++cph) {
     const int cl = cph / nph, ph = cph % nph;
-    unsigned long long items = 0, fitems = 0;
-    long long tiles = 0, ftiles = 0;
-    const int g_begin = (int)groups.size();
+    unsigned long long items = 0;
     // grouped executor: grids of this phase by their plan's stage kernel
     std::vector<const FastVariant *> keys;
     std::vector<std::vector<MultiGroup>> members;
@@ -1461,9 +1451,8 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
       const unsigned skipbit[3] = {D2DHIP_FLAG_SKIP_X, D2DHIP_FLAG_SKIP_Y, D2DHIP_FLAG_SKIP_Z};
       if (p->flags & skipbit[ax]) continue;
       // a queue group: all tiles of this stage of grid g, about
-      // kBatchItemPoints complex points per queue item
-      auto make_group = [&](const AxisDesc *dd, const AxisDesc &h, int kind, long long &fst,
-                            long long &tl, unsigned long long &its) {
+      // item_points complex points per queue item
+      auto make_group = [&](const AxisDesc *dd, const AxisDesc &h, int kind) {
         BatchGroup G;
         G.d = dd;
         G.in = started[g] ? out[g] : in[g];
@@ -1479,20 +1468,14 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
           const long long row_bytes = (long long)h.TX * 2 * b->rb;
           G.per_item = (int)std::max<long long>(G.per_item, (128 + row_bytes - 1) / row_bytes);
         }
-        const long long nitems = (G.ntiles + G.per_item - 1) / G.per_item;
-        G.first = fst;
-        G.tile0 = tl;
-        fst += nitems;
-        tl += G.ntiles;
-        its += (unsigned long long)nitems;
         return G;
       };
-      const BatchGroup G = make_group(b->d_descs + 3 * g + ax, b->h_descs[3 * g + ax], 0, first,
-                                      tiles, items);
+      const BatchGroup G = make_group(b->d_descs + 3 * g + ax, b->h_descs[3 * g + ax], 0);
       groups.push_back(G);
+      items += (unsigned long long)((G.ntiles + G.per_item - 1) / G.per_item);
       if (b->fast)
         fgroups.push_back(make_group(b->d_fdescs + 3 * g + ax, b->h_fdescs[3 * g + ax],
-                                     b->kinds[3 * g + ax], ffirst, ftiles, fitems));
+                                     b->kinds[3 * g + ax]));
       // the same stage as the plan itself runs it
       const FastVariant *fv = p->fast[ax];
       size_t q = 0;
@@ -1518,9 +1501,6 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
     }
     if (items) {
       phase_items.push_back(items);
-      ph_g0.push_back(g_begin);
-      ph_g1.push_back((int)groups.size());
-      ph_tiles.push_back(tiles);
       for (size_t k = 0; k < pp.size(); ++k) {
         pp[k].m0 = (long long)multi.size();
         pp[k].ng = (int)members[k].size();
@@ -1549,8 +1529,6 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
   b->d_multi = nullptr;
   b->d_fgroups = nullptr;
   b->d_fphase_items = nullptr;
-  b->nfgroups = (int)fgroups.size();
-  b->ftotal = ffirst;
   b->parts = parts;
   b->nclusters = ncl;
   b->bound_fast.assign(3 * np, nullptr);
@@ -1558,10 +1536,6 @@ int d2dhip_batch_bind_c2c(d2dhip_batch b, void *const *in, void *const *out, int
     for (int a = 0; a < 3; ++a) b->bound_fast[3 * g + a] = b->plans[g]->fast[a];
   b->ngroups = (int)groups.size();
   b->nphase = (int)phase_items.size();
-  b->total = first;
-  b->ph_g0 = ph_g0;
-  b->ph_g1 = ph_g1;
-  b->ph_tiles = ph_tiles;
   b->bound = false;
   if (b->graph_exec) {
     (void)hipGraphExecDestroy(b->graph_exec);
