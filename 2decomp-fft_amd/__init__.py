@@ -122,6 +122,8 @@ def lib():
             "d2dhip_exec_c2c": ([pl, vp, vp, ci], ci),
             "d2dhip_exec_r2c": ([pl, vp, vp], ci),
             "d2dhip_exec_c2r": ([pl, vp, vp], ci),
+            "d2dhip_exec_r2c_pack": ([pl, vp, vp, ctypes.POINTER(Decomp), ci], ci),
+            "d2dhip_exec_c2r_unpack": ([pl, vp, vp, ctypes.POINTER(Decomp), ci], ci),
             "d2dhip_transpose": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), vp], ci),
             "d2dhip_batch_create": ([ctypes.POINTER(vp), ci, ctypes.POINTER(vp), vp], ci),
             "d2dhip_batch_bind_c2c": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ci], ci),
@@ -303,6 +305,22 @@ class FFTEngine:
     def fft_c2r(self, inp, out):
         self._sync_stream()
         return check(lib().d2dhip_exec_c2r(self._plan, _ptr(inp), _ptr(out)))
+
+    def fft_r2c_pack(self, inp, send, decomp, even=False):
+        """The x stage (r2c along x) of a PHYSICAL_IN_X engine, written in the
+        transpose_x_to_y send-buffer order of `decomp` (the spectral
+        decomposition; pack fused with the FFT stage)."""
+        self._sync_stream()
+        return check(lib().d2dhip_exec_r2c_pack(self._plan, _ptr(inp), _ptr(send),
+                                                ctypes.byref(decomp), int(bool(even))))
+
+    def fft_c2r_unpack(self, recv, out, decomp, even=False):
+        """The x stage (c2r along x) of a PHYSICAL_IN_X engine, read from the
+        transpose_y_to_x receive-buffer order of `decomp` (unpack fused with
+        the FFT stage)."""
+        self._sync_stream()
+        return check(lib().d2dhip_exec_c2r_unpack(self._plan, _ptr(recv), _ptr(out),
+                                                  ctypes.byref(decomp), int(bool(even))))
 
     def fin(self):
         if self._plan:

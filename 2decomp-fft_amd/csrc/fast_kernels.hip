@@ -67,9 +67,14 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
         const int tt = LINES ? w / NH : w % TX;
         const int k = LINES ? w % NH : w / TX;
         const long long cc = 2 * (c0b + tt);
-        const long long base = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
-        if (cc < d->C0r) A[it] = gload<T>(in, base);
-        if (cc + 1 < d->C0r) B[it] = gload<T>(in, base + d->in_s0);
+        long long ia = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
+        long long ib = ia + d->in_s0;
+        if (LINES && d->npack) {  // receive-buffer order (fused unpack; x stages)
+          ia = pack_index(d, k, cc + c1 * d->C0r);
+          ib = pack_index(d, k, cc + 1 + c1 * d->C0r);
+        }
+        if (cc < d->C0r) A[it] = gload<T>(in, ia);
+        if (cc + 1 < d->C0r) B[it] = gload<T>(in, ib);
       }
     }
     // conj(Z) of the merged half spectra into LDS, then the inputs of pass 0
@@ -272,9 +277,14 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
         const int k = LINES ? w % NH : w / TX;
         const long long cc = 2 * (c0b + tt);
         if (cc < d->C0r) {
-          const long long base = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
-          gstore<T>(out, base, oa[it]);
-          if (cc + 1 < d->C0r) gstore<T>(out, base + d->out_s0, ob[it]);
+          long long oa_i = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
+          long long ob_i = oa_i + d->out_s0;
+          if (LINES && d->npack) {  // send-buffer order (fused pack; x stages)
+            oa_i = pack_index(d, k, cc + c1 * d->C0r);
+            ob_i = pack_index(d, k, cc + 1 + c1 * d->C0r);
+          }
+          gstore<T>(out, oa_i, oa[it]);
+          if (cc + 1 < d->C0r) gstore<T>(out, ob_i, ob[it]);
         }
       }
     }

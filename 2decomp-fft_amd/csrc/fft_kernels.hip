@@ -215,9 +215,14 @@ __device__ __forceinline__ void axis_tile(DescPtr d, const AxisIo &io, long long
       const int t = w & tmask, k = w >> lg;
       const long long ca = 2 * (c0b + t);
       const long long base = c1 * d->in_s1 + (long long)k * d->in_sj;
+      long long ia = base + ca * d->in_s0, ib = base + (ca + 1) * d->in_s0;
+      if (d->npack) {  // receive-buffer order (fused unpack)
+        ia = pack_index(d, k, ca + c1 * d->C0r);
+        ib = pack_index(d, k, ca + 1 + c1 * d->C0r);
+      }
       cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
-      if (ca < d->C0r) A = gload<T>(in, base + ca * d->in_s0);
-      if (ca + 1 < d->C0r) B = gload<T>(in, base + (ca + 1) * d->in_s0);
+      if (ca < d->C0r) A = gload<T>(in, ia);
+      if (ca + 1 < d->C0r) B = gload<T>(in, ib);
       if (k == 0 || 2 * k == n) {
         A.y = T(0);
         B.y = T(0);
@@ -245,9 +250,13 @@ __device__ __forceinline__ void axis_tile(DescPtr d, const AxisIo &io, long long
       const int km = (k == 0) ? 0 : n - k;
       const cpx<T> zm = lds[km * ldj + t * ldt];
       const long long base = c1 * d->out_s1 + (long long)k * d->out_sj;
-      gstore<T>(out, base + ca * d->out_s0, cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half});
-      if (ca + 1 < d->C0r)
-        gstore<T>(out, base + (ca + 1) * d->out_s0, cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half});
+      long long oa_i = base + ca * d->out_s0, ob_i = base + (ca + 1) * d->out_s0;
+      if (d->npack) {  // send-buffer order (fused pack)
+        oa_i = pack_index(d, k, ca + c1 * d->C0r);
+        ob_i = pack_index(d, k, ca + 1 + c1 * d->C0r);
+      }
+      gstore<T>(out, oa_i, cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half});
+      if (ca + 1 < d->C0r) gstore<T>(out, ob_i, cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half});
     }
   } else {
     if (np < 2) {
@@ -397,9 +406,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(genera
       const int t = w & tmask, k = w >> lg;
       const long long ca = 2 * (c0b + t);
       const long long base = c1 * d->in_s1 + (long long)k * d->in_sj;
+      long long ia = base + ca * d->in_s0, ib = base + (ca + 1) * d->in_s0;
+      if (d->npack) {  // receive-buffer order (fused unpack)
+        ia = pack_index(desc_ptr(d), k, ca + c1 * d->C0r);
+        ib = pack_index(desc_ptr(d), k, ca + 1 + c1 * d->C0r);
+      }
       cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
-      if (ca < d->C0r) A = gload<T>(in, base + ca * d->in_s0);
-      if (ca + 1 < d->C0r) B = gload<T>(in, base + (ca + 1) * d->in_s0);
+      if (ca < d->C0r) A = gload<T>(in, ia);
+      if (ca + 1 < d->C0r) B = gload<T>(in, ib);
       if (k == 0 || 2 * k == n) {
         A.y = T(0);
         B.y = T(0);
@@ -449,9 +463,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(genera
       const int km = (k == 0) ? 0 : n - k;
       const cpx<T> zm = lds[km * ldj + t * ldt];
       const long long base = c1 * d->out_s1 + (long long)k * d->out_sj;
-      gstore<T>(out, base + ca * d->out_s0, cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half});
-      if (ca + 1 < d->C0r)
-        gstore<T>(out, base + (ca + 1) * d->out_s0, cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half});
+      long long oa_i = base + ca * d->out_s0, ob_i = base + (ca + 1) * d->out_s0;
+      if (d->npack) {  // send-buffer order (fused pack)
+        oa_i = pack_index(desc_ptr(d), k, ca + c1 * d->C0r);
+        ob_i = pack_index(desc_ptr(d), k, ca + 1 + c1 * d->C0r);
+      }
+      gstore<T>(out, oa_i, cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half});
+      if (ca + 1 < d->C0r) gstore<T>(out, ob_i, cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half});
     }
   } else {
     for (int w = tid; w < TX * n; w += nthr) {

@@ -54,7 +54,25 @@ struct AxisDesc {
   int nblue;          // line length of a Bluestein stage
   const void *chirp;  // exp(-i pi j^2 / nblue), j < nblue
   const void *bhat;   // DFT_M of the chirp sequence exp(+i pi m^2 / nblue), wrapped
+  // Complex side of a line stage (r2c output, c2r input) in the x-pencil
+  // send / receive-buffer order of a process grid (a transform fused with
+  // the pack or unpack): npack blocks of a line's points, block m holding
+  // points [b1, b1 + w) described by pack[3m .. 3m+2] = {b1, w, base}, and
+  // point i of line l at base - b1 + i + l * w.  npack = 0: plain layout.
+  const long long *pack;
+  int npack;
 };
+
+// Index of complex point i of line l in the packed order of d (npack > 0).
+__device__ __forceinline__ long long pack_index(const __attribute__((address_space(4))) AxisDesc *d,
+                                                int i, long long l) {
+  const __attribute__((address_space(1))) long long *pk =
+      (const __attribute__((address_space(1))) long long *)d->pack;
+  const int np = d->npack;
+  int m = 0;
+  while (m + 1 < np && (long long)i >= pk[3 * (m + 1)]) ++m;
+  return pk[3 * m + 2] - pk[3 * m] + i + l * pk[3 * m + 1];
+}
 
 // A stage descriptor as the kernels read it: constant address space (it is
 // not written while a kernel runs), so its fields are scalar loads even when

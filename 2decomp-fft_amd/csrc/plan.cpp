@@ -91,6 +91,12 @@ struct d2dhip_plan_s {
   };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
+  // fused pack / unpack: the x stage's descriptor with a block table of the
+  // process grid (d2dhip_exec_r2c_pack, d2dhip_exec_c2r_unpack)
+  long long *d_pack = nullptr;
+  AxisDesc *d_pdesc = nullptr;
+  long long h_pack[3 * D2DHIP_MAX_PROCS] = {};
+  AxisDesc h_pdesc{};
 };
 
 namespace {
@@ -965,6 +971,8 @@ int d2dhip_plan_destroy(d2dhip_plan p) {
   if (p->wk) (void)hipFree(p->wk);
   if (p->d_desc) (void)hipFree(p->d_desc);
   if (p->d_cdesc) (void)hipFree(p->d_cdesc);
+  if (p->d_pack) (void)hipFree(p->d_pack);
+  if (p->d_pdesc) (void)hipFree(p->d_pdesc);
   delete p;
   return D2DHIP_SUCCESS;
 }
@@ -1162,6 +1170,77 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
   if (warn)
     return set_error(D2DHIP_WARN_SKIP_IGNORED, "r2c / c2r transform can not be skipped");
   return D2DHIP_SUCCESS;
+}
+
+namespace {
+
+// The x stage (idx 3: r2c, idx 6: c2r) of a PHYSICAL_IN_X plan with its
+// complex side in the x-pencil send / receive-buffer order of process grid
+// d: blocks of x1dist points at x1disp (or m * x1count with even), the
+// mem_split_xy / mem_merge_yx maps (src/transpose_x_to_y.f90:266-330,
+// src/transpose_y_to_x.f90:384-441) applied by the stage's stores / loads.
+int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
+                       const d2dhip_decomp *d, int even) {
+  if (p->format != D2DHIP_PHYSICAL_IN_X)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "fused pack / unpack needs a PHYSICAL_IN_X plan");
+  const long long nxc = p->nx / 2 + 1;
+  if (d->xsz[0] != nxc || d->xsz[1] != p->ny || d->xsz[2] != p->nz)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition x-pencil is not the plan's spectral x-pencil");
+  const int np = d->p_row;
+  if (np < 1 || np > D2DHIP_MAX_PROCS) return set_error(D2DHIP_ERR_INVALID_VALUE, "p_row out of range");
+  long long b1 = 0;
+  for (int m = 0; m < np; ++m) {
+    if (d->x1dist[m] < 0) return set_error(D2DHIP_ERR_INVALID_VALUE, "negative block size");
+    p->h_pack[3 * m] = b1;
+    p->h_pack[3 * m + 1] = d->x1dist[m];
+    p->h_pack[3 * m + 2] = even ? (long long)m * d->x1count : d->x1disp[m];
+    b1 += d->x1dist[m];
+  }
+  if (b1 != nxc) return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition does not match the pencil");
+  if (!p->d_pack && hipMalloc(&p->d_pack, sizeof(p->h_pack)) != hipSuccess) {
+    p->d_pack = nullptr;
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(pack table)");
+  }
+  if (!p->d_pdesc && hipMalloc(&p->d_pdesc, sizeof(AxisDesc)) != hipSuccess) {
+    p->d_pdesc = nullptr;
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(pack descriptor)");
+  }
+  p->h_pdesc = p->h_desc[idx];
+  p->h_pdesc.pack = p->d_pack;
+  p->h_pdesc.npack = np;
+  int st = hip_check(hipMemcpyAsync(p->d_pack, p->h_pack, sizeof(long long) * 3 * np,
+                                    hipMemcpyHostToDevice, p->stream),
+                     "hipMemcpyAsync(pack table)");
+  if (!st)
+    st = hip_check(hipMemcpyAsync(p->d_pdesc, &p->h_pdesc, sizeof(AxisDesc), hipMemcpyHostToDevice,
+                                  p->stream),
+                   "hipMemcpyAsync(pack descriptor)");
+  if (st) return st;
+  const AxisIo io = {in, out, 0};
+  return p->fast[idx] ? p->fast[idx]->launch(p->d_pdesc, p->h_pdesc, io, p->stream)
+                      : launch_axis(p->rb, p->geom[idx].io, p->d_pdesc, p->h_pdesc, io,
+                                    p->shape[idx], p->stream);
+}
+
+}  // namespace
+
+// The forward x stage of a PHYSICAL_IN_X plan (r2c along x) writing the
+// send buffers of transpose_x_to_y: r2c_1m_x followed by mem_split_xy
+// (src/fft_common_3d.f90:133-140, src/transpose_x_to_y.f90:93-101) in one
+// pass over the data.
+int d2dhip_exec_r2c_pack(d2dhip_plan p, const void *in_r, void *send, const d2dhip_decomp *d,
+                         int even) {
+  if (!p || !in_r || !send || !d) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  return run_x_stage_packed(p, 3, in_r, send, d, even);
+}
+
+// The inverse x stage of a PHYSICAL_IN_X plan (c2r along x) reading the
+// receive buffers of transpose_y_to_x: mem_merge_yx followed by c2r_1m_x
+// (src/transpose_y_to_x.f90:131-133, src/fft_common_3d.f90:259-266).
+int d2dhip_exec_c2r_unpack(d2dhip_plan p, const void *recv, void *out_r, const d2dhip_decomp *d,
+                           int even) {
+  if (!p || !recv || !out_r || !d) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  return run_x_stage_packed(p, 6, recv, out_r, d, even);
 }
 
 }  // extern "C"

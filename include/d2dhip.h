@@ -163,6 +163,24 @@ int d2dhip_exec_r2c(d2dhip_plan plan, const void *in_r, void *out_c);
 /* decomp_2d_fft_3d(in_c, out_r): fft_3d_c2r, src/fft_common_3d.f90:194-296.
  * in_c is preserved unless the plan has D2DHIP_FLAG_INPLACE. */
 int d2dhip_exec_c2r(d2dhip_plan plan, void *in_c, void *out_r);
+/* Transposes fused with the adjacent FFT stage, for one rank of a process
+ * grid (PHYSICAL_IN_X plans of the rank's x-pencil, nx x ny x nz real):
+ * d2dhip_exec_r2c_pack runs the forward x stage (r2c_1m_x) and writes its
+ * (nx/2+1) x ny x nz spectrum straight into the send buffers of
+ * transpose_x_to_y for the spectral decomposition d (mem_split_xy,
+ * src/fft_common_3d.f90:133-140 then src/transpose_x_to_y.f90:93-101);
+ * d2dhip_exec_c2r_unpack reads the receive buffers of transpose_y_to_x
+ * (mem_merge_yx, src/transpose_y_to_x.f90:131-133) and runs the inverse x
+ * stage (c2r_1m_x, src/fft_common_3d.f90:259-266).  d->xsz must be
+ * (nx/2+1, ny, nz); blocks follow x1dist / x1disp (even: m * x1count).
+ * Results equal the x stage followed by d2dhip_pack(D2DHIP_X_TO_Y, ...)
+ * (resp. d2dhip_unpack(D2DHIP_Y_TO_X, ...) then the x stage) bit for bit,
+ * with one read and one write of the data instead of two.  With p_row = 1
+ * the buffer order is the plain spectral x-pencil. */
+int d2dhip_exec_r2c_pack(d2dhip_plan plan, const void *in_r, void *send,
+                         const d2dhip_decomp *d, int even);
+int d2dhip_exec_c2r_unpack(d2dhip_plan plan, const void *recv, void *out_r,
+                           const d2dhip_decomp *d, int even);
 
 /* ---- many grids in one launch ---------------------------------------------
  * The multiple-grid use of the reference (decomp_2d_fft_set_ngrid /
