@@ -411,6 +411,7 @@ static FastVariant make_variant(const char *name) {
                                                             " c2r xs w" #WPE)
 
 using R16_16_4 = RList<16, 16, 4>;
+using R16_16_8 = RList<16, 16, 8>;    // 2048
 using R8_8_8 = RList<8, 8, 8>;
 using R16_9_5 = RList<16, 9, 5>;     // 720
 using R4_9_3_5 = RList<4, 9, 3, 5>;  // 540
@@ -624,6 +625,10 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_SPLIT_ALL(double, 1024, 4, 256, true, R16_16_4, 3),
       D2D_SPLIT_ALL(double, 1024, 8, 512, true, R16_16_4, 3),
       // FP64, 512
+      // FP64 2048 lines (the x stages of 2048-point pencils)
+      D2D_VARIANTS(double, 2048, 1, 128, true, R16_16_8),
+      D2D_VARIANTS(double, 2048, 2, 256, true, R16_16_8),
+      D2D_SPLIT_ALL(double, 2048, 2, 256, true, R16_16_8, 3),
       D2D_VARIANTS(double, 512, 8, 512, false, R8_8_8),
       D2D_VARIANTS(double, 512, 4, 256, false, R8_8_8),
       D2D_VARIANTS(double, 512, 4, 256, true, R8_8_8),

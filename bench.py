@@ -253,6 +253,35 @@ def bench_config(args, torch, d2d):
         cfg = {"workload": "r2c+c2r FP64 %dx%dx%d %s" % (nx, ny, nz, "PHYSICAL_IN_X" if
                                                          fmt == d2d.PHYSICAL_IN_X else "PHYSICAL_IN_Z")}
         extra["kernels"] = e.stage_kernels()
+    elif w == "r2cpack2048":
+        # the same rank: its x stage (r2c along x, 2048 -> 1025) written
+        # straight into the x->y send buffers of the spectral decomposition,
+        # against the x stage (plain spectral x-pencil) followed by the pack
+        dec = d2d.decomp_info_init(1025, 2048, 2048, 4, 4, 0, 0)
+        nx, ny, nz = 2048, dec.xsz[1], dec.xsz[2]
+        one = d2d.decomp_info_init(1025, ny, nz)
+        x = torch.empty(nx * ny * nz, dtype=torch.float64, device="cuda")
+        d2d.fill_uniform(x, x.numel(), d2d.R8, 2)
+        spec = torch.empty(1025 * ny * nz, dtype=torch.complex128, device="cuda")
+        buf = torch.empty(d2d.pack_buffer_elems(d2d.X_TO_Y, dec, 0), dtype=torch.complex128,
+                          device="cuda")
+        e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, nx, ny, nz, tune=not args.no_tune)
+
+        def step():
+            e.fft_r2c_pack(x, buf, dec)
+        ms = timed(step)
+
+        def unfused():
+            e.fft_r2c_pack(x, spec, one)
+            d2d.pack(d2d.X_TO_Y, d2d.C16, spec, buf, dec, 0)
+        ms_unfused = timed(unfused)
+        nbytes = x.numel() * 8 + spec.numel() * 16
+        err = 0.0
+        cfg = {"workload": "r2c x stage fused with the mem_split_xy pack, one rank of 2048^3 on "
+                           "4x4 (2048x512x512 real8 -> 1025x512x512 complex16 send buffers)"}
+        extra["unfused_ms"] = round(ms_unfused, 4)
+        extra["fused_speedup"] = round(ms_unfused / ms, 3)
+        extra["kernels"] = {"r2c": e.stage_kernels()["r2c"]}
     else:  # pack2048
         dec = d2d.decomp_info_init(2048, 2048, 2048, 4, 4, 0, 0)
         xs = dec.xsz
@@ -289,10 +318,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan-time autotuner (use the default kernels)")
     ap.add_argument("--workload", choices=["r2c1024", "batch64", "c2c128", "r2c512", "r2cz720",
-                                           "pack2048"], default="r2c1024",
+                                           "pack2048", "r2cpack2048"], default="r2c1024",
                     help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids; "
                          "c2c128 / r2c512 / r2cz720 / pack2048: the other parity configurations, "
-                         "timed")
+                         "timed; r2cpack2048: the pack2048 rank's x stage fused with its pack")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -307,7 +336,7 @@ def main():
     if args.workload == "batch64":
         bench_batch(args, torch, d2d)
         return
-    if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048"):
+    if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048"):
         bench_config(args, torch, d2d)
         return
     n = args.n

@@ -23,6 +23,7 @@ CASES = [
     ((100, 30, 20), 3, 2, 2, 1),
     ((17, 13, 11), 2, 2, 1, 1),
     ((48, 40, 36), 5, 1, 4, 0),
+    ((2048, 32, 16), 4, 2, 3, 1),
 ]
 
 
