@@ -123,6 +123,9 @@ def lib():
             "d2dhip_exec_r2c": ([pl, vp, vp], ci),
             "d2dhip_exec_c2r": ([pl, vp, vp], ci),
             "d2dhip_exec_r2c_pack": ([pl, vp, vp, ctypes.POINTER(Decomp), ci], ci),
+            "d2dhip_plan_capture_begin": ([pl], ci),
+            "d2dhip_plan_capture_end": ([pl], ci),
+            "d2dhip_plan_graph_launch": ([pl], ci),
             "d2dhip_exec_c2r_unpack": ([pl, vp, vp, ctypes.POINTER(Decomp), ci], ci),
             "d2dhip_transpose": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), vp], ci),
             "d2dhip_batch_create": ([ctypes.POINTER(vp), ci, ctypes.POINTER(vp), vp], ci),
@@ -305,6 +308,19 @@ class FFTEngine:
     def fft_c2r(self, inp, out):
         self._sync_stream()
         return check(lib().d2dhip_exec_c2r(self._plan, _ptr(inp), _ptr(out)))
+
+    def capture_begin(self):
+        """Record the following exec calls on this engine into a hipGraph."""
+        self._sync_stream()
+        check(lib().d2dhip_plan_capture_begin(self._plan))
+
+    def capture_end(self):
+        check(lib().d2dhip_plan_capture_end(self._plan))
+
+    def graph_launch(self):
+        """Replay the recorded exec calls on the current stream."""
+        self._sync_stream()
+        check(lib().d2dhip_plan_graph_launch(self._plan))
 
     def fft_r2c_pack(self, inp, send, decomp, even=False):
         """The x stage (r2c along x) of a PHYSICAL_IN_X engine, written in the

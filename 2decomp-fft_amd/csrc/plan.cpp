@@ -97,6 +97,11 @@ struct d2dhip_plan_s {
   AxisDesc *d_pdesc = nullptr;
   long long h_pack[3 * D2DHIP_MAX_PROCS] = {};
   AxisDesc h_pdesc{};
+  // hipGraph of the exec calls made between capture_begin and capture_end
+  hipStream_t cap_stream = nullptr, cap_keep = nullptr;
+  bool capturing = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
 };
 
 namespace {
@@ -973,13 +978,24 @@ int d2dhip_plan_destroy(d2dhip_plan p) {
   if (p->d_cdesc) (void)hipFree(p->d_cdesc);
   if (p->d_pack) (void)hipFree(p->d_pack);
   if (p->d_pdesc) (void)hipFree(p->d_pdesc);
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->capturing) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(p->cap_stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   delete p;
   return D2DHIP_SUCCESS;
 }
 
 int d2dhip_plan_set_stream(d2dhip_plan p, void *stream) {
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
-  p->stream = (hipStream_t)stream;
+  if (p->capturing)
+    p->cap_keep = (hipStream_t)stream;  // takes effect when the capture ends
+  else
+    p->stream = (hipStream_t)stream;
   return D2DHIP_SUCCESS;
 }
 
@@ -999,6 +1015,8 @@ int d2dhip_plan_get_sizes(d2dhip_plan p, int ph_sz[3], int sp_sz[3]) {
 }
 
 int d2dhip_plan_profile(d2dhip_plan p, int enable) {
+  if (p && p->capturing && enable)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "plan is capturing: no stage profiling");
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
   p->prof = enable ? 1 : 0;
   return D2DHIP_SUCCESS;
@@ -1181,6 +1199,8 @@ namespace {
 // src/transpose_y_to_x.f90:384-441) applied by the stage's stores / loads.
 int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
                        const d2dhip_decomp *d, int even) {
+  if (p->capturing)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "fused pack / unpack calls are not captured into a graph");
   if (p->format != D2DHIP_PHYSICAL_IN_X)
     return set_error(D2DHIP_ERR_UNSUPPORTED, "fused pack / unpack needs a PHYSICAL_IN_X plan");
   const long long nxc = p->nx / 2 + 1;
@@ -1223,6 +1243,60 @@ int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
 }
 
 }  // namespace
+
+// ---- hipGraph of a plan's exec calls --------------------------------------------
+// Between capture_begin and capture_end the plan launches on a capturing
+// stream of its own, so that every exec call (with the buffers given to it)
+// is recorded; graph_launch replays the recording on the plan's stream.
+// The exec paths issue only kernels and device-to-device copies, no memset
+// and no host-synchronous call, so a replay repeats them exactly.
+int d2dhip_plan_capture_begin(d2dhip_plan p) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (p->capturing) return set_error(D2DHIP_ERR_INVALID_VALUE, "plan is already capturing");
+  if (p->prof) return set_error(D2DHIP_ERR_INVALID_VALUE, "stage profiling is on: no capture");
+  if (!p->cap_stream) {
+    const int st = hip_check(hipStreamCreate(&p->cap_stream), "hipStreamCreate");
+    if (st) return st;
+  }
+  // the capture starts after the work already queued on the plan's stream
+  hipEvent_t e = nullptr;
+  int st = hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  if (!st) st = hip_check(hipEventRecord(e, p->stream), "hipEventRecord");
+  if (!st) st = hip_check(hipStreamWaitEvent(p->cap_stream, e, 0), "hipStreamWaitEvent");
+  if (e) (void)hipEventDestroy(e);
+  if (!st)
+    st = hip_check(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal),
+                   "hipStreamBeginCapture");
+  if (st) return st;
+  p->cap_keep = p->stream;
+  p->stream = p->cap_stream;
+  p->capturing = true;
+  return D2DHIP_SUCCESS;
+}
+
+int d2dhip_plan_capture_end(d2dhip_plan p) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (!p->capturing) return set_error(D2DHIP_ERR_INVALID_VALUE, "plan is not capturing");
+  hipGraph_t g = nullptr;
+  int st = hip_check(hipStreamEndCapture(p->cap_stream, &g), "hipStreamEndCapture");
+  p->stream = p->cap_keep;
+  p->capturing = false;
+  if (st) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->graph = g;
+  p->graph_exec = nullptr;
+  return hip_check(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+int d2dhip_plan_graph_launch(d2dhip_plan p) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (!p->graph_exec) return set_error(D2DHIP_ERR_INVALID_VALUE, "no captured graph");
+  return hip_check(hipGraphLaunch(p->graph_exec, p->stream), "hipGraphLaunch");
+}
 
 // The forward x stage of a PHYSICAL_IN_X plan (r2c along x) writing the
 // send buffers of transpose_x_to_y: r2c_1m_x followed by mem_split_xy

@@ -230,6 +230,11 @@ def bench_config(args, torch, d2d):
         err = float(torch.linalg.vector_norm(y / n ** 3 - x) / torch.linalg.vector_norm(x))
         cfg = {"workload": "c2c FP64 128^3 fwd+bwd PHYSICAL_IN_X, uniform seed 0"}
         extra["kernels"] = e.stage_kernels()
+        # the same pair recorded once into a hipGraph and replayed
+        e.capture_begin()
+        step()
+        e.capture_end()
+        extra["graph_ms"] = round(timed(e.graph_launch), 4)
     elif w in ("r2c512", "r2cz720"):
         fmt = d2d.PHYSICAL_IN_X if w == "r2c512" else d2d.PHYSICAL_IN_Z
         nx, ny, nz = (512, 512, 512) if w == "r2c512" else (720, 540, 432)

@@ -163,6 +163,18 @@ int d2dhip_exec_r2c(d2dhip_plan plan, const void *in_r, void *out_c);
 /* decomp_2d_fft_3d(in_c, out_r): fft_3d_c2r, src/fft_common_3d.f90:194-296.
  * in_c is preserved unless the plan has D2DHIP_FLAG_INPLACE. */
 int d2dhip_exec_c2r(d2dhip_plan plan, void *in_c, void *out_r);
+/* hipGraph of a plan's transforms (no reference counterpart; the reference's
+ * GPU backend launches every stage eagerly): between capture_begin and
+ * capture_end the plan records the exec calls made on it, with their buffers
+ * (for example the forward and inverse transforms of a time step), on a
+ * capturing stream of its own that starts after the work already queued on
+ * the plan's stream; graph_launch replays the recording on the plan's
+ * stream.  Stream-ordered like the exec calls; a new capture replaces the
+ * old graph.  The fused pack / unpack calls and stage profiling are refused
+ * while capturing. */
+int d2dhip_plan_capture_begin(d2dhip_plan plan);
+int d2dhip_plan_capture_end(d2dhip_plan plan);
+int d2dhip_plan_graph_launch(d2dhip_plan plan);
 /* Transposes fused with the adjacent FFT stage, for one rank of a process
  * grid (PHYSICAL_IN_X plans of the rank's x-pencil, nx x ny x nz real):
  * d2dhip_exec_r2c_pack runs the forward x stage (r2c_1m_x) and writes its
