@@ -122,6 +122,9 @@ def lib():
             "d2dhip_exec_c2c": ([pl, vp, vp, ci], ci),
             "d2dhip_exec_r2c": ([pl, vp, vp], ci),
             "d2dhip_exec_c2r": ([pl, vp, vp], ci),
+            "d2dhip_plan_set_dtt": ([pl, ci, ctypes.POINTER(ci)], ci),
+            "d2dhip_plan_get_dtt": ([pl, ctypes.POINTER(ci)], ci),
+            "d2dhip_exec_dtt": ([pl, vp, vp, ci], ci),
             "d2dhip_exec_r2c_pack": ([pl, vp, vp, ctypes.POINTER(Decomp), ci], ci),
             "d2dhip_plan_capture_begin": ([pl], ci),
             "d2dhip_plan_capture_end": ([pl], ci),
@@ -221,7 +224,7 @@ class FFTEngine:
     """One decomp_2d_fft engine (src/fft_common.f90:141-242) at one rank."""
 
     def __init__(self, pencil, nx, ny, nz, prec=PREC_F64, inplace=False, skip_xyz=None,
-                 stream=None, tune=False):
+                 stream=None, tune=False, opt_DTT=None):
         flags = FLAG_INPLACE if inplace else 0
         if tune:
             flags |= FLAG_TUNE
@@ -238,6 +241,8 @@ class FFTEngine:
         check(lib().d2dhip_plan_create(ctypes.byref(self._plan), pencil, nx, ny, nz, prec, flags,
                                        st))
         self._explicit_stream = stream is not None
+        if opt_DTT is not None:
+            self.set_dtt(opt_DTT)
 
     def _sync_stream(self):
         if not self._explicit_stream:
@@ -308,6 +313,30 @@ class FFTEngine:
     def fft_c2r(self, inp, out):
         self._sync_stream()
         return check(lib().d2dhip_exec_c2r(self._plan, _ptr(inp), _ptr(out)))
+
+    def set_dtt(self, dtt):
+        """DCT / DST types of the x, y, z axes (3 values, 1-8), or 12 values
+        with ifirst, ndismiss and ofirst (decomp_2d_fft_engine_dtt_init,
+        src/fft_fftw3_f03.f90:341-390)."""
+        v = [int(t) for t in dtt]
+        arr = (ctypes.c_int * len(v))(*v)
+        check(lib().d2dhip_plan_set_dtt(self._plan, len(v), arr))
+
+    @property
+    def dtt(self):
+        """engine%dtt: types, ifirst, ndismiss, ofirst, backward types (15 values)."""
+        v = (ctypes.c_int * 15)()
+        check(lib().d2dhip_plan_get_dtt(self._plan, v))
+        return tuple(v)
+
+    def dtt_input(self):
+        """decomp_2d_fft_get_dtt_input (src/fft_fftw3_f03.f90:629-643)."""
+        return self.dtt[0:3]
+
+    def dtt_3d_r2r(self, inp, out, isign):
+        """decomp_2d_dtt_3d_r2r (src/fft_fftw3_f03.f90:1959-2023); inp is preserved."""
+        self._sync_stream()
+        return check(lib().d2dhip_exec_dtt(self._plan, _ptr(inp), _ptr(out), isign))
 
     def capture_begin(self):
         """Record the following exec calls on this engine into a hipGraph."""

@@ -193,6 +193,98 @@ constexpr int general_wpe() {
   return sizeof(T) == 8 ? 1 : 4;
 }
 
+// ---- real-to-real (DCT / DST) stages ---------------------------------------------
+// A DCT or DST of L points is the DFT of an even or odd real extension e of
+// length m (FFTW's REDFT / RODFT definitions, unnormalised):
+//   type 1 DCT-I    m = 2(L-1)  e[j] = e[m-j] = X_j                 Y_k =  Re E[k]
+//   type 2 DCT-II   m = 4L      e[2j+1] = e[m-2j-1] = X_j           Y_k =  Re E[k]
+//   type 3 DCT-III  m = 4L      e[j] = e[m-j] = X_j (e[0] once)     Y_k =  Re E[2k+1]
+//   type 4 DCT-IV   m = 8L      e[2j+1] = e[m-2j-1] = X_j           Y_k =  Re E[2k+1]
+//   type 5 DST-I    m = 2(L+1)  e[j+1] = -e[m-j-1] = X_j            Y_k = -Im E[k+1]
+//   type 6 DST-II   m = 4L      e[2j+1] = -e[m-2j-1] = X_j          Y_k = -Im E[k+1]
+//   type 7 DST-III  m = 4L      e[j+1] = X_j, e[m-j-1] = -X_j (j < L-1)
+//                                                                   Y_k = -Im E[2k+1]
+//   type 8 DST-IV   m = 8L      e[2j+1] = -e[m-2j-1] = X_j          Y_k = -Im E[2k+1]
+// (other e[i] zero).  Two real lines a, b go through one complex transform
+// as z = e_a + i e_b: an even real extension has a real DFT and an odd one
+// an imaginary DFT, so Z = E_a + i E_b separates as (Re Z, Im Z) for the
+// DCTs and as (Im Z, -Re Z) = (Im E_a, Im E_b) for the DSTs.  Only the
+// points [dtt_first, dtt_first + L) of a line are transformed; the others
+// are copied through, as FFTW's transform of the sub-array leaves them.
+
+// Extension of the tile's line pairs into LDS points [0, m).
+template <typename T>
+__device__ __forceinline__ void dtt_fill(DescPtr d, const void *in, cpx<T> *lds, long long c0b,
+                                         long long c1, int m) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int TX = d->TX, lg = d->log2TX, tmask = TX - 1;
+  const int ldj = d->ldj, ldt = d->ldt;
+  const int type = d->dtt, f = d->dtt_first, L = d->dtt_len;
+  if (type != 1) {  // DCT-I writes every point
+    for (int w = tid; w < TX * m; w += nthr) {
+      const int t = w & tmask, j = w >> lg;
+      lds[j * ldj + t * ldt] = cpx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+  }
+  for (int w = tid; w < TX * L; w += nthr) {
+    const int t = w & tmask, j = w >> lg;
+    const long long ca = 2 * (c0b + t);
+    const long long base = c1 * d->in_s1 + (long long)(f + j) * d->in_sj;
+    T a = T(0), b = T(0);
+    if (ca < d->C0r) a = gload_r<T>(in, base + ca * d->in_s0);
+    if (ca + 1 < d->C0r) b = gload_r<T>(in, base + (ca + 1) * d->in_s0);
+    int p1, p2;
+    switch (type) {
+      case 1: p1 = j; p2 = (j > 0 && j < L - 1) ? m - j : -1; break;
+      case 3: p1 = j; p2 = j > 0 ? m - j : -1; break;
+      case 5: p1 = j + 1; p2 = m - j - 1; break;
+      case 7: p1 = j + 1; p2 = j < L - 1 ? m - j - 1 : -1; break;
+      default: p1 = 2 * j + 1; p2 = m - 2 * j - 1; break;  // types 2, 4, 6, 8
+    }
+    lds[p1 * ldj + t * ldt] = cpx<T>{a, b};
+    if (p2 >= 0) lds[p2 * ldj + t * ldt] = type >= 5 ? cpx<T>{-a, -b} : cpx<T>{a, b};
+  }
+  __syncthreads();
+}
+
+// The transformed lines from the DFT in LDS, and the copied-through points.
+template <typename T>
+__device__ __forceinline__ void dtt_store(DescPtr d, const void *in, void *out, const cpx<T> *lds,
+                                          long long c0b, long long c1) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int TX = d->TX, lg = d->log2TX, tmask = TX - 1;
+  const int ldj = d->ldj, ldt = d->ldt;
+  const int type = d->dtt, f = d->dtt_first, L = d->dtt_len, np = d->dtt_n;
+  for (int w = tid; w < TX * np; w += nthr) {
+    const int t = w & tmask, k = w >> lg;
+    const long long ca = 2 * (c0b + t);
+    if (ca >= d->C0r) continue;
+    const bool two = ca + 1 < d->C0r;
+    const long long ib = c1 * d->in_s1 + (long long)k * d->in_sj;
+    const long long ob = c1 * d->out_s1 + (long long)k * d->out_sj;
+    T a, b = T(0);
+    const int kk = k - f;
+    if (kk >= 0 && kk < L) {
+      const int odd = (type == 3 || type == 4 || type == 7 || type == 8);
+      const int idx = odd ? 2 * kk + 1 : (type == 5 || type == 6) ? kk + 1 : kk;
+      const cpx<T> z = lds[idx * ldj + t * ldt];
+      if (type <= 4) {
+        a = z.x;
+        b = z.y;
+      } else {
+        a = -z.y;
+        b = z.x;
+      }
+    } else {
+      a = gload_r<T>(in, ib + ca * d->in_s0);
+      if (two) b = gload_r<T>(in, ib + (ca + 1) * d->in_s0);
+    }
+    gstore_r<T>(out, ob + ca * d->out_s0, a);
+    if (two) gstore_r<T>(out, ob + (ca + 1) * d->out_s0, b);
+  }
+}
+
 // One tile of a stage, by the whole workgroup (any blockDim at least the
 // stage's planned block size).
 template <typename T, int E, int IO>
@@ -232,15 +324,21 @@ __device__ __forceinline__ void axis_tile(DescPtr d, const AxisIo &io, long long
     }
     __syncthreads();
   }
+  if constexpr (IO == IO_R2R) dtt_fill<T>(d, in, lds, c0b, c1, n);
 
+  // real-to-real stages keep the line in LDS from the fill to the store:
+  // their passes are those of a complex transform
+  constexpr int PIO = IO == IO_R2R ? IO_C2C : IO;
   const int np = d->npass;
   for (int p = 0; p < np; ++p) {
-    const bool sg = (p == 0) && (IO != IO_C2R);
-    const bool dg = (p == np - 1) && (IO != IO_R2C) && (np >= 2);
-    dispatch_pass<T, E, IO>(d, io, p, sg, dg, c0b, c1);
+    const bool sg = (p == 0) && (IO != IO_C2R) && (IO != IO_R2R);
+    const bool dg = (p == np - 1) && (IO != IO_R2C) && (IO != IO_R2R) && (np >= 2);
+    dispatch_pass<T, E, PIO>(d, io, p, sg, dg, c0b, c1);
   }
 
-  if constexpr (IO == IO_R2C) {
+  if constexpr (IO == IO_R2R) {
+    dtt_store<T>(d, in, out, lds, c0b, c1);
+  } else if constexpr (IO == IO_R2C) {
     const T half = T(0.5);
     for (int w = tid; w < TX * nh; w += nthr) {
       const int t = w & tmask, k = w >> lg;
@@ -401,7 +499,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(genera
   const void *bhat = d->bhat;
 
   // line values (conjugated for inverse transforms) into LDS[0, n)
-  if constexpr (IO == IO_C2R) {
+  if constexpr (IO == IO_R2R) {
+    dtt_fill<T>(desc_ptr(d), in, lds, c0b, c1, n);
+  } else if constexpr (IO == IO_C2R) {
     for (int w = tid; w < TX * nh; w += nthr) {
       const int t = w & tmask, k = w >> lg;
       const long long ca = 2 * (c0b + t);
@@ -453,7 +553,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(genera
     e = cpx<T>{y.x * invM, y.y * invM};
   }
   __syncthreads();
-  if constexpr (IO == IO_R2C) {
+  if constexpr (IO == IO_R2R) {
+    dtt_store<T>(desc_ptr(d), in, out, lds, c0b, c1);
+  } else if constexpr (IO == IO_R2C) {
     const T half = T(0.5);
     for (int w = tid; w < TX * nh; w += nthr) {
       const int t = w & tmask, k = w >> lg;
@@ -494,10 +596,12 @@ int axis_kernel_prepare(int prec_bytes) {
     e = set_lds_limit<double, IO_C2C>();
     if (e == hipSuccess) e = set_lds_limit<double, IO_R2C>();
     if (e == hipSuccess) e = set_lds_limit<double, IO_C2R>();
+    if (e == hipSuccess) e = set_lds_limit<double, IO_R2R>();
   } else {
     e = set_lds_limit<float, IO_C2C>();
     if (e == hipSuccess) e = set_lds_limit<float, IO_R2C>();
     if (e == hipSuccess) e = set_lds_limit<float, IO_C2R>();
+    if (e == hipSuccess) e = set_lds_limit<float, IO_R2R>();
   }
   return hip_check(e, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
 }
@@ -526,10 +630,12 @@ int launch_axis(int prec_bytes, int io, const AxisDesc *d_desc, const AxisDesc &
   if (prec_bytes == 8) {
     if (io == IO_C2C) return launch_t<double, IO_C2C>(d_desc, h_desc, io_ptrs, s, st);
     if (io == IO_R2C) return launch_t<double, IO_R2C>(d_desc, h_desc, io_ptrs, s, st);
+    if (io == IO_R2R) return launch_t<double, IO_R2R>(d_desc, h_desc, io_ptrs, s, st);
     return launch_t<double, IO_C2R>(d_desc, h_desc, io_ptrs, s, st);
   }
   if (io == IO_C2C) return launch_t<float, IO_C2C>(d_desc, h_desc, io_ptrs, s, st);
   if (io == IO_R2C) return launch_t<float, IO_R2C>(d_desc, h_desc, io_ptrs, s, st);
+  if (io == IO_R2R) return launch_t<float, IO_R2R>(d_desc, h_desc, io_ptrs, s, st);
   return launch_t<float, IO_C2R>(d_desc, h_desc, io_ptrs, s, st);
 }
 

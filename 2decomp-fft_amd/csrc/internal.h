@@ -17,7 +17,10 @@ constexpr int kMaxPass = 16;
 // four radix-4 and so on.
 constexpr int kRegCap = 18;
 
-enum IoKind : int { IO_C2C = 0, IO_R2C = 1, IO_C2R = 2 };
+// IO_R2R: a real-to-real trigonometric transform (DCT / DST) of each line,
+// computed as a complex FFT of a symmetric extension (two real lines per
+// complex line).
+enum IoKind : int { IO_C2C = 0, IO_R2C = 1, IO_C2R = 2, IO_R2R = 3 };
 
 struct PassInfo {
   int r;       // radix
@@ -61,6 +64,10 @@ struct AxisDesc {
   // point i of line l at base - b1 + i + l * w.  npack = 0: plain layout.
   const long long *pack;
   int npack;
+  // IO_R2R stages: transform type (1-4 DCT-I..IV, 5-8 DST-I..IV), the
+  // transformed points [dtt_first, dtt_first + dtt_len) of each line of
+  // dtt_n points (the others pass through), `n` = the extension's length.
+  int dtt, dtt_first, dtt_len, dtt_n;
 };
 
 // Index of complex point i of line l in the packed order of d (npack > 0).

@@ -43,6 +43,9 @@ struct StageGeom {
   long long in_elems = 0, out_elems = 0;  // elements (real or complex) of each side
   std::vector<const FastVariant *> cands;  // specialised kernels for this stage
   int choice = -1;                         // index into cands, -1 = general kernel
+  // real-to-real stages (io IO_R2R): type 1-8, first transformed point,
+  // transformed points, points per line; n is the extension's length
+  int dtt = 0, dtt_first = 0, dtt_len = 0, dtt_n = 0;
 };
 
 // Chirp tables of a Bluestein length.
@@ -63,8 +66,13 @@ struct d2dhip_plan_s {
   std::map<int, BlueTables> blue;
   // Stage descriptors: 0-2 c2c along x, y, z of the (nx,ny,nz) pencil;
   // 3 the real-to-complex stage; 4 the spectral y stage; 5 the other
-  // spectral c2c stage (z for PHYSICAL_IN_X, x for _Z); 6 the c2r stage.
-  static constexpr int kStages = 7;
+  // spectral c2c stage (z for PHYSICAL_IN_X, x for _Z); 6 the c2r stage;
+  // 7-9 the forward real-to-real (DCT / DST) stages along x, y, z and 10-12
+  // the backward ones, once d2dhip_plan_set_dtt has set them.
+  static constexpr int kFftStages = 7;
+  static constexpr int kStages = 13;
+  bool has_dtt = false;
+  int dtt[15] = {};  // types 1-3, ifirst 4-6, ndismiss 7-9, ofirst 10-12, backward types 13-15
   StageGeom geom[kStages];
   AxisDesc h_desc[kStages];
   AxisShape shape[kStages];
@@ -459,6 +467,10 @@ int configure(d2dhip_plan_s *p, int idx, int c) {
   a.ldj = s.ldj;
   a.ldt = s.ldt;
   a.nh = g.n / 2 + 1;
+  a.dtt = g.dtt;
+  a.dtt_first = g.dtt_first;
+  a.dtt_len = g.dtt_len;
+  a.dtt_n = g.dtt_n;
   p->fast[idx] = fv;
   g.choice = c;
   return D2DHIP_SUCCESS;
@@ -495,6 +507,40 @@ int make_stage(d2dhip_plan_s *p, int idx, int axis, int io, const StageIo &sio, 
     }
   }
   return configure(p, idx, g.cands.empty() ? -1 : 0);
+}
+
+// A real-to-real stage along `axis` of the (nx,ny,nz) real array, in place
+// or not (same layout on both sides): `len` points from `first` (0-based) of
+// every line go through a transform of `type`, computed as the DFT of the
+// extension of length m (the general kernel; Bluestein where m has a prime
+// factor above 17).
+int make_dtt_stage(d2dhip_plan_s *p, int idx, int axis, int type, int first, int len) {
+  StageGeom &g = p->geom[idx];
+  const StageIo v = cols_view(axis, p->nx, p->ny, p->nz);
+  g.axis = axis;
+  g.io = IO_R2R;
+  switch (type) {
+    case 1: g.n = 2 * (len - 1); break;
+    case 5: g.n = 2 * (len + 1); break;
+    case 4:
+    case 8: g.n = 8 * len; break;
+    default: g.n = 4 * len; break;
+  }
+  g.C0r = v.C0r;
+  g.C1 = v.C1;
+  g.in_s0 = v.in_s0;
+  g.in_s1 = v.in_s1;
+  g.in_sj = v.in_sj;
+  g.out_s0 = v.out_s0;
+  g.out_s1 = v.out_s1;
+  g.out_sj = v.out_sj;
+  g.in_elems = g.out_elems = (long long)p->nx * p->ny * p->nz;
+  g.cands.clear();
+  g.dtt = type;
+  g.dtt_first = first;
+  g.dtt_len = len;
+  g.dtt_n = axis == 0 ? p->nx : (axis == 1 ? p->ny : p->nz);
+  return configure(p, idx, -1);
 }
 
 int upload_chunk_descs(d2dhip_plan_s *p);
@@ -1024,33 +1070,43 @@ int d2dhip_plan_profile(d2dhip_plan p, int enable) {
 
 int d2dhip_plan_profile_report(d2dhip_plan p, double ms[7], int launches[7]) {
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
-  for (int i = 0; i < d2dhip_plan_s::kStages; ++i) {
+  constexpr int ns = d2dhip_plan_s::kFftStages;
+  for (int i = 0; i < ns; ++i) {
     if (ms) ms[i] = 0.0;
     if (launches) launches[i] = 0;
   }
   int st = D2DHIP_SUCCESS;
-  double wsum[d2dhip_plan_s::kStages] = {};
+  double wsum[ns] = {};
   for (auto &r : p->recs) {
     if (!st) st = hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
     float t = 0.f;
     if (!st) st = hip_check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
     if (!st) {
-      if (ms) ms[r.stage] += t;
-      if (launches) wsum[r.stage] += r.w;
+      // real-to-real stages count with the other stages along their axis
+      const int slot = r.stage < ns ? r.stage : p->geom[r.stage].axis;
+      if (ms) ms[slot] += t;
+      if (launches) wsum[slot] += r.w;
     }
     p->pool.push_back(r.a);
     p->pool.push_back(r.b);
   }
   p->recs.clear();
   if (launches)
-    for (int i = 0; i < d2dhip_plan_s::kStages; ++i) launches[i] = (int)std::lround(wsum[i]);
+    for (int i = 0; i < ns; ++i) launches[i] = (int)std::lround(wsum[i]);
   return st;
+}
+
+// Stages 7-12 exist once the plan has real-to-real transforms.
+static int check_stage(const d2dhip_plan_s *p, int stage) {
+  if (stage < 0 || stage >= d2dhip_plan_s::kStages ||
+      (stage >= d2dhip_plan_s::kFftStages && !p->has_dtt))
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "stage out of range");
+  return D2DHIP_SUCCESS;
 }
 
 int d2dhip_plan_stage_kernel(d2dhip_plan p, int stage, char *name, int len) {
   if (!p || !name || len <= 0) return set_error(D2DHIP_ERR_NULL, "NULL argument");
-  if (stage < 0 || stage >= d2dhip_plan_s::kStages)
-    return set_error(D2DHIP_ERR_INVALID_VALUE, "stage out of range");
+  if (int st = check_stage(p, stage)) return st;
   const std::string s = choice_name(p, stage);
   std::snprintf(name, (size_t)len, "%s", s.c_str());
   return D2DHIP_SUCCESS;
@@ -1058,8 +1114,7 @@ int d2dhip_plan_stage_kernel(d2dhip_plan p, int stage, char *name, int len) {
 
 int d2dhip_plan_stage_candidate(d2dhip_plan p, int stage, int idx, char *name, int len) {
   if (!p || !name || len <= 0) return set_error(D2DHIP_ERR_NULL, "NULL argument");
-  if (stage < 0 || stage >= d2dhip_plan_s::kStages)
-    return set_error(D2DHIP_ERR_INVALID_VALUE, "stage out of range");
+  if (int st = check_stage(p, stage)) return st;
   const StageGeom &g = p->geom[stage];
   if (idx < 0 || idx > (int)g.cands.size())
     return set_error(D2DHIP_ERR_INVALID_VALUE, "candidate out of range");
@@ -1069,8 +1124,7 @@ int d2dhip_plan_stage_candidate(d2dhip_plan p, int stage, int idx, char *name, i
 
 int d2dhip_plan_set_stage_kernel(d2dhip_plan p, int stage, const char *name) {
   if (!p || !name) return set_error(D2DHIP_ERR_NULL, "NULL argument");
-  if (stage < 0 || stage >= d2dhip_plan_s::kStages)
-    return set_error(D2DHIP_ERR_INVALID_VALUE, "stage out of range");
+  if (int st = check_stage(p, stage)) return st;
   const StageGeom &g = p->geom[stage];
   int c = -2;
   if (std::strcmp(name, "general") == 0) c = -1;
@@ -1187,6 +1241,100 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
   }
   if (warn)
     return set_error(D2DHIP_WARN_SKIP_IGNORED, "r2c / c2r transform can not be skipped");
+  return D2DHIP_SUCCESS;
+}
+
+// decomp_2d_fft_engine_dtt_init (src/fft_fftw3_f03.f90:341-390) with
+// dtt_assign_default (:394-420) and dtt_invert (:424-444): 3 types, or 12
+// values (types, ifirst, ndismiss, ofirst; 1-based), per axis x, y, z.
+int d2dhip_plan_set_dtt(d2dhip_plan p, int ndtt, const int *in_dtt) {
+  if (!p || !in_dtt) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (p->capturing)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "plan is capturing: DTT set-up allocates");
+  if (ndtt != 3 && ndtt != 12) return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid argument");
+  int v[15] = {};
+  for (int k = 0; k < 3; ++k)
+    if (in_dtt[k] < 1 || in_dtt[k] > 8)
+      return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid argument: DTT types are 1 to 8");
+  if (ndtt == 12) {
+    for (int k = 0; k < 12; ++k) v[k] = in_dtt[k];
+    for (int k = 0; k < 3; ++k)
+      if (v[3 + k] != v[9 + k])
+        return set_error(D2DHIP_ERR_INVALID_VALUE,
+                         "Setup is not compatible with in-place r2r transforms");
+  } else {
+    for (int k = 0; k < 3; ++k) {
+      v[k] = in_dtt[k];
+      v[3 + k] = (v[k] == 5 || v[k] == 7) ? 2 : 1;  // ifirst: DST-I and DST-III start at 2
+      v[6 + k] = v[k] == 1 ? 0 : (v[k] == 5 ? 2 : 1);  // ndismiss
+      v[9 + k] = v[3 + k];
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    const int t = v[k];
+    v[12 + k] = t == 2 ? 3 : t == 3 ? 2 : t == 6 ? 7 : t == 7 ? 6 : t;
+  }
+  const int npts[3] = {p->nx, p->ny, p->nz};
+  for (int k = 0; k < 3; ++k) {
+    const int len = npts[k] - v[6 + k], first = v[3 + k] - 1;
+    if (len < (v[k] == 1 ? 2 : 1) || first < 0 || first + len > npts[k])
+      return set_error(D2DHIP_ERR_INVALID_VALUE,
+                       "DTT points out of range along axis " + std::to_string(k));
+  }
+  int st = D2DHIP_SUCCESS;
+  for (int k = 0; k < 3 && !st; ++k) {
+    const int len = npts[k] - v[6 + k];
+    st = make_dtt_stage(p, 7 + k, k, v[k], v[3 + k] - 1, len);
+    if (!st) st = make_dtt_stage(p, 10 + k, k, v[12 + k], v[9 + k] - 1, len);
+  }
+  if (!st) st = upload_descs(p);
+  p->has_dtt = !st;
+  if (!st) std::memcpy(p->dtt, v, sizeof(v));
+  return st;
+}
+
+// engine%dtt after set-up, with the transform types as 1-8 (the reference
+// keeps FFTW's kinds there; decomp_2d_fft_get_dtt_input,
+// src/fft_fftw3_f03.f90:629-643, converts them back to these numbers).
+int d2dhip_plan_get_dtt(d2dhip_plan p, int dtt[15]) {
+  if (!p || !dtt) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (!p->has_dtt) return set_error(D2DHIP_ERR_INVALID_VALUE, "No DTT for the current engine");
+  std::memcpy(dtt, p->dtt, sizeof(p->dtt));
+  return D2DHIP_SUCCESS;
+}
+
+// decomp_2d_dtt_3d_r2r (src/fft_fftw3_f03.f90:1959-2023): PHYSICAL_IN_X
+// forward and PHYSICAL_IN_Z backward run x, y, z, the others z, y, x; the
+// backward direction runs the inverse types over the ofirst points.  The
+// first stage writes `out` and the others work in place on it, so `in` is
+// preserved (the reference transforms it in place along the first axis).
+// Skipped axes (opt_skip_XYZ_c2c) are left out as r2r_1m_x/y/z leave them.
+int d2dhip_exec_dtt(d2dhip_plan p, const void *in, void *out, int isign) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (!in || !out) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
+  if (isign != D2DHIP_FORWARD && isign != D2DHIP_BACKWARD)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid value");
+  if (!p->has_dtt) return set_error(D2DHIP_ERR_PLAN_MISMATCH, "No DTT for the current engine");
+  const bool x_first = (p->format == D2DHIP_PHYSICAL_IN_X && isign == D2DHIP_FORWARD) ||
+                       (p->format == D2DHIP_PHYSICAL_IN_Z && isign == D2DHIP_BACKWARD);
+  const int order[3] = {x_first ? 0 : 2, 1, x_first ? 2 : 0};
+  const unsigned skipbit[3] = {D2DHIP_FLAG_SKIP_X, D2DHIP_FLAG_SKIP_Y, D2DHIP_FLAG_SKIP_Z};
+  const int base = isign == D2DHIP_FORWARD ? 7 : 10;
+  const void *src = in;
+  int ran = 0;
+  for (int s = 0; s < 3; ++s) {
+    const int ax = order[s];
+    if (p->flags & skipbit[ax]) continue;
+    int st = run(p, base + ax, src, out, 0);
+    if (st) return st;
+    src = out;
+    ++ran;
+  }
+  if (!ran && in != out) {
+    const size_t bytes = (size_t)p->nx * p->ny * p->nz * p->rb;
+    return hip_check(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, p->stream),
+                     "hipMemcpyAsync");
+  }
   return D2DHIP_SUCCESS;
 }
 

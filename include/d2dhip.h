@@ -129,8 +129,11 @@ int d2dhip_plan_get_sizes(d2dhip_plan plan, int ph_sz[3], int sp_sz[3]);
  * enabled, every kernel launch of the plan is bracketed by a hipEvent pair
  * on the plan's stream.  The report synchronises on those events, returns
  * the summed milliseconds and launch counts per stage and clears them.
- * Stages: 0-2 c2c along x, y, z; 3 the r2c stage; 4 the spectral y stage;
- * 5 the other spectral c2c stage (z for PHYSICAL_IN_X, x for _Z); 6 c2r. */
+ * Stages: 0-2 c2c along x, y, z (with the real-to-real stages along that
+ * axis); 3 the r2c stage; 4 the spectral y stage; 5 the other spectral c2c
+ * stage (z for PHYSICAL_IN_X, x for _Z); 6 c2r.  The stage-kernel calls
+ * below also take 7-9 (forward real-to-real along x, y, z) and 10-12
+ * (backward) once d2dhip_plan_set_dtt has run. */
 int d2dhip_plan_profile(d2dhip_plan plan, int enable);
 int d2dhip_plan_profile_report(d2dhip_plan plan, double ms[7], int launches[7]);
 /* Name of the kernel a stage runs ("general" or a specialised variant). */
@@ -163,6 +166,25 @@ int d2dhip_exec_r2c(d2dhip_plan plan, const void *in_r, void *out_c);
 /* decomp_2d_fft_3d(in_c, out_r): fft_3d_c2r, src/fft_common_3d.f90:194-296.
  * in_c is preserved unless the plan has D2DHIP_FLAG_INPLACE. */
 int d2dhip_exec_c2r(d2dhip_plan plan, void *in_c, void *out_r);
+/* Real-to-real transforms (DCT / DST types 1-8 = DCT-I..IV, DST-I..IV, FFTW's
+ * REDFT00..11, RODFT00..11, unnormalised).  d2dhip_plan_set_dtt replaces
+ * decomp_2d_fft_engine_dtt_init (src/fft_fftw3_f03.f90:341-390, the opt_DTT
+ * argument of decomp_2d_fft_init): dtt holds ndtt = 3 types for x, y, z, or
+ * 12 values (types, ifirst, ndismiss, ofirst; 1-based, ofirst == ifirst)
+ * whose last nine default as in dtt_assign_default (:394-420).  Along an axis
+ * of n points the transform covers points [ifirst, ifirst + n - ndismiss)
+ * of each line; the other points are copied through.  The backward types
+ * are the inverses (dtt_invert, :424-444: DCT-II <-> DCT-III, DST-II <->
+ * DST-III, the others their own).  The stages run on the Stockham kernels
+ * through the DFT of an even / odd extension of the line. */
+int d2dhip_plan_set_dtt(d2dhip_plan plan, int ndtt, const int *dtt);
+/* The resolved 15 values: types, ifirst, ndismiss, ofirst, backward types
+ * (engine%dtt; types as 1-8, decomp_2d_fft_get_dtt_input :629-643). */
+int d2dhip_plan_get_dtt(d2dhip_plan plan, int dtt[15]);
+/* decomp_2d_dtt_3d_r2r(in, out, isign), src/fft_fftw3_f03.f90:1959-2023, on
+ * the (nx,ny,nz) real array of the plan's precision.  in is preserved; in ==
+ * out allowed. */
+int d2dhip_exec_dtt(d2dhip_plan plan, const void *in, void *out, int isign);
 /* hipGraph of a plan's transforms (no reference counterpart; the reference's
  * GPU backend launches every stage eagerly): between capture_begin and
  * capture_end the plan records the exec calls made on it, with their buffers

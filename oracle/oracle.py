@@ -194,3 +194,108 @@ def _padded_size(direction, d, split):
     if direction in (XY, YX):
         return int(d.x1count) * d.p_row
     return int(d.y2count) * d.p_col
+
+
+# ---- real-to-real transforms (DCT / DST) ----------------------------------------
+# decomp_2d_dtt_3d_r2r (src/fft_fftw3_f03.f90:1959-2023) runs FFTW r2r plans
+# (r2r_1m_x/y/z_plan, :1208-1322) over the points [ifirst, ifirst + n -
+# ndismiss) of every line (wrapper_r2r, :2071-2098).  FFTW 3.3.10 is not in
+# this image; its r2r kinds are restated from their published definitions
+# (FFTW manual, "What FFTW Really Computes", 1d real-even / real-odd DFTs),
+# unnormalised, as direct sums with the angles reduced exactly in integers.
+
+DTT_NAMES = {1: "DCT-I", 2: "DCT-II", 3: "DCT-III", 4: "DCT-IV",
+             5: "DST-I", 6: "DST-II", 7: "DST-III", 8: "DST-IV"}
+
+
+def dtt_matrix(t, L):
+    """Y = A @ X for DTT type t (1-8 = REDFT00, 10, 01, 11, RODFT00, 10, 01, 11)."""
+    j = np.arange(L)[None, :].astype(np.int64)
+    k = np.arange(L)[:, None].astype(np.int64)
+    if t == 1:  # X_0 + (-1)^k X_{L-1} + 2 sum_{j=1}^{L-2} X_j cos(pi j k / (L-1))
+        if L < 2:
+            raise ValueError("DCT-I needs 2 or more points")
+        P = 2 * (L - 1)
+        A = 2 * np.cos(np.pi * ((j * k) % P) / (L - 1))
+        A[:, 0] = 1.0
+        A[:, L - 1] = np.where(k[:, 0] % 2 == 0, 1.0, -1.0)
+        return A
+    if t == 2:  # 2 sum X_j cos(pi (2j+1) k / (2L))
+        return 2 * np.cos(np.pi * (((2 * j + 1) * k) % (4 * L)) / (2 * L))
+    if t == 3:  # X_0 + 2 sum_{j>=1} X_j cos(pi j (2k+1) / (2L))
+        A = 2 * np.cos(np.pi * ((j * (2 * k + 1)) % (4 * L)) / (2 * L))
+        A[:, 0] = 1.0
+        return A
+    if t == 4:  # 2 sum X_j cos(pi (2j+1)(2k+1) / (4L))
+        return 2 * np.cos(np.pi * (((2 * j + 1) * (2 * k + 1)) % (8 * L)) / (4 * L))
+    if t == 5:  # 2 sum X_j sin(pi (j+1)(k+1) / (L+1))
+        return 2 * np.sin(np.pi * (((j + 1) * (k + 1)) % (2 * (L + 1))) / (L + 1))
+    if t == 6:  # 2 sum X_j sin(pi (2j+1)(k+1) / (2L))
+        return 2 * np.sin(np.pi * (((2 * j + 1) * (k + 1)) % (4 * L)) / (2 * L))
+    if t == 7:  # (-1)^k X_{L-1} + 2 sum_{j<L-1} X_j sin(pi (j+1)(2k+1) / (2L))
+        A = 2 * np.sin(np.pi * (((j + 1) * (2 * k + 1)) % (4 * L)) / (2 * L))
+        A[:, L - 1] = np.where(k[:, 0] % 2 == 0, 1.0, -1.0)
+        return A
+    if t == 8:  # 2 sum X_j sin(pi (2j+1)(2k+1) / (4L))
+        return 2 * np.sin(np.pi * (((2 * j + 1) * (2 * k + 1)) % (8 * L)) / (4 * L))
+    raise ValueError("DTT type must be 1-8")
+
+
+def dtt_config(dtt):
+    """engine%dtt from opt_DTT: dtt_assign_default (src/fft_fftw3_f03.f90:394-420)
+    for 3 values, then dtt_invert (:424-444).  Types stay 1-8."""
+    v = [int(a) for a in dtt]
+    if len(v) not in (3, 12) or min(v[:3]) < 1 or max(v[:3]) > 8:
+        raise ValueError("Invalid argument")
+    if len(v) == 3:
+        ifirst = [2 if t in (5, 7) else 1 for t in v]
+        ndismiss = [0 if t == 1 else (2 if t == 5 else 1) for t in v]
+        v = v + ifirst + ndismiss + ifirst
+    inv = {2: 3, 3: 2, 6: 7, 7: 6}
+    return tuple(v + [inv.get(t, t) for t in v[:3]])
+
+
+def dtt_1d(x, axis, t, first, L):
+    """Type-t DTT of points [first, first + L) (0-based) of every line along
+    axis; the other points are kept (FFTW's r2r on the sub-array)."""
+    y = np.array(x, dtype=np.float64, order="F", copy=True)
+    A = dtt_matrix(t, L)
+    sl = [slice(None)] * 3
+    sl[axis] = slice(first, first + L)
+    seg = np.moveaxis(y[tuple(sl)], axis, 0)
+    res = np.tensordot(A, seg, axes=(1, 0))
+    y[tuple(sl)] = np.moveaxis(res, 0, axis)
+    return y
+
+
+def dtt3d(x, dtt, isign, fmt=PHYSICAL_IN_X, skip=None):
+    """One-rank decomp_2d_dtt_3d_r2r (src/fft_fftw3_f03.f90:1959-2023): forward
+    types and ifirst, or the inverse types and ofirst for isign = +1; x, y, z
+    for PHYSICAL_IN_X forward / PHYSICAL_IN_Z backward, else z, y, x."""
+    c = dtt_config(dtt)
+    y = np.asfortranarray(x, dtype=np.float64)
+    n = y.shape
+    x_first = (fmt == PHYSICAL_IN_X) == (isign == -1)
+    for ax in ((0, 1, 2) if x_first else (2, 1, 0)):
+        if skip is not None and skip[ax]:
+            continue
+        t = c[ax] if isign == -1 else c[12 + ax]
+        first = (c[3 + ax] if isign == -1 else c[9 + ax]) - 1
+        y = dtt_1d(y, ax, t, first, n[ax] - c[6 + ax])
+    return y
+
+
+def dtt_norm(dtt, shape):
+    """Round-trip factor of examples/dtt/fftw_f03_dtt_x.f90:124-153: 2n-2 for
+    DCT-I and DCT-IV, 2n+2-2*ndismiss for DST-I, 2n-2*ndismiss otherwise."""
+    c = dtt_config(dtt)
+    f = 1.0
+    for ax in range(3):
+        t, n, nd = c[ax], shape[ax], c[6 + ax]
+        if t in (1, 4):
+            f *= 2 * n - 2
+        elif t == 5:
+            f *= 2 * n + 2 - 2 * nd
+        else:
+            f *= 2 * n - 2 * nd
+    return f

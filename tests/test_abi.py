@@ -59,6 +59,10 @@ def test_null_arguments(d2d):
     assert L.d2dhip_exec_c2c(None, None, None, -1) == d2d.ERR_NULL
     assert L.d2dhip_exec_r2c(None, None, None) == d2d.ERR_NULL
     assert L.d2dhip_exec_c2r(None, None, None) == d2d.ERR_NULL
+    three = (ctypes.c_int * 3)(2, 2, 2)
+    assert L.d2dhip_plan_set_dtt(None, 3, three) == d2d.ERR_NULL
+    assert L.d2dhip_plan_get_dtt(None, (ctypes.c_int * 15)()) == d2d.ERR_NULL
+    assert L.d2dhip_exec_dtt(None, None, None, -1) == d2d.ERR_NULL
     assert L.d2dhip_plan_destroy(None) == 0
     d = d2d.decomp_info_init(8, 8, 8)
     assert L.d2dhip_pack(0, 0, None, None, ctypes.byref(d), 0, None) == d2d.ERR_NULL
