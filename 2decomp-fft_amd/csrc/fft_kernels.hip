@@ -194,25 +194,37 @@ constexpr int general_wpe() {
 }
 
 // ---- real-to-real (DCT / DST) stages ---------------------------------------------
-// A DCT or DST of L points is the DFT of an even or odd real extension e of
-// length m (FFTW's REDFT / RODFT definitions, unnormalised):
+// FFTW's REDFT / RODFT kinds, unnormalised, of the L points [dtt_first,
+// dtt_first + L) of each line; the other points of the line are copied
+// through, as FFTW's transform of the sub-array leaves them.  Two real lines
+// a, b always go through one complex transform, z = a' + i b'.
+//
+// Types 2, 3, 6, 7 (DCT-II / III, DST-II / III) run an L-point DFT
+// (J. Makhoul, "A fast cosine transform in one and two dimensions", IEEE
+// Trans. ASSP 28(1), 1980), with w_k = exp(-i pi k / (2L)):
+//   DCT-II   v[j/2] = x_j (j even), v[L-1-j/2] = x_j (j odd), V = DFT_L(v),
+//            Y_k = 2 Re(w_k V_k); the pair separates as in a real-to-complex
+//            transform, V_a = (Z_k + conj Z_{L-k}) / 2, V_b = (Z_k - conj Z_{L-k}) / 2i.
+//   DST-II   Y_k = DCT-II(x')_{L-1-k} with x'_j = (-1)^j x_j.
+//   DCT-III  V_j = conj(w_j) (X_j - i X_{L-j}) (X_L = 0) is Hermitian, so
+//            v = IDFT_L(V) is real and y_{2n} = v_n, y_{2n+1} = v_{L-1-n};
+//            the pair goes in as Z_j = V_a,j + i V_b,j, the inverse DFT as
+//            conj(DFT(conj Z)).
+//   DST-III  (-1)^k DCT-III of X read backwards.
+// Types 1, 4, 5, 8 run the DFT of an even / odd extension e of length m:
 //   type 1 DCT-I    m = 2(L-1)  e[j] = e[m-j] = X_j                 Y_k =  Re E[k]
-//   type 2 DCT-II   m = 4L      e[2j+1] = e[m-2j-1] = X_j           Y_k =  Re E[k]
-//   type 3 DCT-III  m = 4L      e[j] = e[m-j] = X_j (e[0] once)     Y_k =  Re E[2k+1]
 //   type 4 DCT-IV   m = 8L      e[2j+1] = e[m-2j-1] = X_j           Y_k =  Re E[2k+1]
 //   type 5 DST-I    m = 2(L+1)  e[j+1] = -e[m-j-1] = X_j            Y_k = -Im E[k+1]
-//   type 6 DST-II   m = 4L      e[2j+1] = -e[m-2j-1] = X_j          Y_k = -Im E[k+1]
-//   type 7 DST-III  m = 4L      e[j+1] = X_j, e[m-j-1] = -X_j (j < L-1)
-//                                                                   Y_k = -Im E[2k+1]
 //   type 8 DST-IV   m = 8L      e[2j+1] = -e[m-2j-1] = X_j          Y_k = -Im E[2k+1]
-// (other e[i] zero).  Two real lines a, b go through one complex transform
-// as z = e_a + i e_b: an even real extension has a real DFT and an odd one
+// (other e[i] zero).  An even real extension has a real DFT and an odd one
 // an imaginary DFT, so Z = E_a + i E_b separates as (Re Z, Im Z) for the
-// DCTs and as (Im Z, -Re Z) = (Im E_a, Im E_b) for the DSTs.  Only the
-// points [dtt_first, dtt_first + L) of a line are transformed; the others
-// are copied through, as FFTW's transform of the sub-array leaves them.
+// DCTs and as (Im Z, -Re Z) for the DSTs.
+__device__ __forceinline__ bool dtt_half(int type) {
+  return type == 2 || type == 3 || type == 6 || type == 7;
+}
 
-// Extension of the tile's line pairs into LDS points [0, m).
+// The tile's line pairs as the complex sequence the DFT takes, in LDS
+// points [0, m) (m = the stage's DFT length).
 template <typename T>
 __device__ __forceinline__ void dtt_fill(DescPtr d, const void *in, cpx<T> *lds, long long c0b,
                                          long long c1, int m) {
@@ -220,7 +232,8 @@ __device__ __forceinline__ void dtt_fill(DescPtr d, const void *in, cpx<T> *lds,
   const int TX = d->TX, lg = d->log2TX, tmask = TX - 1;
   const int ldj = d->ldj, ldt = d->ldt;
   const int type = d->dtt, f = d->dtt_first, L = d->dtt_len;
-  if (type != 1) {  // DCT-I writes every point
+  const bool half = dtt_half(type);
+  if (!half && type != 1) {  // the zeros of the extension (DCT-I writes every point)
     for (int w = tid; w < TX * m; w += nthr) {
       const int t = w & tmask, j = w >> lg;
       lds[j * ldj + t * ldt] = cpx<T>{T(0), T(0)};
@@ -234,18 +247,46 @@ __device__ __forceinline__ void dtt_fill(DescPtr d, const void *in, cpx<T> *lds,
     T a = T(0), b = T(0);
     if (ca < d->C0r) a = gload_r<T>(in, base + ca * d->in_s0);
     if (ca + 1 < d->C0r) b = gload_r<T>(in, base + (ca + 1) * d->in_s0);
-    int p1, p2;
+    int p1, p2 = -1;
     switch (type) {
+      case 2: p1 = (j & 1) ? L - 1 - (j >> 1) : (j >> 1); break;
+      case 6:  // (-1)^j x_j in the DCT-II order
+        if (j & 1) {
+          a = -a;
+          b = -b;
+        }
+        p1 = (j & 1) ? L - 1 - (j >> 1) : (j >> 1);
+        break;
+      case 3: p1 = j; break;
+      case 7: p1 = L - 1 - j; break;
       case 1: p1 = j; p2 = (j > 0 && j < L - 1) ? m - j : -1; break;
-      case 3: p1 = j; p2 = j > 0 ? m - j : -1; break;
       case 5: p1 = j + 1; p2 = m - j - 1; break;
-      case 7: p1 = j + 1; p2 = j < L - 1 ? m - j - 1 : -1; break;
-      default: p1 = 2 * j + 1; p2 = m - 2 * j - 1; break;  // types 2, 4, 6, 8
+      default: p1 = 2 * j + 1; p2 = m - 2 * j - 1; break;  // types 4, 8
     }
     lds[p1 * ldj + t * ldt] = cpx<T>{a, b};
     if (p2 >= 0) lds[p2 * ldj + t * ldt] = type >= 5 ? cpx<T>{-a, -b} : cpx<T>{a, b};
   }
   __syncthreads();
+  if (type == 3 || type == 7) {
+    // conj(Z_j) = w_j conj(u_j), u_j = (Xa_j + Xb_{L-j}, Xb_j - Xa_{L-j}):
+    // each thread takes the points j and L-j together
+    const int nj = L / 2 + 1;
+    for (int w = tid; w < TX * nj; w += nthr) {
+      const int t = w & tmask, j = w >> lg, jm = L - j;
+      if (j > jm) continue;
+      const cpx<T> A = lds[j * ldj + t * ldt];
+      const cpx<T> B = (j > 0) ? lds[jm * ldj + t * ldt] : cpx<T>{T(0), T(0)};
+      const cpx<T> cs = gload<T>(d->dtw, j);
+      const T ux = A.x + B.y, uy = A.y - B.x;
+      lds[j * ldj + t * ldt] = cpx<T>{cs.x * ux - cs.y * uy, -(cs.x * uy + cs.y * ux)};
+      if (j > 0 && j < jm) {
+        const cpx<T> cm = gload<T>(d->dtw, jm);
+        const T vx = B.x + A.y, vy = B.y - A.x;
+        lds[jm * ldj + t * ldt] = cpx<T>{cm.x * vx - cm.y * vy, -(cm.x * vy + cm.y * vx)};
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // The transformed lines from the DFT in LDS, and the copied-through points.
@@ -266,15 +307,29 @@ __device__ __forceinline__ void dtt_store(DescPtr d, const void *in, void *out, 
     T a, b = T(0);
     const int kk = k - f;
     if (kk >= 0 && kk < L) {
-      const int odd = (type == 3 || type == 4 || type == 7 || type == 8);
-      const int idx = odd ? 2 * kk + 1 : (type == 5 || type == 6) ? kk + 1 : kk;
-      const cpx<T> z = lds[idx * ldj + t * ldt];
-      if (type <= 4) {
-        a = z.x;
-        b = z.y;
+      if (type == 2 || type == 6) {
+        const int q = type == 2 ? kk : L - 1 - kk;
+        const cpx<T> zk = lds[q * ldj + t * ldt];
+        const cpx<T> zm = lds[(q == 0 ? 0 : L - q) * ldj + t * ldt];
+        const cpx<T> cs = gload<T>(d->dtw, q);
+        a = cs.x * (zk.x + zm.x) + cs.y * (zk.y - zm.y);
+        b = cs.x * (zk.y + zm.y) + cs.y * (zm.x - zk.x);
+      } else if (type == 3 || type == 7) {
+        const int q = (kk & 1) ? L - 1 - (kk >> 1) : (kk >> 1);
+        const cpx<T> r = lds[q * ldj + t * ldt];
+        const bool neg = type == 7 && (kk & 1);
+        a = neg ? -r.x : r.x;
+        b = neg ? r.y : -r.y;
       } else {
-        a = -z.y;
-        b = z.x;
+        const int q = (type == 4 || type == 8) ? 2 * kk + 1 : type == 5 ? kk + 1 : kk;
+        const cpx<T> z = lds[q * ldj + t * ldt];
+        if (type <= 4) {
+          a = z.x;
+          b = z.y;
+        } else {
+          a = -z.y;
+          b = z.x;
+        }
       }
     } else {
       a = gload_r<T>(in, ib + ca * d->in_s0);

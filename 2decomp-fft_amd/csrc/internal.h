@@ -18,8 +18,8 @@ constexpr int kMaxPass = 16;
 constexpr int kRegCap = 18;
 
 // IO_R2R: a real-to-real trigonometric transform (DCT / DST) of each line,
-// computed as a complex FFT of a symmetric extension (two real lines per
-// complex line).
+// computed through a complex FFT of two real lines at once (of the line's
+// length for DCT / DST II and III, of a symmetric extension for the others).
 enum IoKind : int { IO_C2C = 0, IO_R2C = 1, IO_C2R = 2, IO_R2R = 3 };
 
 struct PassInfo {
@@ -66,8 +66,12 @@ struct AxisDesc {
   int npack;
   // IO_R2R stages: transform type (1-4 DCT-I..IV, 5-8 DST-I..IV), the
   // transformed points [dtt_first, dtt_first + dtt_len) of each line of
-  // dtt_n points (the others pass through), `n` = the extension's length.
+  // dtt_n points (the others pass through); `n` is the DFT length (dtt_len,
+  // or the length of the line's symmetric extension).
   int dtt, dtt_first, dtt_len, dtt_n;
+  // (cos, sin)(pi k / (2 dtt_len)), k < dtt_len: the twiddles of the
+  // half-length DCT-II / III and DST-II / III stages (n = dtt_len)
+  const void *dtw;
 };
 
 // Index of complex point i of line l in the packed order of d (npack > 0).

@@ -46,6 +46,7 @@ struct StageGeom {
   // real-to-real stages (io IO_R2R): type 1-8, first transformed point,
   // transformed points, points per line; n is the extension's length
   int dtt = 0, dtt_first = 0, dtt_len = 0, dtt_n = 0;
+  const void *dtw = nullptr;  // half-length types: (cos, sin)(pi k / (2 dtt_len))
 };
 
 // Chirp tables of a Bluestein length.
@@ -64,6 +65,7 @@ struct d2dhip_plan_s {
   hipStream_t stream = nullptr;
   std::map<std::vector<int>, TwTable> tw;
   std::map<int, BlueTables> blue;
+  std::map<int, void *> dtw;  // DCT / DST twiddle tables by length
   // Stage descriptors: 0-2 c2c along x, y, z of the (nx,ny,nz) pencil;
   // 3 the real-to-complex stage; 4 the spectral y stage; 5 the other
   // spectral c2c stage (z for PHYSICAL_IN_X, x for _Z); 6 the c2r stage;
@@ -471,6 +473,7 @@ int configure(d2dhip_plan_s *p, int idx, int c) {
   a.dtt_first = g.dtt_first;
   a.dtt_len = g.dtt_len;
   a.dtt_n = g.dtt_n;
+  a.dtw = g.dtw;
   p->fast[idx] = fv;
   g.choice = c;
   return D2DHIP_SUCCESS;
@@ -509,22 +512,55 @@ int make_stage(d2dhip_plan_s *p, int idx, int axis, int io, const StageIo &sio, 
   return configure(p, idx, g.cands.empty() ? -1 : 0);
 }
 
+// (cos, sin)(pi k / (2L)), k < L, in long double rounded to the plan's
+// precision: the twiddles of the half-length DCT / DST stages.
+int get_dtt_twiddles(d2dhip_plan_s *p, int L, const void **out) {
+  auto it = p->dtw.find(L);
+  if (it != p->dtw.end()) {
+    *out = it->second;
+    return D2DHIP_SUCCESS;
+  }
+  const int rb = p->rb;
+  std::vector<unsigned char> h((size_t)L * 2 * rb);
+  for (int k = 0; k < L; ++k) {
+    long double c, s;
+    unit_root(k, 4LL * L, c, s);
+    if (rb == 8) {
+      reinterpret_cast<double *>(h.data())[2 * k] = (double)c;
+      reinterpret_cast<double *>(h.data())[2 * k + 1] = (double)s;
+    } else {
+      reinterpret_cast<float *>(h.data())[2 * k] = (float)c;
+      reinterpret_cast<float *>(h.data())[2 * k + 1] = (float)s;
+    }
+  }
+  void *d = nullptr;
+  if (hipMalloc(&d, h.size()) != hipSuccess) return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(DTT twiddles)");
+  p->dtw[L] = d;
+  *out = d;
+  return hip_check(hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
 // A real-to-real stage along `axis` of the (nx,ny,nz) real array, in place
 // or not (same layout on both sides): `len` points from `first` (0-based) of
-// every line go through a transform of `type`, computed as the DFT of the
-// extension of length m (the general kernel; Bluestein where m has a prime
-// factor above 17).
+// every line go through a transform of `type`, computed through a DFT of
+// length len (types 2, 3, 6, 7) or of the line's symmetric extension (the
+// general kernel; Bluestein where that length has a prime factor above 17).
 int make_dtt_stage(d2dhip_plan_s *p, int idx, int axis, int type, int first, int len) {
   StageGeom &g = p->geom[idx];
   const StageIo v = cols_view(axis, p->nx, p->ny, p->nz);
   g.axis = axis;
   g.io = IO_R2R;
+  g.dtw = nullptr;
   switch (type) {
     case 1: g.n = 2 * (len - 1); break;
     case 5: g.n = 2 * (len + 1); break;
     case 4:
     case 8: g.n = 8 * len; break;
-    default: g.n = 4 * len; break;
+    default: {  // 2, 3, 6, 7: the half-length algorithm
+      g.n = len;
+      const int st = get_dtt_twiddles(p, len, &g.dtw);
+      if (st) return st;
+    }
   }
   g.C0r = v.C0r;
   g.C1 = v.C1;
@@ -1015,6 +1051,7 @@ int d2dhip_plan_destroy(d2dhip_plan p) {
   for (auto e : p->pool) (void)hipEventDestroy(e);
   for (auto &kv : p->tw)
     if (kv.second.d) (void)hipFree(kv.second.d);
+  for (auto &kv : p->dtw) (void)hipFree(kv.second);
   for (auto &kv : p->blue) {
     if (kv.second.chirp) (void)hipFree(kv.second.chirp);
     if (kv.second.bhat) (void)hipFree(kv.second.bhat);
@@ -1158,6 +1195,7 @@ long long d2dhip_plan_device_bytes(d2dhip_plan p) {
   if (!p) return 0;
   long long b = (long long)p->wk_bytes;
   for (auto &kv : p->tw) b += (long long)kv.second.bytes;
+  for (auto &kv : p->dtw) b += (long long)kv.first * 2 * p->rb;
   return b;
 }
 

@@ -287,6 +287,33 @@ def bench_config(args, torch, d2d):
         extra["unfused_ms"] = round(ms_unfused, 4)
         extra["fused_speedup"] = round(ms_unfused / ms, 3)
         extra["kernels"] = {"r2c": e.stage_kernels()["r2c"]}
+    elif w == "dtt513":
+        # decomp_2d_dtt_3d_r2r forward (DCT-II on x, y, z with the default
+        # ndismiss = 1: 512 transformed points of 513) then backward (DCT-III)
+        n = 513
+        x = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
+        d2d.fill_uniform(x, x.numel(), d2d.R8, 5)
+        f = torch.empty_like(x)
+        y = torch.empty_like(x)
+        e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, tune=not args.no_tune, opt_DTT=(2, 2, 2))
+
+        def step():
+            e.dtt_3d_r2r(x, f, d2d.FORWARD)
+            e.dtt_3d_r2r(f, y, d2d.BACKWARD)
+        ms = timed(step)
+        nbytes = 6 * 2 * n ** 3 * 8
+        # round trip on the transformed points, over (2 * 512)^3
+        core = (slice(0, 512),) * 3
+        xv = x.view(n, n, n)[core]
+        yv = y.view(n, n, n)[core] / float(1024 ** 3)
+        err = float(torch.linalg.vector_norm(yv - xv) / torch.linalg.vector_norm(xv))
+        cfg = {"workload": "DCT-II fwd + DCT-III bwd FP64 513^3 PHYSICAL_IN_X (512 transformed "
+                           "points per line), uniform seed 5"}
+        e.profile(True)
+        step()
+        rep = e.profile_report()
+        e.profile(False)
+        extra["stage_ms"] = {k: round(v[0], 4) for k, v in rep.items()}
     else:  # pack2048
         dec = d2d.decomp_info_init(2048, 2048, 2048, 4, 4, 0, 0)
         xs = dec.xsz
@@ -323,10 +350,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan-time autotuner (use the default kernels)")
     ap.add_argument("--workload", choices=["r2c1024", "batch64", "c2c128", "r2c512", "r2cz720",
-                                           "pack2048", "r2cpack2048"], default="r2c1024",
+                                           "pack2048", "r2cpack2048", "dtt513"],
+                    default="r2c1024",
                     help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids; "
                          "c2c128 / r2c512 / r2cz720 / pack2048: the other parity configurations, "
-                         "timed; r2cpack2048: the pack2048 rank's x stage fused with its pack")
+                         "timed; r2cpack2048: the pack2048 rank's x stage fused with its pack; "
+                         "dtt513: 3D DCT-II + DCT-III round trip")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -341,7 +370,7 @@ def main():
     if args.workload == "batch64":
         bench_batch(args, torch, d2d)
         return
-    if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048"):
+    if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048", "dtt513"):
         bench_config(args, torch, d2d)
         return
     n = args.n
