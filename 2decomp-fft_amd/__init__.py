@@ -329,6 +329,16 @@ class FFTEngine:
         check(lib().d2dhip_plan_get_dtt(self._plan, v))
         return tuple(v)
 
+    def dtt_stage_kernels(self):
+        """{stage: kernel name} of the real-to-real stages (7-9 forward x, y, z;
+        10-12 backward)."""
+        out = {}
+        buf = ctypes.create_string_buffer(128)
+        for i in range(7, 13):
+            check(lib().d2dhip_plan_stage_kernel(self._plan, i, buf, 128))
+            out[i] = buf.value.decode()
+        return out
+
     def dtt_input(self):
         """decomp_2d_fft_get_dtt_input (src/fft_fftw3_f03.f90:629-643)."""
         return self.dtt[0:3]

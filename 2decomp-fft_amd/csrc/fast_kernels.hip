@@ -44,14 +44,100 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   const int r0 = LINES ? tid % RPT : tid / TX;
   const long long c0 = c0b + t;
   const long long ca = 2 * c0;  // first real column of a pair (real IO)
+  constexpr bool DTT = IO == IO_DTT2 || IO == IO_DTT3;
+  static_assert(!DTT || XM == 0, "real-to-real stages use the plain exchange");
   const bool cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
-  const long long in_base = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1;
-  const long long out_base = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1;
+  // real-to-real stages: the transformed points start at dtt_first
+  const long long f = DTT ? d->dtt_first : 0;
+  const long long in_base =
+      (IO == IO_R2C || DTT ? ca : c0) * d->in_s0 + c1 * d->in_s1 + f * d->in_sj;
+  const long long out_base =
+      (IO == IO_C2R || DTT ? ca : c0) * d->out_s0 + c1 * d->out_s1 + f * d->out_sj;
   const void *tw = d->tw;
   cpx<T> v[E];
 
   // ---- inputs of pass 0 -------------------------------------------------
-  if constexpr (IO == IO_C2R) {
+  if constexpr (IO == IO_DTT2) {
+    // v[J] = x_{2J} (J < ceil(N/2)), x_{2N-1-2J} (otherwise); DST-II takes
+    // (-1)^j x_j
+    const bool alt = d->dtt == 6;
+    constexpr int R = RL::r[0], NBR = N / R;
+#pragma unroll
+    for (int q = 0; q < C::nb(0); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(0) || b < NBR) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int J = b + s * NBR;
+          const int j = (2 * J < N) ? 2 * J : 2 * N - 1 - 2 * J;
+          const long long o = in_base + (long long)j * d->in_sj;
+          cpx<T> x = {T(0), T(0)};
+          if (ca < d->C0r) x.x = gload_r<T>(in, o);
+          if (ca + 1 < d->C0r) x.y = gload_r<T>(in, o + d->in_s0);
+          if (alt && (j & 1)) x = cpx<T>{-x.x, -x.y};
+          v[q * R + s] = x;
+        }
+      }
+    }
+  } else if constexpr (IO == IO_DTT3) {
+    // conj(Z_j) = w_j conj(u_j), u_j = (Xa_j + Xb_{N-j}, Xb_j - Xa_{N-j})
+    // (X_N = 0; DST-III reads X backwards), for j and N - j together
+    constexpr int NJ = N / 2 + 1;
+    constexpr int ITER = (TX * NJ + NT - 1) / NT;
+    const bool rev = d->dtt == 7;
+    cpx<T> A[ITER], B[ITER];
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+      const int w = tid + it * NT;
+      A[it] = cpx<T>{T(0), T(0)};
+      B[it] = cpx<T>{T(0), T(0)};
+      if (w < TX * NJ) {
+        const int tt = LINES ? w / NJ : w % TX;
+        const int j = LINES ? w % NJ : w / TX;
+        const long long cc = 2 * (c0b + tt);
+        const long long base = cc * d->in_s0 + c1 * d->in_s1 + f * d->in_sj;
+        const long long o1 = base + (long long)(rev ? N - 1 - j : j) * d->in_sj;
+        const long long o2 = base + (long long)(rev ? j - 1 : N - j) * d->in_sj;
+        if (cc < d->C0r) {
+          A[it].x = gload_r<T>(in, o1);
+          if (j > 0) B[it].x = gload_r<T>(in, o2);
+        }
+        if (cc + 1 < d->C0r) {
+          A[it].y = gload_r<T>(in, o1 + d->in_s0);
+          if (j > 0) B[it].y = gload_r<T>(in, o2 + d->in_s0);
+        }
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+      const int w = tid + it * NT;
+      if (w < TX * NJ) {
+        const int tt = LINES ? w / NJ : w % TX;
+        const int j = LINES ? w % NJ : w / TX;
+        const int jm = N - j;
+        const cpx<T> a = A[it], b = B[it];
+        const cpx<T> cs = gload<T>(d->dtw, j);
+        const T ux = a.x + b.y, uy = a.y - b.x;
+        lds[C::row(j) * LDJ + tt * LDT] = cpx<T>{cs.x * ux - cs.y * uy, -(cs.x * uy + cs.y * ux)};
+        if (j > 0 && j < jm) {
+          const cpx<T> cm = gload<T>(d->dtw, jm);
+          const T vx = b.x + a.y, vy = b.y - a.x;
+          lds[C::row(jm) * LDJ + tt * LDT] =
+              cpx<T>{cm.x * vx - cm.y * vy, -(cm.x * vy + cm.y * vx)};
+        }
+      }
+    }
+    __syncthreads();
+    constexpr int R = RL::r[0], NBR = N / R;
+#pragma unroll
+    for (int q = 0; q < C::nb(0); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(0) || b < NBR) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) v[q * R + s] = lds[C::row(b + s * NBR) * LDJ + t * LDT];
+      }
+    }
+  } else if constexpr (IO == IO_C2R) {
     // conj(Z) of the merged half spectra, all N points, into LDS
     constexpr int NH = N / 2 + 1;
     // all loads first (compile-time trip count), then the LDS writes, so a
@@ -144,7 +230,7 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
     if constexpr (p > 0) {
       // outputs of pass p-1 -> LDS -> inputs of pass p
       constexpr int Rq = RL::r[p - 1], NBq = N / Rq, Lq = RL::L(p - 1);
-      if constexpr (p > 1 || IO == IO_C2R) __syncthreads();
+      if constexpr (p > 1 || IO == IO_C2R || IO == IO_DTT3) __syncthreads();
       if constexpr (XM == 0) {
 #pragma unroll
         for (int q = 0; q < C::nb(p - 1); ++q) {
@@ -215,7 +301,63 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
 
   // ---- outputs of the last pass ---------------------------------------------
   constexpr int RZ = RL::r[P - 1], NBZ = N / RZ, LZ = RL::L(P - 1);
-  if constexpr (IO == IO_R2C) {
+  if constexpr (IO == IO_DTT2) {
+    // Z -> LDS; Y_k = 2 Re(w_q V_q) of each line of the pair, V split from Z
+    // as in a real-to-complex transform, q = k (DCT-II) or N-1-k (DST-II)
+    if constexpr (P > 1) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < C::nb(P - 1); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(P - 1) || b < NBZ) {
+        const int k = b % LZ, g = b / LZ;
+#pragma unroll
+        for (int s = 0; s < RZ; ++s) lds[C::row(g * LZ * RZ + k + s * LZ) * LDJ + t * LDT] = v[q * RZ + s];
+      }
+    }
+    __syncthreads();
+    const bool flip = d->dtt == 6;
+    constexpr int ITER = (TX * N + NT - 1) / NT;
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+      const int w = tid + it * NT;
+      if (w < TX * N) {
+        const int tt = LINES ? w / N : w % TX;
+        const int k = LINES ? w % N : w / TX;
+        const long long cc = 2 * (c0b + tt);
+        if (cc < d->C0r) {
+          const int q = flip ? N - 1 - k : k;
+          const cpx<T> zk = lds[C::row(q) * LDJ + tt * LDT];
+          const cpx<T> zm = lds[C::row(q == 0 ? 0 : N - q) * LDJ + tt * LDT];
+          const cpx<T> cs = gload<T>(d->dtw, q);
+          const long long o = cc * d->out_s0 + c1 * d->out_s1 + (f + k) * d->out_sj;
+          gstore_r<T>(out, o, cs.x * (zk.x + zm.x) + cs.y * (zk.y - zm.y));
+          if (cc + 1 < d->C0r)
+            gstore_r<T>(out, o + d->out_s0, cs.x * (zk.y + zm.y) + cs.y * (zm.x - zk.x));
+        }
+      }
+    }
+  } else if constexpr (IO == IO_DTT3) {
+    // z = conj(last pass) = v_a + i v_b; y_{2n} = v_n, y_{2n+1} = v_{N-1-n}
+    // (DST-III: times (-1)^k)
+    const bool alt = d->dtt == 7;
+#pragma unroll
+    for (int q = 0; q < C::nb(P - 1); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(P - 1) || b < NBZ) {
+        const int k = b % LZ, g = b / LZ;
+#pragma unroll
+        for (int s = 0; s < RZ; ++s) {
+          const int pv = g * LZ * RZ + k + s * LZ;
+          const int kk = (2 * pv < N) ? 2 * pv : 2 * N - 1 - 2 * pv;
+          const cpx<T> r = v[q * RZ + s];
+          const bool neg = alt && (kk & 1);
+          const long long o = out_base + (long long)kk * d->out_sj;
+          if (ca < d->C0r) gstore_r<T>(out, o, neg ? -r.x : r.x);
+          if (ca + 1 < d->C0r) gstore_r<T>(out, o + d->out_s0, neg ? r.y : -r.y);
+        }
+      }
+    }
+  } else if constexpr (IO == IO_R2C) {
     // the last pass's outputs Z -> LDS; A[k] = (Z[k] + conj Z[N-k]) / 2 and
     // B[k] = (Z[k] - conj Z[N-k]) / (2i) for k <= N/2.  Re A and Im B need
     // only real parts of Z, Im A and Re B only imaginary parts, so the split
@@ -298,6 +440,22 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
 #pragma unroll
         for (int s = 0; s < RZ; ++s)
           fstore<T, IO>(d, out, out_base, cvalid, ca, g * LZ * RZ + k + s * LZ, v[q * RZ + s], conj);
+      }
+    }
+  }
+  if constexpr (DTT) {
+    // the points outside [dtt_first, dtt_first + N) are copied through
+    const int npt = d->dtt_n - N, fi = d->dtt_first;
+    for (int w = tid; w < TX * npt; w += NT) {
+      const int tt = LINES ? w / npt : w % TX;
+      const int i = LINES ? w % npt : w / TX;
+      const int k = i < fi ? i : i + N;
+      const long long cc = 2 * (c0b + tt);
+      if (cc < d->C0r) {
+        const long long ib = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
+        const long long ob = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
+        gstore_r<T>(out, ob, gload_r<T>(in, ib));
+        if (cc + 1 < d->C0r) gstore_r<T>(out, ob + d->out_s0, gload_r<T>(in, ib + d->in_s0));
       }
     }
   }
@@ -396,6 +554,11 @@ static FastVariant make_variant(const char *name) {
   make_variant<T, N, TX, NT, IO_C2C, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " c2c"), \
       make_variant<T, N, TX, NT, IO_R2C, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " r2c"), \
       make_variant<T, N, TX, NT, IO_C2R, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " c2r")
+
+// real-to-real stages: DCT-II / DST-II and DCT-III / DST-III of N points
+#define D2D_DTT(T, N, TX, NT, LINES, RL)                                                    \
+  make_variant<T, N, TX, NT, IO_DTT2, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct2"), \
+      make_variant<T, N, TX, NT, IO_DTT3, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct3")
 
 // c2c only, split exchange, at least WPE waves per SIMD
 #define D2D_SPLIT_C2C(T, N, TX, NT, LINES, RL, WPE)                                      \
@@ -698,6 +861,27 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(float, 128, 8, 64, true, R16_8),
       D2D_VARIANTS(float, 64, 16, 128, false, R8_8),
       D2D_VARIANTS(float, 64, 16, 128, true, R8_8),
+      // real-to-real stages of (2^k + 1)-point grids (k = 6..10)
+      D2D_DTT(double, 1024, 4, 256, false, R16_16_4),
+      D2D_DTT(double, 1024, 4, 256, true, R16_16_4),
+      D2D_DTT(double, 512, 8, 512, false, R8_8_8),
+      D2D_DTT(double, 512, 4, 256, true, R8_8_8),
+      D2D_DTT(double, 256, 8, 128, false, R16_16),
+      D2D_DTT(double, 256, 4, 64, true, R16_16),
+      D2D_DTT(double, 128, 8, 64, false, R16_8),
+      D2D_DTT(double, 128, 8, 64, true, R16_8),
+      D2D_DTT(double, 64, 8, 64, false, R8_8),
+      D2D_DTT(double, 64, 16, 128, true, R8_8),
+      D2D_DTT(float, 1024, 8, 512, false, R16_16_4),
+      D2D_DTT(float, 1024, 4, 256, true, R16_16_4),
+      D2D_DTT(float, 512, 16, 512, false, R8_8_8),
+      D2D_DTT(float, 512, 8, 256, true, R8_8_8),
+      D2D_DTT(float, 256, 16, 256, false, R16_16),
+      D2D_DTT(float, 256, 4, 64, true, R16_16),
+      D2D_DTT(float, 128, 16, 128, false, R16_8),
+      D2D_DTT(float, 128, 8, 64, true, R16_8),
+      D2D_DTT(float, 64, 16, 128, false, R8_8),
+      D2D_DTT(float, 64, 16, 128, true, R8_8),
       D2D_VARIANTS(float, 32, 16, 64, false, R8_4),
       D2D_VARIANTS(float, 32, 32, 128, true, R8_4),
       D2D_VARIANTS(float, 48, 16, 64, false, R12_4),

@@ -576,7 +576,14 @@ int make_dtt_stage(d2dhip_plan_s *p, int idx, int axis, int type, int first, int
   g.dtt_first = first;
   g.dtt_len = len;
   g.dtt_n = axis == 0 ? p->nx : (axis == 1 ? p->ny : p->nz);
-  return configure(p, idx, -1);
+  if (g.dtw && !std::getenv("D2DHIP_NO_FAST")) {  // specialised DCT-II / III tiles of length len
+    const int want = (type == 2 || type == 6) ? IO_DTT2 : IO_DTT3;
+    for (const FastVariant &v : fast_registry())
+      if (v.rb == p->rb && v.mf == 0 && v.io == want && v.lines == (axis == 0 ? 1 : 0) &&
+          v.n == len)
+        g.cands.push_back(&v);
+  }
+  return configure(p, idx, g.cands.empty() ? -1 : 0);
 }
 
 int upload_chunk_descs(d2dhip_plan_s *p);

@@ -74,6 +74,12 @@ CASES = [
     ((17, 13, 11), (6, 7, 8)),
     ((40, 24, 18), (2, 6, 1, 3, 2, 1, 4, 3, 0, 3, 2, 1)),
     ((256, 16, 8), (5, 2, 8)),
+    # (2^k + 1)-point axes: the specialised DCT-II / III tiles
+    ((65, 129, 257), (2, 3, 6)),
+    ((257, 65, 129), (7, 6, 2)),
+    ((1025, 33, 16), (3, 2, 7)),
+    ((129, 513, 65), (6, 7, 3)),
+    ((66, 130, 40), (2, 3, 6, 2, 1, 1, 2, 2, 7, 2, 1, 1)),
 ]
 
 
@@ -98,6 +104,40 @@ def test_dtt_against_oracle(d2d, oracle, torch_gpu, case, fmt, prec):
     torch.cuda.synchronize()
     want = oracle.dtt3d(_host(out, shape), dtt, 1, fmt)
     assert rel_l2(_host(back, shape), want) < tol
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("shape,dtt", [((513, 257, 129), (2, 7, 6)),
+                                       ((65, 129, 1025), (3, 6, 2))])
+def test_specialised_dtt_tiles_match_general(d2d, oracle, torch_gpu, prec, shape, dtt):
+    """Every axis runs on a specialised DCT-II / III tile (lines of 512 or 64,
+    columns of 256 to 1024 points); the general kernel gives the same
+    transform to rounding, and the round trip returns the input."""
+    torch = torch_gpu
+    rdt = torch.float64 if prec == 64 else torch.float32
+    tol = 1e-13 if prec == 64 else 2e-6
+    e = d2d.FFTEngine(1, *shape, prec=prec, opt_DTT=dtt)
+    names = e.dtt_stage_kernels()
+    assert all(("dct2" in n or "dct3" in n) for n in names.values()), names
+    x = np.asfortranarray(np.random.default_rng(9).uniform(-1, 1, shape))
+    xin = _dev(torch, x, rdt)
+    out = torch.empty_like(xin)
+    back = torch.empty_like(xin)
+    e.dtt_3d_r2r(xin, out, d2d.FORWARD)
+    e.dtt_3d_r2r(out, back, d2d.BACKWARD)
+    torch.cuda.synchronize()
+    fast_out, fast_back = out.clone(), back.clone()
+    for st in range(7, 13):
+        e.set_stage_kernel(st, "general")
+    e.dtt_3d_r2r(xin, out, d2d.FORWARD)
+    e.dtt_3d_r2r(out, back, d2d.BACKWARD)
+    torch.cuda.synchronize()
+    assert rel_l2(_host(fast_out, shape), _host(out, shape)) < tol
+    assert rel_l2(_host(fast_back, shape), _host(back, shape)) < tol
+    y = _host(fast_back, shape) / oracle.dtt_norm(dtt, shape)
+    c = oracle.dtt_config(dtt)
+    sl = tuple(slice(c[3 + a] - 1, c[3 + a] - 1 + shape[a] - c[6 + a]) for a in range(3))
+    assert rel_l2(y[sl], _host(xin, shape)[sl]) < tol
 
 
 def test_dtt_in_place_and_skip(d2d, oracle, torch_gpu):
@@ -163,4 +203,4 @@ def test_dtt_errors(d2d, torch_gpu):
     with pytest.raises(d2d.D2dError):
         e.dtt_3d_r2r(a, a, 0)
     names = [e.stage_candidates(s) for s in range(7, 13)]
-    assert all(n == ["general"] for n in names)
+    assert all(n == ["general"] for n in names)  # no specialised tiles of 7, 8, 9 points
