@@ -99,6 +99,24 @@ module d2dhip_c
          import :: c_int, c_ptr
          type(c_ptr), value :: plan, din, dout
       end function
+      integer(c_int) function d2dhip_plan_set_dtt(plan, ndtt, dtt) &
+         bind(C, name="d2dhip_plan_set_dtt")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan
+         integer(c_int), value :: ndtt
+         integer(c_int), intent(in) :: dtt(*)
+      end function
+      integer(c_int) function d2dhip_plan_get_dtt(plan, dtt) bind(C, name="d2dhip_plan_get_dtt")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan
+         integer(c_int), intent(out) :: dtt(15)
+      end function
+      integer(c_int) function d2dhip_exec_dtt(plan, din, dout, isign) &
+         bind(C, name="d2dhip_exec_dtt")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan, din, dout
+         integer(c_int), value :: isign
+      end function
       integer(c_int) function d2dhip_transpose(dir, elem, src, dst, d, stream) &
          bind(C, name="d2dhip_transpose")
          import :: c_int, c_ptr, d2dhip_decomp
@@ -473,10 +491,15 @@ module decomp_2d_fft
       logical, private :: skip(3) = .false.
       integer, private :: slot = 0  ! index in the module's engine array, 0 if external
       type(decomp_info), public :: ph, sp
+      ! DCT / DST set-up (src/fft_fftw3_f03.f90:59-62): types 1-3, ifirst
+      ! 4-6, ndismiss 7-9, ofirst 10-12, backward types 13-15, types as 1-8
+      logical, public :: with_dtt = .false.
+      integer, allocatable, public, dimension(:) :: dtt
    contains
       procedure, public :: init => engine_init
       procedure, public :: fin => engine_fin
       procedure, public :: use_it => engine_use_it
+      procedure, public :: dtt_input => engine_dtt_input
       generic, public :: fft => e_c2c, e_r2c, e_c2r
       procedure, private :: e_c2c => engine_fft_c2c
       procedure, private :: e_r2c => engine_fft_r2c
@@ -499,7 +522,7 @@ module decomp_2d_fft
              decomp_2d_fft_get_engine, decomp_2d_fft_get_format, decomp_2d_fft_get_inplace, &
              decomp_2d_fft_get_inplace_r2c, decomp_2d_fft_get_inplace_c2r, &
              decomp_2d_fft_3d_dev, decomp_2d_fft_3d_r2c_dev, decomp_2d_fft_3d_c2r_dev, &
-             decomp_2d_fft_3d_batch_dev
+             decomp_2d_fft_3d_batch_dev, decomp_2d_dtt_3d_r2r, decomp_2d_fft_get_dtt_input
 
    interface decomp_2d_fft_init
       module procedure fft_init_noarg
@@ -536,33 +559,36 @@ contains
       call fft_init_arg(PHYSICAL_IN_X)
    end subroutine
 
-   subroutine fft_init_arg(pencil, opt_skip_XYZ_c2c)
+   subroutine fft_init_arg(pencil, opt_skip_XYZ_c2c, opt_DTT)
       integer, intent(in) :: pencil
       logical, dimension(3), intent(in), optional :: opt_skip_XYZ_c2c
+      integer, dimension(:), intent(in), optional :: opt_DTT
       call fft_init_one_grid(pencil, nx_global, ny_global, nz_global, &
-                             opt_skip_XYZ_c2c=opt_skip_XYZ_c2c)
+                             opt_skip_XYZ_c2c=opt_skip_XYZ_c2c, opt_DTT=opt_DTT)
    end subroutine
 
    subroutine fft_init_one_grid(pencil, nx, ny, nz, opt_inplace, opt_inplace_r2c, &
-                                opt_inplace_c2r, opt_skip_XYZ_c2c)
+                                opt_inplace_c2r, opt_skip_XYZ_c2c, opt_DTT)
       integer, intent(in) :: pencil, nx, ny, nz
       logical, intent(in), optional :: opt_inplace, opt_inplace_r2c, opt_inplace_c2r
       logical, dimension(3), intent(in), optional :: opt_skip_XYZ_c2c
+      integer, dimension(:), intent(in), optional :: opt_DTT
       call decomp_2d_fft_set_ngrid(1)
       call fft_init_multiple_grids(pencil, nx, ny, nz, 1, opt_inplace, opt_inplace_r2c, &
-                                   opt_inplace_c2r, opt_skip_XYZ_c2c)
+                                   opt_inplace_c2r, opt_skip_XYZ_c2c, opt_DTT)
    end subroutine
 
    subroutine fft_init_multiple_grids(pencil, nx, ny, nz, igrid, opt_inplace, &
-                                      opt_inplace_r2c, opt_inplace_c2r, opt_skip_XYZ_c2c)
+                                      opt_inplace_r2c, opt_inplace_c2r, opt_skip_XYZ_c2c, opt_DTT)
       integer, intent(in) :: pencil, nx, ny, nz, igrid
       logical, intent(in), optional :: opt_inplace, opt_inplace_r2c, opt_inplace_c2r
       logical, dimension(3), intent(in), optional :: opt_skip_XYZ_c2c
+      integer, dimension(:), intent(in), optional :: opt_DTT
       if (igrid < 1 .or. igrid > n_grid) &
          call decomp_2d_abort(__FILE__, __LINE__, igrid, "Invalid value for igrid")
       call drop_grid_batch()
       call fft_engines(igrid)%init(pencil, nx, ny, nz, opt_inplace, opt_inplace_r2c, &
-                                   opt_inplace_c2r, opt_skip_XYZ_c2c)
+                                   opt_inplace_c2r, opt_skip_XYZ_c2c, opt_DTT)
       fft_engines(igrid)%slot = igrid
       cur_slot = igrid
       cur_e => fft_engines(igrid)
@@ -570,12 +596,14 @@ contains
 
    ! decomp_2d_fft_engine_init (src/fft_common.f90:141-242)
    subroutine engine_init(engine, pencil, nx, ny, nz, opt_inplace, opt_inplace_r2c, &
-                          opt_inplace_c2r, opt_skip_XYZ_c2c)
+                          opt_inplace_c2r, opt_skip_XYZ_c2c, opt_DTT)
       class(decomp_2d_fft_engine), intent(inout) :: engine
       integer, intent(in) :: pencil, nx, ny, nz
       logical, intent(in), optional :: opt_inplace, opt_inplace_r2c, opt_inplace_c2r
       logical, dimension(3), intent(in), optional :: opt_skip_XYZ_c2c
+      integer, dimension(:), intent(in), optional :: opt_DTT
       integer(c_int) :: flags, st
+      integer(c_int) :: vdtt(15)
       integer(c_size_t) :: ncplx
       if (engine%initialised) &
          call decomp_2d_abort(__FILE__, __LINE__, 4, 'FFT engine should only be initialised once')
@@ -616,8 +644,57 @@ contains
       engine%nbytes = 16_c_size_t * ncplx
       call check(hipMalloc(engine%d_a, engine%nbytes), __FILE__, __LINE__)
       call check(hipMalloc(engine%d_b, engine%nbytes), __FILE__, __LINE__)
+      ! decomp_2d_fft_engine_dtt_init (src/fft_fftw3_f03.f90:341-390)
+      engine%with_dtt = .false.
+      if (allocated(engine%dtt)) deallocate (engine%dtt)
+      if (present(opt_DTT)) then
+         vdtt = 0
+         vdtt(1:min(size(opt_DTT), 15)) = int(opt_DTT(1:min(size(opt_DTT), 15)), c_int)
+         call check(d2dhip_plan_set_dtt(engine%plan, int(size(opt_DTT), c_int), vdtt), &
+                    __FILE__, __LINE__)
+         call check(d2dhip_plan_get_dtt(engine%plan, vdtt), __FILE__, __LINE__)
+         allocate (engine%dtt(15))
+         engine%dtt = int(vdtt)
+         engine%with_dtt = .true.
+      end if
       engine%initialised = .true.
    end subroutine engine_init
+
+   ! decomp_2d_fft_engine_dtt_input (src/fft_fftw3_f03.f90:646-659)
+   function engine_dtt_input(engine)
+      class(decomp_2d_fft_engine), intent(in) :: engine
+      integer, dimension(3) :: engine_dtt_input
+      if (.not. engine%with_dtt) &
+         call decomp_2d_abort(__FILE__, __LINE__, 1, "No DTT for the current engine")
+      engine_dtt_input = engine%dtt(1:3)
+   end function engine_dtt_input
+
+   ! decomp_2d_fft_get_dtt_input (src/fft_fftw3_f03.f90:629-643)
+   function decomp_2d_fft_get_dtt_input()
+      integer, dimension(3) :: decomp_2d_fft_get_dtt_input
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      decomp_2d_fft_get_dtt_input = e%dtt_input()
+   end function decomp_2d_fft_get_dtt_input
+
+   ! decomp_2d_dtt_3d_r2r (src/fft_fftw3_f03.f90:1959-2023) on host arrays of
+   ! the engine's (nx, ny, nz) grid; in is preserved
+   subroutine decomp_2d_dtt_3d_r2r(in, out_real, isign)
+      real(mytype), dimension(:, :, :), contiguous, target, intent(inout) :: in
+      real(mytype), dimension(:, :, :), contiguous, target, intent(out) :: out_real
+      integer, intent(in) :: isign
+      type(decomp_2d_fft_engine), pointer :: e
+      integer(c_size_t) :: b
+      e => current()
+      if (isign /= DECOMP_2D_FFT_FORWARD .and. isign /= DECOMP_2D_FFT_BACKWARD) &
+         call decomp_2d_abort(__FILE__, __LINE__, isign, "Invalid value")
+      b = 8_c_size_t * size(in, kind=c_size_t)
+      if (size(out_real, kind=c_size_t) /= size(in, kind=c_size_t) .or. b > e%nbytes) &
+         call decomp_2d_abort(__FILE__, __LINE__, 1, "r2r array sizes do not match the engine")
+      call check(hipMemcpy(e%d_a, c_loc(in), b, HIP_H2D), __FILE__, __LINE__)
+      call check(d2dhip_exec_dtt(e%plan, e%d_a, e%d_b, int(isign, c_int)), __FILE__, __LINE__)
+      call check(hipMemcpy(c_loc(out_real), e%d_b, b, HIP_D2H), __FILE__, __LINE__)
+   end subroutine decomp_2d_dtt_3d_r2r
 
    subroutine engine_fin(engine)
       class(decomp_2d_fft_engine), intent(inout) :: engine
@@ -627,6 +704,8 @@ contains
       st = hipFree(engine%d_a)
       st = hipFree(engine%d_b)
       engine%plan = c_null_ptr
+      engine%with_dtt = .false.
+      if (allocated(engine%dtt)) deallocate (engine%dtt)
       engine%initialised = .false.
    end subroutine engine_fin
 

@@ -1,5 +1,5 @@
 """The reference's own Fortran checks (fft_c2c_x / fft_r2c_x / their _z
-copies, fft_multiple_grids, test2d) run through the Fortran decomp_2d /
+copies, fft_multiple_grids, test2d, the dtt examples) run through the Fortran decomp_2d /
 decomp_2d_fft modules over the C ABI (2decomp-fft_amd/fortran)."""
 import os
 import subprocess
@@ -21,3 +21,4 @@ def test_fortran_reference_checks(torch_gpu):
     assert "fft_hip_tests completed" in r.stdout
     assert "transposes exact" in r.stdout
     assert r.stdout.count("error / mesh point") == 12
+    assert r.stdout.count("dtt checks passed: 512 type triples") == 2
