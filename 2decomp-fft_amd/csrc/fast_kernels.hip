@@ -44,7 +44,7 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   const int r0 = LINES ? tid % RPT : tid / TX;
   const long long c0 = c0b + t;
   const long long ca = 2 * c0;  // first real column of a pair (real IO)
-  constexpr bool DTT = IO == IO_DTT2 || IO == IO_DTT3;
+  constexpr bool DTT = IO == IO_DTT2 || IO == IO_DTT3 || IO == IO_DTTX;
   static_assert(!DTT || XM == 0, "real-to-real stages use the plain exchange");
   const bool cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
   // real-to-real stages: the transformed points start at dtt_first
@@ -57,7 +57,40 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   cpx<T> v[E];
 
   // ---- inputs of pass 0 -------------------------------------------------
-  if constexpr (IO == IO_DTT2) {
+  if constexpr (IO == IO_DTTX) {
+    // the extension e of the line pair, gathered: e[J] = +-x_src or 0
+    const int type = d->dtt, Lx = d->dtt_len;
+    constexpr int R = RL::r[0], NBR = N / R;
+#pragma unroll
+    for (int q = 0; q < C::nb(0); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(0) || b < NBR) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int J = b + s * NBR;
+          int src = -1;
+          bool neg = false;
+          if (type == 1) {  // e[j] = e[N-j] = x_j
+            src = J < Lx ? J : N - J;
+          } else if (type == 5) {  // e[j+1] = -e[N-j-1] = x_j
+            if (J >= 1 && J <= Lx) src = J - 1;
+            else if (J >= Lx + 2) { src = N - 1 - J; neg = true; }
+          } else if (J & 1) {  // types 4, 8: e[2j+1] = +-e[N-2j-1] = x_j
+            if (J < 2 * Lx) src = J >> 1;
+            else if (J > N - 2 * Lx) { src = (N - 1 - J) >> 1; neg = type == 8; }
+          }
+          cpx<T> x = {T(0), T(0)};
+          if (src >= 0) {
+            const long long o = in_base + (long long)src * d->in_sj;
+            if (ca < d->C0r) x.x = gload_r<T>(in, o);
+            if (ca + 1 < d->C0r) x.y = gload_r<T>(in, o + d->in_s0);
+            if (neg) x = cpx<T>{-x.x, -x.y};
+          }
+          v[q * R + s] = x;
+        }
+      }
+    }
+  } else if constexpr (IO == IO_DTT2) {
     // v[J] = x_{2J} (J < ceil(N/2)), x_{2N-1-2J} (otherwise); DST-II takes
     // (-1)^j x_j
     const bool alt = d->dtt == 6;
@@ -336,6 +369,29 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
         }
       }
     }
+  } else if constexpr (IO == IO_DTTX) {
+    // Y_k from E[p] (E_a + i E_b): DCT-I p = k, DST-I p = k + 1, DCT-IV and
+    // DST-IV p = 2k + 1; cosines take (Re, Im), sines (-Im, Re)
+    const int type = d->dtt, Lx = d->dtt_len;
+#pragma unroll
+    for (int q = 0; q < C::nb(P - 1); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(P - 1) || b < NBZ) {
+        const int k = b % LZ, g = b / LZ;
+#pragma unroll
+        for (int s = 0; s < RZ; ++s) {
+          const int pv = g * LZ * RZ + k + s * LZ;
+          const int kk = type == 1 ? pv : type == 5 ? pv - 1 : ((pv & 1) ? pv >> 1 : -1);
+          if (kk >= 0 && kk < Lx) {
+            const cpx<T> z = v[q * RZ + s];
+            const T a = type <= 4 ? z.x : -z.y, bb = type <= 4 ? z.y : z.x;
+            const long long o = out_base + (long long)kk * d->out_sj;
+            if (ca < d->C0r) gstore_r<T>(out, o, a);
+            if (ca + 1 < d->C0r) gstore_r<T>(out, o + d->out_s0, bb);
+          }
+        }
+      }
+    }
   } else if constexpr (IO == IO_DTT3) {
     // z = conj(last pass) = v_a + i v_b; y_{2n} = v_n, y_{2n+1} = v_{N-1-n}
     // (DST-III: times (-1)^k)
@@ -444,12 +500,12 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
     }
   }
   if constexpr (DTT) {
-    // the points outside [dtt_first, dtt_first + N) are copied through
-    const int npt = d->dtt_n - N, fi = d->dtt_first;
+    // the points outside [dtt_first, dtt_first + dtt_len) are copied through
+    const int npt = d->dtt_n - d->dtt_len, fi = d->dtt_first, Lt = d->dtt_len;
     for (int w = tid; w < TX * npt; w += NT) {
       const int tt = LINES ? w / npt : w % TX;
       const int i = LINES ? w % npt : w / TX;
-      const int k = i < fi ? i : i + N;
+      const int k = i < fi ? i : i + Lt;
       const long long cc = 2 * (c0b + tt);
       if (cc < d->C0r) {
         const long long ib = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
@@ -555,10 +611,12 @@ static FastVariant make_variant(const char *name) {
       make_variant<T, N, TX, NT, IO_R2C, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " r2c"), \
       make_variant<T, N, TX, NT, IO_C2R, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " c2r")
 
-// real-to-real stages: DCT-II / DST-II and DCT-III / DST-III of N points
+// real-to-real stages: DCT-II / DST-II and DCT-III / DST-III of N points,
+// and DCT-I / IV, DST-I / IV through an extension of N points
 #define D2D_DTT(T, N, TX, NT, LINES, RL)                                                    \
   make_variant<T, N, TX, NT, IO_DTT2, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct2"), \
-      make_variant<T, N, TX, NT, IO_DTT3, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct3")
+      make_variant<T, N, TX, NT, IO_DTT3, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct3"), \
+      make_variant<T, N, TX, NT, IO_DTTX, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dctx")
 
 // c2c only, split exchange, at least WPE waves per SIMD
 #define D2D_SPLIT_C2C(T, N, TX, NT, LINES, RL, WPE)                                      \

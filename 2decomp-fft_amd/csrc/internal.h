@@ -20,11 +20,15 @@ constexpr int kRegCap = 18;
 // IO_R2R: a real-to-real trigonometric transform (DCT / DST) of each line,
 // computed through a complex FFT of two real lines at once (of the line's
 // length for DCT / DST II and III, of a symmetric extension for the others).
-// IO_DTT2 / IO_DTT3 are the two I/O forms of the specialised kernels'
-// real-to-real stages: DCT-II / DST-II (a permuted first pass, a twiddled
-// Hermitian split after the last) and DCT-III / DST-III (a twiddled
-// Hermitian merge before the first pass, a permuted last pass).
-enum IoKind : int { IO_C2C = 0, IO_R2C = 1, IO_C2R = 2, IO_R2R = 3, IO_DTT2 = 4, IO_DTT3 = 5 };
+// IO_DTT2 / IO_DTT3 / IO_DTTX are the I/O forms of the specialised
+// kernels' real-to-real stages: DCT-II / DST-II (a permuted first pass, a
+// twiddled Hermitian split after the last), DCT-III / DST-III (a twiddled
+// Hermitian merge before the first pass, a permuted last pass) and
+// DCT-I / IV, DST-I / IV (the symmetric extension gathered by the first
+// pass, the outputs picked from the last).
+enum IoKind : int {
+  IO_C2C = 0, IO_R2C = 1, IO_C2R = 2, IO_R2R = 3, IO_DTT2 = 4, IO_DTT3 = 5, IO_DTTX = 6
+};
 
 struct PassInfo {
   int r;       // radix

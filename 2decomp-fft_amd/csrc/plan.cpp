@@ -576,11 +576,11 @@ int make_dtt_stage(d2dhip_plan_s *p, int idx, int axis, int type, int first, int
   g.dtt_first = first;
   g.dtt_len = len;
   g.dtt_n = axis == 0 ? p->nx : (axis == 1 ? p->ny : p->nz);
-  if (g.dtw && !std::getenv("D2DHIP_NO_FAST")) {  // specialised DCT-II / III tiles of length len
-    const int want = (type == 2 || type == 6) ? IO_DTT2 : IO_DTT3;
+  if (!std::getenv("D2DHIP_NO_FAST")) {  // specialised tiles of the stage's DFT length
+    const int want = (type == 2 || type == 6) ? IO_DTT2 : (type == 3 || type == 7) ? IO_DTT3 : IO_DTTX;
     for (const FastVariant &v : fast_registry())
       if (v.rb == p->rb && v.mf == 0 && v.io == want && v.lines == (axis == 0 ? 1 : 0) &&
-          v.n == len)
+          v.n == g.n)
         g.cands.push_back(&v);
   }
   return configure(p, idx, g.cands.empty() ? -1 : 0);

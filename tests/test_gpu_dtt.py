@@ -80,6 +80,9 @@ CASES = [
     ((1025, 33, 16), (3, 2, 7)),
     ((129, 513, 65), (6, 7, 3)),
     ((66, 130, 40), (2, 3, 6, 2, 1, 1, 2, 2, 7, 2, 1, 1)),
+    # extensions of 2^k points: the specialised DCT-I / IV, DST-I / IV tiles
+    ((129, 65, 33), (4, 8, 5)),
+    ((257, 513, 17), (1, 5, 1)),
 ]
 
 
@@ -108,17 +111,19 @@ def test_dtt_against_oracle(d2d, oracle, torch_gpu, case, fmt, prec):
 
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("shape,dtt", [((513, 257, 129), (2, 7, 6)),
-                                       ((65, 129, 1025), (3, 6, 2))])
+                                       ((65, 129, 1025), (3, 6, 2)),
+                                       ((513, 129, 65), (1, 4, 5)),
+                                       ((65, 257, 33), (8, 1, 4))])
 def test_specialised_dtt_tiles_match_general(d2d, oracle, torch_gpu, prec, shape, dtt):
-    """Every axis runs on a specialised DCT-II / III tile (lines of 512 or 64,
-    columns of 256 to 1024 points); the general kernel gives the same
+    """Every axis runs on a specialised real-to-real tile (DFTs of 64 to 1024
+    points on lines and columns); the general kernel gives the same
     transform to rounding, and the round trip returns the input."""
     torch = torch_gpu
     rdt = torch.float64 if prec == 64 else torch.float32
     tol = 1e-13 if prec == 64 else 2e-6
     e = d2d.FFTEngine(1, *shape, prec=prec, opt_DTT=dtt)
     names = e.dtt_stage_kernels()
-    assert all(("dct2" in n or "dct3" in n) for n in names.values()), names
+    assert all(("dct2" in n or "dct3" in n or "dctx" in n) for n in names.values()), names
     x = np.asfortranarray(np.random.default_rng(9).uniform(-1, 1, shape))
     xin = _dev(torch, x, rdt)
     out = torch.empty_like(xin)
