@@ -314,6 +314,16 @@ def bench_config(args, torch, d2d):
         rep = e.profile_report()
         e.profile(False)
         extra["stage_ms"] = {k: round(v[0], 4) for k, v in rep.items()}
+        extra["kernels"] = {k: v for k, v in e.dtt_stage_kernels().items()}
+        # DCT-I both ways on the same grid (all 513 points, extension of 1024)
+        e1 = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, opt_DTT=(1, 1, 1))
+
+        def step1():
+            e1.dtt_3d_r2r(x, f, d2d.FORWARD)
+            e1.dtt_3d_r2r(f, y, d2d.BACKWARD)
+        extra["dct1_round_trip_ms"] = round(timed(step1), 4)
+        err1 = torch.linalg.vector_norm(y / float(1024 ** 3) - x) / torch.linalg.vector_norm(x)
+        extra["dct1_round_trip_rel_l2"] = float(err1)
     else:  # pack2048
         dec = d2d.decomp_info_init(2048, 2048, 2048, 4, 4, 0, 0)
         xs = dec.xsz
