@@ -147,6 +147,8 @@ def lib():
             "d2dhip_pack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
             "d2dhip_unpack": ([ci, ci, vp, vp, ctypes.POINTER(Decomp), ci, vp], ci),
             "d2dhip_pack_buffer_elems": ([ci, ctypes.POINTER(Decomp), ci], cll),
+            "d2dhip_update_halo": ([ci, vp, vp, ci, ci, ci, ci, ci, ctypes.POINTER(ci), vp], ci),
+            "d2dhip_halo_exchange": ([ci, vp, ci, ci, ci, ci, ci, ctypes.POINTER(ci), vp], ci),
             "d2dhip_fill_uniform": ([vp, cll, ci, ctypes.c_ulonglong, vp], ci),
             "d2dhip_host_fill_uniform": ([vp, cll, ci, ctypes.c_ulonglong], None),
             "d2dhip_column_copy": ([vp, vp, ci, cll, cll, cll, cll, ci, vp], ci),
@@ -514,6 +516,34 @@ def pack_buffer_elems(direction, decomp, even=False):
 
 
 # ---- synthetic inputs ------------------------------------------------------------
+
+def halo_shape(sizes, level, ipencil):
+    """Extents of the halo array of a (s1, s2, s3) pencil (halo_extents_t,
+    src/halo.f90:222-292, without its Fortran lower bounds)."""
+    n = list(sizes)
+    for a in range(3):
+        if a != ipencil - 1:
+            n[a] += 2 * level
+    return tuple(n)
+
+
+def update_halo(elem, src, out, sizes, level, ipencil, periodic=(False, False, False),
+                stream=None):
+    """update_halo (src/halo.f90:112-260) at one rank into the preallocated halo
+    array `out` (halo_shape(sizes, level, ipencil) elements, i fastest)."""
+    per = (ctypes.c_int * 3)(*[int(bool(p)) for p in periodic])
+    s1, s2, s3 = sizes
+    check(lib().d2dhip_update_halo(elem, _ptr(src), _ptr(out), s1, s2, s3, level, ipencil, per,
+                                   _stream(stream)))
+
+
+def halo_exchange(elem, arr, sizes, level, ipencil, periodic=(False, False, False), stream=None):
+    """halo_exchange (src/halo_exchange_{x,y,z}_body.f90) at one rank, in place."""
+    per = (ctypes.c_int * 3)(*[int(bool(p)) for p in periodic])
+    s1, s2, s3 = sizes
+    check(lib().d2dhip_halo_exchange(elem, _ptr(arr), s1, s2, s3, level, ipencil, per,
+                                     _stream(stream)))
+
 
 def column_copy(src, dst, elem, C0, C1, s1, sj, n, stream=None):
     """Copy complex columns with an axis stage's access pattern (bandwidth reference)."""

@@ -17,7 +17,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libd2dhip.so")
-SOURCES = ["fft_kernels.hip", "fast_kernels.hip", "mfma_kernels.hip", "copy_kernels.hip", "plan.cpp",
+SOURCES = ["fft_kernels.hip", "fast_kernels.hip", "mfma_kernels.hip", "copy_kernels.hip",
+           "halo_kernels.hip", "plan.cpp",
            "api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -228,6 +228,40 @@ int d2dhip_unpack(int dir, int elem, const void *recv, void *dst, const d2dhip_d
   return launch_pack(dir, 0, es, recv, dst, d, even, (hipStream_t)stream);
 }
 
+// ---- halo cells ------------------------------------------------------------
+
+static int halo_check(int elem, int s1, int s2, int s3, int level, int ipencil,
+                      const int *periodic) {
+  if (!periodic) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (!elem_size(elem)) return set_error(D2DHIP_ERR_INVALID_VALUE, "bad elem");
+  if (s1 < 0 || s2 < 0 || s3 < 0 || level < 0)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "negative size or level");
+  if (ipencil < 1 || ipencil > 3)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid decomposition size");
+  const int s[3] = {s1, s2, s3};
+  for (int a = 0; a < 3; ++a)
+    if (a != ipencil - 1 && periodic[a] && level > s[a])
+      return set_error(D2DHIP_ERR_INVALID_VALUE,
+                       "halo level exceeds the pencil along a periodic axis");
+  return D2DHIP_SUCCESS;
+}
+
+int d2dhip_update_halo(int elem, const void *in, void *out, int s1, int s2, int s3, int level,
+                       int ipencil, const int periodic[3], void *stream) {
+  if (!in || !out) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (int st = halo_check(elem, s1, s2, s3, level, ipencil, periodic)) return st;
+  return launch_halo(elem_size(elem), in, out, s1, s2, s3, level, ipencil, periodic,
+                     (hipStream_t)stream);
+}
+
+int d2dhip_halo_exchange(int elem, void *arr, int s1, int s2, int s3, int level, int ipencil,
+                         const int periodic[3], void *stream) {
+  if (!arr) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (int st = halo_check(elem, s1, s2, s3, level, ipencil, periodic)) return st;
+  return launch_halo(elem_size(elem), nullptr, arr, s1, s2, s3, level, ipencil, periodic,
+                     (hipStream_t)stream);
+}
+
 // ---- synthetic inputs ------------------------------------------------------
 
 int d2dhip_fill_uniform(void *dst, long long n, int elem, unsigned long long seed, void *stream) {

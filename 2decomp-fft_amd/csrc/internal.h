@@ -220,6 +220,10 @@ int launch_general_multi(int prec_bytes, const MultiGroup *g, int ng, long long 
 int general_multi_prepare(int prec_bytes, size_t lds);
 
 // copy / pack kernels (copy_kernels.hip)
+// halo cells (halo_kernels.hip): in (s1, s2, s3) into the interior of arr
+// (in may be null: exchange only), then the periodic axes' halo slabs
+int launch_halo(int esize, const void *in, void *arr, int s1, int s2, int s3, int level,
+                int ipencil, const int periodic[3], hipStream_t st);
 int launch_pack(int dir, int split, int esize, const void *src, void *dst,
                 const d2dhip_decomp *d, int even, hipStream_t st);
 int launch_fill_uniform(void *dst, long long n, int elem, unsigned long long seed,

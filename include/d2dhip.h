@@ -300,6 +300,25 @@ int d2dhip_unpack(int dir, int elem, const void *recv, void *dst,
 /* Elements a pack/unpack buffer must hold for (dir, d, even). */
 long long d2dhip_pack_buffer_elems(int dir, const d2dhip_decomp *d, int even);
 
+/* ---- halo cells (one rank) -----------------------------------------------
+ * update_halo(in, out, level, opt_pencil=ipencil) (src/halo.f90:112-260,
+ * src/halo_common.f90): in is the local (s1, s2, s3) pencil, out the halo
+ * array with `level` extra cells on both sides of the pencil's two
+ * distributed axes ((s1, s2+2l, s3+2l) for ipencil 1 = x-pencil, (s1+2l, s2,
+ * s3+2l) for 2, (s1+2l, s2+2l, s3) for 3), i fastest; opt_global only moves
+ * Fortran index bounds, not memory.  The interior is copied, then
+ * halo_exchange (src/halo_exchange_{x,y,z}_body.f90) runs axis by axis in
+ * the reference's order at one rank: an axis with periodic[axis] set
+ * (decomp_2d_init's periodic_bc) is its own neighbour, so its halos wrap
+ * round; other axes have no neighbour and their halo cells are left as they
+ * were.  Rejects level > the pencil's extent along a periodic axis. */
+int d2dhip_update_halo(int elem, const void *in, void *out, int s1, int s2, int s3,
+                       int level, int ipencil, const int periodic[3], void *stream);
+/* halo_exchange(arr, ipencil, halo_extents, level, [s1, s2, s3]) on a halo
+ * array already holding its interior. */
+int d2dhip_halo_exchange(int elem, void *arr, int s1, int s2, int s3, int level,
+                         int ipencil, const int periodic[3], void *stream);
+
 /* ---- synthetic inputs (no reference counterpart; used by tests/benchmarks) -
  * SplitMix64 stream keyed by seed, counter = 0-based Fortran linear index;
  * uniform values 2u-1 in [-1,1).  Complex elements use counters 2*idx (re)

@@ -299,3 +299,56 @@ def dtt_norm(dtt, shape):
         else:
             f *= 2 * n - 2 * nd
     return f
+
+
+# ---- halo cells ---------------------------------------------------------------
+# update_halo / halo_exchange at one rank (src/halo.f90:112-260,
+# src/halo_common.f90, src/halo_exchange_{x,y,z}_body.f90): the pencil goes
+# into the interior of the halo array, then the two distributed axes are
+# exchanged in the order of the body files (x-pencil: y then z; y-pencil: x
+# then z; z-pencil: x then y), each over the whole extent of the other axes.
+# MPI_CART_SHIFT on a communicator of size 1 returns the rank itself for a
+# periodic axis (the periodic_bc flags of decomp_2d_init,
+# src/decomp_2d_init_fin.f90:97-112) and MPI_PROC_NULL otherwise, so a
+# periodic axis receives its own opposite interior slabs and a
+# non-periodic one keeps whatever its halo cells held.
+
+HALO_ORDER = {1: (1, 2), 2: (0, 2), 3: (0, 1)}
+
+
+def halo_shape(sizes, level, ipencil):
+    return tuple(s + (2 * level if a != ipencil - 1 else 0) for a, s in enumerate(sizes))
+
+
+def halo_exchange(arr, level, ipencil, periodic):
+    """In place on a halo array (numpy, Fortran order)."""
+    for d in HALO_ORDER[ipencil]:
+        if not periodic[d] or level == 0:
+            continue
+        n = arr.shape[d]
+        lo = [slice(None)] * 3
+        src = [slice(None)] * 3
+        lo[d] = slice(0, level)
+        src[d] = slice(n - 2 * level, n - level)   # MPI_IRECV(low halo) <- neighbour's last slabs
+        hi = [slice(None)] * 3
+        src2 = [slice(None)] * 3
+        hi[d] = slice(n - level, n)
+        src2[d] = slice(level, 2 * level)           # high halo <- neighbour's first slabs
+        a = arr[tuple(src)].copy()
+        b = arr[tuple(src2)].copy()
+        arr[tuple(lo)] = a
+        arr[tuple(hi)] = b
+    return arr
+
+
+def update_halo(x, level, ipencil, periodic, out_init=None):
+    """update_halo(in, out, level, opt_pencil=ipencil) at one rank; out_init is
+    what the halo array held before (the reference allocates it uninitialised)."""
+    x = np.asfortranarray(x)
+    shape = halo_shape(x.shape, level, ipencil)
+    out = (np.zeros(shape, dtype=x.dtype, order="F") if out_init is None
+           else np.array(out_init, dtype=x.dtype, order="F", copy=True))
+    sl = tuple(slice(level, level + x.shape[a]) if a != ipencil - 1 else slice(None)
+               for a in range(3))
+    out[sl] = x
+    return halo_exchange(out, level, ipencil, periodic)
