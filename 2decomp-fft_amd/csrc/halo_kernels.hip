@@ -26,19 +26,19 @@ struct alignas(16) H16 {
   unsigned long long a, b;
 };
 
-// out (n1, n2, n3) at offset (o1, o2, o3) <- in (s1, s2, s3), i fastest.
+// out (n1, n2, n3) at offset (o1, o2, o3) <- in (s1, s2, s3), i fastest: a
+// block per pencil row (j, k), its threads along i.
 template <typename E>
 __global__ void __launch_bounds__(256) halo_interior_kernel(const E *__restrict__ in,
                                                             E *__restrict__ out, int s1, int s2,
                                                             int s3, int o1, int o2, int o3, int n1,
                                                             int n2) {
-  const long long total = (long long)s1 * s2 * s3;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
-    const long long r = t / s1;
-    const int i = (int)(t - r * s1);
-    const int k = (int)(r / s2), j = (int)(r - (long long)k * s2);
-    out[(i + o1) + (long long)(j + o2) * n1 + (long long)(k + o3) * n1 * n2] = in[t];
+  const long long rows = (long long)s2 * s3;
+  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int k = (int)(row / s2), j = (int)(row - (long long)k * s2);
+    const E *src = in + row * s1;
+    E *dst = out + o1 + (long long)(j + o2) * n1 + (long long)(k + o3) * n1 * n2;
+    for (int i = threadIdx.x; i < s1; i += blockDim.x) dst[i] = src[i];
   }
 }
 
@@ -80,11 +80,12 @@ template <typename E>
 static int halo_run(const void *in, void *arr, const int s[3], const int n[3], const int o[3],
                     const int order[2], const int wrap[2], int level, hipStream_t st) {
   if (in) {
-    const long long total = (long long)s[0] * s[1] * s[2];
-    if (total > 0)
-      hipLaunchKernelGGL(halo_interior_kernel<E>, dim3((unsigned)halo_grid(total)), dim3(256), 0,
-                         st, (const E *)in, (E *)arr, s[0], s[1], s[2], o[0], o[1], o[2], n[0],
-                         n[1]);
+    const long long rows = (long long)s[1] * s[2];
+    if (rows > 0 && s[0] > 0) {
+      const long long g = rows < (1LL << 20) ? rows : (1LL << 20);
+      hipLaunchKernelGGL(halo_interior_kernel<E>, dim3((unsigned)g), dim3(256), 0, st,
+                         (const E *)in, (E *)arr, s[0], s[1], s[2], o[0], o[1], o[2], n[0], n[1]);
+    }
   }
   for (int a = 0; a < 2; ++a) {
     if (!wrap[a] || level == 0) continue;

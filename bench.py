@@ -324,6 +324,24 @@ def bench_config(args, torch, d2d):
         extra["dct1_round_trip_ms"] = round(timed(step1), 4)
         err1 = torch.linalg.vector_norm(y / float(1024 ** 3) - x) / torch.linalg.vector_norm(x)
         extra["dct1_round_trip_rel_l2"] = float(err1)
+    elif w == "halo512":
+        # update_halo of a 512^3 real8 x-pencil, 2 halo levels, y and z periodic
+        n, lev = 512, 2
+        x = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
+        d2d.fill_uniform(x, x.numel(), d2d.R8, 6)
+        hs = d2d.halo_shape((n, n, n), lev, 1)
+        out = torch.zeros(hs[0] * hs[1] * hs[2], dtype=torch.float64, device="cuda")
+        per = (False, True, True)
+
+        def step():
+            d2d.update_halo(d2d.R8, x, out, (n, n, n), lev, 1, per)
+        ms = timed(step)
+        # interior read + halo-array write, plus the halo slabs (read + write)
+        halo = out.numel() - x.numel()
+        nbytes = (x.numel() + out.numel() + 2 * halo) * 8
+        err = 0.0
+        cfg = {"workload": "update_halo, 512^3 real8 x-pencil, level 2, y and z periodic "
+                           "(halo array 512x516x516)"}
     else:  # pack2048
         dec = d2d.decomp_info_init(2048, 2048, 2048, 4, 4, 0, 0)
         xs = dec.xsz
@@ -360,12 +378,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan-time autotuner (use the default kernels)")
     ap.add_argument("--workload", choices=["r2c1024", "batch64", "c2c128", "r2c512", "r2cz720",
-                                           "pack2048", "r2cpack2048", "dtt513"],
+                                           "pack2048", "r2cpack2048", "dtt513", "halo512"],
                     default="r2c1024",
                     help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids; "
                          "c2c128 / r2c512 / r2cz720 / pack2048: the other parity configurations, "
                          "timed; r2cpack2048: the pack2048 rank's x stage fused with its pack; "
-                         "dtt513: 3D DCT-II + DCT-III round trip")
+                         "dtt513: 3D DCT-II + DCT-III round trip; halo512: update_halo")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -380,7 +398,8 @@ def main():
     if args.workload == "batch64":
         bench_batch(args, torch, d2d)
         return
-    if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048", "dtt513"):
+    if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048", "dtt513",
+                         "halo512"):
         bench_config(args, torch, d2d)
         return
     n = args.n
