@@ -117,6 +117,14 @@ module d2dhip_c
          type(c_ptr), value :: plan, din, dout
          integer(c_int), value :: isign
       end function
+      integer(c_int) function d2dhip_halo_exchange(elem, arr, s1, s2, s3, level, ipencil, &
+                                                   periodic, stream) &
+         bind(C, name="d2dhip_halo_exchange")
+         import :: c_int, c_ptr
+         integer(c_int), value :: elem, s1, s2, s3, level, ipencil
+         type(c_ptr), value :: arr, stream
+         integer(c_int), intent(in) :: periodic(3)
+      end function
       integer(c_int) function d2dhip_transpose(dir, elem, src, dst, d, stream) &
          bind(C, name="d2dhip_transpose")
          import :: c_int, c_ptr, d2dhip_decomp
@@ -220,6 +228,8 @@ module decomp_2d
    integer, save, dimension(3), public :: xstart, xend, xsize, ystart, yend, ysize, &
                                           zstart, zend, zsize
    type(decomp_info), target, save, public :: decomp_main
+   ! periodic_bc of decomp_2d_init (halo cells, src/decomp_2d_init_fin.f90:63-70)
+   logical, save, public :: periodic_x = .false., periodic_y = .false., periodic_z = .false.
 
    public :: decomp_2d_init, decomp_2d_finalize, decomp_info_init, decomp_info_finalize, &
              decomp_2d_abort, decomp_2d_warning, get_decomp_dims, &
@@ -252,8 +262,9 @@ contains
 
    ! decomp_2d_init (src/decomp_2d_init_fin.f90:15-184) for one rank:
    ! p_row = p_col = 0 runs best_2d_grid(1) = 1 x 1.
-   subroutine decomp_2d_init(nx, ny, nz, p_row, p_col)
+   subroutine decomp_2d_init(nx, ny, nz, p_row, p_col, periodic_bc)
       integer, intent(in) :: nx, ny, nz, p_row, p_col
+      logical, dimension(3), intent(in), optional :: periodic_bc
       integer :: pr, pc
       integer(c_int) :: r, c, st
       if (nx <= 0) call decomp_2d_abort(__FILE__, __LINE__, nx, "Invalid value for nx")
@@ -269,6 +280,14 @@ contains
       if (pr * pc /= nproc) call decomp_2d_abort(__FILE__, __LINE__, pr * pc, &
                                                  "Invalid 2D processor grid - nproc /= p_row*p_col")
       dims = [pr, pc]
+      periodic_x = .false.
+      periodic_y = .false.
+      periodic_z = .false.
+      if (present(periodic_bc)) then
+         periodic_x = periodic_bc(1)
+         periodic_y = periodic_bc(2)
+         periodic_z = periodic_bc(3)
+      end if
       nx_global = nx
       ny_global = ny
       nz_global = nz
@@ -465,6 +484,204 @@ contains
    end function gbl
 
 end module decomp_2d
+
+! m_halo (src/halo.f90): halo cells of one rank's pencil on host arrays,
+! computed on the GPU by d2dhip_halo_exchange.
+module m_halo
+
+   use, intrinsic :: iso_c_binding
+   use decomp_2d_constants
+   use decomp_2d
+   use d2dhip_c
+
+   implicit none
+
+   private
+
+   type, public :: halo_extents_t
+      integer :: xs, xe
+      integer :: ys, ye
+      integer :: zs, ze
+   end type halo_extents_t
+
+   interface update_halo
+      module procedure update_halo_real, update_halo_real_short
+      module procedure update_halo_complex, update_halo_complex_short
+   end interface update_halo
+
+   interface halo_exchange
+      module procedure halo_exchange_real, halo_exchange_complex
+   end interface halo_exchange
+
+   public :: update_halo, halo_exchange, init_halo_extents
+
+contains
+
+   ! get_pencil (src/halo.f90:179-220)
+   integer function get_pencil(sizes, decomp, opt_pencil)
+      integer, dimension(3), intent(in) :: sizes
+      type(decomp_info), intent(in) :: decomp
+      integer, intent(in), optional :: opt_pencil
+      if (present(opt_pencil)) then
+         get_pencil = opt_pencil
+      else if (sizes(1) == decomp%xsz(1)) then
+         get_pencil = 1
+         call decomp_2d_warning(__FILE__, __LINE__, 0, &
+                                "Deprecated interface - calling halo in X without explicit pencil")
+      else if (sizes(2) == decomp%ysz(2)) then
+         get_pencil = 2
+         call decomp_2d_warning(__FILE__, __LINE__, 0, &
+                                "Deprecated interface - calling halo in Y without explicit pencil")
+      else if (sizes(3) == decomp%zsz(3)) then
+         get_pencil = 3
+         call decomp_2d_warning(__FILE__, __LINE__, 0, &
+                                "Deprecated interface - calling halo in Z without explicit pencil")
+      else
+         get_pencil = 0
+         call decomp_2d_abort(__FILE__, __LINE__, 1, "Invalid decomposition size")
+      end if
+   end function get_pencil
+
+   ! init_halo_extents (src/halo.f90:222-292)
+   type(halo_extents_t) function init_halo_extents(ipencil, sizes, decomp, level, global) &
+      result(h)
+      integer, intent(in) :: ipencil, level
+      integer, dimension(3), intent(in) :: sizes
+      type(decomp_info), intent(in) :: decomp
+      logical, intent(in) :: global
+      integer :: st(3), lv(3)
+      st = 1
+      if (global) then
+         if (ipencil == 1) st = decomp%xst
+         if (ipencil == 2) st = decomp%yst
+         if (ipencil == 3) st = decomp%zst
+      end if
+      lv = level
+      lv(ipencil) = 0
+      h%xs = st(1) - lv(1); h%xe = st(1) + sizes(1) - 1 + lv(1)
+      h%ys = st(2) - lv(2); h%ye = st(2) + sizes(2) - 1 + lv(2)
+      h%zs = st(3) - lv(3); h%ze = st(3) + sizes(3) - 1 + lv(3)
+   end function init_halo_extents
+
+   function periodic_flags() result(p)
+      integer(c_int) :: p(3)
+      p = 0
+      if (periodic_x) p(1) = 1
+      if (periodic_y) p(2) = 1
+      if (periodic_z) p(3) = 1
+   end function periodic_flags
+
+   ! the halo array through the GPU: interior already in arr
+   subroutine exchange_dev(arr, nbytes, elem, sizes, level, ipencil)
+      type(c_ptr), intent(in) :: arr
+      integer(c_size_t), intent(in) :: nbytes
+      integer(c_int), intent(in) :: elem
+      integer, intent(in) :: sizes(3), level, ipencil
+      type(c_ptr) :: d
+      integer(c_int) :: st
+      d = c_null_ptr
+      st = hipMalloc(d, nbytes)
+      if (st /= 0) call decomp_2d_abort(__FILE__, __LINE__, int(st), "hipMalloc(halo)")
+      st = hipMemcpy(d, arr, nbytes, HIP_H2D)
+      if (st == 0) st = d2dhip_halo_exchange(elem, d, int(sizes(1), c_int), int(sizes(2), c_int), &
+                                             int(sizes(3), c_int), int(level, c_int), &
+                                             int(ipencil, c_int), periodic_flags(), c_null_ptr)
+      if (st == 0) st = hipMemcpy(arr, d, nbytes, HIP_D2H)
+      if (st /= 0) call decomp_2d_abort(__FILE__, __LINE__, int(st), c_message())
+      st = hipFree(d)
+   end subroutine exchange_dev
+
+   ! update_halo (src/halo.f90:112-260, src/halo_common.f90)
+   subroutine update_halo_real(in, out, level, decomp, opt_global, opt_pencil)
+      real(mytype), dimension(:, :, :), intent(in) :: in
+      real(mytype), allocatable, dimension(:, :, :), target, intent(out) :: out
+      integer, intent(in) :: level
+      type(decomp_info), intent(in) :: decomp
+      logical, intent(in), optional :: opt_global
+      integer, intent(in), optional :: opt_pencil
+      type(halo_extents_t) :: h
+      integer :: ip, sz(3)
+      logical :: global
+      global = .false.
+      if (present(opt_global)) global = opt_global
+      sz = shape(in)
+      ip = get_pencil(sz, decomp, opt_pencil)
+      h = init_halo_extents(ip, sz, decomp, level, global)
+      allocate (out(h%xs:h%xe, h%ys:h%ye, h%zs:h%ze))
+      out = 0._mytype
+      out(h%xs + merge(0, level, ip == 1):h%xs + merge(0, level, ip == 1) + sz(1) - 1, &
+          h%ys + merge(0, level, ip == 2):h%ys + merge(0, level, ip == 2) + sz(2) - 1, &
+          h%zs + merge(0, level, ip == 3):h%zs + merge(0, level, ip == 3) + sz(3) - 1) = in
+      call halo_exchange_real(out, ip, h, level, sz)
+   end subroutine update_halo_real
+
+   subroutine update_halo_real_short(in, out, level, opt_global, opt_pencil)
+      real(mytype), dimension(:, :, :), intent(in) :: in
+      real(mytype), allocatable, dimension(:, :, :), target, intent(out) :: out
+      integer, intent(in) :: level
+      logical, intent(in), optional :: opt_global
+      integer, intent(in), optional :: opt_pencil
+      call update_halo_real(in, out, level, decomp_main, opt_global, opt_pencil)
+   end subroutine update_halo_real_short
+
+   subroutine update_halo_complex(in, out, level, decomp, opt_global, opt_pencil)
+      complex(mytype), dimension(:, :, :), intent(in) :: in
+      complex(mytype), allocatable, dimension(:, :, :), target, intent(out) :: out
+      integer, intent(in) :: level
+      type(decomp_info), intent(in) :: decomp
+      logical, intent(in), optional :: opt_global
+      integer, intent(in), optional :: opt_pencil
+      type(halo_extents_t) :: h
+      integer :: ip, sz(3)
+      logical :: global
+      global = .false.
+      if (present(opt_global)) global = opt_global
+      sz = shape(in)
+      ip = get_pencil(sz, decomp, opt_pencil)
+      h = init_halo_extents(ip, sz, decomp, level, global)
+      allocate (out(h%xs:h%xe, h%ys:h%ye, h%zs:h%ze))
+      out = (0._mytype, 0._mytype)
+      out(h%xs + merge(0, level, ip == 1):h%xs + merge(0, level, ip == 1) + sz(1) - 1, &
+          h%ys + merge(0, level, ip == 2):h%ys + merge(0, level, ip == 2) + sz(2) - 1, &
+          h%zs + merge(0, level, ip == 3):h%zs + merge(0, level, ip == 3) + sz(3) - 1) = in
+      call halo_exchange_complex(out, ip, h, level, sz)
+   end subroutine update_halo_complex
+
+   subroutine update_halo_complex_short(in, out, level, opt_global, opt_pencil)
+      complex(mytype), dimension(:, :, :), intent(in) :: in
+      complex(mytype), allocatable, dimension(:, :, :), target, intent(out) :: out
+      integer, intent(in) :: level
+      logical, intent(in), optional :: opt_global
+      integer, intent(in), optional :: opt_pencil
+      call update_halo_complex(in, out, level, decomp_main, opt_global, opt_pencil)
+   end subroutine update_halo_complex_short
+
+   ! halo_exchange (src/halo.f90:294-399 with the body files)
+   subroutine halo_exchange_real(arr, ipencil, halo_extents, level, sizes)
+      real(mytype), dimension(:, :, :), contiguous, target, intent(inout) :: arr
+      integer, intent(in) :: ipencil, level
+      type(halo_extents_t), intent(in) :: halo_extents
+      integer, dimension(3), intent(in) :: sizes
+      if (size(arr, kind=c_size_t) /= int(halo_extents%xe - halo_extents%xs + 1, c_size_t) * &
+          (halo_extents%ye - halo_extents%ys + 1) * (halo_extents%ze - halo_extents%zs + 1)) &
+         call decomp_2d_abort(__FILE__, __LINE__, 1, "halo array does not match its extents")
+      call exchange_dev(c_loc(arr), 8_c_size_t * size(arr, kind=c_size_t), 0_c_int, sizes, &
+                        level, ipencil)
+   end subroutine halo_exchange_real
+
+   subroutine halo_exchange_complex(arr, ipencil, halo_extents, level, sizes)
+      complex(mytype), dimension(:, :, :), contiguous, target, intent(inout) :: arr
+      integer, intent(in) :: ipencil, level
+      type(halo_extents_t), intent(in) :: halo_extents
+      integer, dimension(3), intent(in) :: sizes
+      if (size(arr, kind=c_size_t) /= int(halo_extents%xe - halo_extents%xs + 1, c_size_t) * &
+          (halo_extents%ye - halo_extents%ys + 1) * (halo_extents%ze - halo_extents%zs + 1)) &
+         call decomp_2d_abort(__FILE__, __LINE__, 1, "halo array does not match its extents")
+      call exchange_dev(c_loc(arr), 16_c_size_t * size(arr, kind=c_size_t), 1_c_int, sizes, &
+                        level, ipencil)
+   end subroutine halo_exchange_complex
+
+end module m_halo
 
 module decomp_2d_fft
 

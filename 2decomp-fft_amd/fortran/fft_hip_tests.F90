@@ -10,7 +10,10 @@
 !!     exact compare;
 !!   examples/dtt/fftw_f03_dtt_x.f90 and _z.f90: every DCT / DST type triple
 !!     on 8 x 9 x 10, forward then backward, normalised, Linf <= 100 epsilon
-!!     on the transformed points.
+!!     on the transformed points;
+!!   examples/halo_test/halo_test.f90: a divergence through halo cells in
+!!     each pencil against the one from transposes, within 2 epsilon, and
+!!     the halo cells of a periodic grid against the wrapped pencil.
 !! Exits with a non-zero status (decomp_2d_abort / error stop) on failure.
 program fft_hip_tests
 
@@ -19,6 +22,7 @@ program fft_hip_tests
    use decomp_2d
    use decomp_2d_fft
    use d2dhip_c, only: hipMalloc, hipFree, hipMemcpy, HIP_H2D, HIP_D2H
+   use m_halo
 
    implicit none
 
@@ -58,6 +62,10 @@ program fft_hip_tests
    call decomp_2d_fft_finalize
    call check_dtt(PHYSICAL_IN_X)
    call check_dtt(PHYSICAL_IN_Z)
+   call check_halo()
+   call decomp_2d_finalize
+   call decomp_2d_init(nx, ny, nz, 0, 0, periodic_bc=[.true., .true., .true.])
+   call check_halo_periodic()
    call decomp_2d_finalize
    write (*, *) 'fft_hip_tests completed'
 
@@ -330,5 +338,126 @@ contains
       write (*, '(a,i0,a,i0)') ' dtt checks passed: ', ncase, ' type triples, format ', fmt
       deallocate (a, b, ref)
    end subroutine check_dtt
+
+   ! examples/halo_test/halo_test.f90:154-480 at one rank (non-periodic)
+   subroutine check_halo()
+      real(mytype), allocatable :: u(:, :, :), v(:, :, :), w(:, :, :), div(:, :, :), dh(:, :, :)
+      real(mytype), allocatable :: uh(:, :, :), vh(:, :, :), wh(:, :, :)
+      integer :: i, j, k, ip, s1, s2, s3
+      real(mytype) :: err, mag
+      s1 = xsize(1); s2 = xsize(2); s3 = xsize(3)
+      allocate (u(s1, s2, s3), v(s1, s2, s3), w(s1, s2, s3), div(s1, s2, s3), dh(s1, s2, s3))
+      do k = 1, s3
+         do j = 1, s2
+            do i = 1, s1
+               u(i, j, k) = sin(0.3_mytype * i + 0.7_mytype * j) * cos(0.2_mytype * k)
+               v(i, j, k) = cos(0.5_mytype * i) * sin(0.9_mytype * j + 0.1_mytype * k)
+               w(i, j, k) = sin(0.4_mytype * i * j) + cos(0.6_mytype * k)
+            end do
+         end do
+      end do
+      ! the reference divergence (test_div_transpose: transposes are copies here)
+      div = 0._mytype
+      do k = 1, s3
+         do j = 1, s2
+            do i = 2, s1 - 1
+               div(i, j, k) = u(i + 1, j, k) - u(i - 1, j, k)
+            end do
+         end do
+      end do
+      do k = 1, s3
+         do j = 2, s2 - 1
+            div(:, j, k) = div(:, j, k) + v(:, j + 1, k) - v(:, j - 1, k)
+         end do
+      end do
+      do k = 2, s3 - 1
+         div(:, :, k) = div(:, :, k) + w(:, :, k + 1) - w(:, :, k - 1)
+      end do
+      do ip = 1, 3
+         dh = 0._mytype
+         if (ip == 1) then  ! test_div_haloX
+            call update_halo(v, vh, 1, opt_pencil=1)
+            call update_halo(w, wh, 1, opt_pencil=1)
+            if (any(shape(vh) /= [s1, s2 + 2, s3 + 2])) error stop "halo X size"
+            do k = 1, s3
+               do j = 1, s2
+                  do i = 2, s1 - 1
+                     dh(i, j, k) = (u(i + 1, j, k) - u(i - 1, j, k)) + (vh(i, j + 1, k) - vh(i, j - 1, k)) &
+                                   + (wh(i, j, k + 1) - wh(i, j, k - 1))
+                  end do
+               end do
+            end do
+            deallocate (vh, wh)
+         else if (ip == 2) then  ! test_div_haloY
+            call update_halo(u, uh, 1, opt_pencil=2)
+            call update_halo(w, wh, 1, opt_pencil=2)
+            if (any(shape(uh) /= [s1 + 2, s2, s3 + 2])) error stop "halo Y size"
+            do k = 1, s3
+               do j = 2, s2 - 1
+                  do i = 1, s1
+                     dh(i, j, k) = (uh(i + 1, j, k) - uh(i - 1, j, k)) + (v(i, j + 1, k) - v(i, j - 1, k)) &
+                                   + (wh(i, j, k + 1) - wh(i, j, k - 1))
+                  end do
+               end do
+            end do
+            deallocate (uh, wh)
+         else  ! test_div_haloZ
+            call update_halo(u, uh, 1, opt_pencil=3)
+            call update_halo(v, vh, 1, opt_pencil=3)
+            if (any(shape(uh) /= [s1 + 2, s2 + 2, s3])) error stop "halo Z size"
+            do k = 2, s3 - 1
+               do j = 1, s2
+                  do i = 1, s1
+                     dh(i, j, k) = (uh(i + 1, j, k) - uh(i - 1, j, k)) + (vh(i, j + 1, k) - vh(i, j - 1, k)) &
+                                   + (w(i, j, k + 1) - w(i, j, k - 1))
+                  end do
+               end do
+            end do
+            deallocate (uh, vh)
+         end if
+         ! check_err: interior points only
+         err = sqrt(sum((dh(2:s1 - 1, 2:s2 - 1, 2:s3 - 1) - div(2:s1 - 1, 2:s2 - 1, 2:s3 - 1))**2))
+         mag = sqrt(sum(div(2:s1 - 1, 2:s2 - 1, 2:s3 - 1)**2))
+         if (.not. (err < 2._mytype * epsilon(mag) * mag)) then
+            write (*, *) "halo divergence, pencil ", ip, err, mag
+            call decomp_2d_abort(__FILE__, __LINE__, ip, "Error in halo_test")
+         end if
+      end do
+      write (*, *) 'halo divergence checks passed'
+      deallocate (u, v, w, div, dh)
+   end subroutine check_halo
+
+   ! periodic_bc = all: the halo cells are the wrapped pencil (global indices too)
+   subroutine check_halo_periodic()
+      real(mytype), allocatable :: a(:, :, :), ah(:, :, :)
+      integer :: s1, s2, s3, i, j, k
+      s1 = xsize(1); s2 = xsize(2); s3 = xsize(3)
+      allocate (a(s1, s2, s3))
+      do k = 1, s3
+         do j = 1, s2
+            do i = 1, s1
+               a(i, j, k) = real(i + 100 * j + 10000 * k, mytype)
+            end do
+         end do
+      end do
+      call update_halo(a, ah, 2, opt_global=.true., opt_pencil=1)
+      if (lbound(ah, 2) /= -1 .or. ubound(ah, 3) /= s3 + 2) error stop "halo global bounds"
+      do k = -1, s3 + 2
+         do j = -1, s2 + 2
+            if (any(ah(:, j, k) /= a(:, modulo(j - 1, s2) + 1, modulo(k - 1, s3) + 1))) &
+               call decomp_2d_abort(__FILE__, __LINE__, 1, "periodic halo cell")
+         end do
+      end do
+      deallocate (ah)
+      call update_halo(a, ah, 1, opt_pencil=3)
+      do j = 0, s2 + 1
+         do i = 0, s1 + 1
+            if (any(ah(i, j, :) /= a(modulo(i - 1, s1) + 1, modulo(j - 1, s2) + 1, :))) &
+               call decomp_2d_abort(__FILE__, __LINE__, 3, "periodic halo cell")
+         end do
+      end do
+      write (*, *) 'periodic halo checks passed'
+      deallocate (a, ah)
+   end subroutine check_halo_periodic
 
 end program fft_hip_tests

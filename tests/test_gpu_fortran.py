@@ -1,5 +1,5 @@
 """The reference's own Fortran checks (fft_c2c_x / fft_r2c_x / their _z
-copies, fft_multiple_grids, test2d, the dtt examples) run through the Fortran decomp_2d /
+copies, fft_multiple_grids, test2d, the dtt and halo examples) run through the Fortran decomp_2d /
 decomp_2d_fft modules over the C ABI (2decomp-fft_amd/fortran)."""
 import os
 import subprocess
@@ -22,3 +22,5 @@ def test_fortran_reference_checks(torch_gpu):
     assert "transposes exact" in r.stdout
     assert r.stdout.count("error / mesh point") == 12
     assert r.stdout.count("dtt checks passed: 512 type triples") == 2
+    assert "halo divergence checks passed" in r.stdout
+    assert "periodic halo checks passed" in r.stdout
