@@ -44,20 +44,43 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   const int r0 = LINES ? tid % RPT : tid / TX;
   const long long c0 = c0b + t;
   const long long ca = 2 * c0;  // first real column of a pair (real IO)
-  constexpr bool DTT = IO == IO_DTT2 || IO == IO_DTT3 || IO == IO_DTTX;
+  constexpr bool DTT = IO == IO_DTT2 || IO == IO_DTT3 || IO == IO_DTTX || IO == IO_DTT4;
+  constexpr bool PAIRED = IO != IO_C2C && IO != IO_DTT4;  // two real columns per tile column
   static_assert(!DTT || XM == 0, "real-to-real stages use the plain exchange");
-  const bool cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
+  const bool cvalid = PAIRED ? (ca < d->C0r) : (c0 < d->C0);
   // real-to-real stages: the transformed points start at dtt_first
   const long long f = DTT ? d->dtt_first : 0;
   const long long in_base =
-      (IO == IO_R2C || DTT ? ca : c0) * d->in_s0 + c1 * d->in_s1 + f * d->in_sj;
+      ((IO == IO_R2C || DTT) && PAIRED ? ca : c0) * d->in_s0 + c1 * d->in_s1 + f * d->in_sj;
   const long long out_base =
-      (IO == IO_C2R || DTT ? ca : c0) * d->out_s0 + c1 * d->out_s1 + f * d->out_sj;
+      ((IO == IO_C2R || DTT) && PAIRED ? ca : c0) * d->out_s0 + c1 * d->out_s1 + f * d->out_sj;
   const void *tw = d->tw;
   cpx<T> v[E];
 
   // ---- inputs of pass 0 -------------------------------------------------
-  if constexpr (IO == IO_DTTX) {
+  if constexpr (IO == IO_DTT4) {
+    // v_J = (x_{2J} + i x_{2N-1-2J}) exp(-i pi (4J+1) / (8N)) of one line of 2N
+    // points; DST-IV takes (-1)^j x_j
+    const bool alt = d->dtt == 8;
+    constexpr int R = RL::r[0], NBR = N / R;
+#pragma unroll
+    for (int q = 0; q < C::nb(0); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(0) || b < NBR) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int J = b + s * NBR;
+          cpx<T> x = {T(0), T(0)};
+          if (cvalid) {
+            x.x = gload_r<T>(in, in_base + (long long)(2 * J) * d->in_sj);
+            x.y = gload_r<T>(in, in_base + (long long)(2 * N - 1 - 2 * J) * d->in_sj);
+            if (alt) x.y = -x.y;
+          }
+          v[q * R + s] = cmul(x, gload<T>(d->dtw, J));
+        }
+      }
+    }
+  } else if constexpr (IO == IO_DTTX) {
     // the extension e of the line pair, gathered: e[J] = +-x_src or 0
     const int type = d->dtt, Lx = d->dtt_len;
     constexpr int R = RL::r[0], NBR = N / R;
@@ -369,6 +392,27 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
         }
       }
     }
+  } else if constexpr (IO == IO_DTT4) {
+    // u_k = V_k exp(-i pi k / (2N)); DCT-IV: Y_{2k} = 2 Re u_k, Y_{2N-1-2k} =
+    // -2 Im u_k; DST-IV the same outputs read backwards
+    const bool alt = d->dtt == 8;
+#pragma unroll
+    for (int q = 0; q < C::nb(P - 1); ++q) {
+      const int b = r0 + q * RPT;
+      if (C::exact(P - 1) || b < NBZ) {
+        const int k = b % LZ, g = b / LZ;
+#pragma unroll
+        for (int s = 0; s < RZ; ++s) {
+          const int pv = g * LZ * RZ + k + s * LZ;
+          const cpx<T> u = cmul(v[q * RZ + s], gload<T>(d->dtw, N + pv));
+          const int ie = alt ? 2 * N - 1 - 2 * pv : 2 * pv, io = alt ? 2 * pv : 2 * N - 1 - 2 * pv;
+          if (cvalid) {
+            gstore_r<T>(out, out_base + (long long)ie * d->out_sj, T(2) * u.x);
+            gstore_r<T>(out, out_base + (long long)io * d->out_sj, T(-2) * u.y);
+          }
+        }
+      }
+    }
   } else if constexpr (IO == IO_DTTX) {
     // Y_k from E[p] (E_a + i E_b): DCT-I p = k, DST-I p = k + 1, DCT-IV and
     // DST-IV p = 2k + 1; cosines take (Re, Im), sines (-Im, Re)
@@ -506,12 +550,13 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
       const int tt = LINES ? w / npt : w % TX;
       const int i = LINES ? w % npt : w / TX;
       const int k = i < fi ? i : i + Lt;
-      const long long cc = 2 * (c0b + tt);
+      const long long cc = PAIRED ? 2 * (c0b + tt) : c0b + tt;
       if (cc < d->C0r) {
         const long long ib = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
         const long long ob = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
         gstore_r<T>(out, ob, gload_r<T>(in, ib));
-        if (cc + 1 < d->C0r) gstore_r<T>(out, ob + d->out_s0, gload_r<T>(in, ib + d->in_s0));
+        if (PAIRED && cc + 1 < d->C0r)
+          gstore_r<T>(out, ob + d->out_s0, gload_r<T>(in, ib + d->in_s0));
       }
     }
   }
@@ -612,11 +657,13 @@ static FastVariant make_variant(const char *name) {
       make_variant<T, N, TX, NT, IO_C2R, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " c2r")
 
 // real-to-real stages: DCT-II / DST-II and DCT-III / DST-III of N points,
-// and DCT-I / IV, DST-I / IV through an extension of N points
+// DCT-I / IV, DST-I / IV through an extension of N points, and DCT-IV /
+// DST-IV of 2N points
 #define D2D_DTT(T, N, TX, NT, LINES, RL)                                                    \
   make_variant<T, N, TX, NT, IO_DTT2, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct2"), \
       make_variant<T, N, TX, NT, IO_DTT3, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct3"), \
-      make_variant<T, N, TX, NT, IO_DTTX, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dctx")
+      make_variant<T, N, TX, NT, IO_DTTX, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dctx"), \
+      make_variant<T, N, TX, NT, IO_DTT4, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct4")
 
 // c2c only, split exchange, at least WPE waves per SIMD
 #define D2D_SPLIT_C2C(T, N, TX, NT, LINES, RL, WPE)                                      \

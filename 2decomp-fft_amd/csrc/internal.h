@@ -25,9 +25,12 @@ constexpr int kRegCap = 18;
 // twiddled Hermitian split after the last), DCT-III / DST-III (a twiddled
 // Hermitian merge before the first pass, a permuted last pass) and
 // DCT-I / IV, DST-I / IV (the symmetric extension gathered by the first
-// pass, the outputs picked from the last).
+// pass, the outputs picked from the last) and DCT-IV / DST-IV of even
+// length L as an L/2-point DFT of one line (IO_DTT4: tile columns are single
+// real lines, not pairs).
 enum IoKind : int {
-  IO_C2C = 0, IO_R2C = 1, IO_C2R = 2, IO_R2R = 3, IO_DTT2 = 4, IO_DTT3 = 5, IO_DTTX = 6
+  IO_C2C = 0, IO_R2C = 1, IO_C2R = 2, IO_R2R = 3, IO_DTT2 = 4, IO_DTT3 = 5, IO_DTTX = 6,
+  IO_DTT4 = 7
 };
 
 struct PassInfo {
@@ -78,7 +81,9 @@ struct AxisDesc {
   // or the length of the line's symmetric extension).
   int dtt, dtt_first, dtt_len, dtt_n;
   // (cos, sin)(pi k / (2 dtt_len)), k < dtt_len: the twiddles of the
-  // half-length DCT-II / III and DST-II / III stages (n = dtt_len)
+  // half-length DCT-II / III and DST-II / III stages (n = dtt_len); for
+  // IO_DTT4 stages exp(-i pi (4j+1) / (4 dtt_len)), j < n, then
+  // exp(-i pi k / dtt_len), k < n (n = dtt_len / 2)
   const void *dtw;
 };
 

@@ -47,6 +47,7 @@ struct StageGeom {
   // transformed points, points per line; n is the extension's length
   int dtt = 0, dtt_first = 0, dtt_len = 0, dtt_n = 0;
   const void *dtw = nullptr;  // half-length types: (cos, sin)(pi k / (2 dtt_len))
+  const void *dtw4 = nullptr;  // DCT-IV / DST-IV on L/2-point tiles: pre- and post-twiddles
 };
 
 // Chirp tables of a Bluestein length.
@@ -431,7 +432,7 @@ int configure(d2dhip_plan_s *p, int idx, int c) {
   st = get_twiddles(p, radices, &tt);
   if (st) return st;
   a.tw = tt->d;
-  a.n = bt ? bt->M : g.n;
+  a.n = fv ? fv->n : (bt ? bt->M : g.n);
   if (bt) {
     a.blue = 1;
     a.nblue = g.n;
@@ -441,7 +442,7 @@ int configure(d2dhip_plan_s *p, int idx, int c) {
   a.npass = (int)tt->passes.size();
   for (int i = 0; i < a.npass; ++i) a.pass[i] = tt->passes[i];
   a.C0r = g.C0r;
-  a.C0 = (g.io == IO_C2C) ? g.C0r : (g.C0r + 1) / 2;
+  a.C0 = (g.io == IO_C2C || (fv && fv->io == IO_DTT4)) ? g.C0r : (g.C0r + 1) / 2;
   a.C1 = g.C1;
   if (fv) {
     s.TX = fv->TX;
@@ -473,7 +474,7 @@ int configure(d2dhip_plan_s *p, int idx, int c) {
   a.dtt_first = g.dtt_first;
   a.dtt_len = g.dtt_len;
   a.dtt_n = g.dtt_n;
-  a.dtw = g.dtw;
+  a.dtw = (fv && fv->io == IO_DTT4) ? g.dtw4 : g.dtw;
   p->fast[idx] = fv;
   g.choice = c;
   return D2DHIP_SUCCESS;
@@ -540,6 +541,38 @@ int get_dtt_twiddles(d2dhip_plan_s *p, int L, const void **out) {
   return hip_check(hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice), "hipMemcpy");
 }
 
+// exp(-i pi (4j+1) / (4L)), j < L/2, then exp(-i pi k / L), k < L/2: the pre-
+// and post-twiddles of DCT-IV / DST-IV through an L/2-point DFT (kept under
+// key -L beside the other tables).
+int get_dtt4_twiddles(d2dhip_plan_s *p, int L, const void **out) {
+  auto it = p->dtw.find(-L);
+  if (it != p->dtw.end()) {
+    *out = it->second;
+    return D2DHIP_SUCCESS;
+  }
+  const int rb = p->rb, H = L / 2;
+  std::vector<unsigned char> h((size_t)L * 2 * rb);
+  for (int k = 0; k < L; ++k) {
+    long double c, s;
+    if (k < H)
+      unit_root(4LL * k + 1, 8LL * L, c, s);
+    else
+      unit_root(k - H, 2LL * L, c, s);
+    if (rb == 8) {
+      reinterpret_cast<double *>(h.data())[2 * k] = (double)c;
+      reinterpret_cast<double *>(h.data())[2 * k + 1] = (double)-s;
+    } else {
+      reinterpret_cast<float *>(h.data())[2 * k] = (float)c;
+      reinterpret_cast<float *>(h.data())[2 * k + 1] = (float)-s;
+    }
+  }
+  void *d = nullptr;
+  if (hipMalloc(&d, h.size()) != hipSuccess) return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(DTT twiddles)");
+  p->dtw[-L] = d;
+  *out = d;
+  return hip_check(hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
 // A real-to-real stage along `axis` of the (nx,ny,nz) real array, in place
 // or not (same layout on both sides): `len` points from `first` (0-based) of
 // every line go through a transform of `type`, computed through a DFT of
@@ -551,6 +584,7 @@ int make_dtt_stage(d2dhip_plan_s *p, int idx, int axis, int type, int first, int
   g.axis = axis;
   g.io = IO_R2R;
   g.dtw = nullptr;
+  g.dtw4 = nullptr;
   switch (type) {
     case 1: g.n = 2 * (len - 1); break;
     case 5: g.n = 2 * (len + 1); break;
@@ -577,10 +611,19 @@ int make_dtt_stage(d2dhip_plan_s *p, int idx, int axis, int type, int first, int
   g.dtt_len = len;
   g.dtt_n = axis == 0 ? p->nx : (axis == 1 ? p->ny : p->nz);
   if (!std::getenv("D2DHIP_NO_FAST")) {  // specialised tiles of the stage's DFT length
+    const int lines = axis == 0 ? 1 : 0;
+    if ((type == 4 || type == 8) && len % 2 == 0) {  // first choice: L/2-point tiles
+      for (const FastVariant &v : fast_registry())
+        if (v.rb == p->rb && v.mf == 0 && v.io == IO_DTT4 && v.lines == lines && v.n == len / 2)
+          g.cands.push_back(&v);
+      if (!g.cands.empty()) {
+        const int st = get_dtt4_twiddles(p, len, &g.dtw4);
+        if (st) return st;
+      }
+    }
     const int want = (type == 2 || type == 6) ? IO_DTT2 : (type == 3 || type == 7) ? IO_DTT3 : IO_DTTX;
     for (const FastVariant &v : fast_registry())
-      if (v.rb == p->rb && v.mf == 0 && v.io == want && v.lines == (axis == 0 ? 1 : 0) &&
-          v.n == g.n)
+      if (v.rb == p->rb && v.mf == 0 && v.io == want && v.lines == lines && v.n == g.n)
         g.cands.push_back(&v);
   }
   return configure(p, idx, g.cands.empty() ? -1 : 0);
@@ -1202,7 +1245,7 @@ long long d2dhip_plan_device_bytes(d2dhip_plan p) {
   if (!p) return 0;
   long long b = (long long)p->wk_bytes;
   for (auto &kv : p->tw) b += (long long)kv.second.bytes;
-  for (auto &kv : p->dtw) b += (long long)kv.first * 2 * p->rb;
+  for (auto &kv : p->dtw) b += (long long)std::abs(kv.first) * 2 * p->rb;
   return b;
 }
 

@@ -83,6 +83,8 @@ CASES = [
     # extensions of 2^k points: the specialised DCT-I / IV, DST-I / IV tiles
     ((129, 65, 33), (4, 8, 5)),
     ((257, 513, 17), (1, 5, 1)),
+    # DCT-IV / DST-IV of even length on L/2-point tiles
+    ((257, 129, 513), (8, 4, 8)),
 ]
 
 
@@ -113,7 +115,8 @@ def test_dtt_against_oracle(d2d, oracle, torch_gpu, case, fmt, prec):
 @pytest.mark.parametrize("shape,dtt", [((513, 257, 129), (2, 7, 6)),
                                        ((65, 129, 1025), (3, 6, 2)),
                                        ((513, 129, 65), (1, 4, 5)),
-                                       ((65, 257, 33), (8, 1, 4))])
+                                       ((65, 257, 33), (8, 1, 4)),
+                                       ((257, 129, 65), (8, 4, 4))])
 def test_specialised_dtt_tiles_match_general(d2d, oracle, torch_gpu, prec, shape, dtt):
     """Every axis runs on a specialised real-to-real tile (DFTs of 64 to 1024
     points on lines and columns); the general kernel gives the same
@@ -123,7 +126,7 @@ def test_specialised_dtt_tiles_match_general(d2d, oracle, torch_gpu, prec, shape
     tol = 1e-13 if prec == 64 else 2e-6
     e = d2d.FFTEngine(1, *shape, prec=prec, opt_DTT=dtt)
     names = e.dtt_stage_kernels()
-    assert all(("dct2" in n or "dct3" in n or "dctx" in n) for n in names.values()), names
+    assert all("dct" in n for n in names.values()), names
     x = np.asfortranarray(np.random.default_rng(9).uniform(-1, 1, shape))
     xin = _dev(torch, x, rdt)
     out = torch.empty_like(xin)
@@ -165,6 +168,25 @@ def test_dtt_in_place_and_skip(d2d, oracle, torch_gpu):
     a.dtt_3d_r2r(xin, out, d2d.FORWARD)
     torch.cuda.synchronize()
     assert torch.equal(out, xin)
+
+
+@pytest.mark.parametrize("fmt", FMTS)
+def test_specialised_dtt_in_place(d2d, oracle, torch_gpu, fmt):
+    """in == out on the specialised tiles of every kind (DCT-II / DCT-I / DST-III
+    forward, their inverses backward), copied-through points included."""
+    torch = torch_gpu
+    shape, dtt = (129, 65, 257), (2, 1, 7)
+    x = np.asfortranarray(np.random.default_rng(21).uniform(-1, 1, shape))
+    e = d2d.FFTEngine(fmt, *shape, opt_DTT=dtt)
+    assert all(("dct" in n) for n in e.dtt_stage_kernels().values())
+    buf = _dev(torch, x, torch.float64)
+    e.dtt_3d_r2r(buf, buf, d2d.FORWARD)
+    torch.cuda.synchronize()
+    fwd = _host(buf, shape)
+    assert rel_l2(fwd, oracle.dtt3d(x, dtt, -1, fmt)) < 1e-13
+    e.dtt_3d_r2r(buf, buf, d2d.BACKWARD)
+    torch.cuda.synchronize()
+    assert rel_l2(_host(buf, shape), oracle.dtt3d(fwd, dtt, 1, fmt)) < 1e-13
 
 
 def test_dtt_graph_replays(d2d, torch_gpu):
