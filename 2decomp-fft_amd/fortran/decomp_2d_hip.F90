@@ -739,7 +739,8 @@ module decomp_2d_fft
              decomp_2d_fft_get_engine, decomp_2d_fft_get_format, decomp_2d_fft_get_inplace, &
              decomp_2d_fft_get_inplace_r2c, decomp_2d_fft_get_inplace_c2r, &
              decomp_2d_fft_3d_dev, decomp_2d_fft_3d_r2c_dev, decomp_2d_fft_3d_c2r_dev, &
-             decomp_2d_fft_3d_batch_dev, decomp_2d_dtt_3d_r2r, decomp_2d_fft_get_dtt_input
+             decomp_2d_fft_3d_batch_dev, decomp_2d_dtt_3d_r2r, decomp_2d_fft_get_dtt_input, &
+             decomp_2d_dtt_3d_r2r_dev
 
    interface decomp_2d_fft_init
       module procedure fft_init_noarg
@@ -1166,6 +1167,16 @@ contains
       type(decomp_2d_fft_engine), pointer :: e
       e => current()
       call check(d2dhip_exec_r2c(e%plan, d_in_r, d_out_c), __FILE__, __LINE__)
+      call check(hipDeviceSynchronize(), __FILE__, __LINE__)
+   end subroutine
+
+   ! decomp_2d_dtt_3d_r2r on device buffers of the engine's real grid
+   subroutine decomp_2d_dtt_3d_r2r_dev(d_in, d_out, isign)
+      type(c_ptr), intent(in) :: d_in, d_out
+      integer, intent(in) :: isign
+      type(decomp_2d_fft_engine), pointer :: e
+      e => current()
+      call check(d2dhip_exec_dtt(e%plan, d_in, d_out, int(isign, c_int)), __FILE__, __LINE__)
       call check(hipDeviceSynchronize(), __FILE__, __LINE__)
    end subroutine
 

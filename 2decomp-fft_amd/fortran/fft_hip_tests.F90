@@ -335,6 +335,11 @@ contains
             end do
          end do
       end do
+      ! the device-pointer form gives the host form's result bit for bit
+      call e%init(fmt, mx, my, mz, opt_DTT=[2, 7, 4])
+      call e%use_it()
+      call check_dtt_dev(ref, mx, my, mz)
+      call e%fin()
       write (*, '(a,i0,a,i0)') ' dtt checks passed: ', ncase, ' type triples, format ', fmt
       deallocate (a, b, ref)
    end subroutine check_dtt
@@ -459,5 +464,26 @@ contains
       write (*, *) 'periodic halo checks passed'
       deallocate (a, ah)
    end subroutine check_halo_periodic
+
+   subroutine check_dtt_dev(ref, mx, my, mz)
+      integer, intent(in) :: mx, my, mz
+      real(mytype), intent(in), target :: ref(mx, my, mz)
+      real(mytype), allocatable, target :: h(:, :, :), hd(:, :, :)
+      type(c_ptr) :: da, db
+      integer(c_size_t) :: nb
+      allocate (h(mx, my, mz), hd(mx, my, mz))
+      h = ref
+      call decomp_2d_dtt_3d_r2r(h, hd, DECOMP_2D_FFT_FORWARD)
+      nb = 8_c_size_t * mx * my * mz
+      da = c_null_ptr
+      db = c_null_ptr
+      if (hipMalloc(da, nb) /= 0 .or. hipMalloc(db, nb) /= 0) error stop "hipMalloc"
+      if (hipMemcpy(da, c_loc(ref), nb, HIP_H2D) /= 0) error stop "hipMemcpy"
+      call decomp_2d_dtt_3d_r2r_dev(da, db, DECOMP_2D_FFT_FORWARD)
+      if (hipMemcpy(c_loc(h), db, nb, HIP_D2H) /= 0) error stop "hipMemcpy"
+      if (any(h /= hd)) call decomp_2d_abort(__FILE__, __LINE__, 1, "DTT device form differs")
+      if (hipFree(da) /= 0 .or. hipFree(db) /= 0) error stop "hipFree"
+      deallocate (h, hd)
+   end subroutine check_dtt_dev
 
 end program fft_hip_tests
