@@ -324,6 +324,20 @@ def bench_config(args, torch, d2d):
         extra["dct1_round_trip_ms"] = round(timed(step1), 4)
         err1 = torch.linalg.vector_norm(y / float(1024 ** 3) - x) / torch.linalg.vector_norm(x)
         extra["dct1_round_trip_rel_l2"] = float(err1)
+        del e1, f, y
+        # the DCT-II / DCT-III round trip in FP32 storage and arithmetic
+        x32 = x.float()
+        f32 = torch.empty_like(x32)
+        y32 = torch.empty_like(x32)
+        e32 = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, prec=d2d.PREC_F32, opt_DTT=(2, 2, 2))
+
+        def step32():
+            e32.dtt_3d_r2r(x32, f32, d2d.FORWARD)
+            e32.dtt_3d_r2r(f32, y32, d2d.BACKWARD)
+        extra["f32_round_trip_ms"] = round(timed(step32), 4)
+        y32v = y32.view(n, n, n)[core].double() / float(1024 ** 3)
+        extra["f32_round_trip_rel_l2"] = float(torch.linalg.vector_norm(y32v - xv) /
+                                               torch.linalg.vector_norm(xv))
     elif w == "halo512":
         # update_halo of a 512^3 real8 x-pencil, 2 halo levels, y and z periodic
         n, lev = 512, 2
