@@ -148,6 +148,22 @@ def test_specialised_dtt_tiles_match_general(d2d, oracle, torch_gpu, prec, shape
     assert rel_l2(y[sl], _host(xin, shape)[sl]) < tol
 
 
+def test_dtt_on_split_bf16_plan(d2d, oracle, torch_gpu):
+    """A split-BF16 plan stores FP32, so its real-to-real stages run the FP32
+    tiles (the matrix-core kernels are complex-to-complex only)."""
+    torch = torch_gpu
+    shape, dtt = (513, 65, 33), (2, 3, 4)
+    e = d2d.FFTEngine(1, *shape, prec=d2d.PREC_SPLIT_BF16, opt_DTT=dtt)
+    names = e.dtt_stage_kernels()
+    assert names[7].startswith("float") and "dct2" in names[7], names
+    x = np.asfortranarray(np.random.default_rng(4).uniform(-1, 1, shape))
+    xin = _dev(torch, x, torch.float32)
+    out = torch.empty_like(xin)
+    e.dtt_3d_r2r(xin, out, d2d.FORWARD)
+    torch.cuda.synchronize()
+    assert rel_l2(_host(out, shape), oracle.dtt3d(_host(xin, shape), dtt, -1)) < 2e-6
+
+
 def test_dtt_in_place_and_skip(d2d, oracle, torch_gpu):
     torch = torch_gpu
     shape, dtt = (48, 40, 36), (4, 1, 6)
