@@ -504,6 +504,12 @@ module m_halo
       integer :: zs, ze
    end type halo_extents_t
 
+   ! the halo_extents_t(ipencil, sizes, decomp, level, global) constructor
+   ! (src/halo.f90:20-22)
+   interface halo_extents_t
+      module procedure init_halo_extents
+   end interface halo_extents_t
+
    interface update_halo
       module procedure update_halo_real, update_halo_real_short
       module procedure update_halo_complex, update_halo_complex_short

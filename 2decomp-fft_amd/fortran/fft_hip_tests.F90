@@ -436,6 +436,7 @@ contains
    subroutine check_halo_periodic()
       real(mytype), allocatable :: a(:, :, :), ah(:, :, :)
       integer :: s1, s2, s3, i, j, k
+      type(halo_extents_t) :: h
       s1 = xsize(1); s2 = xsize(2); s3 = xsize(3)
       allocate (a(s1, s2, s3))
       do k = 1, s3
@@ -454,7 +455,12 @@ contains
          end do
       end do
       deallocate (ah)
-      call update_halo(a, ah, 1, opt_pencil=3)
+      ! exchange_test's form: the halo array filled by hand, halo_exchange in place
+      h = halo_extents_t(3, [s1, s2, s3], decomp_main, 1, .false.)
+      allocate (ah(h%xs:h%xe, h%ys:h%ye, h%zs:h%ze))
+      ah = -1._mytype
+      ah(1:s1, 1:s2, :) = a
+      call halo_exchange(ah, 3, h, 1, [s1, s2, s3])
       do j = 0, s2 + 1
          do i = 0, s1 + 1
             if (any(ah(i, j, :) /= a(modulo(i - 1, s1) + 1, modulo(j - 1, s2) + 1, :))) &
