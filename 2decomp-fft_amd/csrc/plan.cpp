@@ -737,6 +737,11 @@ std::string tune_key(const d2dhip_plan_s *p, int idx) {
                 "v2 pr=%d io=%d ax=%d n=%d C0r=%lld C1=%lld s=%lld,%lld,%lld,%lld,%lld,%lld c=%08x",
                 p->prec, g.io, g.axis, g.n, g.C0r, g.C1, g.in_s0, g.in_s1, g.in_sj, g.out_s0,
                 g.out_s1, g.out_sj, h);
+  if (g.io == IO_R2R) {  // real-to-real stages: the transform and its points too
+    char t[96];
+    std::snprintf(t, sizeof(t), " dtt=%d,%d,%d,%d", g.dtt, g.dtt_first, g.dtt_len, g.dtt_n);
+    return std::string(b) + t;
+  }
   return b;
 }
 
@@ -1376,6 +1381,8 @@ int d2dhip_plan_set_dtt(d2dhip_plan p, int ndtt, const int *in_dtt) {
     if (!st) st = make_dtt_stage(p, 10 + k, k, v[12 + k], v[9 + k] - 1, len);
   }
   if (!st) st = upload_descs(p);
+  // plans made to be tuned time the real-to-real stages' kernels too
+  if (!st && ((p->flags & D2DHIP_FLAG_TUNE) || std::getenv("D2DHIP_TUNE"))) st = tune_stages(p);
   p->has_dtt = !st;
   if (!st) std::memcpy(p->dtt, v, sizeof(v));
   return st;

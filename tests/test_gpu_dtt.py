@@ -205,6 +205,27 @@ def test_specialised_dtt_in_place(d2d, oracle, torch_gpu, fmt):
     assert rel_l2(_host(buf, shape), oracle.dtt3d(fwd, dtt, 1, fmt)) < 1e-13
 
 
+def test_tuned_dtt_plan(d2d, oracle, torch_gpu, tmp_path, monkeypatch):
+    """A plan made to be tuned times its real-to-real stages' kernels too (the
+    general kernel and the specialised tiles), caches the choices with the
+    transform in the key, and computes the same transforms."""
+    torch = torch_gpu
+    cache = tmp_path / "tune.tsv"
+    monkeypatch.setenv("D2DHIP_TUNE_CACHE", str(cache))
+    shape, dtt = (129, 65, 257), (2, 4, 7)
+    e = d2d.FFTEngine(1, *shape, tune=True, opt_DTT=dtt)
+    names = e.dtt_stage_kernels()
+    assert all(n in e.stage_candidates(st) for st, n in names.items()), names
+    x = np.asfortranarray(np.random.default_rng(13).uniform(-1, 1, shape))
+    xin = _dev(torch, x, torch.float64)
+    out = torch.empty_like(xin)
+    e.dtt_3d_r2r(xin, out, d2d.FORWARD)
+    torch.cuda.synchronize()
+    assert rel_l2(_host(out, shape), oracle.dtt3d(x, dtt, -1)) < 1e-13
+    if cache.exists():  # written unless an earlier plan of this process loaded another cache
+        assert "dtt=" in cache.read_text()
+
+
 def test_dtt_graph_replays(d2d, torch_gpu):
     torch = torch_gpu
     shape = (32, 24, 20)
