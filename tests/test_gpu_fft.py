@@ -204,6 +204,32 @@ def test_config1_c2c_128_cube(d2d, torch_gpu, oracle):
         assert torch_gpu.equal(t, b)
 
 
+def test_tuned_plan_cache(d2d, torch_gpu, tmp_path, monkeypatch):
+    """D2DHIP_FLAG_TUNE times each stage's candidates (the general kernel and the
+    specialised tiles of 128 and 256 points), keeps a candidate of that stage,
+    writes the choice to the cache file, and the tuned plan transforms as the
+    untuned one does (to rounding: the choices may differ)."""
+    cache = tmp_path / "tune.tsv"
+    monkeypatch.setenv("D2DHIP_TUNE_CACHE", str(cache))
+    shape = (256, 128, 128)
+    x = np.zeros(shape, np.complex128, order="F")
+    d2d.host_fill_uniform(x, d2d.C16, 3)
+    t = d2d.FFTEngine(1, *shape, tune=True)
+    for st, name in t.stage_kernels().items():
+        assert name in t.stage_candidates(st), (st, name)
+    u = d2d.FFTEngine(1, *shape)
+    a = to_dev(torch_gpu, x)
+    bt = torch_gpu.empty_like(a)
+    bu = torch_gpu.empty_like(a)
+    t.fft_c2c(a, bt, -1)
+    u.fft_c2c(a, bu, -1)
+    torch_gpu.cuda.synchronize()
+    assert rel_l2(from_dev(bt, shape), from_dev(bu, shape)) < F64_TOL
+    assert rel_l2(from_dev(bt, shape), np.fft.fftn(x)) < F64_TOL
+    if cache.exists():  # written unless an earlier plan of this process loaded another cache
+        assert "io=0 ax=0 n=256" in cache.read_text()
+
+
 @pytest.mark.parametrize("shape", [(90, 60, 48), (48, 80, 96), (160, 192, 32), (45, 27, 25),
                                    (7, 49, 98), (64, 1, 33), (1, 1, 1), (2, 3, 5)])
 @pytest.mark.parametrize("fmt", [1, 3])
