@@ -44,6 +44,7 @@ ERR_HIP = 4
 ERR_NULL = 5
 ERR_GRID = 6
 WARN_SKIP_IGNORED = 100
+WARN_TUNE_CACHE = 101
 MAX_PROCS = 64
 
 _lib = None
@@ -315,6 +316,7 @@ class FFTEngine:
     def fft_c2r(self, inp, out):
         self._sync_stream()
         return check(lib().d2dhip_exec_c2r(self._plan, _ptr(inp), _ptr(out)))
+
 
     def set_dtt(self, dtt):
         """DCT / DST types of the x, y, z axes (3 values, 1-8), or 12 values

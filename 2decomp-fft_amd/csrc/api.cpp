@@ -104,6 +104,7 @@ const char *d2dhip_status_string(int status) {
     case D2DHIP_ERR_GRID: return "Invalid 2D processor grid";
     case D2DHIP_ERR_PLAN_MISMATCH: return "call does not match the plan";
     case D2DHIP_WARN_SKIP_IGNORED: return "r2c / c2r transform can not be skipped";
+    case D2DHIP_WARN_TUNE_CACHE: return "tuning choices could not be written to the cache";
     default: return "unknown status";
   }
 }

@@ -111,6 +111,7 @@ struct d2dhip_plan_s {
   // hipGraph of the exec calls made between capture_begin and capture_end
   hipStream_t cap_stream = nullptr, cap_keep = nullptr;
   bool capturing = false;
+  bool tune_write_failed = false;  // a tuning choice could not be written to the cache
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
 };
@@ -631,10 +632,32 @@ int make_dtt_stage(d2dhip_plan_s *p, int idx, int axis, int type, int first, int
 
 int upload_chunk_descs(d2dhip_plan_s *p);
 
+// A captured graph holds the kernels, grids and chunk offsets of the
+// configuration it was recorded with, while the descriptors it reads are
+// rewritten in place; every change of configuration therefore drops it
+// (graph_launch then fails until the plan is captured again).
+void drop_graph(d2dhip_plan_s *p) {
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+}
+
+// Host -> device copy of plan metadata, ordered after the work already
+// queued on the plan's stream (which may still read the old values) and
+// complete on return (the host copy may change right after).
+int upload_sync(d2dhip_plan_s *p, void *dst, const void *src, size_t bytes, const char *what) {
+  int st = hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, p->stream), what);
+  if (!st) st = hip_check(hipStreamSynchronize(p->stream), what);
+  return st;
+}
+
 int upload_descs(d2dhip_plan_s *p) {
-  int st = hip_check(hipMemcpy(p->d_desc, p->h_desc, sizeof(AxisDesc) * d2dhip_plan_s::kStages,
-                               hipMemcpyHostToDevice),
-                     "hipMemcpy(stage descriptors)");
+  if (p->capturing)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "plan is capturing: its configuration is fixed");
+  drop_graph(p);
+  int st = upload_sync(p, p->d_desc, p->h_desc, sizeof(AxisDesc) * d2dhip_plan_s::kStages,
+                       "hipMemcpy(stage descriptors)");
   if (!st) st = upload_chunk_descs(p);
   return st;
 }
@@ -682,6 +705,7 @@ int make_stages(d2dhip_plan_s *p) {
 
 std::mutex g_tune_mu;
 std::map<std::string, std::string> g_tune;
+std::string g_tune_loaded_path;  // cache file g_tune holds; read again when the path changes
 bool g_tune_loaded = false;
 
 std::string tune_path() {
@@ -691,9 +715,12 @@ std::string tune_path() {
 }
 
 void tune_load() {
-  if (g_tune_loaded) return;
+  const std::string path = tune_path();
+  if (g_tune_loaded && path == g_tune_loaded_path) return;
   g_tune_loaded = true;
-  FILE *f = std::fopen(tune_path().c_str(), "r");
+  g_tune_loaded_path = path;
+  g_tune.clear();
+  FILE *f = std::fopen(path.c_str(), "r");
   if (!f) return;
   char line[512];
   while (std::fgets(line, sizeof(line), f)) {
@@ -707,7 +734,9 @@ void tune_load() {
   std::fclose(f);
 }
 
-void tune_store(const std::string &key, const std::string &val) {
+// Records a choice in memory and appends it to the cache file; a failed write
+// marks the plan, whose set-up call then returns D2DHIP_WARN_TUNE_CACHE.
+void tune_store(d2dhip_plan_s *p, const std::string &key, const std::string &val) {
   g_tune[key] = val;
   const std::string path = tune_path();
   const size_t slash = path.rfind('/');
@@ -716,10 +745,13 @@ void tune_store(const std::string &key, const std::string &val) {
     for (size_t i = 1; i <= dir.size(); ++i)
       if (i == dir.size() || dir[i] == '/') (void)mkdir_one(dir.substr(0, i));
   }
-  if (FILE *f = std::fopen(path.c_str(), "a")) {
-    std::fprintf(f, "%s\t%s\n", key.c_str(), val.c_str());
-    std::fclose(f);
+  FILE *f = std::fopen(path.c_str(), "a");
+  bool ok = f != nullptr;
+  if (f) {
+    ok = std::fprintf(f, "%s\t%s\n", key.c_str(), val.c_str()) > 0;
+    ok = (std::fclose(f) == 0) && ok;
   }
+  if (!ok) p->tune_write_failed = true;
 }
 
 // The key also carries a hash of the candidate kernels' names, so that a
@@ -752,9 +784,7 @@ std::string choice_name(const d2dhip_plan_s *p, int idx) {
 
 int time_stage(d2dhip_plan_s *p, int idx, const void *in, void *out, hipEvent_t e0, hipEvent_t e1,
                float *ms) {
-  int st = hip_check(hipMemcpy(p->d_desc + idx, &p->h_desc[idx], sizeof(AxisDesc),
-                               hipMemcpyHostToDevice),
-                     "hipMemcpy(descriptor)");
+  int st = upload_sync(p, p->d_desc + idx, &p->h_desc[idx], sizeof(AxisDesc), "hipMemcpy(descriptor)");
   if (st) return st;
   const AxisIo io = {in, out, 0};
   auto launch = [&]() {
@@ -829,7 +859,7 @@ int tune_stages(d2dhip_plan_s *p) {
       }
     }
     if (!st) st = configure(p, i, best_c);
-    if (!st) tune_store(key, choice_name(p, i));
+    if (!st) tune_store(p, key, choice_name(p, i));
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -916,8 +946,8 @@ int upload_chunk_descs(d2dhip_plan_s *p) {
     p->d_cdesc = nullptr;
     return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(chunk descriptors) failed");
   }
-  return hip_check(hipMemcpy(p->d_cdesc, p->h_cdesc, sizeof(p->h_cdesc), hipMemcpyHostToDevice),
-                   "hipMemcpy(chunk descriptors)");
+  drop_graph(p);
+  return upload_sync(p, p->d_cdesc, p->h_cdesc, sizeof(p->h_cdesc), "hipMemcpy(chunk descriptors)");
 }
 
 // One stage launch over z-planes [k0, k0 + planes) through chunk slot q.
@@ -1032,7 +1062,7 @@ int tune_chunks(d2dhip_plan_s *p) {
   if (!st) {
     char v[64];
     std::snprintf(v, sizeof(v), "%d,%d", best[0], best[1]);
-    tune_store(key, v);
+    tune_store(p, key, v);
   }
   int st2 = upload_chunk_descs(p);
   return st ? st : st2;
@@ -1094,6 +1124,8 @@ int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, in
     return st;
   }
   *plan = p;
+  if (p->tune_write_failed)
+    return set_error(D2DHIP_WARN_TUNE_CACHE, "tuning choices could not be written to " + tune_path());
   return D2DHIP_SUCCESS;
 }
 
@@ -1217,6 +1249,8 @@ int d2dhip_plan_stage_candidate(d2dhip_plan p, int stage, int idx, char *name, i
 int d2dhip_plan_set_stage_kernel(d2dhip_plan p, int stage, const char *name) {
   if (!p || !name) return set_error(D2DHIP_ERR_NULL, "NULL argument");
   if (int st = check_stage(p, stage)) return st;
+  if (p->capturing)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "plan is capturing: its kernels are fixed");
   const StageGeom &g = p->geom[stage];
   int c = -2;
   if (std::strcmp(name, "general") == 0) c = -1;
@@ -1234,6 +1268,8 @@ int d2dhip_plan_set_chunking(d2dhip_plan p, int fwd_planes, int bwd_planes) {
     return set_error(D2DHIP_ERR_INVALID_VALUE, "chunk size must be >= 0");
   if (p->format != D2DHIP_PHYSICAL_IN_X && (fwd_planes || bwd_planes))
     return set_error(D2DHIP_ERR_UNSUPPORTED, "plane chunking is for PHYSICAL_IN_X plans");
+  if (p->capturing)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "plan is capturing: its chunking is fixed");
   p->chunk_fwd = fwd_planes >= p->nz ? 0 : fwd_planes;
   p->chunk_bwd = bwd_planes >= p->nz ? 0 : bwd_planes;
   return upload_chunk_descs(p);
@@ -1310,13 +1346,14 @@ int d2dhip_exec_r2c(d2dhip_plan p, const void *in_r, void *out_c) {
 // fft_3d_c2r: src/fft_common_3d.f90:194-296.  The c2c stages write the
 // plan's workspace (or in_c itself with D2DHIP_FLAG_INPLACE), so that in_c
 // is preserved as the reference's wk1 copy preserves it, without the copy.
-int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
-  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
-  if (!in_c || !out_r) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
+namespace {
+
+// The c2r stages with the spectral workspace wk (the plan's own, in_c itself,
+// or the library's static scratch array).
+int c2r_stages(d2dhip_plan_s *p, void *in_c, void *out_r, void *wk) {
   const bool fx = p->format == D2DHIP_PHYSICAL_IN_X;
   const int warn = (p->flags & (fx ? D2DHIP_FLAG_SKIP_X : D2DHIP_FLAG_SKIP_Z)) ? 1 : 0;
   const unsigned skip_first = fx ? D2DHIP_FLAG_SKIP_Z : D2DHIP_FLAG_SKIP_X;
-  void *wk = (p->flags & D2DHIP_FLAG_INPLACE) ? in_c : p->wk;
   const void *src = in_c;
   int st;
   if (!(p->flags & skip_first)) {  // z (X format) or x (Z format), inverse
@@ -1336,6 +1373,15 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
     return set_error(D2DHIP_WARN_SKIP_IGNORED, "r2c / c2r transform can not be skipped");
   return D2DHIP_SUCCESS;
 }
+
+}  // namespace
+
+int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (!in_c || !out_r) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
+  return c2r_stages(p, in_c, out_r, (p->flags & D2DHIP_FLAG_INPLACE) ? in_c : p->wk);
+}
+
 
 // decomp_2d_fft_engine_dtt_init (src/fft_fftw3_f03.f90:341-390) with
 // dtt_assign_default (:394-420) and dtt_invert (:424-444): 3 types, or 12
@@ -1385,6 +1431,8 @@ int d2dhip_plan_set_dtt(d2dhip_plan p, int ndtt, const int *in_dtt) {
   if (!st && ((p->flags & D2DHIP_FLAG_TUNE) || std::getenv("D2DHIP_TUNE"))) st = tune_stages(p);
   p->has_dtt = !st;
   if (!st) std::memcpy(p->dtt, v, sizeof(v));
+  if (!st && p->tune_write_failed)
+    return set_error(D2DHIP_WARN_TUNE_CACHE, "tuning choices could not be written to " + tune_path());
   return st;
 }
 
@@ -1788,6 +1836,12 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
   if (!st && (hipMalloc(&b->d_head, sizeof(unsigned long long) * kBatchQueues * kHeadStride) != hipSuccess ||
               hipMalloc(&b->d_err, sizeof(unsigned int)) != hipSuccess))
     st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(batch counters)");
+  // the error flag starts clear: only the persistent launches reset and set
+  // it, and check reads it whatever the executor
+  if (!st) {
+    const unsigned int zero = 0;
+    st = hip_check(hipMemcpy(b->d_err, &zero, sizeof(zero), hipMemcpyHostToDevice), "hipMemcpy(batch flag)");
+  }
   if (st) {
     d2dhip_batch_destroy(b);
     return st;

@@ -662,15 +662,34 @@ contains
       call update_halo_complex(in, out, level, decomp_main, opt_global, opt_pencil)
    end subroutine update_halo_complex_short
 
+   ! The kernel indexes the halo array as sizes plus 2*level cells on the two
+   ! axes other than ipencil: the extents and the array must be exactly that.
+   subroutine check_halo_shape(nelem, ipencil, h, level, sizes)
+      integer(c_size_t), intent(in) :: nelem
+      integer, intent(in) :: ipencil, level
+      type(halo_extents_t), intent(in) :: h
+      integer, dimension(3), intent(in) :: sizes
+      integer :: want(3), ext(3), a
+      if (ipencil < 1 .or. ipencil > 3 .or. level < 0) &
+         call decomp_2d_abort(__FILE__, __LINE__, 1, "halo_exchange: bad ipencil or level")
+      ext = [h%xe - h%xs + 1, h%ye - h%ys + 1, h%ze - h%zs + 1]
+      do a = 1, 3
+         want(a) = sizes(a) + merge(0, 2 * level, a == ipencil)
+      end do
+      if (any(ext /= want)) &
+         call decomp_2d_abort(__FILE__, __LINE__, 1, &
+                              "halo extents do not equal sizes + 2*level on the halo axes")
+      if (nelem /= int(ext(1), c_size_t) * ext(2) * ext(3)) &
+         call decomp_2d_abort(__FILE__, __LINE__, 1, "halo array does not match its extents")
+   end subroutine check_halo_shape
+
    ! halo_exchange (src/halo.f90:294-399 with the body files)
    subroutine halo_exchange_real(arr, ipencil, halo_extents, level, sizes)
       real(mytype), dimension(:, :, :), contiguous, target, intent(inout) :: arr
       integer, intent(in) :: ipencil, level
       type(halo_extents_t), intent(in) :: halo_extents
       integer, dimension(3), intent(in) :: sizes
-      if (size(arr, kind=c_size_t) /= int(halo_extents%xe - halo_extents%xs + 1, c_size_t) * &
-          (halo_extents%ye - halo_extents%ys + 1) * (halo_extents%ze - halo_extents%zs + 1)) &
-         call decomp_2d_abort(__FILE__, __LINE__, 1, "halo array does not match its extents")
+      call check_halo_shape(size(arr, kind=c_size_t), ipencil, halo_extents, level, sizes)
       call exchange_dev(c_loc(arr), 8_c_size_t * size(arr, kind=c_size_t), 0_c_int, sizes, &
                         level, ipencil)
    end subroutine halo_exchange_real
@@ -680,9 +699,7 @@ contains
       integer, intent(in) :: ipencil, level
       type(halo_extents_t), intent(in) :: halo_extents
       integer, dimension(3), intent(in) :: sizes
-      if (size(arr, kind=c_size_t) /= int(halo_extents%xe - halo_extents%xs + 1, c_size_t) * &
-          (halo_extents%ye - halo_extents%ys + 1) * (halo_extents%ze - halo_extents%zs + 1)) &
-         call decomp_2d_abort(__FILE__, __LINE__, 1, "halo array does not match its extents")
+      call check_halo_shape(size(arr, kind=c_size_t), ipencil, halo_extents, level, sizes)
       call exchange_dev(c_loc(arr), 16_c_size_t * size(arr, kind=c_size_t), 1_c_int, sizes, &
                         level, ipencil)
    end subroutine halo_exchange_complex

@@ -43,7 +43,8 @@ extern "C" {
 #define D2DHIP_FLAG_TUNE 0x10u   /* time the candidate kernels of each stage at plan
                                     creation (also with env D2DHIP_TUNE=1); choices
                                     are cached in $D2DHIP_TUNE_CACHE or
-                                    ~/.cache/d2dhip/tuning.tsv */
+                                    ~/.cache/d2dhip/tuning.tsv, read again
+                                    whenever that path changes */
 
 /* element kinds for transposes and packs */
 #define D2DHIP_R8 0  /* real(real64)    */
@@ -69,6 +70,8 @@ extern "C" {
 #define D2DHIP_ERR_PLAN_MISMATCH 7 /* exec kind does not match the plan */
 #define D2DHIP_WARN_SKIP_IGNORED 100 /* "r2c / c2r transform can not be skipped"
                                         (src/fft_generic.f90:222-223) */
+#define D2DHIP_WARN_TUNE_CACHE 101 /* plan created (or DTT set up) and tuned, but the
+                                      choices could not be written to the tuning cache */
 
 #define D2DHIP_MAX_PROCS 64
 
@@ -315,7 +318,10 @@ long long d2dhip_pack_buffer_elems(int dir, const d2dhip_decomp *d, int even);
 int d2dhip_update_halo(int elem, const void *in, void *out, int s1, int s2, int s3,
                        int level, int ipencil, const int periodic[3], void *stream);
 /* halo_exchange(arr, ipencil, halo_extents, level, [s1, s2, s3]) on a halo
- * array already holding its interior. */
+ * array already holding its interior.  arr must hold exactly the halo
+ * array's elements: (s1, s2, s3) plus 2*level on each axis other than
+ * ipencil (there is no size argument to check it against).  Likewise `out`
+ * of d2dhip_update_halo. */
 int d2dhip_halo_exchange(int elem, void *arr, int s1, int s2, int s3, int level,
                          int ipencil, const int periodic[3], void *stream);
 

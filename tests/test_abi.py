@@ -23,6 +23,7 @@ def test_library_exports_every_header_function(d2d):
 def test_status_strings(d2d):
     assert d2d.status_string(0) == "success"
     assert "skipped" in d2d.status_string(d2d.WARN_SKIP_IGNORED)
+    assert "cache" in d2d.status_string(d2d.WARN_TUNE_CACHE)
     assert d2d.status_string(12345) == "unknown status"
     assert d2d.lib().d2dhip_version() >= 100
 

@@ -74,3 +74,55 @@ def test_c2c_graph_and_refusals(d2d, torch_gpu):
     e.fft_c2c(b, b, 1)
     torch.cuda.synchronize()
     assert torch.equal(b, eager)
+
+
+def test_reconfiguring_drops_the_graph(d2d, torch_gpu):
+    """A captured graph holds the kernels and chunk offsets it was recorded
+    with: changing a stage kernel or the chunking is refused while capturing
+    and drops the graph afterwards, so a replay cannot run a stale
+    configuration; after a new capture the replay equals the eager calls."""
+    torch = torch_gpu
+    nx, ny, nz = 64, 48, 40
+    e = d2d.FFTEngine(1, nx, ny, nz)
+    x = torch.rand(nx * ny * nz, dtype=torch.float64, device="cuda")
+    sp = torch.empty((nx // 2 + 1) * ny * nz, dtype=torch.complex128, device="cuda")
+    y = torch.empty_like(x)
+
+    def capture():
+        e.capture_begin()
+        e.fft_r2c(x, sp)
+        e.fft_c2r(sp, y)
+        e.capture_end()
+
+    capture()
+    e.graph_launch()
+    torch.cuda.synchronize()
+    cands = e.stage_candidates(4)
+    assert len(cands) >= 2
+    e.capture_begin()
+    with pytest.raises(d2d.D2dError):
+        e.set_stage_kernel(4, cands[-1])
+    with pytest.raises(d2d.D2dError):
+        e.set_chunking(8, 8)
+    e.fft_r2c(x, sp)
+    e.capture_end()
+    e.set_stage_kernel(4, "general")
+    with pytest.raises(d2d.D2dError):
+        e.graph_launch()  # dropped by the kernel change
+    capture()
+    e.set_chunking(8, 8)
+    with pytest.raises(d2d.D2dError):
+        e.graph_launch()  # dropped by the chunking change
+    e.fft_r2c(x, sp)
+    e.fft_c2r(sp, y)
+    torch.cuda.synchronize()
+    sp_eager, y_eager = sp.clone(), y.clone()
+    capture()
+    for rep in range(3):
+        sp.zero_()
+        y.zero_()
+        torch.cuda.synchronize()
+        e.graph_launch()
+        torch.cuda.synchronize()
+        assert torch.equal(sp, sp_eager), rep
+        assert torch.equal(y, y_eager), rep
