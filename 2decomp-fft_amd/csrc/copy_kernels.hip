@@ -15,6 +15,8 @@
 // read and written in consecutive addresses.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 #include "codelets.h"
 #include "gen.h"
 #include "internal.h"
@@ -86,6 +88,79 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a, long long units, 
           dst[pos[e]] = v[e];
         else
           dst[lin0 + i] = v[e];
+      }
+    }
+  }
+}
+
+// Block of a position along the block axis: the m with b1[m] <= key <
+// b1[m+1], by a binary search of fixed depth.
+__device__ __forceinline__ int pack_block(const PackArgs &a, int key) {
+  int lo = 0, hi = a.np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (key >= a.b1[mid]) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Vector form for maps whose runs all start on 16-byte boundaries (checked
+// on the host: n1, every block start / width on the i axis and every
+// displacement multiples of the vector): 16 bytes per lane.  A unit is rpu
+// whole pencil rows (rows of fewer than 1024 vectors) or one chunk of 1024
+// vectors of a row; each thread moves 4 vectors, loads first.  The block
+// of a row (blocks on j or k) or of a vector (blocks on i) is looked up
+// once, not per element.
+template <int AX>
+__global__ void __launch_bounds__(256) pack_vec_kernel(PackArgs a, int vpe, long long units,
+                                                       int cpr, int rpu, long long rows) {
+  constexpr int NV = 4, CH = 256 * NV;
+  const E16 *src = reinterpret_cast<const E16 *>(a.src);
+  E16 *dst = reinterpret_cast<E16 *>(a.dst);
+  const int nvr = a.n1 / vpe;  // vectors per pencil row
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    long long pv[NV], bv[NV];  // pencil and buffer vector index
+    E16 v[NV];
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+      const int t = e * 256 + (int)threadIdx.x;
+      long long row;
+      int iv;
+      if (rpu > 1) {
+        row = u * rpu + t / nvr;
+        iv = t % nvr;
+        if (t >= rpu * nvr) row = rows;  // past the unit
+      } else {
+        row = u / cpr;
+        iv = (int)(u - row * cpr) * CH + t;
+        if (iv >= nvr) row = rows;
+      }
+      pv[e] = -1;
+      if (row < rows) {
+        const int j = (int)(row % a.n2), k = (int)(row / a.n2);
+        const int i = iv * vpe;
+        long long pos;
+        if constexpr (AX == 0) {
+          const int m = pack_block(a, i);
+          pos = a.init[m] + (long long)(i - a.b1[m]) + (long long)j * a.w[m] +
+                (long long)k * a.n2 * a.w[m];
+        } else {
+          const int m = pack_block(a, AX == 1 ? j : k);
+          pos = a.init[m] + i + (AX == 1 ? (long long)(j - a.b1[m]) * a.n1 + (long long)k * a.w[m] * a.n1
+                                         : (long long)j * a.n1 + (long long)(k - a.b1[m]) * a.n1 * a.n2);
+        }
+        pv[e] = row * nvr + iv;
+        bv[e] = pos / vpe;
+        v[e] = a.split ? src[pv[e]] : src[bv[e]];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+      if (pv[e] >= 0) {
+        if (a.split)
+          dst[bv[e]] = v[e];
+        else
+          dst[pv[e]] = v[e];
       }
     }
   }
@@ -168,6 +243,31 @@ int launch_pack(int dir, int split, int esize, const void *src, void *dst, const
   if (wsum != axlen)
     return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition does not match the pencil");
   if ((long long)a.n1 * a.n2 * a.n3 == 0) return D2DHIP_SUCCESS;
+  // 16-byte vectors when every run starts on a vector boundary
+  const int vpe = 16 / esize;
+  bool vec = (a.n1 % vpe == 0) && ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
+  for (int m = 0; m < np && vec; ++m) {
+    vec = a.init[m] % vpe == 0;
+    if (a.ax == 0) vec = vec && a.b1[m] % vpe == 0 && a.w[m] % vpe == 0;
+  }
+  if (vec) {
+    const long long rows = (long long)a.n2 * a.n3;
+    const int nvr = a.n1 / vpe;
+    const int rpu = nvr >= 1024 ? 1 : 1024 / nvr;
+    const int vcpr = (nvr + 1023) / 1024;
+    const long long vunits = rpu > 1 ? (rows + rpu - 1) / rpu : rows * vcpr;
+    const long long vgrid = vunits < (1LL << 20) ? vunits : (1LL << 20);
+    if (a.ax == 0)
+      hipLaunchKernelGGL((pack_vec_kernel<0>), dim3((unsigned)vgrid), dim3(256), 0, st, a, vpe, vunits,
+                         vcpr, rpu, rows);
+    else if (a.ax == 1)
+      hipLaunchKernelGGL((pack_vec_kernel<1>), dim3((unsigned)vgrid), dim3(256), 0, st, a, vpe, vunits,
+                         vcpr, rpu, rows);
+    else
+      hipLaunchKernelGGL((pack_vec_kernel<2>), dim3((unsigned)vgrid), dim3(256), 0, st, a, vpe, vunits,
+                         vcpr, rpu, rows);
+    return hip_check(hipGetLastError(), "pack_vec_kernel launch");
+  }
   const int cpr = (a.n1 + 1023) / 1024;
   const long long units = (long long)a.n2 * a.n3 * cpr;
   const long long grid = units < (1LL << 20) ? units : (1LL << 20);

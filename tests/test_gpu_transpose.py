@@ -9,7 +9,9 @@ pytestmark = pytest.mark.gpu
 
 DT = {0: np.float64, 1: np.complex128, 2: np.float32, 3: np.complex64}
 CASES = [(17, 13, 11, 2, 2), (17, 13, 11, 3, 2), (33, 15, 11, 4, 3), (16, 12, 10, 2, 2),
-         (9, 8, 7, 1, 3), (9, 8, 7, 3, 1), (64, 32, 16, 4, 4)]
+         (9, 8, 7, 1, 3), (9, 8, 7, 3, 1), (64, 32, 16, 4, 4),
+         # rows of more than 1024 16-byte vectors (chunked rows on the vector path)
+         (4096, 6, 4, 2, 2), (2050, 6, 4, 2, 2)]
 
 
 def to_dev(torch, a):
