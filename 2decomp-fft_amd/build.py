@@ -2,6 +2,7 @@
 
     python 2decomp-fft_amd/build.py            # compile + link
     python 2decomp-fft_amd/build.py --jobs 4
+    python 2decomp-fft_amd/build.py --incremental   # only stale objects
 
 Every HIP source is compiled for gfx950 with packed FP32 instructions
 disabled, then linked into 2decomp-fft_amd/libd2dhip.so next to this file.
@@ -43,10 +44,27 @@ def _compile(src):
     return src, r.returncode, err, out
 
 
-def build(jobs=4, verbose=False):
+def _stale(src):
+    """True when src's object is missing or older than src or any header."""
+    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    if not os.path.exists(out):
+        return True
+    deps = [os.path.join(CSRC, src)]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    deps += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    t = os.path.getmtime(out)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(jobs=4, verbose=False, incremental=False):
+    """Compiles every source (only the stale ones with incremental=True) and
+    links the library."""
     os.makedirs(OBJ, exist_ok=True)
+    todo = [s for s in SOURCES if not incremental or _stale(s)]
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(_compile, todo))
+    results += [(s, 0, "", os.path.join(OBJ, os.path.splitext(s)[0] + ".o"))
+                for s in SOURCES if s not in todo]
     objs = []
     for src, rc, err, out in results:
         if rc != 0:
@@ -92,5 +110,7 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources whose object is older than them or a header")
     args = ap.parse_args()
-    print(build(args.jobs, args.verbose))
+    print(build(args.jobs, args.verbose, args.incremental))

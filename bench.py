@@ -8,11 +8,16 @@ field of 32 Fourier modes with wavenumbers in [-128,128]^3, amplitudes
 10^U(-6,2) with random signs, plus N(0, 1e-6) noise, seed 3 (synthetic).
 
 Printed as one JSON line: ms per fwd+bwd (`value`), the 3-pass HBM model's
-effective bandwidth and roofline fraction, per-stage times from hipEvents
-recorded around every kernel launch of the timed region, the FP32 variant,
-and a CPU baseline: the Glassman restatement of the reference's generic
-backend (oracle/) timed on a bounded sample of the same workload's lines on
-the host cores and scaled to the full round trip.
+effective bandwidth and roofline fraction, the stricter compulsory-traffic
+fraction (one read of the input and one write of the output per direction),
+per-stage times from hipEvents recorded around every kernel launch of a
+second set of round trips, the FP32 and split-BF16 variants, `configs` (the
+other configurations, each timed as its own --workload line: c2c 128^3, r2c
+512^3, r2c 720x540x432 PHYSICAL_IN_Z, the 2048^3 4x4 rank pack, 64 uneven
+c2c grids in FP64 and FP32), and a CPU baseline: the Glassman restatement of
+the reference's generic backend (oracle/) timed on a bounded sample of the
+same workload's lines on all host cores and on one, scaled to the full round
+trip.
 """
 import argparse
 import json
@@ -59,6 +64,12 @@ def round_trip_bytes(n, rb):
     return 2 * (b["r2c"] + b["spec_y"] + b["spec_last"])
 
 
+def compulsory_bytes(n, rb):
+    """One read of the input and one write of the output per direction."""
+    b = stage_bytes(n, rb)
+    return 2 * b["r2c"]
+
+
 def time_round_trips(torch, eng, x, spec, y, steps, warmup, profile):
     """ms per r2c+c2r over `steps` timed steps (no per-launch events inside
     the timed region); with `profile`, the same number of further steps run
@@ -95,27 +106,50 @@ def stage_ms(rep):
 
 
 def cpu_baseline(n, budget_lines):
+    """The Glassman restatement timed on a bounded sample of the round trip's
+    lines, once on every host core this process may run on (P) and once on
+    one core, each scaled to the full round trip by lines per stage."""
     import oracle as O
-    ncores = min(os.cpu_count() or 1, 16)
-    t0 = time.time()
-    t, used = O.sample_stage_times(n, n, n, budget_lines, ncores)
-    wall = time.time() - t0
+    ncores = len(os.sched_getaffinity(0))
     nxc = n // 2 + 1
     lines = [n * n, nxc * n, nxc * n, nxc * n, nxc * n, n * n]  # x, y, z, z, y, x
-    total_s = sum(tl * nl for tl, nl in zip(t, lines))
+
+    def run(nthreads, nlines):
+        t0 = time.time()
+        t, used = O.sample_stage_times(n, n, n, nlines, nthreads)
+        return sum(tl * nl for tl, nl in zip(t, lines)) * 1e3, used, time.time() - t0
+
+    ms_p, used, wall_p = run(ncores, budget_lines)
+    lines1 = max(4096, budget_lines // max(1, used) // 2)
+    ms_1, _, wall_1 = run(1, lines1)
     return {
-        "value": total_s * 1e3,
+        "value": ms_p,
         "unit": "ms per fwd+bwd (extrapolated)",
         "cores": used,
         "kind": "port",
+        "P1_value": ms_1,
+        "P1_over_P_speedup": round(ms_1 / ms_p, 2),
         "sample": ("Glassman restatement of the generic backend (oracle/glassman.c), "
                    "%d lines of each of the 6 stages of the %d^3 r2c+c2r round trip at their "
-                   "real strides, %d OpenMP threads, %.1f s wall, scaled by lines per stage"
-                   % (budget_lines, n, used, wall)),
+                   "real strides on P = %d OpenMP threads (all cores of this process's "
+                   "affinity mask, %.1f s wall), and %d lines on P = 1 (%.1f s wall), each "
+                   "scaled by lines per stage" % (budget_lines, n, used, wall_p, lines1, wall_1)),
     }
 
 
-def bench_batch(args, torch, d2d):
+def config_summary(line):
+    """The fields of a configuration's own bench line kept in the headline's
+    `configs` entry."""
+    keep = ("ms_per_step", "GBps_model", "hbm_frac_model", "GBps_3pass", "hbm_frac_3pass",
+            "round_trip_rel_l2", "round_trip_rel_l2_max", "graph_ms", "sequential_ms",
+            "persistent_ms", "persistent_graph_ms", "speedup_vs_sequential", "grouped_graph_ms",
+            "executor", "value", "unit")
+    out = {"workload": line["config"]["workload"]}
+    out.update({k: line[k] for k in keep if k in line})
+    return out
+
+
+def bench_batch(args, torch, d2d, batch_prec=None):
     """64 independent c2c grids, each dimension from {32,...,256} (seed 4),
     inputs uniform[-1,1): forward+backward of every grid as 64 sequential
     decomp_2d_fft_3d pairs, and as one batch with either executor (one launch
@@ -124,7 +158,7 @@ def bench_batch(args, torch, d2d):
     import numpy as np
     rng = np.random.default_rng(4)
     sizes = [32, 48, 64, 80, 96, 128, 160, 192, 256]
-    prec = d2d.PREC_F64 if args.batch_prec == 64 else d2d.PREC_F32
+    prec = d2d.PREC_F64 if (batch_prec or args.batch_prec) == 64 else d2d.PREC_F32
     cdt = torch.complex128 if prec == d2d.PREC_F64 else torch.complex64
     elem = d2d.C16 if prec == d2d.PREC_F64 else d2d.C8
     shapes = [tuple(int(v) for v in rng.choice(sizes, 3)) for _ in range(64)]
@@ -189,18 +223,18 @@ def bench_batch(args, torch, d2d):
         "grouped_ms": round(ms_grouped, 4), "grouped_graph_ms": round(ms_grouped_graph, 4),
         "grouped_speedup_vs_sequential": round(ms_seq / ms_grouped_graph, 3),
         "GBps_3pass": round(nbytes / (ms_graph * 1e-3) / 1e9, 1),
+        "hbm_frac_3pass": round(nbytes / (ms_graph * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
         "round_trip_rel_l2_max": rt,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
-def bench_config(args, torch, d2d):
+def bench_config(args, torch, d2d, w):
     """One of the other configurations, timed like the headline (FP64, tuned
     plans, inputs in HBM): c2c 128^3 forward+backward (PHYSICAL_IN_X, uniform
     inputs seed 0); r2c+c2r 512^3 PHYSICAL_IN_X (modal field seed 1); r2c+c2r
     720x540x432 PHYSICAL_IN_Z (uniform seed 2); the mem_split_xy pack of one
     rank of 2048^3 on a 4x4 grid (x-pencil 2048x512x512 real8, seed 2)."""
-    w = args.workload
 
     def timed(fn):
         for _ in range(args.warmup):
@@ -377,7 +411,7 @@ def bench_config(args, torch, d2d):
             "hbm_frac_model": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
             "round_trip_rel_l2": err}
     line.update(extra)
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def main():
@@ -387,6 +421,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="headline only: skip the other configurations' timings")
     ap.add_argument("--cpu-lines", type=int, default=524288,
                     help="lines of each stage in the CPU baseline sample (about 13 s on 16 cores)")
     ap.add_argument("--no-tune", action="store_true",
@@ -410,11 +446,11 @@ def main():
     import torch
     d2d = importlib.import_module("2decomp-fft_amd")
     if args.workload == "batch64":
-        bench_batch(args, torch, d2d)
+        print(json.dumps(bench_batch(args, torch, d2d)), flush=True)
         return
     if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048", "dtt513",
                          "halo512"):
-        bench_config(args, torch, d2d)
+        print(json.dumps(bench_config(args, torch, d2d, args.workload)), flush=True)
         return
     n = args.n
     nxc = n // 2 + 1
@@ -540,6 +576,8 @@ def main():
                    "grid": [n, n, n], "format": "PHYSICAL_IN_X", "precision": "f64"},
         "GBps_3pass": round(gbs, 1),
         "hbm_frac_3pass": round(gbs / PEAK_HBM_GBS, 4),
+        "compulsory_GBps": round(compulsory_bytes(n, 8) / (ms * 1e-3) / 1e9, 1),
+        "compulsory_frac": round(compulsory_bytes(n, 8) / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
         "copy_GBps": round(copy_gbs, 1),
         "pattern_copy_GBps": pattern_gbs,
         "round_trip_rel_l2": rt_err,
@@ -550,11 +588,29 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "%s (%s stage)" % (kernels64.get(dom), dom),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("rocprofv3 FETCH_SIZE/WRITE_SIZE of this kernel, "
+                                        "profiles/hbm_traffic_r2c_c2r_1024cube.md"
+                                        if traffic else "kernel not in the profiled table"),
                      "frac_of_pattern_copy": (round(achieved / pattern_gbs[dom], 4)
                                               if dom in pattern_gbs else None),
                      "algorithmic_bytes": sb[dom]},
         "variants": variants,
     }
+    # the other configurations, each timed as its own --workload line does
+    del x, spec, spec64, eng
+    torch.cuda.empty_cache()
+    if not args.no_configs:
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = min(args.steps, 10), min(args.warmup, 3)
+        cfgs = {}
+        for key, w in (("C1_c2c128", "c2c128"), ("C2_r2c512", "r2c512"),
+                       ("C3a_r2cz720", "r2cz720"), ("C3b_pack2048", "pack2048")):
+            cfgs[key] = config_summary(bench_config(sub, torch, d2d, w))
+            torch.cuda.empty_cache()
+        for prec in (64, 32):
+            cfgs["C5_batch64_f%d" % prec] = config_summary(bench_batch(sub, torch, d2d, prec))
+            torch.cuda.empty_cache()
+        line["configs"] = cfgs
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(n, args.cpu_lines)
