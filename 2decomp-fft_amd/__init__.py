@@ -45,6 +45,7 @@ ERR_NULL = 5
 ERR_GRID = 6
 WARN_SKIP_IGNORED = 100
 WARN_TUNE_CACHE = 101
+SCRATCH_BYTES = 64 << 20  # D2DHIP_SCRATCH_BYTES
 MAX_PROCS = 64
 
 _lib = None
@@ -123,6 +124,7 @@ def lib():
             "d2dhip_exec_c2c": ([pl, vp, vp, ci], ci),
             "d2dhip_exec_r2c": ([pl, vp, vp], ci),
             "d2dhip_exec_c2r": ([pl, vp, vp], ci),
+            "d2dhip_exec_c2r_noalloc": ([pl, vp, vp], ci),
             "d2dhip_plan_set_dtt": ([pl, ci, ctypes.POINTER(ci)], ci),
             "d2dhip_plan_get_dtt": ([pl, ctypes.POINTER(ci)], ci),
             "d2dhip_exec_dtt": ([pl, vp, vp, ci], ci),
@@ -317,6 +319,12 @@ class FFTEngine:
         self._sync_stream()
         return check(lib().d2dhip_exec_c2r(self._plan, _ptr(inp), _ptr(out)))
 
+    def fft_c2r_noalloc(self, inp, out):
+        """fft_c2r with the workspace in the library's static scratch array
+        (no allocation; spectra up to SCRATCH_BYTES; concurrent calls on one
+        GPU race)."""
+        self._sync_stream()
+        return check(lib().d2dhip_exec_c2r_noalloc(self._plan, _ptr(inp), _ptr(out)))
 
     def set_dtt(self, dtt):
         """DCT / DST types of the x, y, z axes (3 values, 1-8), or 12 values

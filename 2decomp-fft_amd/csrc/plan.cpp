@@ -1374,6 +1374,36 @@ int c2r_stages(d2dhip_plan_s *p, void *in_c, void *out_r, void *wk) {
   return D2DHIP_SUCCESS;
 }
 
+// Workspace of d2dhip_exec_c2r_noalloc: one array per GPU, living as long as
+// the process.  Host code never passes `scratch` itself to a kernel (on the
+// host it is only a stand-in for the symbol); it uses the device address
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[D2DHIP_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device.
+int scratch_address(void **ptr) {
+  int dev = 0;
+  int st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (st) return st;
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *a = nullptr;
+    st = hip_check(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (st) return st;
+    if (!a) return set_error(D2DHIP_ERR_HIP, "hipGetSymbolAddress(scratch) gave NULL");
+    g_scratch_ptr[dev] = a;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return D2DHIP_SUCCESS;
+}
+
 }  // namespace
 
 int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
@@ -1382,6 +1412,27 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
   return c2r_stages(p, in_c, out_r, (p->flags & D2DHIP_FLAG_INPLACE) ? in_c : p->wk);
 }
 
+// d2dhip_exec_c2r with the workspace in the library's static scratch array:
+// no allocation of any kind, in_c preserved whatever the plan's flags.
+int d2dhip_exec_c2r_noalloc(d2dhip_plan p, void *in_c, void *out_r) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (!in_c || !out_r) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
+  if (p->prof)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "stage profiling records events: not with noalloc");
+  const long long nsp = (p->format == D2DHIP_PHYSICAL_IN_X)
+                            ? (long long)(p->nx / 2 + 1) * p->ny * p->nz
+                            : (long long)p->nx * p->ny * (p->nz / 2 + 1);
+  const unsigned long long need = (unsigned long long)nsp * 2 * p->rb;
+  if (need > (unsigned long long)D2DHIP_SCRATCH_BYTES)
+    return set_error(D2DHIP_ERR_UNSUPPORTED,
+                     "spectral pencil of " + std::to_string(need) + " bytes exceeds the " +
+                         std::to_string((long long)D2DHIP_SCRATCH_BYTES) +
+                         "-byte scratch array of d2dhip_exec_c2r_noalloc");
+  void *wk = nullptr;
+  const int st = scratch_address(&wk);
+  if (st) return st;
+  return c2r_stages(p, in_c, out_r, wk);
+}
 
 // decomp_2d_fft_engine_dtt_init (src/fft_fftw3_f03.f90:341-390) with
 // dtt_assign_default (:394-420) and dtt_invert (:424-444): 3 types, or 12

@@ -169,6 +169,22 @@ int d2dhip_exec_r2c(d2dhip_plan plan, const void *in_r, void *out_c);
 /* decomp_2d_fft_3d(in_c, out_r): fft_3d_c2r, src/fft_common_3d.f90:194-296.
  * in_c is preserved unless the plan has D2DHIP_FLAG_INPLACE. */
 int d2dhip_exec_c2r(d2dhip_plan plan, void *in_c, void *out_r);
+/* d2dhip_exec_c2r that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory at call time (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc), and for plans created with D2DHIP_FLAG_INPLACE, which hold no
+ * workspace, when in_c must survive.  The spectral workspace of the c2r
+ * stages is a static device array of D2DHIP_SCRATCH_BYTES that the library
+ * holds on each GPU it runs on until the process ends; in_c is preserved
+ * whatever the plan's flags, and the results equal d2dhip_exec_c2r's bit
+ * for bit.  Returns D2DHIP_ERR_UNSUPPORTED when the spectral pencil
+ * ((nx/2+1)*ny*nz or nx*ny*(nz/2+1) complex values of the plan's precision)
+ * exceeds D2DHIP_SCRATCH_BYTES, and D2DHIP_ERR_INVALID_VALUE while stage
+ * profiling is on.  Every call on one GPU shares that GPU's array, which
+ * keeps the last call's contents (the stages write it before reading it):
+ * calls that overlap on one GPU, from any plan or stream, race. */
+#define D2DHIP_SCRATCH_BYTES (64ll << 20)
+int d2dhip_exec_c2r_noalloc(d2dhip_plan plan, void *in_c, void *out_r);
 /* Real-to-real transforms (DCT / DST types 1-8 = DCT-I..IV, DST-I..IV, FFTW's
  * REDFT00..11, RODFT00..11, unnormalised).  d2dhip_plan_set_dtt replaces
  * decomp_2d_fft_engine_dtt_init (src/fft_fftw3_f03.f90:341-390, the opt_DTT
