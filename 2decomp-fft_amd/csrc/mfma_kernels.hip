@@ -29,6 +29,14 @@
 // Passes with another radix (the 4 of 1024 = 16*16*4, the 2 of
 // 512 = 16*16*2) run on the VALU with the FP32 codelets, one butterfly per
 // thread, as in fast_kernels.hip.
+//
+// What the instruction count comes to: the lane maps below make every LDS
+// and HBM address a lane base plus a compile-time offset (a first version
+// that derived them per value issued 2560 VALU instructions per lane and
+// tile against 806 for the FP32 kernel; now about 860), the split takes 7
+// VALU instructions per complex value (v_cvt_pk_bf16_f32 and exact
+// v_dot2c_f32_bf16 residuals), and the HBM loads of a tile are all issued
+// before the first butterfly.
 #include <hip/hip_runtime.h>
 
 #include "codelets.h"
@@ -115,33 +123,54 @@ __device__ __forceinline__ f32x4 mfma6(const Split3 &a, const Split3 &b) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.h, acc, 0, 0, 0);
 }
 
-// The bf16 pair of one 32-bit word back to FP32 (element 0 in the low half).
-__device__ __forceinline__ float bf_lo(unsigned w) { return __uint_as_float(w << 16); }
-__device__ __forceinline__ float bf_hi(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
 __device__ __forceinline__ unsigned bf_pack(float a, float b) {
   const float __attribute__((ext_vector_type(2))) v = {a, b};
   return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
 }
 
-// (re, im) -> the three bf16 parts of both, each part one packed word (one
-// v_cvt_pk_bf16_f32 per part).
-__device__ __forceinline__ void split_pair(float re, float im, unsigned &h, unsigned &m, unsigned &l) {
+// The three bf16 parts of (re, im), each part one packed word (re in the low
+// half).  The residual of a part is taken with v_dot2c_f32_bf16 against
+// (-1, -0) or (0, -1): the products with the packed part are exact and so
+// is their sum with the value (the difference of a value and its
+// round-to-nearest bf16 fits FP32), one instruction per residual instead of
+// an unpack and a subtraction.  (-1, -0) rather than (-1, 0): the compiler
+// encodes the latter as the inline constant -1.0, whose two halves the
+// hardware reads the other way round.
+__device__ __forceinline__ float res_lo(unsigned w, float x) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w),
+                                         __builtin_bit_cast(bf16x2, 0x8000bf80u), x, false);
+}
+__device__ __forceinline__ float res_hi(unsigned w, float x) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w),
+                                         __builtin_bit_cast(bf16x2, 0xbf800000u), x, false);
+}
+__device__ __forceinline__ void split3(float re, float im, unsigned &h, unsigned &m, unsigned &l) {
   h = bf_pack(re, im);
-  const float r1 = re - bf_lo(h), i1 = im - bf_hi(h);  // exact
+  const float r1 = res_lo(h, re), i1 = res_hi(h, im);
   m = bf_pack(r1, i1);
-  l = bf_pack(r1 - bf_lo(m), i1 - bf_hi(m));  // exact differences
+  l = bf_pack(res_lo(m, r1), res_hi(m, i1));
+}
+
+// The parts of (im, -re) from those of (re, im): halves exchanged, the new
+// high half negated (bf16 rounding commutes with negation).
+__device__ __forceinline__ unsigned swap_neg(unsigned w) {
+  return __builtin_amdgcn_alignbit(w, w, 16) ^ 0x80000000u;
 }
 
 // Radix-16 DFT of 16 butterflies spread over the wave: x holds this lane's
 // inputs 4*(l/16) .. +3 (already twiddled), y receives its outputs
-// 4*(l/16) .. +3.
+// 4*(l/16) .. +3.  SW = 0: yr = Ar x, yi = Ai x.  SW = 1: Ai is not held;
+// yi = Ar x' with x'_s = (im_s, -re_s), since Ai[i][2s] = -Ar[i][2s+1] and
+// Ai[i][2s+1] = Ar[i][2s] (12 fewer registers, 12 more VALU operations per
+// group).
+template <int SW>
 __device__ __forceinline__ void mf_dft16(const cpx<float> (&x)[4], cpx<float> (&y)[4],
                                          const Split3 &ar, const Split3 &ai) {
   u32x4 h, m, l;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     unsigned a, b, c;
-    split_pair(x[e].x, x[e].y, a, b, c);
+    split3(x[e].x, x[e].y, a, b, c);
     h[e] = a;
     m[e] = b;
     l[e] = c;
@@ -151,80 +180,93 @@ __device__ __forceinline__ void mf_dft16(const cpx<float> (&x)[4], cpx<float> (&
   bs.m = __builtin_bit_cast(bf16x8, m);
   bs.l = __builtin_bit_cast(bf16x8, l);
   const f32x4 yr = mfma6(ar, bs);
-  const f32x4 yi = mfma6(ai, bs);
+  f32x4 yi;
+  if constexpr (SW) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      h[e] = swap_neg(h[e]);
+      m[e] = swap_neg(m[e]);
+      l[e] = swap_neg(l[e]);
+    }
+    bs.h = __builtin_bit_cast(bf16x8, h);
+    bs.m = __builtin_bit_cast(bf16x8, m);
+    bs.l = __builtin_bit_cast(bf16x8, l);
+    yi = mfma6(ar, bs);
+  } else {
+    yi = mfma6(ai, bs);
+  }
 #pragma unroll
   for (int e = 0; e < 4; ++e) y[e] = cpx<float>{yr[e], yi[e]};
 }
 
-// Where element i of a thread's registers sits in pass p: butterfly (t, b),
-// butterfly point s, and whether it exists.  Matrix-core passes (radix 16)
-// hold groups of 16 butterflies per wave, lane l taking points
-// 4*(l/16) .. +3 of butterfly l%16 of each group (i = 4 q + e); VALU passes
-// hold whole butterflies, one per thread and q (i = q R + s).
-template <int N, int TX, int NT, bool LINES, class RL, int p>
-struct MfPass {
-  static constexpr int R = RL::r[p], NBR = N / R, L = RL::L(p);
-  static constexpr bool MF = (R == 16);
+// Lane maps.  A stage of length N = 16 * 16 * R2 (R2 <= 8) runs two
+// radix-16 passes on the matrix cores (L = 1, 16) and one radix-R2 pass on
+// the VALU (L = 256).
+//
+// Matrix-core passes: a wave holds groups of 16 butterflies, lane l taking
+// butterfly l%16 of each group and its points s = 4*(l/16) + e, e < 4.  The
+// q-th group of a lane has butterfly b = b0 + BQ*q of tile column
+// t = t0 + TQ*q, with (b0, t0) fixed per lane and BQ, TQ compile-time:
+//   columns: t0 = (l%16) % TX, b0 = wave*16/TX + (l%16)/TX, BQ = W*16/TX, TQ = 0
+//   lines:   t0 = wave / NG,  b0 = (wave%NG)*16 + l%16,      BQ = 0, TQ = W/NG
+// (NG = N/256 groups per line).  With BQ a multiple of 16, a point index
+// j = b + s*N/16 and its LDS row j + j/16 are affine in (q, e), and b%16 is
+// the same for every group of the lane, so every LDS address is a lane base
+// plus an immediate and the second pass's twiddles are four loads per lane.
+//
+// VALU pass: one butterfly per thread and q, as in fast_kernels.hip:
+// t = tid%TX (columns) or tid/RPT (lines), b = (tid/TX or tid%RPT) + q*RPT.
+template <int N, int TX, int NT, bool LINES>
+struct MfMap {
   static constexpr int W = NT / 64;
-  static constexpr int RPT = NT / TX;
-  static constexpr int G = MF ? TX * NBR / 16 : 0;  // groups of 16 butterflies
-  static constexpr int GW = MF ? G / W : 0;
-  static constexpr int NB = MF ? 0 : (NBR + RPT - 1) / RPT;
-  static constexpr bool EXACT = MF || (NBR % RPT == 0);
-  static constexpr int COUNT = MF ? 4 * GW : NB * R;
-  static_assert(!MF || G % W == 0, "groups must divide over the waves");
-  __device__ static __forceinline__ void elem(int tid, int i, int &t, int &b, int &s, bool &valid) {
-    if constexpr (MF) {
-      const int lane = tid & 63, wave = tid >> 6, q = i >> 2;
-      const int u = (wave + W * q) * 16 + (lane & 15);
-      t = LINES ? u / NBR : u % TX;
-      b = LINES ? u % NBR : u / TX;
-      s = 4 * (lane >> 4) + (i & 3);
-      valid = true;
-    } else {
-      const int q = i / R;
-      s = i % R;
-      t = LINES ? tid / RPT : tid % TX;
-      b = (LINES ? tid % RPT : tid / TX) + q * RPT;
-      valid = EXACT || b < NBR;
-    }
-  }
-  __device__ static __forceinline__ int jin(int b, int s) { return b + s * NBR; }
-  __device__ static __forceinline__ int jout(int b, int s) {
-    const int k = b % L, g = b / L;
-    return g * L * R + k + s * L;
+  static constexpr int NG = N / 256;
+  static constexpr int E = TX * N / NT;  // values per lane
+  static constexpr int GW = E / 4;       // groups per lane in a matrix-core pass
+  static constexpr int BQ = LINES ? 0 : W * 16 / TX;
+  static constexpr int TQ = LINES ? W / NG : 0;
+  static_assert(E * NT == TX * N && E % 4 == 0, "whole groups per lane");
+  static_assert(LINES ? (W % NG == 0) : (16 % TX == 0 && W % TX == 0), "affine lane map");
+};
+
+// Compile-time shape of a split-BF16 stage tile: the values a lane fetches
+// from HBM (the pass-0 inputs, or for c2r the two half-spectrum points of
+// each ITER slot), and the LDS layout (the tile, then optionally the
+// twiddles of passes 1 and 2).
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM>
+struct MfShape {
+  using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
+  using M = MfMap<N, TX, NT, LINES>;
+  static constexpr int E = M::E, NH = N / 2 + 1, ITER = (TX * NH + NT - 1) / NT;
+  static constexpr int F = IO == IO_C2R ? 2 * ITER : E;
+  static constexpr int TW0 = RL::tw_off(1), TWN = RL::tw_off(3) - RL::tw_off(1);
+  static constexpr size_t tw_lds_off = (C::lds_bytes() + 15) / 16 * 16;
+  static constexpr size_t lds_bytes(bool twl) {
+    return twl ? tw_lds_off + (size_t)TWN * sizeof(cpx<float>) : C::lds_bytes();
   }
 };
 
-template <int N, int TX, int NT, bool LINES, class RL, int p = 0>
-constexpr int mf_regs() {
-  if constexpr (p == RL::P) {
-    return 0;
-  } else {
-    constexpr int c = MfPass<N, TX, NT, LINES, RL, p>::COUNT;
-    constexpr int r = mf_regs<N, TX, NT, LINES, RL, p + 1>();
-    return c > r ? c : r;
-  }
-}
-
-// One tile of a stage (tile < tiles0 * C1), by the whole workgroup.
+// The HBM loads of one tile (tile < tiles0 * C1) into f, issued and not
+// waited for.
 template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM>
-__device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out,
-                                        int conj, long long tile) {
+__device__ __forceinline__ void mf_fetch(DescPtr d, const void *in, int conj, long long tile,
+                                         cpx<float> (&f)[MfShape<N, TX, NT, IO, LINES, RL, XM>::F]) {
   using T = float;
   using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
-  constexpr int P = RL::P, LDT = C::LDT, LDJ = C::LDJ;
-  constexpr int E = mf_regs<N, TX, NT, LINES, RL>();
+  using M = MfMap<N, TX, NT, LINES>;
+  constexpr int LDT = C::LDT, LDJ = C::LDJ, RPT = C::RPT, E = M::E, GW = M::GW;
+  constexpr int R2 = RL::r[2], NB2 = N / R2, L2 = RL::L(2);
+  static_assert(RL::P == 3 && RL::r[0] == 16 && RL::r[1] == 16 && R2 <= 8, "16 x 16 x R2");
+  static_assert(NB2 % RPT == 0 && (NB2 / RPT) * R2 == E, "VALU pass covers the tile");
   static_assert(NT % 64 == 0, "whole waves");
-  static_assert(P >= 2, "at least two passes");
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
-  T *ldr = reinterpret_cast<T *>(lds_raw);
 
   const long long tiles0 = d->tiles0;
   const long long c1 = tile / tiles0;
   const long long c0b = (tile - c1 * tiles0) * TX;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, i16 = lane & 15, sq = lane >> 4;
+  const int t0 = LINES ? wave / M::NG : i16 % TX;
+  const int b0 = LINES ? (wave % M::NG) * 16 + i16 : wave * (16 / TX) + i16 / TX;
+  const int tv = LINES ? tid / RPT : tid % TX;
+  const int rv = LINES ? tid % RPT : tid / TX;
   const void *tw = d->tw;
 
   // column t of the tile: first pencil column, validity and element bases
@@ -236,23 +278,18 @@ __device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out,
     ib = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1;
     ob = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1;
   };
+  // LDS element of point j of tile column t
+  auto at = [&](int j, int t) __attribute__((always_inline)) { return C::row(j) * LDJ + t * LDT; };
 
-  Split3 ar, ai;
-  w16_operands(tid & 63, ar, ai);
-  cpx<T> v[E];
-
-  // ---- inputs of pass 0 -----------------------------------------------------
-  using P0 = MfPass<N, TX, NT, LINES, RL, 0>;
+  constexpr int NB16 = N / 16;
   if constexpr (IO == IO_C2R) {
-    // conj(Z) of the merged half spectra into LDS, then the inputs of pass 0
     constexpr int NH = N / 2 + 1;
     constexpr int ITER = (TX * NH + NT - 1) / NT;
-    cpx<T> A[ITER], B[ITER];
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
       const int w = tid + it * NT;
-      A[it] = cpx<T>{T(0), T(0)};
-      B[it] = cpx<T>{T(0), T(0)};
+      f[it] = cpx<T>{T(0), T(0)};
+      f[ITER + it] = cpx<T>{T(0), T(0)};
       if (w < TX * NH) {
         const int tt = LINES ? w / NH : w % TX;
         const int k = LINES ? w % NH : w / TX;
@@ -263,10 +300,101 @@ __device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out,
           ia = pack_index(d, k, cc + c1 * d->C0r);
           ib = pack_index(d, k, cc + 1 + c1 * d->C0r);
         }
-        if (cc < d->C0r) A[it] = gload<T>(in, ia);
-        if (cc + 1 < d->C0r) B[it] = gload<T>(in, ib);
+        if (cc < d->C0r) f[it] = gload<T>(in, ia);
+        if (cc + 1 < d->C0r) f[ITER + it] = gload<T>(in, ib);
       }
     }
+  } else {
+    static_assert(IO == IO_C2C || IO == IO_R2C, "c2c, r2c or c2r");
+    // Point j = b0 + 4 sq N/16 (lane part) + BQ q + e N/16 (compile-time part)
+    // of column t: the lane part of the offset is computed once, the rest is a
+    // uniform multiple of the point stride.  A column past the pencil's edge
+    // reads column 0 of the tile instead (always present) and its values are
+    // never stored, so the loads need no branches; the second real column
+    // of an r2c pair is zeroed when absent.
+    const long long sj = d->in_sj;
+    const int jl = b0 + 4 * sq * NB16;
+#pragma unroll
+    for (int q = 0; q < GW; ++q) {
+      long long ca, ib, ob;
+      bool cv;
+      const int t = t0 + M::TQ * q;
+      col(t, ca, cv, ib, ob);
+      bool second = (IO == IO_R2C) && (ca + 1 < d->C0r);
+      if (!cv) {
+        col(0, ca, cv, ib, ob);
+        second = false;
+      }
+      const long long lb = ib + (long long)jl * sj;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const long long o = lb + (long long)(M::BQ * q + e * NB16) * sj;
+        cpx<T> x;
+        if constexpr (IO == IO_R2C) {
+          x.x = gload_r<T>(in, o);
+          x.y = gload_r<T>(in, o + (second ? d->in_s0 : 0));
+          if (!second) x.y = T(0);
+        } else {
+          x = gload<T>(in, o);
+          if (conj) x.y = -x.y;
+        }
+        f[4 * q + e] = x;
+      }
+    }
+  }
+
+  (void)conj;
+}
+
+// The rest of one tile from its fetched values f: (c2r: the half spectra
+// into LDS,) three passes with two LDS exchanges, the stores.  TWL: the
+// twiddles of passes 1 and 2 are read from LDS past the tile (loaded there
+// by the caller) instead of from HBM.
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int SW, bool TWL>
+__device__ __forceinline__ void mf_run(DescPtr d, void *out, int conj, long long tile,
+                                       cpx<float> (&f)[MfShape<N, TX, NT, IO, LINES, RL, XM>::F],
+                                       const Split3 &ar, const Split3 &ai) {
+  using S = MfShape<N, TX, NT, IO, LINES, RL, XM>;
+  using T = float;
+  using C = FastCfg<T, N, TX, NT, LINES, RL, XM>;
+  using M = MfMap<N, TX, NT, LINES>;
+  constexpr int LDT = C::LDT, LDJ = C::LDJ, RPT = C::RPT, E = M::E, GW = M::GW;
+  constexpr int R2 = RL::r[2], NB2 = N / R2, L2 = RL::L(2);
+  static_assert(RL::P == 3 && RL::r[0] == 16 && RL::r[1] == 16 && R2 <= 8, "16 x 16 x R2");
+  static_assert(NB2 % RPT == 0 && (NB2 / RPT) * R2 == E, "VALU pass covers the tile");
+  static_assert(NT % 64 == 0, "whole waves");
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(lds_raw);
+  T *ldr = reinterpret_cast<T *>(lds_raw);
+
+  const long long tiles0 = d->tiles0;
+  const long long c1 = tile / tiles0;
+  const long long c0b = (tile - c1 * tiles0) * TX;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, i16 = lane & 15, sq = lane >> 4;
+  const int t0 = LINES ? wave / M::NG : i16 % TX;
+  const int b0 = LINES ? (wave % M::NG) * 16 + i16 : wave * (16 / TX) + i16 / TX;
+  const int tv = LINES ? tid / RPT : tid % TX;
+  const int rv = LINES ? tid % RPT : tid / TX;
+  const void *tw = d->tw;
+
+  // column t of the tile: first pencil column, validity and element bases
+  auto col = [&](int t, long long &ca, bool &cvalid, long long &ib, long long &ob)
+      __attribute__((always_inline)) {
+    const long long c0 = c0b + t;
+    ca = 2 * c0;
+    cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
+    ib = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1;
+    ob = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1;
+  };
+  // LDS element of point j of tile column t
+  auto at = [&](int j, int t) __attribute__((always_inline)) { return C::row(j) * LDJ + t * LDT; };
+
+  const cpx<T> *ldtw = reinterpret_cast<const cpx<T> *>(lds_raw + S::tw_lds_off);
+  constexpr int NB16 = N / 16;
+  cpx<T> v[E];
+  if constexpr (IO == IO_C2R) {
+    constexpr int NH = N / 2 + 1;
+    constexpr int ITER = (TX * NH + NT - 1) / NT;
 #pragma unroll
     for (int half = 0; half < (XM ? 2 : 1); ++half) {
       if (half) __syncthreads();
@@ -276,7 +404,7 @@ __device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out,
         if (w < TX * NH) {
           const int tt = LINES ? w / NH : w % TX;
           const int k = LINES ? w % NH : w / TX;
-          cpx<T> a = A[it], b = B[it];
+          cpx<T> a = f[it], b = f[ITER + it];
           if (k == 0 || 2 * k == N) {
             a.y = T(0);
             b.y = T(0);
@@ -284,119 +412,137 @@ __device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out,
           const cpx<T> zk = {a.x - b.y, -(a.y + b.x)}, zm = {a.x + b.y, a.y - b.x};
           const bool mirror = k > 0 && 2 * k < N;
           if constexpr (XM == 0) {
-            lds[C::row(k) * LDJ + tt * LDT] = zk;
-            if (mirror) lds[C::row(N - k) * LDJ + tt * LDT] = zm;
+            lds[at(k, tt)] = zk;
+            if (mirror) lds[at(N - k, tt)] = zm;
           } else {
-            ldr[C::row(k) * LDJ + tt * LDT] = half ? zk.y : zk.x;
-            if (mirror) ldr[C::row(N - k) * LDJ + tt * LDT] = half ? zm.y : zm.x;
+            ldr[at(k, tt)] = half ? zk.y : zk.x;
+            if (mirror) ldr[at(N - k, tt)] = half ? zm.y : zm.x;
           }
         }
       }
       __syncthreads();
 #pragma unroll
-      for (int i = 0; i < P0::COUNT; ++i) {
-        int t, b, sp;
-        bool ok;
-        P0::elem(tid, i, t, b, sp, ok);
-        if (ok) {
-          const int a = C::row(P0::jin(b, sp)) * LDJ + t * LDT;
+      for (int q = 0; q < GW; ++q) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int a = at(b0 + M::BQ * q + (4 * sq + e) * NB16, t0 + M::TQ * q);
           if constexpr (XM == 0) {
-            v[i] = lds[a];
+            v[4 * q + e] = lds[a];
           } else {
-            if (half) v[i].y = ldr[a]; else v[i].x = ldr[a];
+            if (half) v[4 * q + e].y = ldr[a]; else v[4 * q + e].x = ldr[a];
           }
         }
       }
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < P0::COUNT; ++i) {
-      int t, b, sp;
-      bool ok;
-      P0::elem(tid, i, t, b, sp, ok);
-      if (ok) {
-        long long ca, ib, ob;
-        bool cv;
-        col(t, ca, cv, ib, ob);
-        v[i] = fload<T, IO>(d, in, ib, cv, true, ca, P0::jin(b, sp), conj);
+    for (int i = 0; i < E; ++i) v[i] = f[i];
+  }
+
+  // ---- pass 0: radix 16, L = 1 (no twiddles) ---------------------------------
+#pragma unroll
+  for (int q = 0; q < GW; ++q) {
+    cpx<T> x[4] = {v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]}, y[4];
+    mf_dft16<SW>(x, y, ar, ai);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[4 * q + e] = y[e];
+  }
+
+  // ---- exchange 0 -> 1: outputs at 16 b + s, inputs at b + s N/16 -------------
+  if constexpr (IO == IO_C2R) __syncthreads();
+#pragma unroll
+  for (int half = 0; half < (XM ? 2 : 1); ++half) {
+    if (half) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < GW; ++q) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int a = at(16 * (b0 + M::BQ * q) + 4 * sq + e, t0 + M::TQ * q);
+        if constexpr (XM == 0) lds[a] = v[4 * q + e];
+        else ldr[a] = half ? v[4 * q + e].y : v[4 * q + e].x;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < GW; ++q) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int a = at(b0 + M::BQ * q + (4 * sq + e) * NB16, t0 + M::TQ * q);
+        if constexpr (XM == 0) {
+          v[4 * q + e] = lds[a];
+        } else {
+          if (half) v[4 * q + e].y = ldr[a]; else v[4 * q + e].x = ldr[a];
+        }
       }
     }
   }
 
-  // ---- passes -------------------------------------------------------------
-  static_for<0, P>([&](auto pc) __attribute__((always_inline)) {
-    constexpr int p = decltype(pc)::value;
-    using M = MfPass<N, TX, NT, LINES, RL, p>;
-    constexpr int R = M::R, L = M::L;
-    if constexpr (p > 0) {
-      // outputs of pass p-1 -> LDS -> inputs of pass p (real parts, then
-      // imaginary parts, for the split exchange)
-      using Q = MfPass<N, TX, NT, LINES, RL, p - 1>;
-      if constexpr (p > 1 || IO == IO_C2R) __syncthreads();
+  // ---- pass 1: radix 16, L = 16; butterfly b of every group has b%16 = b0%16 --
+  {
+    const int k1 = b0 & 15;
+    cpx<T> w1[4];
 #pragma unroll
-      for (int half = 0; half < (XM ? 2 : 1); ++half) {
-        if (half) __syncthreads();
-#pragma unroll
-        for (int i = 0; i < Q::COUNT; ++i) {
-          int t, b, sp;
-          bool ok;
-          Q::elem(tid, i, t, b, sp, ok);
-          if (ok) {
-            const int a = C::row(Q::jout(b, sp)) * LDJ + t * LDT;
-            if constexpr (XM == 0)
-              lds[a] = v[i];
-            else
-              ldr[a] = half ? v[i].y : v[i].x;
-          }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < M::COUNT; ++i) {
-          int t, b, sp;
-          bool ok;
-          M::elem(tid, i, t, b, sp, ok);
-          if (ok) {
-            const int a = C::row(M::jin(b, sp)) * LDJ + t * LDT;
-            if constexpr (XM == 0) {
-              v[i] = lds[a];
-            } else {
-              if (half) v[i].y = ldr[a]; else v[i].x = ldr[a];
-            }
-          }
-        }
-      }
+    for (int e = 0; e < 4; ++e) {
+      const int s = 4 * sq + e;
+      const int ti = RL::tw_off(1) + k1 * 15 + (s > 0 ? s - 1 : 0);
+      const cpx<T> wl = TWL ? ldtw[ti - S::TW0] : gload<T>(tw, ti);
+      w1[e] = s > 0 ? wl : cpx<T>{T(1), T(0)};
     }
-    // twiddles, then the butterflies
-    if constexpr (L > 1) {
 #pragma unroll
-      for (int i = 0; i < M::COUNT; ++i) {
-        int t, b, sp;
-        bool ok;
-        M::elem(tid, i, t, b, sp, ok);
-        if (ok && sp > 0) v[i] = cmul(v[i], gload<T>(tw, RL::tw_off(p) + (b % L) * (R - 1) + sp - 1));
-      }
+    for (int q = 0; q < GW; ++q) {
+      cpx<T> x[4], y[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[e] = cmul(v[4 * q + e], w1[e]);
+      mf_dft16<SW>(x, y, ar, ai);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[4 * q + e] = y[e];
     }
-    if constexpr (M::MF) {
-#pragma unroll
-      for (int q = 0; q < M::GW; ++q) {
-        cpx<T> x[4] = {v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]}, y[4];
-        mf_dft16(x, y, ar, ai);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) v[4 * q + e] = y[e];
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < M::NB; ++q) {
-        int t, b, sp;
-        bool ok;
-        M::elem(tid, q * R, t, b, sp, ok);
-        if (ok) dft<T, R>(&v[q * R]);
-      }
-    }
-  });
+  }
 
-  // ---- outputs of the last pass ---------------------------------------------
-  using Z = MfPass<N, TX, NT, LINES, RL, P - 1>;
+  // ---- exchange 1 -> 2: outputs at 256 (b/16) + b%16 + 16 s; VALU-map inputs --
+  __syncthreads();
+#pragma unroll
+  for (int half = 0; half < (XM ? 2 : 1); ++half) {
+    if (half) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < GW; ++q) {
+      const int b = b0 + M::BQ * q;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int a = at(256 * (b >> 4) + (b & 15) + 16 * (4 * sq + e), t0 + M::TQ * q);
+        if constexpr (XM == 0) lds[a] = v[4 * q + e];
+        else ldr[a] = half ? v[4 * q + e].y : v[4 * q + e].x;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NB2 / RPT; ++q) {
+#pragma unroll
+      for (int s = 0; s < R2; ++s) {
+        const int a = at(rv + q * RPT + s * NB2, tv);
+        if constexpr (XM == 0) {
+          v[q * R2 + s] = lds[a];
+        } else {
+          if (half) v[q * R2 + s].y = ldr[a]; else v[q * R2 + s].x = ldr[a];
+        }
+      }
+    }
+  }
+
+  // ---- pass 2: radix R2 on the VALU, L = 256 ----------------------------------
+#pragma unroll
+  for (int q = 0; q < NB2 / RPT; ++q) {
+    const int b = rv + q * RPT;  // < NB2 = L2: k = b, g = 0
+#pragma unroll
+    for (int s = 1; s < R2; ++s)
+{
+      const int ti = RL::tw_off(2) + (b % L2) * (R2 - 1) + s - 1;
+      v[q * R2 + s] = cmul(v[q * R2 + s], TWL ? ldtw[ti - S::TW0] : gload<T>(tw, ti));
+    }
+    dft<T, R2>(&v[q * R2]);
+  }
+
+  // ---- outputs: point b + s*256 of column tv ------------------------------------
   if constexpr (IO == IO_R2C) {
     constexpr int NH = N / 2 + 1;
     constexpr int ITER = (TX * NH + NT - 1) / NT;
@@ -406,16 +552,12 @@ __device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out,
     for (int half = 0; half < (XM ? 2 : 1); ++half) {
       __syncthreads();
 #pragma unroll
-      for (int i = 0; i < Z::COUNT; ++i) {
-        int t, b, sp;
-        bool ok;
-        Z::elem(tid, i, t, b, sp, ok);
-        if (ok) {
-          const int a = C::row(Z::jout(b, sp)) * LDJ + t * LDT;
-          if constexpr (XM == 0)
-            lds[a] = v[i];
-          else
-            ldr[a] = half ? v[i].y : v[i].x;
+      for (int q = 0; q < NB2 / RPT; ++q) {
+#pragma unroll
+        for (int s = 0; s < R2; ++s) {
+          const int a = at(rv + q * RPT + s * L2, tv);
+          if constexpr (XM == 0) lds[a] = v[q * R2 + s];
+          else ldr[a] = half ? v[q * R2 + s].y : v[q * R2 + s].x;
         }
       }
       __syncthreads();
@@ -426,7 +568,7 @@ __device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out,
           const int tt = LINES ? w / NH : w % TX;
           const int k = LINES ? w % NH : w / TX;
           const int km = (k == 0) ? 0 : N - k;
-          const int a1 = C::row(k) * LDJ + tt * LDT, a2 = C::row(km) * LDJ + tt * LDT;
+          const int a1 = at(k, tt), a2 = at(km, tt);
           if constexpr (XM == 0) {
             const cpx<T> zk = lds[a1], zm = lds[a2];
             oa[it] = cpx<T>{(zk.x + zm.x) * half_, (zk.y - zm.y) * half_};
@@ -464,44 +606,73 @@ __device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out,
       }
     }
   } else {
+    // point j = rv + q RPT + s L2 of column tv: one branch for the column,
+    // the lane part of the offset once, the rest uniform
+    long long ca, ib, ob;
+    bool cv;
+    col(tv, ca, cv, ib, ob);
+    if (cv) {
+      const long long sj = d->out_sj;
+      const long long lb = ob + (long long)rv * sj;
+      const bool second = (IO == IO_C2R) && (ca + 1 < d->C0r);
 #pragma unroll
-    for (int i = 0; i < Z::COUNT; ++i) {
-      int t, b, sp;
-      bool ok;
-      Z::elem(tid, i, t, b, sp, ok);
-      if (ok) {
-        long long ca, ib, ob;
-        bool cv;
-        col(t, ca, cv, ib, ob);
-        fstore<T, IO>(d, out, ob, cv, ca, Z::jout(b, sp), v[i], conj);
+      for (int q = 0; q < NB2 / RPT; ++q) {
+#pragma unroll
+        for (int s = 0; s < R2; ++s) {
+          const long long o = lb + (long long)(q * RPT + s * L2) * sj;
+          cpx<T> y = v[q * R2 + s];
+          if constexpr (IO == IO_C2R) {
+            gstore_r<T>(out, o, y.x);
+            if (second) gstore_r<T>(out, o + d->out_s0, -y.y);
+          } else {
+            if (conj) y.y = -y.y;
+            gstore<T>(out, o, y);
+          }
+        }
       }
     }
   }
 }
 
-template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+
+// One tile of a stage, by the whole workgroup: every load issued, then the
+// passes and stores.
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int SW>
+__device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out, int conj, long long tile) {
+  cpx<float> f[MfShape<N, TX, NT, IO, LINES, RL, XM>::F];
+  mf_fetch<N, TX, NT, IO, LINES, RL, XM>(d, in, conj, tile, f);
+  // every load of the tile in flight before the first butterfly: without
+  // this the scheduler sinks each load next to its first use, and the stage
+  // loses most of its memory-level parallelism
+  __builtin_amdgcn_sched_barrier(0);
+  Split3 ar, ai;
+  w16_operands(threadIdx.x & 63, ar, ai);
+  mf_run<N, TX, NT, IO, LINES, RL, XM, SW, false>(d, out, conj, tile, f, ar, ai);
+}
+
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
     mf_axis_kernel(const AxisDesc *__restrict__ d, const void *in, void *out, int conj) {
   const long long ntiles = (long long)d->tiles0 * d->C1;
   const long long tile = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, ntiles);
   if (tile >= ntiles) return;  // whole block, before any barrier
-  mf_tile<N, TX, NT, IO, LINES, RL, XM>(desc_ptr(d), in, out, conj, tile);
+  mf_tile<N, TX, NT, IO, LINES, RL, XM, SW>(desc_ptr(d), in, out, conj, tile);
 }
 
 // The same stage kernel over the tiles of several grids.
-template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
     mf_multi_kernel(const MultiGroup *__restrict__ g, int ng, long long tiles) {
   const long long it = xcd_tile((long long)blockIdx.x + (long long)blockIdx.y * gridDim.x, tiles);
   if (it >= tiles) return;  // whole block, before any barrier
   const int gi = find_multi_group(g, ng, it);
-  mf_tile<N, TX, NT, IO, LINES, RL, XM>(desc_ptr(g[gi].d), g[gi].in, g[gi].out, g[gi].conj,
+  mf_tile<N, TX, NT, IO, LINES, RL, XM, SW>(desc_ptr(g[gi].d), g[gi].in, g[gi].out, g[gi].conj,
                                         it - g[gi].tile0);
 }
 
 // ---- registry -----------------------------------------------------------------
 
-template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 static int launch_mf(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, hipStream_t st) {
   using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
   const long long ntiles = xcd_grid((long long)h.tiles0 * h.C1);
@@ -509,19 +680,19 @@ static int launch_mf(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, hi
   const long long gx = ntiles < (1LL << 20) ? ntiles : (1LL << 20);
   const long long gy = (ntiles + gx - 1) / gx;
   if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "axis stage has too many tiles");
-  hipLaunchKernelGGL((mf_axis_kernel<N, TX, NT, IO, LINES, RL, XM, WPE>), dim3((unsigned)gx, (unsigned)gy),
+  hipLaunchKernelGGL((mf_axis_kernel<N, TX, NT, IO, LINES, RL, XM, WPE, SW>), dim3((unsigned)gx, (unsigned)gy),
                      dim3(NT), C::lds_bytes(), st, dd, io.in, io.out, io.conj);
   return hip_check(hipGetLastError(), "mf_axis_kernel launch");
 }
 
-template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 static hipError_t prepare_mf() {
   using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
-  return hipFuncSetAttribute((const void *)mf_axis_kernel<N, TX, NT, IO, LINES, RL, XM, WPE>,
+  return hipFuncSetAttribute((const void *)mf_axis_kernel<N, TX, NT, IO, LINES, RL, XM, WPE, SW>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
 }
 
-template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 static int launch_mf_multi(const MultiGroup *g, int ng, long long tiles, hipStream_t st) {
   using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
   const long long nb = xcd_grid(tiles);
@@ -529,15 +700,15 @@ static int launch_mf_multi(const MultiGroup *g, int ng, long long tiles, hipStre
   const long long gx = nb < (1LL << 20) ? nb : (1LL << 20);
   const long long gy = (nb + gx - 1) / gx;
   if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "launch has too many tiles");
-  hipLaunchKernelGGL((mf_multi_kernel<N, TX, NT, IO, LINES, RL, XM, WPE>), dim3((unsigned)gx, (unsigned)gy),
+  hipLaunchKernelGGL((mf_multi_kernel<N, TX, NT, IO, LINES, RL, XM, WPE, SW>), dim3((unsigned)gx, (unsigned)gy),
                      dim3(NT), C::lds_bytes(), st, g, ng, tiles);
   return hip_check(hipGetLastError(), "mf_multi_kernel launch");
 }
 
-template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 static hipError_t prepare_mf_multi() {
   using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
-  return hipFuncSetAttribute((const void *)mf_multi_kernel<N, TX, NT, IO, LINES, RL, XM, WPE>,
+  return hipFuncSetAttribute((const void *)mf_multi_kernel<N, TX, NT, IO, LINES, RL, XM, WPE, SW>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
 }
 
@@ -546,7 +717,7 @@ static std::vector<int> mf_rvec(RList<Rs...>) {
   return {Rs...};
 }
 
-template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE>
+template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 static FastVariant make_mf(const char *name) {
   using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
   FastVariant v;
@@ -561,19 +732,20 @@ static FastVariant make_mf(const char *name) {
   v.ldj = C::LDJ;
   v.ldt = C::LDT;
   v.radices = mf_rvec(RL{});
-  v.launch = &launch_mf<N, TX, NT, IO, LINES, RL, XM, WPE>;
-  v.prepare = &prepare_mf<N, TX, NT, IO, LINES, RL, XM, WPE>;
-  v.launch_multi = &launch_mf_multi<N, TX, NT, IO, LINES, RL, XM, WPE>;
-  v.prepare_multi = &prepare_mf_multi<N, TX, NT, IO, LINES, RL, XM, WPE>;
+  v.launch = &launch_mf<N, TX, NT, IO, LINES, RL, XM, WPE, SW>;
+  v.prepare = &prepare_mf<N, TX, NT, IO, LINES, RL, XM, WPE, SW>;
+  v.launch_multi = &launch_mf_multi<N, TX, NT, IO, LINES, RL, XM, WPE, SW>;
+  v.prepare_multi = &prepare_mf_multi<N, TX, NT, IO, LINES, RL, XM, WPE, SW>;
   v.name = name;
   return v;
 }
 
-// c2c, r2c and c2r of one shape; XM = 1 adds " xs", WPE > 1 " wN" to the name
-#define D2D_MF_VARIANTS(N, TX, NT, LINES, RL, XM, WPE, TAG)                                      \
-  make_mf<N, TX, NT, IO_C2C, LINES, RL, XM, WPE>("mfma n=" #N " tx=" #TX " nt=" #NT " c2c" TAG),   \
-      make_mf<N, TX, NT, IO_R2C, LINES, RL, XM, WPE>("mfma n=" #N " tx=" #TX " nt=" #NT " r2c" TAG), \
-      make_mf<N, TX, NT, IO_C2R, LINES, RL, XM, WPE>("mfma n=" #N " tx=" #TX " nt=" #NT " c2r" TAG)
+// c2c, r2c and c2r of one shape; XM = 1 adds " xs", WPE > 1 " wN", SW = 1
+// " sw" to the name
+#define D2D_MF_VARIANTS(N, TX, NT, LINES, RL, XM, WPE, SW, TAG)                                         \
+  make_mf<N, TX, NT, IO_C2C, LINES, RL, XM, WPE, SW>("mfma n=" #N " tx=" #TX " nt=" #NT " c2c" TAG),   \
+      make_mf<N, TX, NT, IO_R2C, LINES, RL, XM, WPE, SW>("mfma n=" #N " tx=" #TX " nt=" #NT " r2c" TAG), \
+      make_mf<N, TX, NT, IO_C2R, LINES, RL, XM, WPE, SW>("mfma n=" #N " tx=" #TX " nt=" #NT " c2r" TAG)
 
 using M16_16_4 = RList<16, 16, 4>;
 using M16_16_2 = RList<16, 16, 2>;
@@ -581,25 +753,20 @@ using M16_16_2 = RList<16, 16, 2>;
 std::vector<FastVariant> mf_variants() {
   return {
       // 1024 = 16 * 16 * 4: strided columns
-      D2D_MF_VARIANTS(1024, 16, 1024, false, M16_16_4, 0, 4, ""),
-      D2D_MF_VARIANTS(1024, 8, 512, false, M16_16_4, 0, 4, ""),
-      D2D_MF_VARIANTS(1024, 8, 512, false, M16_16_4, 1, 5, " xs w5"),
-      D2D_MF_VARIANTS(1024, 4, 256, false, M16_16_4, 1, 5, " xs w5"),
-      D2D_MF_VARIANTS(1024, 16, 1024, false, M16_16_4, 1, 4, " xs"),
-      // two groups per wave (8 values per lane): 64-74 VGPRs, 6-8 waves per SIMD
-      D2D_MF_VARIANTS(1024, 4, 512, false, M16_16_4, 1, 6, " xs w6"),
-      D2D_MF_VARIANTS(1024, 2, 256, false, M16_16_4, 1, 6, " xs w6"),
-      D2D_MF_VARIANTS(1024, 8, 1024, false, M16_16_4, 1, 8, " xs w8"),
-      D2D_MF_VARIANTS(1024, 8, 1024, false, M16_16_4, 0, 4, ""),
+      D2D_MF_VARIANTS(1024, 8, 512, false, M16_16_4, 0, 4, 0, ""),
+      D2D_MF_VARIANTS(1024, 8, 512, false, M16_16_4, 1, 6, 1, " xs w6 sw"),
+      D2D_MF_VARIANTS(1024, 8, 1024, false, M16_16_4, 0, 8, 1, " w8 sw"),
+      D2D_MF_VARIANTS(1024, 16, 1024, false, M16_16_4, 1, 4, 0, " xs"),
       // contiguous lines
-      D2D_MF_VARIANTS(1024, 4, 256, true, M16_16_4, 0, 4, ""),
-      D2D_MF_VARIANTS(1024, 8, 512, true, M16_16_4, 0, 4, ""),
-      D2D_MF_VARIANTS(1024, 4, 256, true, M16_16_4, 1, 5, " xs w5"),
-      D2D_MF_VARIANTS(1024, 8, 512, true, M16_16_4, 1, 5, " xs w5"),
-      D2D_MF_VARIANTS(1024, 4, 512, true, M16_16_4, 1, 6, " xs w6"),
+      D2D_MF_VARIANTS(1024, 4, 256, true, M16_16_4, 0, 4, 0, ""),
+      D2D_MF_VARIANTS(1024, 4, 256, true, M16_16_4, 1, 6, 1, " xs w6 sw"),
+      D2D_MF_VARIANTS(1024, 4, 512, true, M16_16_4, 0, 8, 1, " w8 sw"),
+      D2D_MF_VARIANTS(1024, 8, 512, true, M16_16_4, 1, 6, 1, " xs w6 sw"),
       // 512 = 16 * 16 * 2
-      D2D_MF_VARIANTS(512, 16, 512, false, M16_16_2, 0, 4, ""),
-      D2D_MF_VARIANTS(512, 8, 256, true, M16_16_2, 0, 4, ""),
+      D2D_MF_VARIANTS(512, 16, 1024, false, M16_16_2, 1, 8, 1, " xs w8 sw"),
+      D2D_MF_VARIANTS(512, 8, 512, false, M16_16_2, 0, 4, 0, ""),
+      D2D_MF_VARIANTS(512, 8, 256, true, M16_16_2, 0, 4, 0, ""),
+      D2D_MF_VARIANTS(512, 8, 512, true, M16_16_2, 0, 8, 1, " w8 sw"),
   };
 }
 
