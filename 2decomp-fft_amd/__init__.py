@@ -109,6 +109,12 @@ def lib():
             "d2dhip_decomp_info_init": ([ci, ci, ci, ci, ci, ci, ci, ctypes.POINTER(Decomp)], ci),
             "d2dhip_best_2d_grid": ([ci, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
             "d2dhip_plan_create": ([ctypes.POINTER(pl), ci, ci, ci, ci, ci, cu, vp], ci),
+            "d2dhip_plan_create_rank": ([ctypes.POINTER(pl), ci, ci, ci, ci, ci, ci, ci, ci, ci,
+                                         cu, vp], ci),
+            "d2dhip_plan_get_rank": ([pl, ctypes.POINTER(Decomp), ctypes.POINTER(Decomp)], ci),
+            "d2dhip_exec_1m": ([pl, ci, vp, vp, ci], ci),
+            "d2dhip_exec_1m_pack": ([pl, ci, vp, vp, ci, ctypes.POINTER(Decomp), ci, ci], ci),
+            "d2dhip_exec_1m_unpack": ([pl, ci, vp, vp, ci, ctypes.POINTER(Decomp), ci, ci], ci),
             "d2dhip_plan_destroy": ([pl], ci),
             "d2dhip_plan_set_stream": ([pl, vp], ci),
             "d2dhip_plan_get_sizes": ([pl, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
@@ -229,7 +235,10 @@ class FFTEngine:
     """One decomp_2d_fft engine (src/fft_common.f90:141-242) at one rank."""
 
     def __init__(self, pencil, nx, ny, nz, prec=PREC_F64, inplace=False, skip_xyz=None,
-                 stream=None, tune=False, opt_DTT=None):
+                 stream=None, tune=False, opt_DTT=None, rank=None):
+        """rank = (p_row, p_col, coord_row, coord_col): the engine of one rank
+        of a process grid (its stages on that rank's pencils; run them with
+        exec_1m)."""
         flags = FLAG_INPLACE if inplace else 0
         if tune:
             flags |= FLAG_TUNE
@@ -243,8 +252,12 @@ class FFTEngine:
         self.prec = prec
         self.flags = flags
         st = ctypes.c_void_p(stream) if stream is not None else ctypes.c_void_p()
-        check(lib().d2dhip_plan_create(ctypes.byref(self._plan), pencil, nx, ny, nz, prec, flags,
-                                       st))
+        if rank is None:
+            check(lib().d2dhip_plan_create(ctypes.byref(self._plan), pencil, nx, ny, nz, prec,
+                                           flags, st))
+        else:
+            check(lib().d2dhip_plan_create_rank(ctypes.byref(self._plan), pencil, nx, ny, nz, prec,
+                                                *rank, flags, st))
         self._explicit_stream = stream is not None
         if opt_DTT is not None:
             self.set_dtt(opt_DTT)
@@ -252,6 +265,32 @@ class FFTEngine:
     def _sync_stream(self):
         if not self._explicit_stream:
             check(lib().d2dhip_plan_set_stream(self._plan, _stream(None)))
+
+    def decomps(self):
+        """(ph, sp): the plan's physical and spectral decompositions."""
+        ph, sp = Decomp(), Decomp()
+        check(lib().d2dhip_plan_get_rank(self._plan, ctypes.byref(ph), ctypes.byref(sp)))
+        return ph, sp
+
+    def exec_1m(self, stage, inp, out, isign):
+        """One 1D stage (STAGE_*) on the plan's pencil: c2c_1m_*, r2c_1m_*, c2r_1m_*."""
+        self._sync_stream()
+        i = self.STAGES.index(stage) if isinstance(stage, str) else stage
+        return check(lib().d2dhip_exec_1m(self._plan, i, _ptr(inp), _ptr(out), isign))
+
+    def exec_1m_pack(self, stage, inp, send, direction, decomp, isign, even=False):
+        """A y stage with its output in the send order of `direction` (Y_TO_Z, Y_TO_X)."""
+        self._sync_stream()
+        i = self.STAGES.index(stage) if isinstance(stage, str) else stage
+        return check(lib().d2dhip_exec_1m_pack(self._plan, i, _ptr(inp), _ptr(send), direction,
+                                               ctypes.byref(decomp), int(bool(even)), isign))
+
+    def exec_1m_unpack(self, stage, recv, out, direction, decomp, isign, even=False):
+        """A y stage reading its input in the receive order of `direction` (Z_TO_Y, X_TO_Y)."""
+        self._sync_stream()
+        i = self.STAGES.index(stage) if isinstance(stage, str) else stage
+        return check(lib().d2dhip_exec_1m_unpack(self._plan, i, _ptr(recv), _ptr(out), direction,
+                                                 ctypes.byref(decomp), int(bool(even)), isign))
 
     def get_sizes(self):
         ph = (ctypes.c_int * 3)()

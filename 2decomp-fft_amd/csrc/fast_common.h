@@ -55,11 +55,18 @@ struct FastCfg {
   }
 };
 
+// base: the column's plain element offset; c1: its outer column (used by
+// the y-pencil buffer forms).
 template <typename T, int IO>
 __device__ __forceinline__ cpx<T> fload(DescPtr d, const void *in,
                                         long long base, bool cvalid, bool bvalid, long long ca,
-                                        int j, int conj) {
-  if constexpr (IO == IO_R2C) {
+                                        int j, int conj, long long c1 = 0) {
+  if constexpr (IO == IO_YUN) {
+    cpx<T> v = {T(0), T(0)};
+    if (cvalid) v = gload<T>(in, base + (long long)j * d->in_sj + packj_offset(d, j, c1));
+    if (conj) v.y = -v.y;
+    return v;
+  } else if constexpr (IO == IO_R2C) {
     const long long o = base + (long long)j * d->in_sj;
     cpx<T> v = {T(0), T(0)};
     if (ca < d->C0r) v.x = gload_r<T>(in, o);
@@ -76,8 +83,14 @@ __device__ __forceinline__ cpx<T> fload(DescPtr d, const void *in,
 
 template <typename T, int IO>
 __device__ __forceinline__ void fstore(DescPtr d, void *out, long long base,
-                                       bool cvalid, long long ca, int j, cpx<T> v, int conj) {
-  if constexpr (IO == IO_C2R) {
+                                       bool cvalid, long long ca, int j, cpx<T> v, int conj,
+                                       long long c1 = 0) {
+  if constexpr (IO == IO_YPK) {
+    if (cvalid) {
+      if (conj) v.y = -v.y;
+      gstore<T>(out, base + (long long)j * d->out_sj + packj_offset(d, j, c1), v);
+    }
+  } else if constexpr (IO == IO_C2R) {
     const long long o = base + (long long)j * d->out_sj;
     if (ca < d->C0r) gstore_r<T>(out, o, v.x);
     if (ca + 1 < d->C0r) gstore_r<T>(out, o + d->out_s0, -v.y);

@@ -45,7 +45,7 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   const long long c0 = c0b + t;
   const long long ca = 2 * c0;  // first real column of a pair (real IO)
   constexpr bool DTT = IO == IO_DTT2 || IO == IO_DTT3 || IO == IO_DTTX || IO == IO_DTT4;
-  constexpr bool PAIRED = IO != IO_C2C && IO != IO_DTT4;  // two real columns per tile column
+  constexpr bool PAIRED = IO != IO_C2C && IO != IO_DTT4 && IO != IO_YPK && IO != IO_YUN;  // two real columns per tile column
   static_assert(!DTT || XM == 0, "real-to-real stages use the plain exchange");
   const bool cvalid = PAIRED ? (ca < d->C0r) : (c0 < d->C0);
   // real-to-real stages: the transformed points start at dtt_first
@@ -274,7 +274,7 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
       if (C::exact(0) || b < NBR) {
 #pragma unroll
         for (int s = 0; s < R; ++s)
-          v[q * R + s] = fload<T, IO>(d, in, in_base, cvalid, true, ca, b + s * NBR, conj);
+          v[q * R + s] = fload<T, IO>(d, in, in_base, cvalid, true, ca, b + s * NBR, conj, c1);
       }
     }
   }
@@ -539,7 +539,7 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
         const int k = b % LZ, g = b / LZ;
 #pragma unroll
         for (int s = 0; s < RZ; ++s)
-          fstore<T, IO>(d, out, out_base, cvalid, ca, g * LZ * RZ + k + s * LZ, v[q * RZ + s], conj);
+          fstore<T, IO>(d, out, out_base, cvalid, ca, g * LZ * RZ + k + s * LZ, v[q * RZ + s], conj, c1);
       }
     }
   }
@@ -664,6 +664,12 @@ static FastVariant make_variant(const char *name) {
       make_variant<T, N, TX, NT, IO_DTT3, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct3"), \
       make_variant<T, N, TX, NT, IO_DTTX, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dctx"), \
       make_variant<T, N, TX, NT, IO_DTT4, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct4")
+
+// the y-pencil stage with its output in pack order (" ypk") and with its
+// input in unpack order (" yun"), twins of the c2c variant of the same name
+#define D2D_YPACK(T, N, TX, NT, RL, XM, WPE, TAG)                                                     \
+  make_variant<T, N, TX, NT, IO_YPK, false, RL, XM, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT " ypk" TAG), \
+      make_variant<T, N, TX, NT, IO_YUN, false, RL, XM, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT " yun" TAG)
 
 // c2c only, split exchange, at least WPE waves per SIMD
 #define D2D_SPLIT_C2C(T, N, TX, NT, LINES, RL, WPE)                                      \
@@ -883,6 +889,14 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(double, 1024, 1, 64, false, R16_16_4),
       D2D_VARIANTS(double, 1024, 4, 256, true, R16_16_4),
       D2D_VARIANTS(double, 1024, 2, 128, true, R16_16_4),
+      // y-pencil stages fused with the pack / unpack of the y-pencil's exchange
+      D2D_YPACK(double, 1024, 16, 1024, R16_16_4, 1, 4, " xs w4"),
+      D2D_YPACK(double, 1024, 8, 512, R16_16_4, 1, 4, " xs w4"),
+      D2D_YPACK(double, 1024, 8, 512, R16_16_4, 0, 1, ""),
+      D2D_YPACK(float, 1024, 16, 1024, R16_16_4, 1, 8, " xs w8"),
+      D2D_YPACK(float, 1024, 8, 512, R16_16_4, 0, 1, ""),
+      D2D_YPACK(double, 512, 8, 512, R8_8_8, 0, 1, ""),
+      D2D_YPACK(float, 512, 16, 512, R8_8_8, 0, 1, ""),
       D2D_SPLIT_C2C(double, 1024, 2, 128, false, R16_16_4, 4),
       D2D_SPLIT_C2C(double, 1024, 4, 256, false, R16_16_4, 4),
       D2D_SPLIT_C2C(double, 1024, 8, 512, false, R16_16_4, 4),

@@ -114,6 +114,24 @@ struct d2dhip_plan_s {
   bool tune_write_failed = false;  // a tuning choice could not be written to the cache
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // Pencils the stages run on: x, y, z of the physical decomposition, then
+  // x, y, z of the spectral one (sizes, i fastest).  At one rank every one
+  // is the whole grid (or its spectral shape); a plan of one rank of a
+  // p_row x p_col grid (d2dhip_plan_create_rank) has that rank's pencils,
+  // and its 3D calls are refused (they need the exchange between stages).
+  long long pen[6][3] = {};
+  bool multi = false;
+  d2dhip_decomp ph{}, sp{};
+  // y-pencil stages fused with the y-pencil exchange's pack / unpack
+  // (d2dhip_exec_1m_pack / _unpack): block table and descriptor; and, for a
+  // stage kernel without a fused form, a y-pencil workspace for stage then
+  // pack (allocated on first use)
+  long long *d_ypack = nullptr;
+  AxisDesc *d_ydesc = nullptr;
+  long long h_ypack[3 * D2DHIP_MAX_PROCS] = {};
+  AxisDesc h_ydesc{};
+  void *ywk = nullptr;
+  size_t ywk_bytes = 0;
 };
 
 namespace {
@@ -663,30 +681,67 @@ int upload_descs(d2dhip_plan_s *p) {
 }
 
 // Builds the seven stage descriptors of a plan and uploads them.
-int make_stages(d2dhip_plan_s *p) {
+long long pencil_elems(const long long (&v)[3]) { return v[0] * v[1] * v[2]; }
+
+StageIo pencil_view(int axis, const long long (&v)[3]) { return cols_view(axis, v[0], v[1], v[2]); }
+
+// The pencils of a plan: the whole grid at one rank, else the rank's
+// physical and spectral pencils.
+void set_pencils(d2dhip_plan_s *p) {
   const long long nx = p->nx, ny = p->ny, nz = p->nz;
-  const long long nph = nx * ny * nz;
+  if (!p->multi) {
+    const long long sx = p->format == D2DHIP_PHYSICAL_IN_X ? nx / 2 + 1 : nx;
+    const long long sz = p->format == D2DHIP_PHYSICAL_IN_X ? nz : nz / 2 + 1;
+    for (int q = 0; q < 3; ++q) {
+      p->pen[q][0] = nx, p->pen[q][1] = ny, p->pen[q][2] = nz;
+      p->pen[3 + q][0] = sx, p->pen[3 + q][1] = ny, p->pen[3 + q][2] = sz;
+    }
+    return;
+  }
+  const d2dhip_decomp *dd[2] = {&p->ph, &p->sp};
+  for (int h = 0; h < 2; ++h)
+    for (int a = 0; a < 3; ++a) {
+      p->pen[3 * h + 0][a] = dd[h]->xsz[a];
+      p->pen[3 * h + 1][a] = dd[h]->ysz[a];
+      p->pen[3 * h + 2][a] = dd[h]->zsz[a];
+    }
+}
+
+// Builds the seven stage descriptors of a plan on its pencils and uploads
+// them: c2c along x, y, z of the physical pencils; the r2c stage (physical
+// x- or z-pencil to the spectral one); the spectral y stage; the other
+// spectral c2c stage; the c2r stage.
+int make_stages(d2dhip_plan_s *p) {
+  set_pencils(p);
+  const auto &px = p->pen[0], &py = p->pen[1], &pz = p->pen[2];
+  const auto &sx = p->pen[3], &sy = p->pen[4], &sz = p->pen[5];
   int st;
   for (int a = 0; a < 3; ++a)
-    if ((st = make_stage(p, a, a, IO_C2C, cols_view(a, nx, ny, nz), nph, nph))) return st;
+    if ((st = make_stage(p, a, a, IO_C2C, pencil_view(a, p->pen[a]), pencil_elems(p->pen[a]),
+                         pencil_elems(p->pen[a]))))
+      return st;
   if (p->format == D2DHIP_PHYSICAL_IN_X) {
-    const long long nxc = nx / 2 + 1, nsp = nxc * ny * nz;
-    StageIo r = cols_view(0, nx, ny, nz);  // real lines of nx -> complex lines of nxc
-    r.out_s0 = nxc;
-    if ((st = make_stage(p, 3, 0, IO_R2C, r, nph, nsp))) return st;
-    if ((st = make_stage(p, 4, 1, IO_C2C, cols_view(1, nxc, ny, nz), nsp, nsp))) return st;
-    if ((st = make_stage(p, 5, 2, IO_C2C, cols_view(2, nxc, ny, nz), nsp, nsp))) return st;
-    StageIo c = cols_view(0, nx, ny, nz);
-    c.in_s0 = nxc;
-    if ((st = make_stage(p, 6, 0, IO_C2R, c, nsp, nph))) return st;
+    StageIo r = pencil_view(0, px);  // real lines of nx -> complex lines of nx/2+1
+    r.out_s0 = sx[0];
+    if ((st = make_stage(p, 3, 0, IO_R2C, r, pencil_elems(px), pencil_elems(sx)))) return st;
+    if ((st = make_stage(p, 4, 1, IO_C2C, pencil_view(1, sy), pencil_elems(sy), pencil_elems(sy))))
+      return st;
+    if ((st = make_stage(p, 5, 2, IO_C2C, pencil_view(2, sz), pencil_elems(sz), pencil_elems(sz))))
+      return st;
+    StageIo c = pencil_view(0, px);
+    c.in_s0 = sx[0];
+    if ((st = make_stage(p, 6, 0, IO_C2R, c, pencil_elems(sx), pencil_elems(px)))) return st;
   } else {
-    const long long nzc = nz / 2 + 1, nsp = nx * ny * nzc;
-    // real columns i + nx*j of the z axis; the complex output has the same
+    // real columns i + n1*j of the z axis; the complex output has the same
     // column and point strides
-    if ((st = make_stage(p, 3, 2, IO_R2C, cols_view(2, nx, ny, nz), nph, nsp))) return st;
-    if ((st = make_stage(p, 4, 1, IO_C2C, cols_view(1, nx, ny, nzc), nsp, nsp))) return st;
-    if ((st = make_stage(p, 5, 0, IO_C2C, cols_view(0, nx, ny, nzc), nsp, nsp))) return st;
-    if ((st = make_stage(p, 6, 2, IO_C2R, cols_view(2, nx, ny, nz), nsp, nph))) return st;
+    if ((st = make_stage(p, 3, 2, IO_R2C, pencil_view(2, pz), pencil_elems(pz), pencil_elems(sz))))
+      return st;
+    if ((st = make_stage(p, 4, 1, IO_C2C, pencil_view(1, sy), pencil_elems(sy), pencil_elems(sy))))
+      return st;
+    if ((st = make_stage(p, 5, 0, IO_C2C, pencil_view(0, sx), pencil_elems(sx), pencil_elems(sx))))
+      return st;
+    if ((st = make_stage(p, 6, 2, IO_C2R, pencil_view(2, pz), pencil_elems(sz), pencil_elems(pz))))
+      return st;
   }
   const size_t bytes = sizeof(AxisDesc) * d2dhip_plan_s::kStages;
   if (hipMalloc(&p->d_desc, bytes) != hipSuccess) {
@@ -1072,8 +1127,12 @@ int tune_chunks(d2dhip_plan_s *p) {
 
 extern "C" {
 
-int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec,
-                       unsigned flags, void *stream) {
+namespace {
+
+// Plan creation for one rank: the whole grid (ph == nullptr), or the
+// pencils of the rank whose physical and spectral decompositions are ph, sp.
+int plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec, unsigned flags,
+                void *stream, const d2dhip_decomp *ph, const d2dhip_decomp *sp) {
   if (!plan) return set_error(D2DHIP_ERR_NULL, "plan pointer is NULL");
   *plan = nullptr;
   if (nx <= 0) return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid value for nx");
@@ -1093,11 +1152,16 @@ int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, in
   p->rb = (prec == D2DHIP_PREC_F64) ? 8 : 4;
   p->flags = flags;
   p->stream = (hipStream_t)stream;
+  if (ph) {
+    p->ph = *ph;
+    p->sp = *sp;
+    p->multi = ph->p_row * ph->p_col > 1;
+  }
   int st = axis_kernel_prepare(p->rb);
   if (!st) st = make_stages(p);
   const bool tune = (flags & D2DHIP_FLAG_TUNE) || std::getenv("D2DHIP_TUNE");
   if (!st && tune) st = tune_stages(p);
-  if (!st) {
+  if (!st && !p->multi) {
     // D2DHIP_CHUNK_PLANES="f,b" fixes the plane chunks (0 = whole stages)
     const char *e = std::getenv("D2DHIP_CHUNK_PLANES");
     int f = 0, b = 0;
@@ -1109,7 +1173,7 @@ int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, in
       st = tune_chunks(p);
     }
   }
-  if (!st && !(flags & D2DHIP_FLAG_INPLACE)) {
+  if (!st && !p->multi && !(flags & D2DHIP_FLAG_INPLACE)) {
     const long long nsp = (format == D2DHIP_PHYSICAL_IN_X)
                               ? (long long)(nx / 2 + 1) * ny * nz
                               : (long long)nx * ny * (nz / 2 + 1);
@@ -1127,6 +1191,49 @@ int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, in
   if (p->tune_write_failed)
     return set_error(D2DHIP_WARN_TUNE_CACHE, "tuning choices could not be written to " + tune_path());
   return D2DHIP_SUCCESS;
+}
+
+}  // namespace
+
+int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec,
+                       unsigned flags, void *stream) {
+  return plan_create(plan, format, nx, ny, nz, prec, flags, stream, nullptr, nullptr);
+}
+
+// decomp_2d_fft_init on a p_row x p_col process grid, for one rank: the
+// physical decomposition of (nx, ny, nz) and the spectral one of
+// (nx/2+1, ny, nz) or (nx, ny, nz/2+1) (src/fft_common.f90:198-216).
+int d2dhip_plan_create_rank(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec,
+                            int p_row, int p_col, int coord_row, int coord_col, unsigned flags,
+                            void *stream) {
+  if (!plan) return set_error(D2DHIP_ERR_NULL, "plan pointer is NULL");
+  *plan = nullptr;
+  if (format != D2DHIP_PHYSICAL_IN_X && format != D2DHIP_PHYSICAL_IN_Z)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid value for pencil");
+  if (nx <= 0 || ny <= 0 || nz <= 0) return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid grid");
+  d2dhip_decomp ph, sp;
+  int st = d2dhip_decomp_info_init(nx, ny, nz, p_row, p_col, coord_row, coord_col, &ph);
+  if (!st)
+    st = format == D2DHIP_PHYSICAL_IN_X
+             ? d2dhip_decomp_info_init(nx / 2 + 1, ny, nz, p_row, p_col, coord_row, coord_col, &sp)
+             : d2dhip_decomp_info_init(nx, ny, nz / 2 + 1, p_row, p_col, coord_row, coord_col, &sp);
+  if (st) return st;
+  return plan_create(plan, format, nx, ny, nz, prec, flags, stream, &ph, &sp);
+}
+
+int d2dhip_plan_get_rank(d2dhip_plan p, d2dhip_decomp *ph, d2dhip_decomp *sp) {
+  if (!p || !ph || !sp) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (p->multi) {
+    *ph = p->ph;
+    *sp = p->sp;
+    return D2DHIP_SUCCESS;
+  }
+  int st = d2dhip_decomp_info_init(p->nx, p->ny, p->nz, 1, 1, 0, 0, ph);
+  if (!st)
+    st = p->format == D2DHIP_PHYSICAL_IN_X
+             ? d2dhip_decomp_info_init(p->nx / 2 + 1, p->ny, p->nz, 1, 1, 0, 0, sp)
+             : d2dhip_decomp_info_init(p->nx, p->ny, p->nz / 2 + 1, 1, 1, 0, 0, sp);
+  return st;
 }
 
 int d2dhip_plan_destroy(d2dhip_plan p) {
@@ -1148,6 +1255,9 @@ int d2dhip_plan_destroy(d2dhip_plan p) {
   if (p->d_cdesc) (void)hipFree(p->d_cdesc);
   if (p->d_pack) (void)hipFree(p->d_pack);
   if (p->d_pdesc) (void)hipFree(p->d_pdesc);
+  if (p->d_ypack) (void)hipFree(p->d_ypack);
+  if (p->d_ydesc) (void)hipFree(p->d_ydesc);
+  if (p->ywk) (void)hipFree(p->ywk);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->capturing) {
@@ -1270,6 +1380,8 @@ int d2dhip_plan_set_chunking(d2dhip_plan p, int fwd_planes, int bwd_planes) {
     return set_error(D2DHIP_ERR_UNSUPPORTED, "plane chunking is for PHYSICAL_IN_X plans");
   if (p->capturing)
     return set_error(D2DHIP_ERR_UNSUPPORTED, "plan is capturing: its chunking is fixed");
+  if (p->multi && (fwd_planes || bwd_planes))
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "plane chunking is for whole-grid plans");
   p->chunk_fwd = fwd_planes >= p->nz ? 0 : fwd_planes;
   p->chunk_bwd = bwd_planes >= p->nz ? 0 : bwd_planes;
   return upload_chunk_descs(p);
@@ -1295,6 +1407,10 @@ long long d2dhip_plan_device_bytes(d2dhip_plan p) {
 // (opt_skip_XYZ_c2c) are left out; the first stage that runs reads `in`.
 int d2dhip_exec_c2c(d2dhip_plan p, void *in, void *out, int isign) {
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (p->multi)
+    return set_error(D2DHIP_ERR_UNSUPPORTED,
+                     "plan of one rank of a process grid: the 3D transform needs the exchange "
+                     "between its stages (use d2dhip_exec_1m)");
   if (!in || !out) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
   if (isign != D2DHIP_FORWARD && isign != D2DHIP_BACKWARD)
     return set_error(D2DHIP_ERR_INVALID_VALUE, "isign must be -1 or 1");
@@ -1325,6 +1441,10 @@ int d2dhip_exec_c2c(d2dhip_plan p, void *in, void *out, int isign) {
 // then y, then z; PHYSICAL_IN_Z: r2c along z, then y, then x.
 int d2dhip_exec_r2c(d2dhip_plan p, const void *in_r, void *out_c) {
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (p->multi)
+    return set_error(D2DHIP_ERR_UNSUPPORTED,
+                     "plan of one rank of a process grid: the 3D transform needs the exchange "
+                     "between its stages (use d2dhip_exec_1m)");
   if (!in_r || !out_c) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
   const bool fx = p->format == D2DHIP_PHYSICAL_IN_X;
   // the r2c axis cannot be skipped (src/fft_generic.f90:222-223, :265-266)
@@ -1408,6 +1528,10 @@ int scratch_address(void **ptr) {
 
 int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (p->multi)
+    return set_error(D2DHIP_ERR_UNSUPPORTED,
+                     "plan of one rank of a process grid: the 3D transform needs the exchange "
+                     "between its stages (use d2dhip_exec_1m)");
   if (!in_c || !out_r) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
   return c2r_stages(p, in_c, out_r, (p->flags & D2DHIP_FLAG_INPLACE) ? in_c : p->wk);
 }
@@ -1416,6 +1540,10 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
 // no allocation of any kind, in_c preserved whatever the plan's flags.
 int d2dhip_exec_c2r_noalloc(d2dhip_plan p, void *in_c, void *out_r) {
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (p->multi)
+    return set_error(D2DHIP_ERR_UNSUPPORTED,
+                     "plan of one rank of a process grid: the 3D transform needs the exchange "
+                     "between its stages (use d2dhip_exec_1m)");
   if (!in_c || !out_r) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
   if (p->prof)
     return set_error(D2DHIP_ERR_INVALID_VALUE, "stage profiling records events: not with noalloc");
@@ -1439,6 +1567,10 @@ int d2dhip_exec_c2r_noalloc(d2dhip_plan p, void *in_c, void *out_r) {
 // values (types, ifirst, ndismiss, ofirst; 1-based), per axis x, y, z.
 int d2dhip_plan_set_dtt(d2dhip_plan p, int ndtt, const int *in_dtt) {
   if (!p || !in_dtt) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (p->multi)
+    return set_error(D2DHIP_ERR_UNSUPPORTED,
+                     "plan of one rank of a process grid: the 3D transform needs the exchange "
+                     "between its stages (use d2dhip_exec_1m)");
   if (p->capturing)
     return set_error(D2DHIP_ERR_UNSUPPORTED, "plan is capturing: DTT set-up allocates");
   if (ndtt != 3 && ndtt != 12) return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid argument");
@@ -1546,7 +1678,7 @@ int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
   if (p->format != D2DHIP_PHYSICAL_IN_X)
     return set_error(D2DHIP_ERR_UNSUPPORTED, "fused pack / unpack needs a PHYSICAL_IN_X plan");
   const long long nxc = p->nx / 2 + 1;
-  if (d->xsz[0] != nxc || d->xsz[1] != p->ny || d->xsz[2] != p->nz)
+  if (d->xsz[0] != p->pen[3][0] || d->xsz[1] != p->pen[3][1] || d->xsz[2] != p->pen[3][2])
     return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition x-pencil is not the plan's spectral x-pencil");
   const int np = d->p_row;
   if (np < 1 || np > D2DHIP_MAX_PROCS) return set_error(D2DHIP_ERR_INVALID_VALUE, "p_row out of range");
@@ -1584,7 +1716,135 @@ int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
                                     p->shape[idx], p->stream);
 }
 
+// The y stage of a plan (1: on the physical y-pencil, 4: on the spectral
+// one) with its output in the send order of exchange dir (D2DHIP_Y_TO_Z:
+// mem_split_yz, src/transpose_y_to_z.f90:370-428; D2DHIP_Y_TO_X:
+// mem_split_yx, src/transpose_y_to_x.f90:266-323) or its input in the
+// receive order of dir (D2DHIP_Z_TO_Y: mem_merge_zy,
+// src/transpose_z_to_y.f90:489-547; D2DHIP_X_TO_Y: mem_merge_xy,
+// src/transpose_x_to_y.f90:384-441).  All four keep natural order inside
+// blocks of the y axis: point (i, j, k) of block m (b1 <= j < b1 + w) at
+// disp_m + i + (j - b1) n1 + k w n1 (disp_m = m * count under EVEN).  The
+// stage kernel's fused form (" ypk" / " yun") or the general kernel applies
+// that map in its stores / loads; a specialised kernel without a fused form
+// runs unfused through a y-pencil workspace.  Results equal the stage
+// followed by d2dhip_pack (d2dhip_unpack followed by the stage) bit for bit.
+int run_y_stage_packed(d2dhip_plan_s *p, int stage, const void *in, void *out, int dir,
+                       const d2dhip_decomp *d, int even, int isign, bool pack) {
+  if (p->capturing)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "fused pack / unpack calls are not captured into a graph");
+  if (stage != 1 && stage != 4)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "fused y-pencil pack / unpack: stage 1 or 4");
+  if (isign != D2DHIP_FORWARD && isign != D2DHIP_BACKWARD)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid value of isign");
+  const bool two = pack ? dir == D2DHIP_Y_TO_Z : dir == D2DHIP_Z_TO_Y;
+  const bool one = pack ? dir == D2DHIP_Y_TO_X : dir == D2DHIP_X_TO_Y;
+  if (!two && !one)
+    return set_error(D2DHIP_ERR_INVALID_VALUE,
+                     pack ? "pack after a y stage: D2DHIP_Y_TO_Z or D2DHIP_Y_TO_X"
+                          : "unpack before a y stage: D2DHIP_Z_TO_Y or D2DHIP_X_TO_Y");
+  const auto &pen = p->pen[stage];
+  if (d->ysz[0] != pen[0] || d->ysz[1] != pen[1] || d->ysz[2] != pen[2])
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition y-pencil is not the stage's pencil");
+  const int np = two ? d->p_col : d->p_row;
+  if (np < 1 || np > D2DHIP_MAX_PROCS) return set_error(D2DHIP_ERR_INVALID_VALUE, "process count out of range");
+  const int *dist = two ? d->y2dist : d->y1dist;
+  const long long *disp = two ? d->y2disp : d->y1disp;
+  const long long count = two ? d->y2count : d->y1count;
+  const long long n1 = pen[0], ny = pen[1];
+  long long b1 = 0;
+  for (int m = 0; m < np; ++m) {
+    if (dist[m] < 0) return set_error(D2DHIP_ERR_INVALID_VALUE, "negative block size");
+    p->h_ypack[3 * m] = b1;
+    p->h_ypack[3 * m + 1] = n1 * (dist[m] - ny);
+    p->h_ypack[3 * m + 2] = (even ? (long long)m * count : disp[m]) - b1 * n1;
+    b1 += dist[m];
+  }
+  if (b1 != ny) return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition does not match the pencil");
+  const int conj = isign == D2DHIP_BACKWARD ? 1 : 0;
+  const int esize = 2 * p->rb;
+  // the specialised kernel's fused form, if it has one
+  const FastVariant *v = p->fast[stage], *twin = nullptr;
+  if (v) {
+    std::string want = v->name;
+    const size_t at = want.find(" c2c");
+    if (at != std::string::npos) {
+      want.replace(at, 4, pack ? " ypk" : " yun");
+      for (const FastVariant &c : fast_registry())
+        if (c.rb == p->rb && c.io == (pack ? IO_YPK : IO_YUN) && want == c.name) twin = &c;
+    }
+  }
+  int st;
+  if (v && !twin) {  // stage then pack (unpack then stage) through a workspace
+    const size_t need = (size_t)pencil_elems(pen) * esize;
+    if (p->ywk_bytes < need) {
+      if (p->ywk) (void)hipFree(p->ywk);
+      p->ywk = nullptr;
+      p->ywk_bytes = 0;
+      if (hipMalloc(&p->ywk, need) != hipSuccess) return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(y-pencil workspace)");
+      p->ywk_bytes = need;
+    }
+    if (pack) {
+      if ((st = run(p, stage, in, p->ywk, conj))) return st;
+      return launch_pack(dir, 1, esize, p->ywk, out, d, even, p->stream);
+    }
+    if ((st = launch_pack(dir, 0, esize, in, p->ywk, d, even, p->stream))) return st;
+    return run(p, stage, p->ywk, out, conj);
+  }
+  if (!p->d_ypack && hipMalloc(&p->d_ypack, sizeof(p->h_ypack)) != hipSuccess) {
+    p->d_ypack = nullptr;
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(y pack table)");
+  }
+  if (!p->d_ydesc && hipMalloc(&p->d_ydesc, sizeof(AxisDesc)) != hipSuccess) {
+    p->d_ydesc = nullptr;
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(y pack descriptor)");
+  }
+  p->h_ydesc = p->h_desc[stage];
+  p->h_ydesc.pack = p->d_ypack;
+  p->h_ydesc.npack = np;
+  p->h_ydesc.packj = pack ? 1 : 2;
+  st = upload_sync(p, p->d_ypack, p->h_ypack, sizeof(long long) * 3 * np, "hipMemcpy(y pack table)");
+  if (!st) st = upload_sync(p, p->d_ydesc, &p->h_ydesc, sizeof(AxisDesc), "hipMemcpy(y pack descriptor)");
+  if (st) return st;
+  const AxisIo io = {in, out, conj};
+  if (twin) {
+    if ((st = hip_check(twin->prepare(), "hipFuncSetAttribute(fast kernel)"))) return st;
+    return twin->launch(p->d_ydesc, p->h_ydesc, io, p->stream);
+  }
+  return launch_axis(p->rb, IO_C2C, p->d_ydesc, p->h_ydesc, io, p->shape[stage], p->stream);
+}
+
 }  // namespace
+
+// One 1D stage of a plan on its pencil: c2c_1m_x / _y / _z, r2c_1m_x / _z,
+// c2r_1m_x / _z (src/fft_generic.f90:112-384, FFTW plans
+// src/fft_fftw3_f03.f90:1414-1552).
+int d2dhip_exec_1m(d2dhip_plan p, int stage, const void *in, void *out, int isign) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  if (!in || !out) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
+  if (stage < 0 || stage >= d2dhip_plan_s::kFftStages)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "stage out of range (0-6)");
+  if (isign != D2DHIP_FORWARD && isign != D2DHIP_BACKWARD)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid value of isign");
+  if ((stage == 3 && isign != D2DHIP_FORWARD) || (stage == 6 && isign != D2DHIP_BACKWARD))
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "r2c is forward, c2r backward");
+  if ((stage == 3 || stage == 6) && in == out)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "r2c / c2r stages are not in place");
+  const int conj = stage == 6 ? 1 : (stage == 3 ? 0 : (isign == D2DHIP_BACKWARD ? 1 : 0));
+  return run(p, stage, in, out, conj);
+}
+
+int d2dhip_exec_1m_pack(d2dhip_plan p, int stage, const void *in, void *send, int dir,
+                        const d2dhip_decomp *d, int even, int isign) {
+  if (!p || !in || !send || !d) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  return run_y_stage_packed(p, stage, in, send, dir, d, even, isign, true);
+}
+
+int d2dhip_exec_1m_unpack(d2dhip_plan p, int stage, const void *recv, void *out, int dir,
+                          const d2dhip_decomp *d, int even, int isign) {
+  if (!p || !recv || !out || !d) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  return run_y_stage_packed(p, stage, recv, out, dir, d, even, isign, false);
+}
 
 // ---- hipGraph of a plan's exec calls --------------------------------------------
 // Between capture_begin and capture_end the plan launches on a capturing
@@ -1789,6 +2049,7 @@ int d2dhip_batch_create(d2dhip_batch *out, int nplans, const d2dhip_plan *plans,
   for (int i = 0; i < nplans && !st; ++i) {
     d2dhip_plan_s *p = plans[i];
     if (!p) st = set_error(D2DHIP_ERR_NULL, "NULL plan in batch");
+    else if (p->multi) st = set_error(D2DHIP_ERR_UNSUPPORTED, "batches take whole-grid plans");
     else if (i > 0 && p->rb != b->rb) st = set_error(D2DHIP_ERR_INVALID_VALUE, "mixed precisions in batch");
     if (st) break;
     b->rb = p->rb;

@@ -235,6 +235,53 @@ int d2dhip_exec_r2c_pack(d2dhip_plan plan, const void *in_r, void *send,
 int d2dhip_exec_c2r_unpack(d2dhip_plan plan, const void *recv, void *out_r,
                            const d2dhip_decomp *d, int even);
 
+/* ---- one rank of a process grid ------------------------------------------
+ * d2dhip_plan_create_rank is decomp_2d_fft_init for rank (coord_row,
+ * coord_col) of a p_row x p_col grid (src/fft_common.f90:141-242 after
+ * decomp_2d_init, src/decomp_2d_init_fin.f90:15-184): its stages run on the
+ * rank's pencils of the physical decomposition of (nx, ny, nz) and the
+ * spectral one of (nx/2+1, ny, nz) (PHYSICAL_IN_X) or (nx, ny, nz/2+1)
+ * (PHYSICAL_IN_Z), as decomp_info_init gives them.  With p_row*p_col > 1
+ * the 3D calls (exec_c2c / r2c / c2r / dtt) return D2DHIP_ERR_UNSUPPORTED:
+ * between their stages the reference exchanges pencils over MPI
+ * (src/fft_common_3d.f90:43-68, 143-159); the caller runs the stages one
+ * at a time with d2dhip_exec_1m and moves the alltoallv buffers that
+ * d2dhip_pack / d2dhip_unpack and the fused calls below write and read. */
+int d2dhip_plan_create_rank(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec,
+                            int p_row, int p_col, int coord_row, int coord_col, unsigned flags,
+                            void *stream);
+/* The plan's physical and spectral decompositions (one rank: the whole
+ * grid). */
+int d2dhip_plan_get_rank(d2dhip_plan plan, d2dhip_decomp *ph, d2dhip_decomp *sp);
+/* Stage slots of d2dhip_exec_1m, as d2dhip_plan_profile numbers them. */
+#define D2DHIP_STAGE_C2C_X 0     /* c2c_1m_x on the physical x-pencil */
+#define D2DHIP_STAGE_C2C_Y 1     /* c2c_1m_y on the physical y-pencil */
+#define D2DHIP_STAGE_C2C_Z 2     /* c2c_1m_z on the physical z-pencil */
+#define D2DHIP_STAGE_R2C 3       /* r2c_1m_x (X) / r2c_1m_z (Z): physical -> spectral pencil */
+#define D2DHIP_STAGE_SPEC_Y 4    /* c2c_1m_y on the spectral y-pencil */
+#define D2DHIP_STAGE_SPEC_LAST 5 /* c2c_1m_z (X) / c2c_1m_x (Z) on the spectral pencil */
+#define D2DHIP_STAGE_C2R 6       /* c2r_1m_x (X) / c2r_1m_z (Z): spectral -> physical pencil */
+/* One 1D stage on the plan's pencil: c2c_1m_x / _y / _z (isign -1 forward,
+ * +1 backward, unnormalised; in == out allowed), r2c_1m_x / _z (isign -1),
+ * c2r_1m_x / _z (isign +1) -- src/fft_generic.f90:112-384, FFTW execs
+ * src/fft_fftw3_f03.f90:1414-1552. */
+int d2dhip_exec_1m(d2dhip_plan plan, int stage, const void *in, void *out, int isign);
+/* A y stage (D2DHIP_STAGE_C2C_Y or _SPEC_Y) fused with the pack of the
+ * exchange that follows it (dir D2DHIP_Y_TO_Z: mem_split_yz,
+ * src/transpose_y_to_z.f90:370-428; D2DHIP_Y_TO_X: mem_split_yx,
+ * src/transpose_y_to_x.f90:266-323), or with the unpack of the exchange
+ * before it (dir D2DHIP_Z_TO_Y: mem_merge_zy,
+ * src/transpose_z_to_y.f90:489-547; D2DHIP_X_TO_Y: mem_merge_xy,
+ * src/transpose_x_to_y.f90:384-441).  d->ysz must be the stage's pencil;
+ * even selects the padded offsets.  Results equal d2dhip_exec_1m followed
+ * by d2dhip_pack (d2dhip_unpack then d2dhip_exec_1m) bit for bit.  A stage
+ * kernel without a fused form runs unfused through a workspace the plan
+ * allocates on first use. */
+int d2dhip_exec_1m_pack(d2dhip_plan plan, int stage, const void *in, void *send, int dir,
+                        const d2dhip_decomp *d, int even, int isign);
+int d2dhip_exec_1m_unpack(d2dhip_plan plan, int stage, const void *recv, void *out, int dir,
+                          const d2dhip_decomp *d, int even, int isign);
+
 /* ---- many grids in one launch ---------------------------------------------
  * The multiple-grid use of the reference (decomp_2d_fft_set_ngrid /
  * decomp_2d_fft_use_grid, src/fft_common.f90:381-443, exercised by
