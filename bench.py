@@ -143,7 +143,7 @@ def config_summary(line):
     keep = ("ms_per_step", "GBps_model", "hbm_frac_model", "GBps_3pass", "hbm_frac_3pass",
             "round_trip_rel_l2", "round_trip_rel_l2_max", "graph_ms", "sequential_ms",
             "persistent_ms", "persistent_graph_ms", "speedup_vs_sequential", "grouped_graph_ms",
-            "executor", "value", "unit")
+            "executor", "value", "unit", "unfused_ms", "fused_speedup")
     out = {"workload": line["config"]["workload"]}
     out.update({k: line[k] for k in keep if k in line})
     return out
@@ -235,6 +235,7 @@ def bench_config(args, torch, d2d, w):
     inputs seed 0); r2c+c2r 512^3 PHYSICAL_IN_X (modal field seed 1); r2c+c2r
     720x540x432 PHYSICAL_IN_Z (uniform seed 2); the mem_split_xy pack of one
     rank of 2048^3 on a 4x4 grid (x-pencil 2048x512x512 real8, seed 2)."""
+    import numpy as np
 
     def timed(fn):
         for _ in range(args.warmup):
@@ -372,6 +373,59 @@ def bench_config(args, torch, d2d, w):
         y32v = y32.view(n, n, n)[core].double() / float(1024 ** 3)
         extra["f32_round_trip_rel_l2"] = float(torch.linalg.vector_norm(y32v - xv) /
                                                torch.linalg.vector_norm(xv))
+    elif w == "rank1024":
+        # rank (1, 1) of 1024^3 on a 4 x 2 grid: the forward r2c's per-rank
+        # stages and the packs / unpacks of its two exchanges (the exchanges
+        # themselves need other GPUs; each receive buffer here is the send
+        # buffer), unfused against the x and y stages fused with their packs
+        n = 1024
+        e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, rank=(4, 2, 1, 1), tune=not args.no_tune)
+        ph, sp = e.decomps()
+        # the y stage on a kernel with a fused pack form (or the general kernel)
+        fused_forms = ("double n=1024 tx=16 nt=1024 c2c xs w4", "double n=1024 tx=8 nt=512 c2c xs w4",
+                       "double n=1024 tx=8 nt=512 c2c", "general")
+        if e.stage_kernels()["spec_y"] not in fused_forms:
+            e.set_stage_kernel("spec_y", fused_forms[0])
+        nel = lambda v: int(np.prod(list(v)))
+        x = torch.empty(nel(ph.xsz), dtype=torch.float64, device="cuda")
+        d2d.fill_uniform(x, x.numel(), d2d.R8, 3)
+        sx = torch.empty(nel(sp.xsz), dtype=torch.complex128, device="cuda")
+        sy = torch.empty(nel(sp.ysz), dtype=torch.complex128, device="cuda")
+        sz = torch.empty(nel(sp.zsz), dtype=torch.complex128, device="cuda")
+        b1 = torch.empty(d2d.pack_buffer_elems(d2d.X_TO_Y, sp), dtype=torch.complex128, device="cuda")
+        b2 = torch.empty(max(d2d.pack_buffer_elems(d2d.Y_TO_Z, sp), nel(sp.zsz)),
+                         dtype=torch.complex128, device="cuda")
+        C16 = d2d.C16
+
+        def unfused():
+            e.exec_1m("r2c", x, sx, d2d.FORWARD)
+            d2d.pack(d2d.X_TO_Y, C16, sx, b1, sp)
+            d2d.unpack(d2d.X_TO_Y, C16, b1, sy, sp)
+            e.exec_1m("spec_y", sy, sy, d2d.FORWARD)
+            d2d.pack(d2d.Y_TO_Z, C16, sy, b2, sp)
+            d2d.unpack(d2d.Y_TO_Z, C16, b2, sz, sp)
+            e.exec_1m("spec_last", sz, sz, d2d.FORWARD)
+
+        def step():
+            e.fft_r2c_pack(x, b1, sp)
+            d2d.unpack(d2d.X_TO_Y, C16, b1, sy, sp)
+            e.exec_1m_pack("spec_y", sy, b2, d2d.Y_TO_Z, sp, d2d.FORWARD)
+            d2d.unpack(d2d.Y_TO_Z, C16, b2, sz, sp)
+            e.exec_1m("spec_last", sz, sz, d2d.FORWARD)
+        ms = timed(step)
+        ms_unfused = timed(unfused)
+        rs, cs = x.numel() * 8, sx.numel() * 16
+        # fused: r2c_pack (real + spectrum), two unpacks, y_pack and the z stage
+        nbytes = (rs + cs) + 2 * cs + 2 * cs + 2 * cs + 2 * cs
+        err = 0.0
+        cfg = {"workload": "forward r2c stages of rank (1,1) of 1024^3 on 4x2 (x-pencil "
+                           "1024x256x512 real8, spectral y-pencil 128x1024x512) with the packs and "
+                           "unpacks of its two exchanges, x and y stages fused with their packs; "
+                           "the exchanges themselves omitted (one GPU)"}
+        extra["unfused_ms"] = round(ms_unfused, 4)
+        extra["fused_speedup"] = round(ms_unfused / ms, 3)
+        extra["kernels"] = {k: v for k, v in e.stage_kernels().items() if k in ("r2c", "spec_y",
+                                                                               "spec_last")}
     elif w == "halo512":
         # update_halo of a 512^3 real8 x-pencil, 2 halo levels, y and z periodic
         n, lev = 512, 2
@@ -428,7 +482,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan-time autotuner (use the default kernels)")
     ap.add_argument("--workload", choices=["r2c1024", "batch64", "c2c128", "r2c512", "r2cz720",
-                                           "pack2048", "r2cpack2048", "dtt513", "halo512"],
+                                           "pack2048", "r2cpack2048", "dtt513", "halo512",
+                                           "rank1024"],
                     default="r2c1024",
                     help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids; "
                          "c2c128 / r2c512 / r2cz720 / pack2048: the other parity configurations, "
@@ -449,7 +504,7 @@ def main():
         print(json.dumps(bench_batch(args, torch, d2d)), flush=True)
         return
     if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048", "dtt513",
-                         "halo512"):
+                         "halo512", "rank1024"):
         print(json.dumps(bench_config(args, torch, d2d, args.workload)), flush=True)
         return
     n = args.n
@@ -604,7 +659,8 @@ def main():
         sub.steps, sub.warmup = min(args.steps, 10), min(args.warmup, 3)
         cfgs = {}
         for key, w in (("C1_c2c128", "c2c128"), ("C2_r2c512", "r2c512"),
-                       ("C3a_r2cz720", "r2cz720"), ("C3b_pack2048", "pack2048")):
+                       ("C3a_r2cz720", "r2cz720"), ("C3b_pack2048", "pack2048"),
+                       ("C4_rank_4x2_fused", "rank1024")):
             cfgs[key] = config_summary(bench_config(sub, torch, d2d, w))
             torch.cuda.empty_cache()
         for prec in (64, 32):
