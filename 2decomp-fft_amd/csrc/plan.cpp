@@ -2385,7 +2385,10 @@ int d2dhip_batch_launch(d2dhip_batch b) {
   if (!b) return set_error(D2DHIP_ERR_NULL, "batch is NULL");
   if (!b->bound) return set_error(D2DHIP_ERR_INVALID_VALUE, "batch not bound");
   if (b->ngroups == 0) return D2DHIP_SUCCESS;
-  if (b->mode == D2DHIP_BATCH_GROUPED) {
+  // the default executor runs grouped launches when some length has no
+  // persistent tile: the general kernel's tiles in one persistent launch
+  // are about 3x slower than the plans' own kernels
+  if (b->mode == D2DHIP_BATCH_GROUPED || (b->mode == D2DHIP_BATCH_PERSISTENT && !b->fast)) {
     for (size_t g = 0; g < b->plans.size(); ++g)
       for (int a = 0; a < 3; ++a)
         if (b->plans[g]->fast[a] != b->bound_fast[3 * g + a])

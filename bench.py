@@ -143,13 +143,14 @@ def config_summary(line):
     keep = ("ms_per_step", "GBps_model", "hbm_frac_model", "GBps_3pass", "hbm_frac_3pass",
             "round_trip_rel_l2", "round_trip_rel_l2_max", "graph_ms", "sequential_ms",
             "persistent_ms", "persistent_graph_ms", "speedup_vs_sequential", "grouped_graph_ms",
-            "executor", "value", "unit", "unfused_ms", "fused_speedup")
+            "executor", "value", "unit", "unfused_ms", "fused_speedup",
+            "persistent_general_graph_ms")
     out = {"workload": line["config"]["workload"]}
     out.update({k: line[k] for k in keep if k in line})
     return out
 
 
-def bench_batch(args, torch, d2d, batch_prec=None):
+def bench_batch(args, torch, d2d, batch_prec=None, mixed=False):
     """64 independent c2c grids, each dimension from {32,...,256} (seed 4),
     inputs uniform[-1,1): forward+backward of every grid as 64 sequential
     decomp_2d_fft_3d pairs, and as one batch with either executor (one launch
@@ -162,6 +163,8 @@ def bench_batch(args, torch, d2d, batch_prec=None):
     cdt = torch.complex128 if prec == d2d.PREC_F64 else torch.complex64
     elem = d2d.C16 if prec == d2d.PREC_F64 else d2d.C8
     shapes = [tuple(int(v) for v in rng.choice(sizes, 3)) for _ in range(64)]
+    if mixed:  # the reference's multi-grid shapes (fft_multiple_grids.f90:51-54) among them
+        shapes[:4] = [(17, 13, 11), (17, 13, 11), (17, 15, 27), (33, 15, 11)]
     engines = [d2d.FFTEngine(d2d.PHYSICAL_IN_X, *s, prec=prec) for s in shapes]
     ins, outs = [], []
     for g, s in enumerate(shapes):
@@ -207,6 +210,35 @@ def bench_batch(args, torch, d2d, batch_prec=None):
              for a, o, s in zip(ins, outs, shapes))
     rb = 16 if prec == d2d.PREC_F64 else 8
     nbytes = sum(2 * 6 * int(np.prod(s)) * rb for s in shapes)  # 3-pass model, both directions
+    if mixed:  # no persistent tiles for 11..33: the default executor runs grouped launches
+        assert not batch.persistent_tiles()
+        batch.set_mode(d2d.BATCH_PERSISTENT)
+        batch.capture_graph()
+        ms_default = timed(batch.graph_launch)
+        batch.check()
+        batch.set_mode(d2d.BATCH_PERSISTENT_GENERAL)
+        batch.capture_graph()
+        ms_general = timed(batch.graph_launch)
+        batch.check()
+        rb = 16 if prec == d2d.PREC_F64 else 8
+        nbytes = sum(2 * 6 * int(np.prod(s)) * rb for s in shapes)
+        return {
+            "metric": "grids/s, 64 c2c grids fwd+bwd incl. 17x13x11, 17x15x27, 33x15x11 "
+                      "(default executor, hipGraph replay)",
+            "value": round(64 / (ms_default * 1e-3), 1), "unit": "grids/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_default, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64" if prec == d2d.PREC_F64 else "f32", "data": "synthetic",
+            "config": {"workload": "64 c2c grids: the reference's 3 multi-grid shapes + 60 with "
+                                   "dims from {32..256} seed 4, fwd+bwd"},
+            "executor": "grouped (no persistent tile for lengths 11-33)",
+            "sequential_ms": round(ms_seq, 4), "persistent_general_graph_ms": round(ms_general, 4),
+            "speedup_vs_sequential": round(ms_seq / ms_default, 3),
+            "GBps_3pass": round(nbytes / (ms_default * 1e-3) / 1e9, 1),
+            "round_trip_rel_l2_max": max(
+                float(torch.linalg.vector_norm(o / float(np.prod(s)) - a) / torch.linalg.vector_norm(a))
+                for a, o, s in zip(ins, outs, shapes)),
+        }
     line = {
         "metric": "grids/s, 64 uneven c2c grids fwd+bwd (one persistent launch, hipGraph replay)",
         "value": round(64 / (ms_graph * 1e-3), 1), "unit": "grids/s", "n_gpus": 1,
@@ -666,6 +698,9 @@ def main():
         for prec in (64, 32):
             cfgs["C5_batch64_f%d" % prec] = config_summary(bench_batch(sub, torch, d2d, prec))
             torch.cuda.empty_cache()
+        cfgs["C5_mixed_with_reference_grids_f64"] = config_summary(
+            bench_batch(sub, torch, d2d, 64, mixed=True))
+        torch.cuda.empty_cache()
         line["configs"] = cfgs
     if not args.no_cpu:
         try:

@@ -310,13 +310,15 @@ int d2dhip_batch_graph_launch(d2dhip_batch batch);
 int d2dhip_batch_check(d2dhip_batch batch);
 int d2dhip_batch_set_stream(d2dhip_batch batch, void *stream);
 /* Executor of a batch: D2DHIP_BATCH_PERSISTENT (default) is the single
- * persistent launch.  When every c2c stage of every plan has a length with a
- * persistent tile (32, 48, 64, 80, 96, 128, 160, 192, 256), its workgroups
+ * persistent launch when every c2c stage of every plan has a length with a
+ * persistent tile (32, 48, 64, 80, 96, 128, 160, 192, 256): its workgroups
  * run those specialised tiles (d2dhip_batch_persistent_tiles reports 1), and
  * results are bit-identical to the plans' calls on any specialised kernel of
- * each length; otherwise, and always with D2DHIP_BATCH_PERSISTENT_GENERAL,
- * they run the general kernel's tiles, bit-identical to the plans' calls
- * with every stage on the general kernel.
+ * each length.  Otherwise the default runs as D2DHIP_BATCH_GROUPED (one
+ * persistent launch on general-kernel tiles is about 3x slower than the
+ * plans' own kernels).  D2DHIP_BATCH_PERSISTENT_GENERAL forces the
+ * persistent launch on the general kernel's tiles, bit-identical to the
+ * plans' calls with every stage on the general kernel.
  * D2DHIP_BATCH_GROUPED runs, per stage phase, one ordinary launch per
  * distinct stage kernel of the grids' own plans, covering the tiles of every
  * grid that runs that kernel in that phase (the kernel boundaries order the

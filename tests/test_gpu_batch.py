@@ -209,6 +209,44 @@ def test_config5_64_uneven_grids_fp32(d2d, torch_gpu):
             assert torch.equal(o, q), (mode, s)
 
 
+def test_default_executor_on_lengths_without_tiles(d2d, torch_gpu):
+    """A batch with the reference's multi-grid shapes (fft_multiple_grids.f90:51-54:
+    17x13x11, 17x15x27, 33x15x11) among specialised ones has no persistent
+    tiles: the default executor then runs the plans' own kernels in grouped
+    launches, bit-identical to the sequential calls, eager and replayed."""
+    torch = torch_gpu
+    shapes = [(17, 13, 11), (17, 15, 27), (33, 15, 11)] + grids(9, 5, SIZES)
+    engines = [d2d.FFTEngine(1, *s) for s in shapes]
+    ins, outs = [], []
+    for g, s in enumerate(shapes):
+        a = torch.empty(int(np.prod(s)), dtype=torch.complex128, device="cuda")
+        d2d.fill_uniform(a, a.numel(), d2d.C16, 60 + g)
+        ins.append(a)
+        outs.append(torch.empty_like(a))
+    seq = [torch.empty_like(a) for a in ins]
+    for e, a, q in zip(engines, ins, seq):
+        e.fft_c2c(a, q, -1)
+        e.fft_c2c(q, q, 1)
+    b = d2d.Batch(engines)
+    assert not b.persistent_tiles()
+    b.bind(ins, outs, mode=0)
+    b.launch()
+    b.check()
+    for o, q in zip(outs, seq):
+        assert torch.equal(o, q)
+    b.capture_graph()
+    for rep in range(3):
+        for o in outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        b.graph_launch()
+        b.check()
+        for o, q in zip(outs, seq):
+            assert torch.equal(o, q), rep
+    x = ins[1].cpu().numpy().reshape(shapes[1], order="F")
+    assert rel_l2(outs[1].cpu().numpy().reshape(shapes[1], order="F") / x.size, x) < 1e-13
+
+
 def test_grouped_rejects_changed_plan(d2d, torch_gpu):
     """A plan whose stage kernel changes after bind must be bound again."""
     torch = torch_gpu
