@@ -25,6 +25,71 @@
 
 namespace d2dhip {
 
+// Cross-lane word swaps of gfx950 (v_permlane32_swap_b32,
+// v_permlane16_swap_b32), on every 32-bit word of a complex value:
+// swap32 trades lanes 32-63 of a for lanes 0-31 of b; swap16 trades the odd
+// 16-lane rows of a (lanes 16-31, 48-63) for the even rows of b (0-15,
+// 32-47).  Register moves only, no LDS.
+template <typename T>
+__device__ __forceinline__ void swap32(cpx<T> &a, cpx<T> &b) {
+  constexpr int W = sizeof(cpx<T>) / 4;
+  unsigned wa[W], wb[W];
+  __builtin_memcpy(wa, &a, sizeof(a));
+  __builtin_memcpy(wb, &b, sizeof(b));
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const auto r = __builtin_amdgcn_permlane32_swap(wa[i], wb[i], false, false);
+    wa[i] = r[0];
+    wb[i] = r[1];
+  }
+  __builtin_memcpy(&a, wa, sizeof(a));
+  __builtin_memcpy(&b, wb, sizeof(b));
+}
+template <typename T>
+__device__ __forceinline__ void swap16(cpx<T> &a, cpx<T> &b) {
+  constexpr int W = sizeof(cpx<T>) / 4;
+  unsigned wa[W], wb[W];
+  __builtin_memcpy(wa, &a, sizeof(a));
+  __builtin_memcpy(wb, &b, sizeof(b));
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const auto r = __builtin_amdgcn_permlane16_swap(wa[i], wb[i], false, false);
+    wa[i] = r[0];
+    wb[i] = r[1];
+  }
+  __builtin_memcpy(&a, wa, sizeof(a));
+  __builtin_memcpy(&b, wb, sizeof(b));
+}
+
+// The four lanes l, l+16, l+32, l+48 (rows 0-3 of the wave) each hold 16
+// values as four blocks of 4; afterwards row r holds block r of every row,
+// row g's block in its register block g: u[4g + i] = (row g's v)[4r + i].
+// Two rounds of swaps, 8 value pairs each, the same register pattern in
+// every lane.
+// Then value q of block s is input s of butterfly q: reordered to v[4q + s].
+template <typename T, int E>
+__device__ __forceinline__ void block_transpose4(cpx<T> (&v)[E]) {
+  if constexpr (E == 16) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      swap32(v[i], v[8 + i]);       // rows 2,3 of block 0 <-> rows 0,1 of block 2
+      swap32(v[4 + i], v[12 + i]);  // rows 2,3 of block 1 <-> rows 0,1 of block 3
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      swap16(v[i], v[4 + i]);       // odd rows of block 0 <-> even rows of block 1
+      swap16(v[8 + i], v[12 + i]);  // odd rows of block 2 <-> even rows of block 3
+    }
+    cpx<T> u[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) u[4 * q + s] = v[4 * s + q];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = u[i];
+  }
+}
+
 // WPE: minimum waves per SIMD the register allocation must allow (1 = no
 // constraint beyond the block size).
 // One tile of a stage (tile < tiles0 * C1), by the whole workgroup.
@@ -56,6 +121,22 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
       ((IO == IO_C2R || DTT) && PAIRED ? ca : c0) * d->out_s0 + c1 * d->out_s1 + f * d->out_sj;
   const void *tw = d->tw;
   cpx<T> v[E];
+
+  // XM = 2 (columns of 1024 = 16 x 16 x 4, 16 values per thread): the first
+  // exchange as XM = 1, the last inside the wave.  The radix-4 butterfly
+  // b = k + 16 s' (k < 16, s' < 16) takes output s' of the radix-16
+  // butterflies k + 16 g, g < 4; those four run in lanes l, l+16, l+32,
+  // l+48 of one wave (rows gq = 0..3; k from the wave and the row within
+  // its quarter, spread so that the pass-1 LDS reads hit distinct banks), and
+  // block_transpose4 hands each its four butterflies 16 (4 gq + q) + k.
+  constexpr bool SHUF = XM == 2;
+  constexpr int HQ = SHUF ? 64 / TX / 4 : 1;  // tile rows per quarter wave
+  static_assert(!SHUF || (!LINES && P == 3 && RL::r[0] == 16 && RL::r[1] == 16 && RL::r[2] == 4 &&
+                          RPT == 64 && (HQ == 1 || HQ == 2 || HQ == 4) &&
+                          (IO == IO_C2C || IO == IO_YPK || IO == IO_YUN)),
+                "in-wave last exchange: c2c columns of 1024 points, 4 <= TX <= 16");
+  const int wq = tid >> 6, rq = (tid & 63) / TX;
+  const int gq = rq / HQ, kq = (wq & 3) + 4 * HQ * (wq >> 2) + 4 * (rq % HQ);
 
   // ---- inputs of pass 0 -------------------------------------------------
   if constexpr (IO == IO_DTT4) {
@@ -283,7 +364,9 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   static_for<0, P>([&](auto pc) __attribute__((always_inline)) {
     constexpr int p = decltype(pc)::value;
     constexpr int R = RL::r[p], NBR = N / R, L = RL::L(p);
-    if constexpr (p > 0) {
+    if constexpr (SHUF && p == P - 1) {
+      block_transpose4(v);  // radix-16 outputs -> radix-4 inputs, in the wave
+    } else if constexpr (p > 0) {
       // outputs of pass p-1 -> LDS -> inputs of pass p
       constexpr int Rq = RL::r[p - 1], NBq = N / Rq, Lq = RL::L(p - 1);
       if constexpr (p > 1 || IO == IO_C2R || IO == IO_DTT3) __syncthreads();
@@ -328,7 +411,7 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
           __syncthreads();
 #pragma unroll
           for (int q = 0; q < C::nb(p); ++q) {
-            const int b = r0 + q * RPT;
+            const int b = (SHUF && p == P - 2) ? kq + 16 * gq : r0 + q * RPT;
             if (C::exact(p) || b < NBR) {
 #pragma unroll
               for (int s = 0; s < R; ++s) {
@@ -342,7 +425,7 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
     }
 #pragma unroll
     for (int q = 0; q < C::nb(p); ++q) {
-      const int b = r0 + q * RPT;
+      const int b = !SHUF ? r0 + q * RPT : p == P - 2 ? kq + 16 * gq : p == P - 1 ? kq + 16 * (4 * gq + q) : r0 + q * RPT;
       if (C::exact(p) || b < NBR) {
         if constexpr (L > 1) {
           const int k = b % L;
@@ -534,7 +617,7 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
     static_assert(P >= 2, "single-pass lengths go through the general kernel");
 #pragma unroll
     for (int q = 0; q < C::nb(P - 1); ++q) {
-      const int b = r0 + q * RPT;
+      const int b = SHUF ? kq + 16 * (4 * gq + q) : r0 + q * RPT;
       if (C::exact(P - 1) || b < NBZ) {
         const int k = b % LZ, g = b / LZ;
 #pragma unroll
@@ -675,6 +758,12 @@ static FastVariant make_variant(const char *name) {
 #define D2D_SPLIT_C2C(T, N, TX, NT, LINES, RL, WPE)                                      \
   make_variant<T, N, TX, NT, IO_C2C, LINES, RL, 1, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT \
                                                         " c2c xs w" #WPE)
+
+// c2c columns, split first exchange and in-wave last exchange, at least WPE
+// waves per SIMD
+#define D2D_SHUF_C2C(T, N, TX, NT, RL, WPE)                                             \
+  make_variant<T, N, TX, NT, IO_C2C, false, RL, 2, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT \
+                                                        " c2c xsh w" #WPE)
 
 // c2c, r2c and c2r, split exchange, at least WPE waves per SIMD
 #define D2D_SPLIT_ALL(T, N, TX, NT, LINES, RL, WPE)                                           \
@@ -895,12 +984,18 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_YPACK(double, 1024, 8, 512, R16_16_4, 0, 1, ""),
       D2D_YPACK(float, 1024, 16, 1024, R16_16_4, 1, 8, " xs w8"),
       D2D_YPACK(float, 1024, 8, 512, R16_16_4, 0, 1, ""),
+      D2D_YPACK(double, 1024, 16, 1024, R16_16_4, 2, 4, " xsh w4"),
+      D2D_YPACK(double, 1024, 8, 512, R16_16_4, 2, 4, " xsh w4"),
+      D2D_YPACK(float, 1024, 16, 1024, R16_16_4, 2, 8, " xsh w8"),
+      D2D_YPACK(float, 1024, 8, 512, R16_16_4, 2, 6, " xsh w6"),
       D2D_YPACK(double, 512, 8, 512, R8_8_8, 0, 1, ""),
       D2D_YPACK(float, 512, 16, 512, R8_8_8, 0, 1, ""),
       D2D_SPLIT_C2C(double, 1024, 2, 128, false, R16_16_4, 4),
       D2D_SPLIT_C2C(double, 1024, 4, 256, false, R16_16_4, 4),
       D2D_SPLIT_C2C(double, 1024, 8, 512, false, R16_16_4, 4),
       D2D_SPLIT_C2C(double, 1024, 16, 1024, false, R16_16_4, 4),
+      D2D_SHUF_C2C(double, 1024, 8, 512, R16_16_4, 4),
+      D2D_SHUF_C2C(double, 1024, 16, 1024, R16_16_4, 4),
       D2D_SPLIT_C2C(double, 1024, 4, 256, false, R16_16_4, 3),
       D2D_SPLIT_C2C(double, 1024, 8, 512, false, R16_16_4, 3),
       D2D_SPLIT_ALL(double, 1024, 2, 128, true, R16_16_4, 3),
@@ -922,6 +1017,8 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_SPLIT_C2C(float, 1024, 8, 512, false, R16_16_4, 6),
       D2D_SPLIT_C2C(float, 1024, 4, 256, false, R16_16_4, 6),
       D2D_SPLIT_C2C(float, 1024, 16, 1024, false, R16_16_4, 8),
+      D2D_SHUF_C2C(float, 1024, 16, 1024, R16_16_4, 8),
+      D2D_SHUF_C2C(float, 1024, 8, 512, R16_16_4, 6),
       D2D_SPLIT_ALL(float, 1024, 4, 256, true, R16_16_4, 5),
       D2D_VARIANTS(float, 1024, 8, 512, true, R16_16_4),
       D2D_VARIANTS(float, 1024, 4, 256, true, R16_16_4),
