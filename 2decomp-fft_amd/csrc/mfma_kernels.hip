@@ -499,6 +499,55 @@ __device__ __forceinline__ void mf_run(DescPtr d, void *out, int conj, long long
     }
   }
 
+  // ---- columns: the last pass straight from the matrix-core layout ------------
+  // With groups of 16 butterflies 16 apart (BQ = 16) and as many groups per
+  // lane as the last radix, group q of a lane is radix-16 butterfly
+  // b0 + 16 q, whose output s sits at 256 q + b0 + 16 s: the last pass's
+  // butterfly b0 + 16 s takes output s of the lane's groups q = 0 .. R2-1,
+  // all held by the lane.  No exchange; each lane runs the last-pass
+  // butterflies b0 + 16 (4 sq + e) and stores their points + 256 q.
+  constexpr bool DIRECT = !LINES && GW == R2 && M::BQ == 16 && IO != IO_R2C;
+  if constexpr (DIRECT) {
+    cpx<T> z[4 * R2];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int bp = b0 + 16 * (4 * sq + e);  // < L2 = 256
+      cpx<T> x[R2];
+#pragma unroll
+      for (int q = 0; q < R2; ++q) x[q] = v[4 * q + e];
+#pragma unroll
+      for (int s = 1; s < R2; ++s)
+        x[s] = cmul(x[s], gload<T>(tw, RL::tw_off(2) + bp * (R2 - 1) + s - 1));
+      dft<T, R2>(x);
+#pragma unroll
+      for (int s = 0; s < R2; ++s) z[e * R2 + s] = x[s];
+    }
+    long long ca, ib, ob;
+    bool cv;
+    col(t0, ca, cv, ib, ob);
+    if (cv) {
+      const long long sj = d->out_sj;
+      const long long lb = ob + (long long)(b0 + 64 * sq) * sj;
+      const bool second = (IO == IO_C2R) && (ca + 1 < d->C0r);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+#pragma unroll
+        for (int s = 0; s < R2; ++s) {
+          const long long o = lb + (long long)(16 * e + L2 * s) * sj;
+          cpx<T> y = z[e * R2 + s];
+          if constexpr (IO == IO_C2R) {
+            gstore_r<T>(out, o, y.x);
+            if (second) gstore_r<T>(out, o + d->out_s0, -y.y);
+          } else {
+            if (conj) y.y = -y.y;
+            gstore<T>(out, o, y);
+          }
+        }
+      }
+    }
+    return;
+  }
+
   // ---- exchange 1 -> 2: outputs at 256 (b/16) + b%16 + 16 s; VALU-map inputs --
   __syncthreads();
 #pragma unroll
