@@ -106,6 +106,9 @@ def lib():
             "d2dhip_status_string": ([ci], ctypes.c_char_p),
             "d2dhip_last_message": ([], ctypes.c_char_p),
             "d2dhip_version": ([], ci),
+            "d2dhip_profiler_prep": ([ci, ci, ci, ci], ci),
+            "d2dhip_profiler_start": ([ctypes.c_char_p], ci),
+            "d2dhip_profiler_end": ([ctypes.c_char_p], ci),
             "d2dhip_decomp_info_init": ([ci, ci, ci, ci, ci, ci, ci, ctypes.POINTER(Decomp)], ci),
             "d2dhip_best_2d_grid": ([ci, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
             "d2dhip_plan_create": ([ctypes.POINTER(pl), ci, ci, ci, ci, ci, cu, vp], ci),
@@ -227,6 +230,25 @@ def best_2d_grid(nproc):
     r, c = ctypes.c_int(), ctypes.c_int()
     check(lib().d2dhip_best_2d_grid(nproc, ctypes.byref(r), ctypes.byref(c)))
     return r.value, c.value
+
+
+# ---- profiler regions (decomp_2d_profiler, src/profiler_caliper.f90) ------------
+
+def profiler_prep(transpose=True, io=True, fft=True, d2d=True):
+    """decomp_profiler_prep (src/profiler_caliper.f90:138-150): the region
+    categories that the library opens (rocTX ranges)."""
+    check(lib().d2dhip_profiler_prep(int(transpose), int(io), int(fft), int(d2d)))
+
+
+def profiler_start(name):
+    """decomp_profiler_start (src/profiler_caliper.f90:156-164)."""
+    check(lib().d2dhip_profiler_start(name.encode()))
+
+
+def profiler_end(name):
+    """decomp_profiler_end (src/profiler_caliper.f90:169-177): closes the
+    innermost region this thread opened."""
+    check(lib().d2dhip_profiler_end(name.encode()))
 
 
 # ---- FFT engine ----------------------------------------------------------------

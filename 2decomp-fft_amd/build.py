@@ -22,6 +22,7 @@ SOURCES = ["fft_kernels.hip", "fast_kernels.hip", "mfma_kernels.hip", "copy_kern
            "halo_kernels.hip", "plan.cpp",
            "api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM_LIB = os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "lib")
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -73,7 +74,9 @@ def build(jobs=4, verbose=False, incremental=False):
             print(err, file=sys.stderr)
         objs.append(out)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    # rocTX for the profiler regions, found at run time where it was at link time
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [
+        "-L" + ROCM_LIB, "-lrocprofiler-sdk-roctx", "-Wl,-rpath," + ROCM_LIB]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)

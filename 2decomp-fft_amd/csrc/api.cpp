@@ -1,7 +1,9 @@
 // api.cpp -- status handling, decomposition metadata, transposes/packs and
 // synthetic-input generators of libd2dhip's C ABI.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -12,6 +14,18 @@
 namespace d2dhip {
 
 static thread_local std::string g_last_message;
+
+// profiler categories that open regions; all on by default, as the
+// reference's default_profiler = .true. (src/profiler_caliper.f90:23-27)
+static std::atomic<unsigned> g_regions{PROF_TRANSPOSE | PROF_IO | PROF_FFT | PROF_D2D};
+
+Region::Region(int category, const char *name)
+    : on((g_regions.load(std::memory_order_relaxed) & (unsigned)category) != 0) {
+  if (on) roctxRangePushA(name);
+}
+Region::~Region() {
+  if (on) roctxRangePop();
+}
 
 int set_error(int status, const std::string &msg) {
   g_last_message = msg;
@@ -113,11 +127,39 @@ const char *d2dhip_last_message(void) { return g_last_message.c_str(); }
 
 int d2dhip_version(void) { return 100; }
 
+// decomp_profiler_prep (src/profiler_caliper.f90:138-150): which categories
+// open regions, in the reference's order (transpose, io, fft, d2d).
+int d2dhip_profiler_prep(int transpose, int io, int fft, int d2d) {
+  g_regions.store((transpose ? PROF_TRANSPOSE : 0u) | (io ? PROF_IO : 0u) | (fft ? PROF_FFT : 0u) |
+                      (d2d ? PROF_D2D : 0u),
+                  std::memory_order_relaxed);
+  return D2DHIP_SUCCESS;
+}
+
+// decomp_profiler_start / _end (src/profiler_caliper.f90:156-177): a caller's
+// own region, always opened; regions nest, and _end closes the innermost one
+// that is open on the calling thread.
+static thread_local int g_open_regions = 0;  // opened by d2dhip_profiler_start
+int d2dhip_profiler_start(const char *name) {
+  if (!name) return set_error(D2DHIP_ERR_NULL, "region name is NULL");
+  roctxRangePushA(name);
+  ++g_open_regions;
+  return D2DHIP_SUCCESS;
+}
+int d2dhip_profiler_end(const char *name) {
+  if (!name) return set_error(D2DHIP_ERR_NULL, "region name is NULL");
+  if (g_open_regions == 0) return set_error(D2DHIP_ERR_INVALID_VALUE, "no region is open");
+  --g_open_regions;
+  roctxRangePop();
+  return D2DHIP_SUCCESS;
+}
+
 // decomp_info_init: src/decomp_2d.f90:382-490 with get_dist (:1108-1130),
 // partition x (1,2,3), y (2,1,3), z (2,3,1) (:413-418), prepare_buffer
 // (:1135-1206) and the even test (:443-454).
 int d2dhip_decomp_info_init(int nx, int ny, int nz, int p_row, int p_col, int coord_row,
                             int coord_col, d2dhip_decomp *d) {
+  Region region(PROF_D2D, "decomp_2d_init");
   if (!d) return set_error(D2DHIP_ERR_NULL, "decomp pointer is NULL");
   std::memset(d, 0, sizeof(*d));
   if (nx <= 0 || ny <= 0 || nz <= 0)
@@ -192,6 +234,14 @@ int d2dhip_transpose(int dir, int elem, const void *src, void *dst, const d2dhip
   if (!d || !src || !dst) return set_error(D2DHIP_ERR_NULL, "NULL argument");
   const int es = elem_size(elem);
   if (!es || dir < 0 || dir > 3) return set_error(D2DHIP_ERR_INVALID_VALUE, "bad dir/elem");
+  // region names of src/transpose_*.f90 (transp_x_y_r at transpose_x_to_y.f90:39, ...):
+  // _r for real data, _c for complex
+  static const char *const kTransp[4][2] = {{"transp_x_y_r", "transp_x_y_c"},
+                                            {"transp_y_z_r", "transp_y_z_c"},
+                                            {"transp_z_y_r", "transp_z_y_c"},
+                                            {"transp_y_x_r", "transp_y_x_c"}};
+  const bool cplx = elem == D2DHIP_C16 || elem == D2DHIP_C8;
+  Region region(PROF_TRANSPOSE, kTransp[dir][cplx ? 1 : 0]);
   const bool xy = (dir == D2DHIP_X_TO_Y || dir == D2DHIP_Y_TO_X);
   const int np = xy ? d->p_row : d->p_col;
   if (np != 1)

@@ -210,6 +210,20 @@ __device__ inline long long xcd_tile(long long block, long long ntiles) {
   return (block % kXcds) * per + block / kXcds;
 }
 
+// Profiler regions: the reference's decomp_profiler_start / _end around its
+// entry points (src/profiler_caliper.f90:156-177), here as rocTX ranges that
+// `rocprofv3 --marker-trace` records, with the reference's region names and
+// its four categories (src/profiler_caliper.f90:24-27).  A Region opens one
+// for its scope when its category is on (api.cpp).
+enum { PROF_TRANSPOSE = 1, PROF_IO = 2, PROF_FFT = 4, PROF_D2D = 8 };
+struct Region {
+  bool on;
+  Region(int category, const char *name);
+  ~Region();
+  Region(const Region &) = delete;
+  Region &operator=(const Region &) = delete;
+};
+
 // status helpers (api.cpp)
 int set_error(int status, const std::string &msg);
 int hip_check(hipError_t e, const char *what);

@@ -1230,6 +1230,7 @@ int plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec,
 
 int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec,
                        unsigned flags, void *stream) {
+  Region region(PROF_FFT, "fft_init");
   return plan_create(plan, format, nx, ny, nz, prec, flags, stream, nullptr, nullptr);
 }
 
@@ -1239,6 +1240,7 @@ int d2dhip_plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, in
 int d2dhip_plan_create_rank(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec,
                             int p_row, int p_col, int coord_row, int coord_col, unsigned flags,
                             void *stream) {
+  Region region(PROF_FFT, "fft_init");
   if (!plan) return set_error(D2DHIP_ERR_NULL, "plan pointer is NULL");
   *plan = nullptr;
   if (format != D2DHIP_PHYSICAL_IN_X && format != D2DHIP_PHYSICAL_IN_Z)
@@ -1270,6 +1272,7 @@ int d2dhip_plan_get_rank(d2dhip_plan p, d2dhip_decomp *ph, d2dhip_decomp *sp) {
 }
 
 int d2dhip_plan_destroy(d2dhip_plan p) {
+  Region region(PROF_FFT, "fft_fin");
   if (!p) return D2DHIP_SUCCESS;
   for (auto &r : p->recs) {
     (void)hipEventDestroy(r.a);
@@ -1439,6 +1442,7 @@ long long d2dhip_plan_device_bytes(d2dhip_plan p) {
 // x, y, z; X-backward and Z-forward run z, y, x.  Skipped axes
 // (opt_skip_XYZ_c2c) are left out; the first stage that runs reads `in`.
 int d2dhip_exec_c2c(d2dhip_plan p, void *in, void *out, int isign) {
+  Region region(PROF_FFT, "fft_c2c");
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
   if (p->multi)
     return set_error(D2DHIP_ERR_UNSUPPORTED,
@@ -1473,6 +1477,7 @@ int d2dhip_exec_c2c(d2dhip_plan p, void *in, void *out, int isign) {
 // fft_3d_r2c: src/fft_common_3d.f90:121-189.  PHYSICAL_IN_X: r2c along x,
 // then y, then z; PHYSICAL_IN_Z: r2c along z, then y, then x.
 int d2dhip_exec_r2c(d2dhip_plan p, const void *in_r, void *out_c) {
+  Region region(PROF_FFT, "fft_r2c");
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
   if (p->multi)
     return set_error(D2DHIP_ERR_UNSUPPORTED,
@@ -1560,6 +1565,7 @@ int scratch_address(void **ptr) {
 }  // namespace
 
 int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
+  Region region(PROF_FFT, "fft_c2r");
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
   if (p->multi)
     return set_error(D2DHIP_ERR_UNSUPPORTED,
@@ -1572,6 +1578,7 @@ int d2dhip_exec_c2r(d2dhip_plan p, void *in_c, void *out_r) {
 // d2dhip_exec_c2r with the workspace in the library's static scratch array:
 // no allocation of any kind, in_c preserved whatever the plan's flags.
 int d2dhip_exec_c2r_noalloc(d2dhip_plan p, void *in_c, void *out_r) {
+  Region region(PROF_FFT, "fft_c2r");
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
   if (p->multi)
     return set_error(D2DHIP_ERR_UNSUPPORTED,
@@ -1669,6 +1676,7 @@ int d2dhip_plan_get_dtt(d2dhip_plan p, int dtt[15]) {
 // preserved (the reference transforms it in place along the first axis).
 // Skipped axes (opt_skip_XYZ_c2c) are left out as r2r_1m_x/y/z leave them.
 int d2dhip_exec_dtt(d2dhip_plan p, const void *in, void *out, int isign) {
+  Region region(PROF_FFT, "decomp_2d_dtt_3d_r2r");
   if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
   if (!in || !out) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
   if (isign != D2DHIP_FORWARD && isign != D2DHIP_BACKWARD)

@@ -135,6 +135,20 @@ module d2dhip_c
       type(c_ptr) function d2dhip_last_message() bind(C, name="d2dhip_last_message")
          import :: c_ptr
       end function
+      ! profiler regions (rocTX)
+      integer(c_int) function d2dhip_profiler_prep(transpose, io, fft, d2d) &
+         bind(C, name="d2dhip_profiler_prep")
+         import :: c_int
+         integer(c_int), value :: transpose, io, fft, d2d
+      end function
+      integer(c_int) function d2dhip_profiler_start(name) bind(C, name="d2dhip_profiler_start")
+         import :: c_int, c_char
+         character(kind=c_char), dimension(*), intent(in) :: name
+      end function
+      integer(c_int) function d2dhip_profiler_end(name) bind(C, name="d2dhip_profiler_end")
+         import :: c_int, c_char
+         character(kind=c_char), dimension(*), intent(in) :: name
+      end function
       ! many grids in one persistent launch
       integer(c_int) function d2dhip_batch_create(batch, nplans, plans, stream) &
          bind(C, name="d2dhip_batch_create")
@@ -204,11 +218,89 @@ contains
 
 end module d2dhip_c
 
+!! decomp_2d_profiler (src/profiler_caliper.f90) over rocTX: the region
+!! categories and the caller's own regions; the library opens its regions
+!! (fft_init, fft_c2c, fft_r2c, fft_c2r, transp_x_y_r, ...) itself.
+module decomp_2d_profiler
+
+   use, intrinsic :: iso_c_binding
+   use d2dhip_c, only: d2dhip_profiler_prep, d2dhip_profiler_start, d2dhip_profiler_end, &
+                       c_message
+
+   implicit none
+
+   logical, parameter :: default_profiler = .true.
+   logical, save, public :: decomp_profiler_transpose = default_profiler
+   logical, save, public :: decomp_profiler_io = default_profiler
+   logical, save, public :: decomp_profiler_fft = default_profiler
+   logical, save, public :: decomp_profiler_d2d = default_profiler
+
+   private
+   public :: decomp_profiler_init, decomp_profiler_fin, decomp_profiler_prep, &
+             decomp_profiler_log, decomp_profiler_start, decomp_profiler_end
+
+contains
+
+   ! rocTX needs no set-up: the tool that records the ranges (rocprofv3
+   ! --marker-trace) attaches to the process
+   subroutine decomp_profiler_init()
+   end subroutine decomp_profiler_init
+
+   subroutine decomp_profiler_fin()
+      decomp_profiler_transpose = default_profiler
+      decomp_profiler_io = default_profiler
+      decomp_profiler_fft = default_profiler
+      decomp_profiler_d2d = default_profiler
+      call apply()
+   end subroutine decomp_profiler_fin
+
+   subroutine decomp_profiler_log(io_unit)
+      integer, intent(in) :: io_unit
+      write (io_unit, *) "rocTX profiling regions active"
+      write (io_unit, *) "   record them with rocprofv3 --marker-trace"
+   end subroutine decomp_profiler_log
+
+   ! (transpose, io, fft, d2d), as src/profiler_caliper.f90:138-150
+   subroutine decomp_profiler_prep(profiler_setup)
+      logical, dimension(4), intent(in), optional :: profiler_setup
+      if (present(profiler_setup)) then
+         decomp_profiler_transpose = profiler_setup(1)
+         decomp_profiler_io = profiler_setup(2)
+         decomp_profiler_fft = profiler_setup(3)
+         decomp_profiler_d2d = profiler_setup(4)
+      end if
+      call apply()
+   end subroutine decomp_profiler_prep
+
+   subroutine decomp_profiler_start(timer_name)
+      character(len=*), intent(in) :: timer_name
+      if (d2dhip_profiler_start(trim(timer_name)//c_null_char) /= 0) &
+         error stop "decomp_profiler_start: "//c_message()
+   end subroutine decomp_profiler_start
+
+   subroutine decomp_profiler_end(timer_name)
+      character(len=*), intent(in) :: timer_name
+      if (d2dhip_profiler_end(trim(timer_name)//c_null_char) /= 0) &
+         error stop "decomp_profiler_end: "//c_message()
+   end subroutine decomp_profiler_end
+
+   subroutine apply()
+      integer(c_int) :: st
+      st = d2dhip_profiler_prep(merge(1, 0, decomp_profiler_transpose), &
+                                merge(1, 0, decomp_profiler_io), &
+                                merge(1, 0, decomp_profiler_fft), &
+                                merge(1, 0, decomp_profiler_d2d))
+   end subroutine apply
+
+end module decomp_2d_profiler
+
 module decomp_2d
 
    use, intrinsic :: iso_c_binding
    use decomp_2d_constants
    use d2dhip_c
+   use decomp_2d_profiler, only: decomp_profiler_transpose, decomp_profiler_start, &
+                                 decomp_profiler_end
 
    implicit none
 
@@ -237,16 +329,16 @@ module decomp_2d
              alloc_x, alloc_y, alloc_z
 
    interface transpose_x_to_y
-      module procedure tr_real_short, tr_real_xy, tr_cplx_short, tr_cplx_xy
+      module procedure tr_real_xy_s, tr_real_xy, tr_cplx_xy_s, tr_cplx_xy
    end interface
    interface transpose_y_to_z
-      module procedure tr_real_short, tr_real_yz, tr_cplx_short, tr_cplx_yz
+      module procedure tr_real_yz_s, tr_real_yz, tr_cplx_yz_s, tr_cplx_yz
    end interface
    interface transpose_z_to_y
-      module procedure tr_real_short, tr_real_zy, tr_cplx_short, tr_cplx_zy
+      module procedure tr_real_zy_s, tr_real_zy, tr_cplx_zy_s, tr_cplx_zy
    end interface
    interface transpose_y_to_x
-      module procedure tr_real_short, tr_real_yx, tr_cplx_short, tr_cplx_yx
+      module procedure tr_real_yx_s, tr_real_yx, tr_cplx_yx_s, tr_cplx_yx
    end interface
    interface alloc_x
       module procedure alloc_x_real, alloc_x_cplx
@@ -347,71 +439,121 @@ contains
 
    ! One-rank transposes: dst = src (src/transpose_x_to_y.f90:41-50 and the
    ! matching branches of the other three files).
-   subroutine tr_real_short(src, dst)
+   ! At one rank every transpose is a copy of the whole pencil
+   ! (src/transpose_x_to_y.f90:41-50 and its three siblings), inside the
+   ! reference's profiler region of that transpose (transp_x_y_r at
+   ! src/transpose_x_to_y.f90:39, transp_x_y_c at :162, ...).
+   subroutine tr_real_xy_s(src, dst)
       real(mytype), dimension(:, :, :), intent(in) :: src
       real(mytype), dimension(:, :, :), intent(out) :: dst
+      if (decomp_profiler_transpose) call decomp_profiler_start("transp_x_y_r")
       dst = src
-   end subroutine
-   subroutine tr_cplx_short(src, dst)
-      complex(mytype), dimension(:, :, :), intent(in) :: src
-      complex(mytype), dimension(:, :, :), intent(out) :: dst
-      dst = src
+      if (decomp_profiler_transpose) call decomp_profiler_end("transp_x_y_r")
    end subroutine
    subroutine tr_real_xy(src, dst, decomp)
       real(mytype), dimension(:, :, :), intent(in) :: src
       real(mytype), dimension(:, :, :), intent(out) :: dst
       type(decomp_info), intent(in) :: decomp
       if (any(shape(src) /= decomp%xsz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
-      dst = src
+      call tr_real_xy_s(src, dst)
    end subroutine
-   subroutine tr_real_yz(src, dst, decomp)
-      real(mytype), dimension(:, :, :), intent(in) :: src
-      real(mytype), dimension(:, :, :), intent(out) :: dst
-      type(decomp_info), intent(in) :: decomp
-      if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+   subroutine tr_cplx_xy_s(src, dst)
+      complex(mytype), dimension(:, :, :), intent(in) :: src
+      complex(mytype), dimension(:, :, :), intent(out) :: dst
+      if (decomp_profiler_transpose) call decomp_profiler_start("transp_x_y_c")
       dst = src
-   end subroutine
-   subroutine tr_real_zy(src, dst, decomp)
-      real(mytype), dimension(:, :, :), intent(in) :: src
-      real(mytype), dimension(:, :, :), intent(out) :: dst
-      type(decomp_info), intent(in) :: decomp
-      if (any(shape(src) /= decomp%zsz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
-      dst = src
-   end subroutine
-   subroutine tr_real_yx(src, dst, decomp)
-      real(mytype), dimension(:, :, :), intent(in) :: src
-      real(mytype), dimension(:, :, :), intent(out) :: dst
-      type(decomp_info), intent(in) :: decomp
-      if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
-      dst = src
+      if (decomp_profiler_transpose) call decomp_profiler_end("transp_x_y_c")
    end subroutine
    subroutine tr_cplx_xy(src, dst, decomp)
       complex(mytype), dimension(:, :, :), intent(in) :: src
       complex(mytype), dimension(:, :, :), intent(out) :: dst
       type(decomp_info), intent(in) :: decomp
       if (any(shape(src) /= decomp%xsz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      call tr_cplx_xy_s(src, dst)
+   end subroutine
+   subroutine tr_real_yz_s(src, dst)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      if (decomp_profiler_transpose) call decomp_profiler_start("transp_y_z_r")
       dst = src
+      if (decomp_profiler_transpose) call decomp_profiler_end("transp_y_z_r")
+   end subroutine
+   subroutine tr_real_yz(src, dst, decomp)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      call tr_real_yz_s(src, dst)
+   end subroutine
+   subroutine tr_cplx_yz_s(src, dst)
+      complex(mytype), dimension(:, :, :), intent(in) :: src
+      complex(mytype), dimension(:, :, :), intent(out) :: dst
+      if (decomp_profiler_transpose) call decomp_profiler_start("transp_y_z_c")
+      dst = src
+      if (decomp_profiler_transpose) call decomp_profiler_end("transp_y_z_c")
    end subroutine
    subroutine tr_cplx_yz(src, dst, decomp)
       complex(mytype), dimension(:, :, :), intent(in) :: src
       complex(mytype), dimension(:, :, :), intent(out) :: dst
       type(decomp_info), intent(in) :: decomp
       if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      call tr_cplx_yz_s(src, dst)
+   end subroutine
+   subroutine tr_real_zy_s(src, dst)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      if (decomp_profiler_transpose) call decomp_profiler_start("transp_z_y_r")
       dst = src
+      if (decomp_profiler_transpose) call decomp_profiler_end("transp_z_y_r")
+   end subroutine
+   subroutine tr_real_zy(src, dst, decomp)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%zsz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      call tr_real_zy_s(src, dst)
+   end subroutine
+   subroutine tr_cplx_zy_s(src, dst)
+      complex(mytype), dimension(:, :, :), intent(in) :: src
+      complex(mytype), dimension(:, :, :), intent(out) :: dst
+      if (decomp_profiler_transpose) call decomp_profiler_start("transp_z_y_c")
+      dst = src
+      if (decomp_profiler_transpose) call decomp_profiler_end("transp_z_y_c")
    end subroutine
    subroutine tr_cplx_zy(src, dst, decomp)
       complex(mytype), dimension(:, :, :), intent(in) :: src
       complex(mytype), dimension(:, :, :), intent(out) :: dst
       type(decomp_info), intent(in) :: decomp
       if (any(shape(src) /= decomp%zsz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      call tr_cplx_zy_s(src, dst)
+   end subroutine
+   subroutine tr_real_yx_s(src, dst)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      if (decomp_profiler_transpose) call decomp_profiler_start("transp_y_x_r")
       dst = src
+      if (decomp_profiler_transpose) call decomp_profiler_end("transp_y_x_r")
+   end subroutine
+   subroutine tr_real_yx(src, dst, decomp)
+      real(mytype), dimension(:, :, :), intent(in) :: src
+      real(mytype), dimension(:, :, :), intent(out) :: dst
+      type(decomp_info), intent(in) :: decomp
+      if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
+      call tr_real_yx_s(src, dst)
+   end subroutine
+   subroutine tr_cplx_yx_s(src, dst)
+      complex(mytype), dimension(:, :, :), intent(in) :: src
+      complex(mytype), dimension(:, :, :), intent(out) :: dst
+      if (decomp_profiler_transpose) call decomp_profiler_start("transp_y_x_c")
+      dst = src
+      if (decomp_profiler_transpose) call decomp_profiler_end("transp_y_x_c")
    end subroutine
    subroutine tr_cplx_yx(src, dst, decomp)
       complex(mytype), dimension(:, :, :), intent(in) :: src
       complex(mytype), dimension(:, :, :), intent(out) :: dst
       type(decomp_info), intent(in) :: decomp
       if (any(shape(src) /= decomp%ysz)) call decomp_2d_abort(__FILE__, __LINE__, 1, "shape")
-      dst = src
+      call tr_cplx_yx_s(src, dst)
    end subroutine
 
    ! alloc_x / alloc_y / alloc_z (src/alloc.f90:10-135): natural (i,j,k) order,

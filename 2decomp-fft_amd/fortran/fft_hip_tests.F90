@@ -23,6 +23,7 @@ program fft_hip_tests
    use decomp_2d_fft
    use d2dhip_c, only: hipMalloc, hipFree, hipMemcpy, HIP_H2D, HIP_D2H
    use m_halo
+   use decomp_2d_profiler
 
    implicit none
 
@@ -30,6 +31,9 @@ program fft_hip_tests
    type(decomp_2d_fft_engine) :: local_engine
    integer :: igrid
 
+   ! the library's regions on (transpose, fft, d2d; no io), plus one of our own
+   call decomp_profiler_prep([.true., .false., .true., .true.])
+   if (decomp_profiler_d2d) call decomp_profiler_start("fft_hip_tests")
    call decomp_2d_init(nx + 1, ny + 1, nz + 1, 0, 0)
    call check_transposes()
 
@@ -67,6 +71,8 @@ program fft_hip_tests
    call decomp_2d_init(nx, ny, nz, 0, 0, periodic_bc=[.true., .true., .true.])
    call check_halo_periodic()
    call decomp_2d_finalize
+   if (decomp_profiler_d2d) call decomp_profiler_end("fft_hip_tests")
+   call decomp_profiler_fin()
    write (*, *) 'fft_hip_tests completed'
 
 contains

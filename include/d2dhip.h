@@ -102,6 +102,26 @@ const char *d2dhip_last_message(void);
 /* Library version as major*10000 + minor*100 + patch. */
 int d2dhip_version(void);
 
+/* ---- profiler regions (host only) -----------------------------------------
+ * The reference's decomp_2d_profiler (src/profiler_caliper.f90): regions
+ * around its entry points, here rocTX ranges (rocprofv3 --marker-trace
+ * records them) with the reference's names: fft_init / fft_fin around plan
+ * creation and destruction, fft_c2c / fft_r2c / fft_c2r around the 3D
+ * transforms (src/fft_common_3d.f90:22,130,206), decomp_2d_dtt_3d_r2r
+ * around d2dhip_exec_dtt, transp_x_y_r / _c ... around d2dhip_transpose
+ * (src/transpose_x_to_y.f90:39,162 and the other three files), and
+ * decomp_2d_init around d2dhip_decomp_info_init.
+ * d2dhip_profiler_prep replaces decomp_profiler_prep
+ * (src/profiler_caliper.f90:138-150): which categories open regions, in the
+ * reference's order; all four are on until it is called.
+ * d2dhip_profiler_start / _end replace decomp_profiler_start / _end
+ * (:156-177): a caller's own region; regions nest and _end closes the
+ * innermost one the calling thread opened (D2DHIP_ERR_INVALID_VALUE when it
+ * opened none). */
+int d2dhip_profiler_prep(int transpose, int io, int fft, int d2d);
+int d2dhip_profiler_start(const char *name);
+int d2dhip_profiler_end(const char *name);
+
 /* ---- decomposition metadata (host only) ----------------------------------
  * Replaces decomp_info_init / get_dist / partition / prepare_buffer,
  * src/decomp_2d.f90:382-490 and :1016-1206, for rank (coord_row, coord_col). */

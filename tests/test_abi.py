@@ -130,3 +130,24 @@ def test_modal_log_amplitudes_and_noise(d2d):
     a = np.zeros((16, 8, 4), np.float64, order="F")
     d2d.host_fill_modal(a, 16, 8, 4, 0, m)
     assert np.isfinite(a).all()
+
+
+def test_profiler_regions_host_api(d2d):
+    """decomp_profiler_prep / _start / _end (src/profiler_caliper.f90:138-177)
+    on the host: nested caller regions open and close, closing with none
+    open is an error, and the category switches take the reference's four
+    flags."""
+    d2d.profiler_prep(transpose=False, io=False, fft=True, d2d=False)
+    d2d.profiler_start("outer")
+    d2d.profiler_start("inner")
+    d2d.profiler_end("inner")
+    d2d.profiler_end("outer")
+    L = d2d.lib()
+    assert L.d2dhip_profiler_end(b"none") == d2d.ERR_INVALID_VALUE
+    assert "no region" in d2d.last_message()
+    assert L.d2dhip_profiler_start(None) == d2d.ERR_NULL
+    # host-only calls with regions on: the metadata and a refused transpose
+    d2d.profiler_prep()
+    d = d2d.decomp_info_init(17, 13, 11, 2, 3, 1, 2)
+    assert L.d2dhip_transpose(0, d2d.R8, ctypes.c_void_p(8), ctypes.c_void_p(16),
+                              ctypes.byref(d), None) == d2d.ERR_UNSUPPORTED
