@@ -249,6 +249,7 @@ struct MultiGroup {
   long long tile0;
 };
 
+#ifdef __HIPCC__  // (a host compiler, for the host-only sanitizer build, skips it)
 // Group of launch tile `it` (groups sorted by tile0), wave-uniform.
 __device__ inline int find_multi_group(const MultiGroup *__restrict__ g, int n, long long it) {
   int lo = 0, hi = n - 1;
@@ -258,6 +259,7 @@ __device__ inline int find_multi_group(const MultiGroup *__restrict__ g, int n, 
   }
   return __builtin_amdgcn_readfirstlane(lo);
 }
+#endif
 
 // The general kernel over the tiles of several grids (their own descriptors).
 int launch_general_multi(int prec_bytes, const MultiGroup *g, int ng, long long tiles,
