@@ -802,9 +802,12 @@ using R4_4_12 = RList<4, 4, 12>;      // 192, 16 rows (or 8)
 // the radix list of the registered variants of its length, so its results
 // equal the plans' own executions on any specialised kernel bit for bit.
 //   FP64: tiles of up to 35 KB (4 blocks, 8 waves per CU), at least 8
-//         columns (128-byte rows) on strided axes;
-//   FP32: the same tile bytes, twice the columns (at least 16: 128-byte rows).
-// Both use 2-wave blocks.
+//         columns (128-byte rows) on strided axes, 2-wave blocks;
+//   FP32: the same tile bytes, twice the columns (at least 16: 128-byte
+//         rows), 4-wave blocks (16 waves per CU; 2.95 -> 2.78 ms on the
+//         64-grid batch against 2-wave blocks).  FP64 on 4-wave blocks gave
+//         results that differ in the last bits from the standalone kernels
+//         of the same tiles, so it keeps 2-wave blocks.
 // Some kinds leave lanes idle in a pass (N / r not a multiple of the rows
 // per column).
 template <int N_, int TX_, bool LINES_, class RL_>
@@ -826,7 +829,7 @@ using PersistSetD =
          PV<64, 16, false, R8_8>, PV<64, 16, true, R8_8>, PV<48, 32, false, R12_4>,
          PV<48, 32, true, R12_4>, PV<32, 32, false, R8_4>, PV<32, 32, true, R8_4>>;
 using PersistSetF =
-    PSet<128, 2, PV<256, 16, false, R16_16>, PV<256, 16, true, R16_16>, PV<192, 16, false, R4_4_12>,
+    PSet<256, 4, PV<256, 16, false, R16_16>, PV<256, 16, true, R16_16>, PV<192, 16, false, R4_4_12>,
          PV<192, 16, true, R4_4_12>, PV<160, 16, false, R4_4_10>, PV<160, 16, true, R4_4_10>,
          PV<128, 32, false, R16_8>, PV<128, 32, true, R16_8>, PV<96, 32, false, R12_8>,
          PV<96, 32, true, R12_8>, PV<80, 32, false, R4_4_5>, PV<80, 32, true, R4_4_5>,
@@ -1110,23 +1113,23 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(float, 160, 8, 64, true, R4_4_10),
       D2D_VARIANTS(float, 192, 8, 128, false, R4_4_12),
       D2D_VARIANTS(float, 192, 4, 64, true, R4_4_12),
-      D2D_C2C(float, 256, 16, 128, false, R16_16),
-      D2D_C2C(float, 256, 16, 128, true, R16_16),
-      D2D_C2C(float, 192, 16, 128, false, R4_4_12),
-      D2D_C2C(float, 192, 16, 128, true, R4_4_12),
-      D2D_C2C(float, 160, 16, 128, true, R4_4_10),
-      D2D_C2C(float, 128, 32, 128, false, R16_8),
-      D2D_C2C(float, 128, 32, 128, true, R16_8),
-      D2D_C2C(float, 96, 32, 128, false, R12_8),
-      D2D_C2C(float, 96, 32, 128, true, R12_8),
-      D2D_C2C(float, 80, 32, 128, false, R4_4_5),
-      D2D_C2C(float, 80, 32, 128, true, R4_4_5),
-      D2D_C2C(float, 64, 32, 128, false, R8_8),
-      D2D_C2C(float, 64, 32, 128, true, R8_8),
-      D2D_C2C(float, 48, 64, 128, false, R12_4),
-      D2D_C2C(float, 48, 64, 128, true, R12_4),
-      D2D_C2C(float, 32, 64, 128, false, R8_4),
-      D2D_C2C(float, 32, 64, 128, true, R8_4),
+      D2D_C2C(float, 256, 16, 256, true, R16_16),
+      D2D_C2C(float, 192, 16, 256, false, R4_4_12),
+      D2D_C2C(float, 192, 16, 256, true, R4_4_12),
+      D2D_C2C(float, 160, 16, 256, false, R4_4_10),
+      D2D_C2C(float, 160, 16, 256, true, R4_4_10),
+      D2D_C2C(float, 128, 32, 256, false, R16_8),
+      D2D_C2C(float, 128, 32, 256, true, R16_8),
+      D2D_C2C(float, 96, 32, 256, false, R12_8),
+      D2D_C2C(float, 96, 32, 256, true, R12_8),
+      D2D_C2C(float, 80, 32, 256, false, R4_4_5),
+      D2D_C2C(float, 80, 32, 256, true, R4_4_5),
+      D2D_C2C(float, 64, 32, 256, false, R8_8),
+      D2D_C2C(float, 64, 32, 256, true, R8_8),
+      D2D_C2C(float, 48, 64, 256, false, R12_4),
+      D2D_C2C(float, 48, 64, 256, true, R12_4),
+      D2D_C2C(float, 32, 64, 256, false, R8_4),
+      D2D_C2C(float, 32, 64, 256, true, R8_4),
     };
     for (const FastVariant &v : mf_variants()) r.push_back(v);
     return r;
