@@ -301,14 +301,16 @@ class FFTEngine:
         return check(lib().d2dhip_exec_1m(self._plan, i, _ptr(inp), _ptr(out), isign))
 
     def exec_1m_pack(self, stage, inp, send, direction, decomp, isign, even=False):
-        """A y stage with its output in the send order of `direction` (Y_TO_Z, Y_TO_X)."""
+        """A y stage with its output in the send order of `direction` (Y_TO_Z,
+        Y_TO_X), or a z stage in that of Z_TO_Y."""
         self._sync_stream()
         i = self.STAGES.index(stage) if isinstance(stage, str) else stage
         return check(lib().d2dhip_exec_1m_pack(self._plan, i, _ptr(inp), _ptr(send), direction,
                                                ctypes.byref(decomp), int(bool(even)), isign))
 
     def exec_1m_unpack(self, stage, recv, out, direction, decomp, isign, even=False):
-        """A y stage reading its input in the receive order of `direction` (Z_TO_Y, X_TO_Y)."""
+        """A y stage reading its input in the receive order of `direction`
+        (Z_TO_Y, X_TO_Y), or a z stage in that of Y_TO_Z."""
         self._sync_stream()
         i = self.STAGES.index(stage) if isinstance(stage, str) else stage
         return check(lib().d2dhip_exec_1m_unpack(self._plan, i, _ptr(recv), _ptr(out), direction,

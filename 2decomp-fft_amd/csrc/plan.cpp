@@ -1770,38 +1770,58 @@ int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
 // that map in its stores / loads; a specialised kernel without a fused form
 // runs unfused through a y-pencil workspace.  Results equal the stage
 // followed by d2dhip_pack (d2dhip_unpack followed by the stage) bit for bit.
-int run_y_stage_packed(d2dhip_plan_s *p, int stage, const void *in, void *out, int dir,
-                       const d2dhip_decomp *d, int even, int isign, bool pack) {
+int run_stage_packed(d2dhip_plan_s *p, int stage, const void *in, void *out, int dir,
+                     const d2dhip_decomp *d, int even, int isign, bool pack) {
   if (p->capturing)
     return set_error(D2DHIP_ERR_UNSUPPORTED, "fused pack / unpack calls are not captured into a graph");
-  if (stage != 1 && stage != 4)
-    return set_error(D2DHIP_ERR_INVALID_VALUE, "fused y-pencil pack / unpack: stage 1 or 4");
+  // y stages: the four y-pencil maps; z stages (c2c_1m_z on the physical
+  // z-pencil, and the spectral one under PHYSICAL_IN_X): mem_split_zy after,
+  // mem_merge_yz before.  Every one of these maps cuts its pencil into blocks
+  // along the stage's own axis.
+  const bool ystage = stage == 1 || stage == 4;
+  const bool zstage = stage == 2 || (stage == 5 && p->format == D2DHIP_PHYSICAL_IN_X);
+  if (!ystage && !zstage)
+    return set_error(D2DHIP_ERR_INVALID_VALUE,
+                     "fused pack / unpack: a y stage (1, 4) or a z stage (2; 5 under PHYSICAL_IN_X)");
   if (isign != D2DHIP_FORWARD && isign != D2DHIP_BACKWARD)
     return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid value of isign");
-  const bool two = pack ? dir == D2DHIP_Y_TO_Z : dir == D2DHIP_Z_TO_Y;
-  const bool one = pack ? dir == D2DHIP_Y_TO_X : dir == D2DHIP_X_TO_Y;
+  const bool two = ystage ? (pack ? dir == D2DHIP_Y_TO_Z : dir == D2DHIP_Z_TO_Y)
+                          : (pack ? dir == D2DHIP_Z_TO_Y : dir == D2DHIP_Y_TO_Z);
+  const bool one = ystage && (pack ? dir == D2DHIP_Y_TO_X : dir == D2DHIP_X_TO_Y);
   if (!two && !one)
     return set_error(D2DHIP_ERR_INVALID_VALUE,
-                     pack ? "pack after a y stage: D2DHIP_Y_TO_Z or D2DHIP_Y_TO_X"
-                          : "unpack before a y stage: D2DHIP_Z_TO_Y or D2DHIP_X_TO_Y");
+                     ystage ? (pack ? "pack after a y stage: D2DHIP_Y_TO_Z or D2DHIP_Y_TO_X"
+                                    : "unpack before a y stage: D2DHIP_Z_TO_Y or D2DHIP_X_TO_Y")
+                            : (pack ? "pack after a z stage: D2DHIP_Z_TO_Y"
+                                    : "unpack before a z stage: D2DHIP_Y_TO_Z"));
   const auto &pen = p->pen[stage];
-  if (d->ysz[0] != pen[0] || d->ysz[1] != pen[1] || d->ysz[2] != pen[2])
-    return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition y-pencil is not the stage's pencil");
+  const int *dsz = ystage ? d->ysz : d->zsz;
+  if (dsz[0] != pen[0] || dsz[1] != pen[1] || dsz[2] != pen[2])
+    return set_error(D2DHIP_ERR_INVALID_VALUE,
+                     ystage ? "decomposition y-pencil is not the stage's pencil"
+                            : "decomposition z-pencil is not the stage's pencil");
   const int np = two ? d->p_col : d->p_row;
   if (np < 1 || np > D2DHIP_MAX_PROCS) return set_error(D2DHIP_ERR_INVALID_VALUE, "process count out of range");
-  const int *dist = two ? d->y2dist : d->y1dist;
-  const long long *disp = two ? d->y2disp : d->y1disp;
-  const long long count = two ? d->y2count : d->y1count;
-  const long long n1 = pen[0], ny = pen[1];
+  // block m of the stage's axis (b1 <= j < b1 + w) of the pencil starts at
+  // disp_m in the buffer, and its point (i, j, k) sits at
+  //   y-pencil: disp_m + i + (j - b1) n1 + k w n1      (blocks along j)
+  //   z-pencil: disp_m + i + j n1 + (k - b1) n1 n2     (blocks along k)
+  // i.e. at the plain pencil offset plus table[2] + c1 table[1] (c1 = k for
+  // a y stage, 0 for a z stage, whose columns cover the (i, j) plane)
+  const int *dist = ystage ? (two ? d->y2dist : d->y1dist) : d->z2dist;
+  const long long *disp = ystage ? (two ? d->y2disp : d->y1disp) : d->z2disp;
+  const long long count = ystage ? (two ? d->y2count : d->y1count) : d->z2count;
+  const long long n1 = pen[0], naxis = ystage ? pen[1] : pen[2];
+  const long long row = ystage ? n1 : n1 * pen[1];
   long long b1 = 0;
   for (int m = 0; m < np; ++m) {
     if (dist[m] < 0) return set_error(D2DHIP_ERR_INVALID_VALUE, "negative block size");
     p->h_ypack[3 * m] = b1;
-    p->h_ypack[3 * m + 1] = n1 * (dist[m] - ny);
-    p->h_ypack[3 * m + 2] = (even ? (long long)m * count : disp[m]) - b1 * n1;
+    p->h_ypack[3 * m + 1] = ystage ? n1 * (dist[m] - naxis) : 0;
+    p->h_ypack[3 * m + 2] = (even ? (long long)m * count : disp[m]) - b1 * row;
     b1 += dist[m];
   }
-  if (b1 != ny) return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition does not match the pencil");
+  if (b1 != naxis) return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition does not match the pencil");
   const int conj = isign == D2DHIP_BACKWARD ? 1 : 0;
   const int esize = 2 * p->rb;
   // the specialised kernel's fused form, if it has one
@@ -1822,7 +1842,7 @@ int run_y_stage_packed(d2dhip_plan_s *p, int stage, const void *in, void *out, i
       if (p->ywk) (void)hipFree(p->ywk);
       p->ywk = nullptr;
       p->ywk_bytes = 0;
-      if (hipMalloc(&p->ywk, need) != hipSuccess) return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(y-pencil workspace)");
+      if (hipMalloc(&p->ywk, need) != hipSuccess) return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(stage workspace)");
       p->ywk_bytes = need;
     }
     if (pack) {
@@ -1878,13 +1898,13 @@ int d2dhip_exec_1m(d2dhip_plan p, int stage, const void *in, void *out, int isig
 int d2dhip_exec_1m_pack(d2dhip_plan p, int stage, const void *in, void *send, int dir,
                         const d2dhip_decomp *d, int even, int isign) {
   if (!p || !in || !send || !d) return set_error(D2DHIP_ERR_NULL, "NULL argument");
-  return run_y_stage_packed(p, stage, in, send, dir, d, even, isign, true);
+  return run_stage_packed(p, stage, in, send, dir, d, even, isign, true);
 }
 
 int d2dhip_exec_1m_unpack(d2dhip_plan p, int stage, const void *recv, void *out, int dir,
                           const d2dhip_decomp *d, int even, int isign) {
   if (!p || !recv || !out || !d) return set_error(D2DHIP_ERR_NULL, "NULL argument");
-  return run_y_stage_packed(p, stage, recv, out, dir, d, even, isign, false);
+  return run_stage_packed(p, stage, recv, out, dir, d, even, isign, false);
 }
 
 // ---- hipGraph of a plan's exec calls --------------------------------------------

@@ -416,11 +416,14 @@ def bench_config(args, torch, d2d, w):
         n = 1024
         e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, rank=(4, 2, 1, 1), tune=not args.no_tune)
         ph, sp = e.decomps()
-        # the y stage on a kernel with a fused pack form (or the general kernel)
-        fused_forms = ("double n=1024 tx=16 nt=1024 c2c xs w4", "double n=1024 tx=8 nt=512 c2c xs w4",
+        # the y and z stages on kernels with a fused pack / unpack form (or
+        # the general kernel)
+        fused_forms = ("double n=1024 tx=16 nt=1024 c2c xsh w4", "double n=1024 tx=8 nt=512 c2c xsh w4",
+                       "double n=1024 tx=16 nt=1024 c2c xs w4", "double n=1024 tx=8 nt=512 c2c xs w4",
                        "double n=1024 tx=8 nt=512 c2c", "general")
-        if e.stage_kernels()["spec_y"] not in fused_forms:
-            e.set_stage_kernel("spec_y", fused_forms[0])
+        for st in ("spec_y", "spec_last"):
+            if e.stage_kernels()[st] not in fused_forms:
+                e.set_stage_kernel(st, fused_forms[0])
         nel = lambda v: int(np.prod(list(v)))
         x = torch.empty(nel(ph.xsz), dtype=torch.float64, device="cuda")
         d2d.fill_uniform(x, x.numel(), d2d.R8, 3)
@@ -445,18 +448,19 @@ def bench_config(args, torch, d2d, w):
             e.fft_r2c_pack(x, b1, sp)
             d2d.unpack(d2d.X_TO_Y, C16, b1, sy, sp)
             e.exec_1m_pack("spec_y", sy, b2, d2d.Y_TO_Z, sp, d2d.FORWARD)
-            d2d.unpack(d2d.Y_TO_Z, C16, b2, sz, sp)
-            e.exec_1m("spec_last", sz, sz, d2d.FORWARD)
+            e.exec_1m_unpack("spec_last", b2, sz, d2d.Y_TO_Z, sp, d2d.FORWARD)
         ms = timed(step)
         ms_unfused = timed(unfused)
         rs, cs = x.numel() * 8, sx.numel() * 16
-        # fused: r2c_pack (real + spectrum), two unpacks, y_pack and the z stage
-        nbytes = (rs + cs) + 2 * cs + 2 * cs + 2 * cs + 2 * cs
+        # fused: r2c_pack (real + spectrum), the x->y unpack, y_pack, and the
+        # z stage reading the y->z receive buffer
+        nbytes = (rs + cs) + 2 * cs + 2 * cs + 2 * cs
         err = 0.0
         cfg = {"workload": "forward r2c stages of rank (1,1) of 1024^3 on 4x2 (x-pencil "
-                           "1024x256x512 real8, spectral y-pencil 128x1024x512) with the packs and "
-                           "unpacks of its two exchanges, x and y stages fused with their packs; "
-                           "the exchanges themselves omitted (one GPU)"}
+                           "1024x256x512 real8, spectral y-pencil 128x1024x512, z-pencil "
+                           "128x512x1024) with the packs and unpacks of its two exchanges; x and y "
+                           "stages fused with their packs, z stage with its unpack; the exchanges "
+                           "themselves omitted (one GPU)"}
         extra["unfused_ms"] = round(ms_unfused, 4)
         extra["fused_speedup"] = round(ms_unfused / ms, 3)
         extra["kernels"] = {k: v for k, v in e.stage_kernels().items() if k in ("r2c", "spec_y",

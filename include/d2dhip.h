@@ -292,11 +292,15 @@ int d2dhip_exec_1m(d2dhip_plan plan, int stage, const void *in, void *out, int i
  * src/transpose_y_to_x.f90:266-323), or with the unpack of the exchange
  * before it (dir D2DHIP_Z_TO_Y: mem_merge_zy,
  * src/transpose_z_to_y.f90:489-547; D2DHIP_X_TO_Y: mem_merge_xy,
- * src/transpose_x_to_y.f90:384-441).  d->ysz must be the stage's pencil;
- * even selects the padded offsets.  Results equal d2dhip_exec_1m followed
- * by d2dhip_pack (d2dhip_unpack then d2dhip_exec_1m) bit for bit.  A stage
- * kernel without a fused form runs unfused through a workspace the plan
- * allocates on first use. */
+ * src/transpose_x_to_y.f90:384-441); or a z stage (D2DHIP_STAGE_C2C_Z, and
+ * D2DHIP_STAGE_SPEC_LAST of a PHYSICAL_IN_X plan) fused with the pack after
+ * it (dir D2DHIP_Z_TO_Y: mem_split_zy, src/transpose_z_to_y.f90:370-427) or
+ * the unpack before it (dir D2DHIP_Y_TO_Z: mem_merge_yz,
+ * src/transpose_y_to_z.f90:490-548).  d->ysz (d->zsz) must be the stage's
+ * pencil; even selects the padded offsets.  Results equal d2dhip_exec_1m
+ * followed by d2dhip_pack (d2dhip_unpack then d2dhip_exec_1m) bit for bit.
+ * A stage kernel without a fused form runs unfused through a workspace the
+ * plan allocates on first use. */
 int d2dhip_exec_1m_pack(d2dhip_plan plan, int stage, const void *in, void *send, int dir,
                         const d2dhip_decomp *d, int even, int isign);
 int d2dhip_exec_1m_unpack(d2dhip_plan plan, int stage, const void *recv, void *out, int dir,

@@ -1,6 +1,6 @@
 """Plans of one rank of a process grid (d2dhip_plan_create_rank), their single
 1D stages (d2dhip_exec_1m: c2c_1m_*, r2c_1m_*, c2r_1m_*, src/fft_generic.f90:112-384)
-and the y stage fused with the y-pencil exchange's pack or unpack
+and the y and z stages fused with their exchanges' pack or unpack
 (d2dhip_exec_1m_pack / _unpack).
 
 - Every stage of every rank of a 2 x 3 grid against numpy along its axis.
@@ -9,9 +9,9 @@ and the y stage fused with the y-pencil exchange's pack or unpack
   reference's counts and displacements (x1cnts/x1disp ... z2cnts/z2disp,
   src/transpose_x_to_y.f90:93-133 and the other three files): the gathered
   spectrum equals numpy's rfftn and the round trip returns N times the input.
-- At the 1024^3 rank shape of a 4 x 2 grid: the fused y stage equals the
-  stage followed by d2dhip_pack (d2dhip_unpack followed by the stage) bit for
-  bit, for every candidate kernel of the stage (the fused forms of the
+- At the 1024^3 rank shape of a 4 x 2 grid: the fused y and z stages equal
+  the stage followed by d2dhip_pack (d2dhip_unpack followed by the stage) bit
+  for bit, for every candidate kernel of the stage (the fused forms of the
   specialised kernels, the general kernel, and the unfused fallback).
 """
 import itertools
@@ -157,11 +157,15 @@ def test_distributed_round_trip_on_one_gpu(d2d, torch_gpu, fmt, grid, pg):
     recv = alltoallv(from_y, send, SP, pr, pc)
     spec = np.zeros((nx // 2 + 1, ny, nz) if fmt == 1 else (nx, ny, nz // 2 + 1), np.complex128)
     lastpen = {}
-    for rc, e in E.items():
+    for i, (rc, e) in enumerate(E.items()):
         d = SP[rc]
         z = torch.empty(int(np.prod(pen(d, last))), dtype=torch.complex128, device="cuda")
-        d2d.unpack(from_y, C16, torch.from_numpy(recv[rc]).to("cuda"), z, d)
-        e.exec_1m("spec_last", z, z, d2d.FORWARD)
+        r = torch.from_numpy(recv[rc]).to("cuda")
+        if fmt == 1 and i % 2:  # the unpack fused into the z stage
+            e.exec_1m_unpack("spec_last", r, z, from_y, d, d2d.FORWARD)
+        else:
+            d2d.unpack(from_y, C16, r, z, d)
+            e.exec_1m("spec_last", z, z, d2d.FORWARD)
         lastpen[rc] = z
         put(spec, getattr(d, last + "st"), getattr(d, last + "en"), host(z, pen(d, last)))
     want = np.fft.rfftn(u, axes=(2, 1, 0) if fmt == 1 else (0, 1, 2))
@@ -169,11 +173,14 @@ def test_distributed_round_trip_on_one_gpu(d2d, torch_gpu, fmt, grid, pg):
 
     # ---- backward: last axis, exchange, y, exchange, c2r ----------------------
     send = {}
-    for rc, e in E.items():
+    for i, (rc, e) in enumerate(E.items()):
         d, z = SP[rc], lastpen[rc]
-        e.exec_1m("spec_last", z, z, d2d.BACKWARD)
         buf = torch.empty(d2d.pack_buffer_elems(back_to_y, d), dtype=torch.complex128, device="cuda")
-        d2d.pack(back_to_y, C16, z, buf, d)
+        if fmt == 1 and i % 2 == 0:  # the z stage fused with its pack
+            e.exec_1m_pack("spec_last", z, buf, back_to_y, d, d2d.BACKWARD)
+        else:
+            e.exec_1m("spec_last", z, z, d2d.BACKWARD)
+            d2d.pack(back_to_y, C16, z, buf, d)
         send[rc] = buf.cpu().numpy()
     recv = alltoallv(back_to_y, send, SP, pr, pc)
     send = {}
@@ -207,45 +214,73 @@ def test_distributed_round_trip_on_one_gpu(d2d, torch_gpu, fmt, grid, pg):
 
 
 @pytest.mark.parametrize("prec", [64, 32, 16])
-def test_fused_y_stage_bit_exact_at_1024_rank(d2d, torch_gpu, prec):
-    """Rank (1, 1) of 1024^3 on a 4 x 2 grid, spectral y-pencil 128 x 1024 x 512:
-    every candidate kernel of the y stage, fused with the pack of y->z and
-    y->x (plain and EVEN offsets) and with the unpack of z->y and x->y."""
+@pytest.mark.parametrize("stage", ["spec_y", "spec_last"])
+def test_fused_stage_bit_exact_at_1024_rank(d2d, torch_gpu, prec, stage):
+    """Rank (1, 1) of 1024^3 on a 4 x 2 grid, spectral y-pencil 128 x 1024 x 512
+    and z-pencil 128 x 512 x 1024: every candidate kernel of the y stage fused
+    with the pack of y->z and y->x and the unpack of z->y and x->y, and of the
+    z stage fused with the pack of z->y and the unpack of y->z, plain and EVEN
+    offsets."""
     torch = torch_gpu
     n = 1024
     e = d2d.FFTEngine(1, n, n, n, prec=prec, rank=(4, 2, 1, 1))
     ph, sp = e.decomps()
-    shape = tuple(sp.ysz)
-    assert shape == (128, 1024, 512)
+    XY, YZ, ZY, YX = d2d.X_TO_Y, d2d.Y_TO_Z, d2d.Z_TO_Y, d2d.Y_TO_X
+    # pencil, exchanges after the stage (pack), exchanges before it (unpack)
+    p, packs, unpacks = {
+        "spec_y": ("y", ((YZ, False, -1), (YX, True, 1)), ((ZY, False, 1), (XY, True, -1))),
+        "spec_last": ("z", ((ZY, False, 1), (ZY, True, -1)), ((YZ, False, -1), (YZ, True, 1))),
+    }[stage]
+    shape = tuple(getattr(sp, p + "sz"))
+    assert shape == ((128, 1024, 512) if p == "y" else (128, 512, 1024))
     cdt = torch.complex128 if prec == 64 else torch.complex64
     elem = d2d.C16 if prec == 64 else d2d.C8
     g = torch.Generator(device="cuda")
     g.manual_seed(prec)
     y = torch.randn(int(np.prod(shape)), dtype=cdt, device="cuda", generator=g)
     tmp = torch.empty_like(y)
-    cands = e.stage_candidates("spec_y")
+    cands = e.stage_candidates(stage)
     assert "general" in cands and len(cands) >= 2
     for name in cands:
-        e.set_stage_kernel("spec_y", name)
-        for direction, even, isign in ((d2d.Y_TO_Z, False, -1), (d2d.Y_TO_X, True, 1)):
+        e.set_stage_kernel(stage, name)
+        for direction, even, isign in packs:
             nb = d2d.pack_buffer_elems(direction, sp, even)
             # the same sentinel in both: positions neither writes (EVEN padding)
             # must keep it
             fused = torch.full((nb,), 7.0, dtype=cdt, device="cuda")
             plain = torch.full((nb,), 7.0, dtype=cdt, device="cuda")
-            e.exec_1m_pack("spec_y", y, fused, direction, sp, isign, even)
-            e.exec_1m("spec_y", y, tmp, isign)
+            e.exec_1m_pack(stage, y, fused, direction, sp, isign, even)
+            e.exec_1m(stage, y, tmp, isign)
             d2d.pack(direction, elem, tmp, plain, sp, even)
             torch.cuda.synchronize()
             assert torch.equal(fused, plain), (name, direction, even)
-        for direction, even, isign in ((d2d.Z_TO_Y, False, 1), (d2d.X_TO_Y, True, -1)):
-            # a receive buffer holds the y-pencil (padded under EVEN)
+        for direction, even, isign in unpacks:
+            # a receive buffer holds the pencil (padded under EVEN)
             nb = max(d2d.pack_buffer_elems(direction, sp, even), int(np.prod(shape)))
             recv = torch.randn(nb, dtype=cdt, device="cuda", generator=g)
             fused = torch.empty_like(y)
             plain = torch.empty_like(y)
-            e.exec_1m_unpack("spec_y", recv, fused, direction, sp, isign, even)
+            e.exec_1m_unpack(stage, recv, fused, direction, sp, isign, even)
             d2d.unpack(direction, elem, recv, tmp, sp, even)
-            e.exec_1m("spec_y", tmp, plain, isign)
+            e.exec_1m(stage, tmp, plain, isign)
             torch.cuda.synchronize()
             assert torch.equal(fused, plain), (name, direction, even)
+
+
+def test_fused_stage_refusals(d2d, torch_gpu):
+    """The fused calls take only the maps that border their stage."""
+    torch = torch_gpu
+    e = d2d.FFTEngine(3, 30, 24, 20, rank=(2, 3, 1, 2))  # PHYSICAL_IN_Z: stage 5 is x
+    ph, sp = e.decomps()
+    a = torch.zeros(int(np.prod(sp.zsz)) + 64, dtype=torch.complex128, device="cuda")
+    b = torch.zeros_like(a)
+    with pytest.raises(d2d.D2dError):
+        e.exec_1m_pack("spec_last", a, b, d2d.X_TO_Y, sp, d2d.FORWARD)
+    e1 = d2d.FFTEngine(1, 30, 24, 20, rank=(2, 3, 1, 2))
+    ph, sp = e1.decomps()
+    with pytest.raises(d2d.D2dError):  # a z stage packs only for z->y
+        e1.exec_1m_pack("spec_last", a, b, d2d.Y_TO_X, sp, d2d.FORWARD)
+    with pytest.raises(d2d.D2dError):  # and unpacks only y->z
+        e1.exec_1m_unpack("spec_last", a, b, d2d.Z_TO_Y, sp, d2d.FORWARD)
+    with pytest.raises(d2d.D2dError):  # the x stages have their own fused calls
+        e1.exec_1m_pack("c2c_x", a, b, d2d.X_TO_Y, ph, d2d.FORWARD)
