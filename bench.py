@@ -465,6 +465,51 @@ def bench_config(args, torch, d2d, w):
         extra["fused_speedup"] = round(ms_unfused / ms, 3)
         extra["kernels"] = {k: v for k, v in e.stage_kernels().items() if k in ("r2c", "spec_y",
                                                                                "spec_last")}
+    elif w == "host512":
+        # the boundary handing over host arrays (the Fortran layer's
+        # decomp_2d_fft_3d on host arrays): real input copied in, r2c + c2r,
+        # real output copied back, PCIe included; pageable and pinned host
+        # memory
+        n = 512
+        nxc = n // 2 + 1
+        e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, tune=not args.no_tune)
+        xd = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
+        d2d.fill_uniform(xd, xd.numel(), d2d.R8, 1)
+        sp = torch.empty(nxc * n * n, dtype=torch.complex128, device="cuda")
+        yd = torch.empty_like(xd)
+        res = {}
+        for kind, pin in (("pageable", False), ("pinned", True)):
+            xh = xd.cpu()
+            yh = torch.empty(n ** 3, dtype=torch.float64)
+            if pin:
+                xh, yh = xh.pin_memory(), yh.pin_memory()
+
+            def step():
+                xd.copy_(xh, non_blocking=pin)
+                e.fft_r2c(xd, sp)
+                e.fft_c2r(sp, yd)
+                yh.copy_(yd, non_blocking=pin)
+            res[kind] = timed(step)
+            torch.cuda.synchronize()
+            res[kind + "_rel_l2"] = float(torch.linalg.vector_norm(yh / float(n ** 3) - xh) /
+                                          torch.linalg.vector_norm(xh))
+
+        def device_only():
+            e.fft_r2c(xd, sp)
+            e.fft_c2r(sp, yd)
+        ms_dev = timed(device_only)
+        ms = res["pinned"]
+        xfer = 2 * xd.numel() * 8
+        nbytes = round_trip_bytes(n, 8)
+        err = max(res["pageable_rel_l2"], res["pinned_rel_l2"])
+        cfg = {"workload": "r2c+c2r FP64 512^3 PHYSICAL_IN_X on host arrays: host-to-device copy "
+                           "of the real input, the round trip, device-to-host copy of the real "
+                           "output (value: pinned host memory)"}
+        extra["pageable_ms"] = round(res["pageable"], 4)
+        extra["pinned_ms"] = round(res["pinned"], 4)
+        extra["device_only_ms"] = round(ms_dev, 4)
+        extra["pcie_GBps_pinned"] = round(xfer / ((res["pinned"] - ms_dev) * 1e-3) / 1e9, 1)
+        extra["pcie_GBps_pageable"] = round(xfer / ((res["pageable"] - ms_dev) * 1e-3) / 1e9, 1)
     elif w == "halo512":
         # update_halo of a 512^3 real8 x-pencil, 2 halo levels, y and z periodic
         n, lev = 512, 2
@@ -522,12 +567,14 @@ def main():
                     help="skip the plan-time autotuner (use the default kernels)")
     ap.add_argument("--workload", choices=["r2c1024", "batch64", "c2c128", "r2c512", "r2cz720",
                                            "pack2048", "r2cpack2048", "dtt513", "halo512",
-                                           "rank1024"],
+                                           "rank1024", "host512"],
                     default="r2c1024",
                     help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids; "
                          "c2c128 / r2c512 / r2cz720 / pack2048: the other parity configurations, "
                          "timed; r2cpack2048: the pack2048 rank's x stage fused with its pack; "
-                         "dtt513: 3D DCT-II + DCT-III round trip; halo512: update_halo")
+                         "dtt513: 3D DCT-II + DCT-III round trip; halo512: update_halo; "
+                         "rank1024: one rank's stages with their packs and unpacks; host512: the "
+                         "512^3 round trip on host arrays, PCIe included")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -543,7 +590,7 @@ def main():
         print(json.dumps(bench_batch(args, torch, d2d)), flush=True)
         return
     if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048", "dtt513",
-                         "halo512", "rank1024"):
+                         "halo512", "rank1024", "host512"):
         print(json.dumps(bench_config(args, torch, d2d, args.workload)), flush=True)
         return
     n = args.n
