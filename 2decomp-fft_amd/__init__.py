@@ -118,6 +118,7 @@ def lib():
             "d2dhip_exec_1m": ([pl, ci, vp, vp, ci], ci),
             "d2dhip_exec_1m_pack": ([pl, ci, vp, vp, ci, ctypes.POINTER(Decomp), ci, ci], ci),
             "d2dhip_exec_1m_unpack": ([pl, ci, vp, vp, ci, ctypes.POINTER(Decomp), ci, ci], ci),
+            "d2dhip_exec_1m_unpack_pack": ([pl, ci, vp, vp, ci, ci, ctypes.POINTER(Decomp), ci, ci], ci),
             "d2dhip_plan_destroy": ([pl], ci),
             "d2dhip_plan_set_stream": ([pl, vp], ci),
             "d2dhip_plan_get_sizes": ([pl, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
@@ -315,6 +316,16 @@ class FFTEngine:
         i = self.STAGES.index(stage) if isinstance(stage, str) else stage
         return check(lib().d2dhip_exec_1m_unpack(self._plan, i, _ptr(recv), _ptr(out), direction,
                                                  ctypes.byref(decomp), int(bool(even)), isign))
+
+    def exec_1m_unpack_pack(self, stage, recv, send, dir_in, dir_out, decomp, isign,
+                            even=False):
+        """A y stage reading the receive order of `dir_in` (X_TO_Y, Z_TO_Y) and
+        writing the send order of `dir_out` (Y_TO_Z, Y_TO_X)."""
+        self._sync_stream()
+        i = self.STAGES.index(stage) if isinstance(stage, str) else stage
+        return check(lib().d2dhip_exec_1m_unpack_pack(self._plan, i, _ptr(recv), _ptr(send),
+                                                      dir_in, dir_out, ctypes.byref(decomp),
+                                                      int(bool(even)), isign))
 
     def get_sizes(self):
         ph = (ctypes.c_int * 3)()

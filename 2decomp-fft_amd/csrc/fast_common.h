@@ -61,7 +61,7 @@ template <typename T, int IO>
 __device__ __forceinline__ cpx<T> fload(DescPtr d, const void *in,
                                         long long base, bool cvalid, bool bvalid, long long ca,
                                         int j, int conj, long long c1 = 0) {
-  if constexpr (IO == IO_YUN) {
+  if constexpr (IO == IO_YUN || IO == IO_YUP) {
     cpx<T> v = {T(0), T(0)};
     if (cvalid) v = gload<T>(in, base + (long long)j * d->in_sj + packj_offset(d, j, c1));
     if (conj) v.y = -v.y;
@@ -85,10 +85,11 @@ template <typename T, int IO>
 __device__ __forceinline__ void fstore(DescPtr d, void *out, long long base,
                                        bool cvalid, long long ca, int j, cpx<T> v, int conj,
                                        long long c1 = 0) {
-  if constexpr (IO == IO_YPK) {
+  if constexpr (IO == IO_YPK || IO == IO_YUP) {
     if (cvalid) {
       if (conj) v.y = -v.y;
-      gstore<T>(out, base + (long long)j * d->out_sj + packj_offset(d, j, c1), v);
+      const long long o = IO == IO_YPK ? packj_offset(d, j, c1) : packj_offset2(d, j, c1);
+      gstore<T>(out, base + (long long)j * d->out_sj + o, v);
     }
   } else if constexpr (IO == IO_C2R) {
     const long long o = base + (long long)j * d->out_sj;

@@ -110,7 +110,8 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   const long long c0 = c0b + t;
   const long long ca = 2 * c0;  // first real column of a pair (real IO)
   constexpr bool DTT = IO == IO_DTT2 || IO == IO_DTT3 || IO == IO_DTTX || IO == IO_DTT4;
-  constexpr bool PAIRED = IO != IO_C2C && IO != IO_DTT4 && IO != IO_YPK && IO != IO_YUN;  // two real columns per tile column
+  constexpr bool PAIRED = IO != IO_C2C && IO != IO_DTT4 && IO != IO_YPK && IO != IO_YUN &&
+                          IO != IO_YUP;  // two real columns per tile column
   static_assert(!DTT || XM == 0, "real-to-real stages use the plain exchange");
   const bool cvalid = PAIRED ? (ca < d->C0r) : (c0 < d->C0);
   // real-to-real stages: the transformed points start at dtt_first
@@ -133,7 +134,7 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   constexpr int HQ = SHUF ? 64 / TX / 4 : 1;  // tile rows per quarter wave
   static_assert(!SHUF || (!LINES && P == 3 && RL::r[0] == 16 && RL::r[1] == 16 && RL::r[2] == 4 &&
                           RPT == 64 && (HQ == 1 || HQ == 2 || HQ == 4) &&
-                          (IO == IO_C2C || IO == IO_YPK || IO == IO_YUN)),
+                          (IO == IO_C2C || IO == IO_YPK || IO == IO_YUN || IO == IO_YUP)),
                 "in-wave last exchange: c2c columns of 1024 points, 4 <= TX <= 16");
   const int wq = tid >> 6, rq = (tid & 63) / TX;
   const int gq = rq / HQ, kq = (wq & 3) + 4 * HQ * (wq >> 2) + 4 * (rq % HQ);
@@ -748,11 +749,13 @@ static FastVariant make_variant(const char *name) {
       make_variant<T, N, TX, NT, IO_DTTX, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dctx"), \
       make_variant<T, N, TX, NT, IO_DTT4, LINES, RL>(#T " n=" #N " tx=" #TX " nt=" #NT " dct4")
 
-// the y-pencil stage with its output in pack order (" ypk") and with its
-// input in unpack order (" yun"), twins of the c2c variant of the same name
+// the y- / z-pencil stage with its output in pack order (" ypk"), with its
+// input in unpack order (" yun"), and with both (" yup"), twins of the c2c
+// variant of the same name
 #define D2D_YPACK(T, N, TX, NT, RL, XM, WPE, TAG)                                                     \
   make_variant<T, N, TX, NT, IO_YPK, false, RL, XM, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT " ypk" TAG), \
-      make_variant<T, N, TX, NT, IO_YUN, false, RL, XM, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT " yun" TAG)
+      make_variant<T, N, TX, NT, IO_YUN, false, RL, XM, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT " yun" TAG), \
+      make_variant<T, N, TX, NT, IO_YUP, false, RL, XM, WPE>(#T " n=" #N " tx=" #TX " nt=" #NT " yup" TAG)
 
 // c2c only, split exchange, at least WPE waves per SIMD
 #define D2D_SPLIT_C2C(T, N, TX, NT, LINES, RL, WPE)                                      \

@@ -28,14 +28,16 @@ constexpr int kRegCap = 18;
 // pass, the outputs picked from the last) and DCT-IV / DST-IV of even
 // length L as an L/2-point DFT of one line (IO_DTT4: tile columns are single
 // real lines, not pairs).
-// IO_YPK / IO_YUN: a c2c stage on the columns of a y-pencil whose output
-// (YPK) is written in, or whose input (YUN) is read from, the alltoallv
-// buffer order of the y-pencil's exchange (mem_split_yz / mem_split_yx,
-// mem_merge_zy / mem_merge_xy): the transpose's pack or unpack fused with
-// the stage.
+// IO_YPK / IO_YUN: a c2c stage on the columns of a y- or z-pencil whose
+// output (YPK) is written in, or whose input (YUN) is read from, the
+// alltoallv buffer order of the pencil's exchange (mem_split_yz /
+// mem_split_yx / mem_split_zy, mem_merge_zy / mem_merge_xy / mem_merge_yz):
+// the transpose's pack or unpack fused with the stage.  IO_YUP: both, the
+// input in one exchange's receive order and the output in the next one's
+// send order.
 enum IoKind : int {
   IO_C2C = 0, IO_R2C = 1, IO_C2R = 2, IO_R2R = 3, IO_DTT2 = 4, IO_DTT3 = 5, IO_DTTX = 6,
-  IO_DTT4 = 7, IO_YPK = 8, IO_YUN = 9
+  IO_DTT4 = 7, IO_YPK = 8, IO_YUN = 9, IO_YUP = 10
 };
 
 struct PassInfo {
@@ -80,12 +82,17 @@ struct AxisDesc {
   // point i of line l at base - b1 + i + l * w.  npack = 0: plain layout.
   const long long *pack;
   int npack;
-  // Columns of a y-pencil in buffer order (fused y-pencil pack / unpack):
-  // packj = 1 on the output side, 2 on the input side, 0 plain.  The npack
-  // blocks of points along j (the column axis) are pack[3m .. 3m+2] =
-  // {b1, dc1, dj}: point j of column (c0, c1) of block m (b1 <= j < the
-  // next b1) sits at the plain index plus dj + c1 * dc1.
+  // Columns of a y- or z-pencil in buffer order (fused pack / unpack):
+  // packj = 1 on the output side, 2 on the input side, 3 on both (input
+  // through pack, output through pack2), 0 plain.  The table holds one
+  // entry per point j of the column axis, pack[2j .. 2j+1] = {dj, dc1}:
+  // point j of column (c0, c1) sits at the plain index plus dj + c1 * dc1
+  // (dj, dc1 those of the exchange block that holds j; one load per point,
+  // no search, so that the loads of an unpacking stage wait on a single
+  // cached read).  npack / npack2: the number of entries.
   int packj;
+  const long long *pack2;
+  int npack2;
   // IO_R2R stages: transform type (1-4 DCT-I..IV, 5-8 DST-I..IV), the
   // transformed points [dtt_first, dtt_first + dtt_len) of each line of
   // dtt_n points (the others pass through); `n` is the DFT length (dtt_len,
@@ -109,20 +116,26 @@ __device__ __forceinline__ long long pack_index(const __attribute__((address_spa
   return pk[3 * m + 2] - pk[3 * m] + i + l * pk[3 * m + 1];
 }
 
+#ifdef __HIPCC__  // (a host compiler, for the host-only sanitizer build, skips these)
 // Offset of point j of outer column c1 from its plain position in the
-// j-block buffer order of d (packj != 0): the block by a binary search of
-// fixed depth over the block starts.
+// block buffer order of d (packj != 0): one 16-byte read of the point's entry.
+__device__ __forceinline__ long long packj_offset_in(const long long *table, int j, long long c1) {
+  typedef long long ll2 __attribute__((ext_vector_type(2)));
+  const __attribute__((address_space(1))) ll2 *pk =
+      (const __attribute__((address_space(1))) ll2 *)table;
+  const ll2 e = pk[j];
+  return e.x + c1 * e.y;
+}
 __device__ __forceinline__ long long packj_offset(const __attribute__((address_space(4))) AxisDesc *d,
                                                   int j, long long c1) {
-  const __attribute__((address_space(1))) long long *pk =
-      (const __attribute__((address_space(1))) long long *)d->pack;
-  int lo = 0, hi = d->npack - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if ((long long)j >= pk[3 * mid]) lo = mid; else hi = mid - 1;
-  }
-  return pk[3 * lo + 2] + c1 * pk[3 * lo + 1];
+  return packj_offset_in(d->pack, j, c1);
 }
+// the output side of a packj = 3 stage
+__device__ __forceinline__ long long packj_offset2(const __attribute__((address_space(4))) AxisDesc *d,
+                                                   int j, long long c1) {
+  return packj_offset_in(d->pack2, j, c1);
+}
+#endif
 
 // A stage descriptor as the kernels read it: constant address space (it is
 // not written while a kernel runs), so its fields are scalar loads even when
@@ -249,7 +262,7 @@ struct MultiGroup {
   long long tile0;
 };
 
-#ifdef __HIPCC__  // (a host compiler, for the host-only sanitizer build, skips it)
+#ifdef __HIPCC__  // (skipped by a host compiler, as above)
 // Group of launch tile `it` (groups sorted by tile0), wave-uniform.
 __device__ inline int find_multi_group(const MultiGroup *__restrict__ g, int n, long long it) {
   int lo = 0, hi = n - 1;

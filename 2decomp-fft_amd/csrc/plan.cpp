@@ -122,13 +122,14 @@ struct d2dhip_plan_s {
   long long pen[6][3] = {};
   bool multi = false;
   d2dhip_decomp ph{}, sp{};
-  // y-pencil stages fused with the y-pencil exchange's pack / unpack
-  // (d2dhip_exec_1m_pack / _unpack): block table and descriptor; and, for a
-  // stage kernel without a fused form, a y-pencil workspace for stage then
-  // pack (allocated on first use)
-  long long *d_ypack = nullptr;
+  // y and z stages fused with their exchanges' pack / unpack
+  // (d2dhip_exec_1m_pack / _unpack / _unpack_pack): block tables and
+  // descriptor; and, for a stage kernel without a fused form, a pencil
+  // workspace for unpack, stage, pack (allocated on first use)
+  long long *d_ypack = nullptr, *d_ypack2 = nullptr;  // per-point tables (input / output side)
+  size_t ypack_cap = 0, ypack2_cap = 0;                // their sizes in entries
   AxisDesc *d_ydesc = nullptr;
-  long long h_ypack[3 * D2DHIP_MAX_PROCS] = {};
+  std::vector<long long> h_ypack, h_ypack2;
   AxisDesc h_ydesc{};
   void *ywk = nullptr;
   size_t ywk_bytes = 0;
@@ -1292,6 +1293,7 @@ int d2dhip_plan_destroy(d2dhip_plan p) {
   if (p->d_pack) (void)hipFree(p->d_pack);
   if (p->d_pdesc) (void)hipFree(p->d_pdesc);
   if (p->d_ypack) (void)hipFree(p->d_ypack);
+  if (p->d_ypack2) (void)hipFree(p->d_ypack2);
   if (p->d_ydesc) (void)hipFree(p->d_ydesc);
   if (p->ywk) (void)hipFree(p->ywk);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
@@ -1770,21 +1772,20 @@ int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
 // that map in its stores / loads; a specialised kernel without a fused form
 // runs unfused through a y-pencil workspace.  Results equal the stage
 // followed by d2dhip_pack (d2dhip_unpack followed by the stage) bit for bit.
-int run_stage_packed(d2dhip_plan_s *p, int stage, const void *in, void *out, int dir,
-                     const d2dhip_decomp *d, int even, int isign, bool pack) {
-  if (p->capturing)
-    return set_error(D2DHIP_ERR_UNSUPPORTED, "fused pack / unpack calls are not captured into a graph");
-  // y stages: the four y-pencil maps; z stages (c2c_1m_z on the physical
-  // z-pencil, and the spectral one under PHYSICAL_IN_X): mem_split_zy after,
-  // mem_merge_yz before.  Every one of these maps cuts its pencil into blocks
-  // along the stage's own axis.
+// The block table of map `dir` on the output (pack) or input (unpack) side
+// of stage `stage` into tab (3 entries per block), np blocks.
+//
+// y stages: the four y-pencil maps; z stages (c2c_1m_z on the physical
+// z-pencil, and the spectral one under PHYSICAL_IN_X): mem_split_zy after,
+// mem_merge_yz before.  Every one of these maps cuts its pencil into blocks
+// along the stage's own axis.
+int stage_table(const d2dhip_plan_s *p, int stage, int dir, const d2dhip_decomp *d, int even,
+                bool pack, std::vector<long long> &tab) {
   const bool ystage = stage == 1 || stage == 4;
   const bool zstage = stage == 2 || (stage == 5 && p->format == D2DHIP_PHYSICAL_IN_X);
   if (!ystage && !zstage)
     return set_error(D2DHIP_ERR_INVALID_VALUE,
                      "fused pack / unpack: a y stage (1, 4) or a z stage (2; 5 under PHYSICAL_IN_X)");
-  if (isign != D2DHIP_FORWARD && isign != D2DHIP_BACKWARD)
-    return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid value of isign");
   const bool two = ystage ? (pack ? dir == D2DHIP_Y_TO_Z : dir == D2DHIP_Z_TO_Y)
                           : (pack ? dir == D2DHIP_Z_TO_Y : dir == D2DHIP_Y_TO_Z);
   const bool one = ystage && (pack ? dir == D2DHIP_Y_TO_X : dir == D2DHIP_X_TO_Y);
@@ -1816,27 +1817,55 @@ int run_stage_packed(d2dhip_plan_s *p, int stage, const void *in, void *out, int
   long long b1 = 0;
   for (int m = 0; m < np; ++m) {
     if (dist[m] < 0) return set_error(D2DHIP_ERR_INVALID_VALUE, "negative block size");
-    p->h_ypack[3 * m] = b1;
-    p->h_ypack[3 * m + 1] = ystage ? n1 * (dist[m] - naxis) : 0;
-    p->h_ypack[3 * m + 2] = (even ? (long long)m * count : disp[m]) - b1 * row;
     b1 += dist[m];
   }
   if (b1 != naxis) return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition does not match the pencil");
+  // one entry {dj, dc1} per point j of the stage's axis
+  tab.assign(2 * (size_t)naxis, 0);
+  b1 = 0;
+  for (int m = 0; m < np; ++m) {
+    const long long dj = (even ? (long long)m * count : disp[m]) - b1 * row;
+    const long long dc1 = ystage ? n1 * (dist[m] - naxis) : 0;
+    for (long long j = b1; j < b1 + dist[m]; ++j) {
+      tab[2 * j] = dj;
+      tab[2 * j + 1] = dc1;
+    }
+    b1 += dist[m];
+  }
+  return D2DHIP_SUCCESS;
+}
+
+// A y or z stage with its input in the receive order of `dir_in` (-1: plain)
+// and its output in the send order of `dir_out` (-1: plain): packj 2, 1 or
+// (both) 3.
+int run_stage_packed(d2dhip_plan_s *p, int stage, const void *in, void *out, int dir_in,
+                     int dir_out, const d2dhip_decomp *d, int even, int isign) {
+  if (p->capturing)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "fused pack / unpack calls are not captured into a graph");
+  if (isign != D2DHIP_FORWARD && isign != D2DHIP_BACKWARD)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "Invalid value of isign");
+  const bool unpack = dir_in >= 0, pack = dir_out >= 0;
+  int st;
+  if (unpack && (st = stage_table(p, stage, dir_in, d, even, false, p->h_ypack))) return st;
+  if (pack && (st = stage_table(p, stage, dir_out, d, even, true, unpack ? p->h_ypack2 : p->h_ypack)))
+    return st;
+  const int packj = unpack ? (pack ? 3 : 2) : 1;
   const int conj = isign == D2DHIP_BACKWARD ? 1 : 0;
   const int esize = 2 * p->rb;
+  const auto &pen = p->pen[stage];
   // the specialised kernel's fused form, if it has one
+  const int io = packj == 1 ? IO_YPK : packj == 2 ? IO_YUN : IO_YUP;
   const FastVariant *v = p->fast[stage], *twin = nullptr;
   if (v) {
     std::string want = v->name;
     const size_t at = want.find(" c2c");
     if (at != std::string::npos) {
-      want.replace(at, 4, pack ? " ypk" : " yun");
+      want.replace(at, 4, packj == 1 ? " ypk" : packj == 2 ? " yun" : " yup");
       for (const FastVariant &c : fast_registry())
-        if (c.rb == p->rb && c.io == (pack ? IO_YPK : IO_YUN) && want == c.name) twin = &c;
+        if (c.rb == p->rb && c.io == io && want == c.name) twin = &c;
     }
   }
-  int st;
-  if (v && !twin) {  // stage then pack (unpack then stage) through a workspace
+  if (v && !twin) {  // unpack, stage, pack through a workspace
     const size_t need = (size_t)pencil_elems(pen) * esize;
     if (p->ywk_bytes < need) {
       if (p->ywk) (void)hipFree(p->ywk);
@@ -1845,34 +1874,52 @@ int run_stage_packed(d2dhip_plan_s *p, int stage, const void *in, void *out, int
       if (hipMalloc(&p->ywk, need) != hipSuccess) return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(stage workspace)");
       p->ywk_bytes = need;
     }
-    if (pack) {
-      if ((st = run(p, stage, in, p->ywk, conj))) return st;
-      return launch_pack(dir, 1, esize, p->ywk, out, d, even, p->stream);
+    const void *src = in;
+    if (unpack) {
+      if ((st = launch_pack(dir_in, 0, esize, in, p->ywk, d, even, p->stream))) return st;
+      src = p->ywk;
     }
-    if ((st = launch_pack(dir, 0, esize, in, p->ywk, d, even, p->stream))) return st;
-    return run(p, stage, p->ywk, out, conj);
+    if (!pack) return run(p, stage, src, out, conj);
+    if ((st = run(p, stage, src, p->ywk, conj))) return st;
+    return launch_pack(dir_out, 1, esize, p->ywk, out, d, even, p->stream);
   }
-  if (!p->d_ypack && hipMalloc(&p->d_ypack, sizeof(p->h_ypack)) != hipSuccess) {
-    p->d_ypack = nullptr;
-    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(y pack table)");
-  }
+  auto fit = [&](long long *&buf, size_t &cap, size_t n) {
+    if (cap >= n) return D2DHIP_SUCCESS;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    if (hipMalloc(&buf, n * sizeof(long long)) != hipSuccess) {
+      buf = nullptr;
+      return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(pack table)");
+    }
+    cap = n;
+    return D2DHIP_SUCCESS;
+  };
+  if ((st = fit(p->d_ypack, p->ypack_cap, p->h_ypack.size()))) return st;
+  if (packj == 3 && (st = fit(p->d_ypack2, p->ypack2_cap, p->h_ypack2.size()))) return st;
   if (!p->d_ydesc && hipMalloc(&p->d_ydesc, sizeof(AxisDesc)) != hipSuccess) {
     p->d_ydesc = nullptr;
-    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(y pack descriptor)");
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(pack descriptor)");
   }
   p->h_ydesc = p->h_desc[stage];
   p->h_ydesc.pack = p->d_ypack;
-  p->h_ydesc.npack = np;
-  p->h_ydesc.packj = pack ? 1 : 2;
-  st = upload_sync(p, p->d_ypack, p->h_ypack, sizeof(long long) * 3 * np, "hipMemcpy(y pack table)");
-  if (!st) st = upload_sync(p, p->d_ydesc, &p->h_ydesc, sizeof(AxisDesc), "hipMemcpy(y pack descriptor)");
+  p->h_ydesc.npack = (int)(p->h_ypack.size() / 2);
+  p->h_ydesc.pack2 = packj == 3 ? p->d_ypack2 : nullptr;
+  p->h_ydesc.npack2 = packj == 3 ? (int)(p->h_ypack2.size() / 2) : 0;
+  p->h_ydesc.packj = packj;
+  st = upload_sync(p, p->d_ypack, p->h_ypack.data(), sizeof(long long) * p->h_ypack.size(),
+                   "hipMemcpy(pack table)");
+  if (!st && packj == 3)
+    st = upload_sync(p, p->d_ypack2, p->h_ypack2.data(), sizeof(long long) * p->h_ypack2.size(),
+                     "hipMemcpy(pack table)");
+  if (!st) st = upload_sync(p, p->d_ydesc, &p->h_ydesc, sizeof(AxisDesc), "hipMemcpy(pack descriptor)");
   if (st) return st;
-  const AxisIo io = {in, out, conj};
+  const AxisIo aio = {in, out, conj};
   if (twin) {
     if ((st = hip_check(twin->prepare(), "hipFuncSetAttribute(fast kernel)"))) return st;
-    return twin->launch(p->d_ydesc, p->h_ydesc, io, p->stream);
+    return twin->launch(p->d_ydesc, p->h_ydesc, aio, p->stream);
   }
-  return launch_axis(p->rb, IO_C2C, p->d_ydesc, p->h_ydesc, io, p->shape[stage], p->stream);
+  return launch_axis(p->rb, IO_C2C, p->d_ydesc, p->h_ydesc, aio, p->shape[stage], p->stream);
 }
 
 }  // namespace
@@ -1898,13 +1945,23 @@ int d2dhip_exec_1m(d2dhip_plan p, int stage, const void *in, void *out, int isig
 int d2dhip_exec_1m_pack(d2dhip_plan p, int stage, const void *in, void *send, int dir,
                         const d2dhip_decomp *d, int even, int isign) {
   if (!p || !in || !send || !d) return set_error(D2DHIP_ERR_NULL, "NULL argument");
-  return run_stage_packed(p, stage, in, send, dir, d, even, isign, true);
+  return run_stage_packed(p, stage, in, send, -1, dir, d, even, isign);
 }
 
 int d2dhip_exec_1m_unpack(d2dhip_plan p, int stage, const void *recv, void *out, int dir,
                           const d2dhip_decomp *d, int even, int isign) {
   if (!p || !recv || !out || !d) return set_error(D2DHIP_ERR_NULL, "NULL argument");
-  return run_stage_packed(p, stage, recv, out, dir, d, even, isign, false);
+  return run_stage_packed(p, stage, recv, out, dir, -1, d, even, isign);
+}
+
+int d2dhip_exec_1m_unpack_pack(d2dhip_plan p, int stage, const void *recv, void *send,
+                               int dir_in, int dir_out, const d2dhip_decomp *d, int even,
+                               int isign) {
+  if (!p || !recv || !send || !d) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  if (stage != 1 && stage != 4)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "unpack and pack around one stage: a y stage (1, 4)");
+  if (dir_in < 0 || dir_out < 0) return set_error(D2DHIP_ERR_INVALID_VALUE, "bad exchange direction");
+  return run_stage_packed(p, stage, recv, send, dir_in, dir_out, d, even, isign);
 }
 
 // ---- hipGraph of a plan's exec calls --------------------------------------------

@@ -446,21 +446,21 @@ def bench_config(args, torch, d2d, w):
 
         def step():
             e.fft_r2c_pack(x, b1, sp)
-            d2d.unpack(d2d.X_TO_Y, C16, b1, sy, sp)
-            e.exec_1m_pack("spec_y", sy, b2, d2d.Y_TO_Z, sp, d2d.FORWARD)
+            e.exec_1m_unpack_pack("spec_y", b1, b2, d2d.X_TO_Y, d2d.Y_TO_Z, sp, d2d.FORWARD)
             e.exec_1m_unpack("spec_last", b2, sz, d2d.Y_TO_Z, sp, d2d.FORWARD)
         ms = timed(step)
         ms_unfused = timed(unfused)
         rs, cs = x.numel() * 8, sx.numel() * 16
-        # fused: r2c_pack (real + spectrum), the x->y unpack, y_pack, and the
-        # z stage reading the y->z receive buffer
-        nbytes = (rs + cs) + 2 * cs + 2 * cs + 2 * cs
+        # fused: r2c_pack (real + spectrum), the y stage from the x->y receive
+        # buffer to the y->z send buffer, the z stage from the y->z receive
+        # buffer: three passes
+        nbytes = (rs + cs) + 2 * cs + 2 * cs
         err = 0.0
         cfg = {"workload": "forward r2c stages of rank (1,1) of 1024^3 on 4x2 (x-pencil "
                            "1024x256x512 real8, spectral y-pencil 128x1024x512, z-pencil "
-                           "128x512x1024) with the packs and unpacks of its two exchanges; x and y "
-                           "stages fused with their packs, z stage with its unpack; the exchanges "
-                           "themselves omitted (one GPU)"}
+                           "128x512x1024) with the packs and unpacks of its two exchanges, every "
+                           "one fused into a stage (x: pack; y: unpack and pack; z: unpack); the "
+                           "exchanges themselves omitted (one GPU)"}
         extra["unfused_ms"] = round(ms_unfused, 4)
         extra["fused_speedup"] = round(ms_unfused / ms, 3)
         extra["kernels"] = {k: v for k, v in e.stage_kernels().items() if k in ("r2c", "spec_y",

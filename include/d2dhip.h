@@ -305,6 +305,16 @@ int d2dhip_exec_1m_pack(d2dhip_plan plan, int stage, const void *in, void *send,
                         const d2dhip_decomp *d, int even, int isign);
 int d2dhip_exec_1m_unpack(d2dhip_plan plan, int stage, const void *recv, void *out, int dir,
                           const d2dhip_decomp *d, int even, int isign);
+/* A y stage between two exchanges with both fused: its input read in the
+ * receive order of dir_in (D2DHIP_X_TO_Y: mem_merge_xy, or D2DHIP_Z_TO_Y:
+ * mem_merge_zy) and its output written in the send order of dir_out
+ * (D2DHIP_Y_TO_Z: mem_split_yz, or D2DHIP_Y_TO_X: mem_split_yx), as
+ * src/fft_common_3d.f90:145-158 does with a transpose on each side of
+ * c2c_1m_y.  Equals d2dhip_unpack, d2dhip_exec_1m and d2dhip_pack bit for
+ * bit. */
+int d2dhip_exec_1m_unpack_pack(d2dhip_plan plan, int stage, const void *recv, void *send,
+                               int dir_in, int dir_out, const d2dhip_decomp *d, int even,
+                               int isign);
 
 /* ---- many grids in one launch ---------------------------------------------
  * The multiple-grid use of the reference (decomp_2d_fft_set_ngrid /

@@ -10,9 +10,10 @@ and the y and z stages fused with their exchanges' pack or unpack
   src/transpose_x_to_y.f90:93-133 and the other three files): the gathered
   spectrum equals numpy's rfftn and the round trip returns N times the input.
 - At the 1024^3 rank shape of a 4 x 2 grid: the fused y and z stages equal
-  the stage followed by d2dhip_pack (d2dhip_unpack followed by the stage) bit
-  for bit, for every candidate kernel of the stage (the fused forms of the
-  specialised kernels, the general kernel, and the unfused fallback).
+  the stage followed by d2dhip_pack (d2dhip_unpack followed by the stage, and
+  for the y stage unpack, stage and pack) bit for bit, for every candidate
+  kernel of the stage (the fused forms of the specialised kernels, the general
+  kernel, and the unfused fallback).
 """
 import itertools
 
@@ -147,7 +148,9 @@ def test_distributed_round_trip_on_one_gpu(d2d, torch_gpu, fmt, grid, pg):
         y = torch.empty(int(np.prod(pen(d, "y"))), dtype=torch.complex128, device="cuda")
         buf = torch.empty(d2d.pack_buffer_elems(from_y, d), dtype=torch.complex128, device="cuda")
         r = torch.from_numpy(recv[rc]).to("cuda")
-        if i % 2 == 0:  # unpack fused into the y stage, then the plain pack
+        if i % 3 == 2:  # both fused into the y stage
+            e.exec_1m_unpack_pack("spec_y", r, buf, to_y, from_y, d, d2d.FORWARD)
+        elif i % 2 == 0:  # unpack fused into the y stage, then the plain pack
             e.exec_1m_unpack("spec_y", r, y, to_y, d, d2d.FORWARD)
             d2d.pack(from_y, C16, y, buf, d)
         else:  # plain unpack, then the y stage fused with the pack
@@ -190,7 +193,9 @@ def test_distributed_round_trip_on_one_gpu(d2d, torch_gpu, fmt, grid, pg):
         buf = torch.empty(d2d.pack_buffer_elems(back_from_y, d), dtype=torch.complex128,
                           device="cuda")
         r = torch.from_numpy(recv[rc]).to("cuda")
-        if i % 2:
+        if i % 3 == 1:
+            e.exec_1m_unpack_pack("spec_y", r, buf, back_to_y, back_from_y, d, d2d.BACKWARD)
+        elif i % 2:
             e.exec_1m_unpack("spec_y", r, y, back_to_y, d, d2d.BACKWARD)
             d2d.pack(back_from_y, C16, y, buf, d)
         else:
@@ -265,6 +270,22 @@ def test_fused_stage_bit_exact_at_1024_rank(d2d, torch_gpu, prec, stage):
             e.exec_1m(stage, tmp, plain, isign)
             torch.cuda.synchronize()
             assert torch.equal(fused, plain), (name, direction, even)
+        if stage != "spec_y":
+            continue
+        # both sides at once: the forward's x->y unpack and y->z pack, the
+        # backward's z->y unpack and y->x pack
+        for d_in, d_out, even, isign in ((XY, YZ, False, -1), (ZY, YX, True, 1)):
+            nb = max(d2d.pack_buffer_elems(d_in, sp, even), int(np.prod(shape)))
+            recv = torch.randn(nb, dtype=cdt, device="cuda", generator=g)
+            ns = d2d.pack_buffer_elems(d_out, sp, even)
+            fused = torch.full((ns,), 7.0, dtype=cdt, device="cuda")
+            plain = torch.full((ns,), 7.0, dtype=cdt, device="cuda")
+            e.exec_1m_unpack_pack(stage, recv, fused, d_in, d_out, sp, isign, even)
+            d2d.unpack(d_in, elem, recv, tmp, sp, even)
+            e.exec_1m(stage, tmp, tmp, isign)
+            d2d.pack(d_out, elem, tmp, plain, sp, even)
+            torch.cuda.synchronize()
+            assert torch.equal(fused, plain), (name, d_in, d_out, even)
 
 
 def test_fused_stage_refusals(d2d, torch_gpu):
