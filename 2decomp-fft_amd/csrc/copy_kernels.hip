@@ -107,17 +107,22 @@ __device__ __forceinline__ int pack_block(const PackArgs &a, int key) {
 // Vector form for maps whose runs all start on 16-byte boundaries (checked
 // on the host: n1, every block start / width on the i axis and every
 // displacement multiples of the vector): 16 bytes per lane.  A unit is rpu
-// whole pencil rows (rows of fewer than 1024 vectors) or one chunk of 1024
-// vectors of a row; each thread moves 4 vectors, loads first.  The block
-// of a row (blocks on j or k) or of a vector (blocks on i) is looked up
-// once, not per element.
+// whole pencil rows (rows of fewer than 1024 vectors), a 1024-vector chunk
+// of a row (rows of a multiple of 1024 vectors) or, for other long rows,
+// 1024 consecutive vectors of the pencil taken as one flat run (a row of
+// 1025 vectors would otherwise leave a unit with one), the row found by a
+// division through the FP64 reciprocal of the row length, corrected to
+// exact; each thread moves 4 vectors, loads first.  The block of a row
+// (blocks on j or k) or of a vector (blocks on i) is looked up once, not per
+// element.
 template <int AX>
 __global__ void __launch_bounds__(256) pack_vec_kernel(PackArgs a, int vpe, long long units,
-                                                       int cpr, int rpu, long long rows) {
+                                                       int rpu, long long rows, double inv_nvr) {
   constexpr int NV = 4, CH = 256 * NV;
   const E16 *src = reinterpret_cast<const E16 *>(a.src);
   E16 *dst = reinterpret_cast<E16 *>(a.dst);
   const int nvr = a.n1 / vpe;  // vectors per pencil row
+  const long long total = rows * nvr;
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
     long long pv[NV], bv[NV];  // pencil and buffer vector index
     E16 v[NV];
@@ -130,10 +135,23 @@ __global__ void __launch_bounds__(256) pack_vec_kernel(PackArgs a, int vpe, long
         row = u * rpu + t / nvr;
         iv = t % nvr;
         if (t >= rpu * nvr) row = rows;  // past the unit
-      } else {
+      } else if (nvr % CH == 0) {  // whole chunks of a row
+        const int cpr = nvr / CH;
         row = u / cpr;
         iv = (int)(u - row * cpr) * CH + t;
-        if (iv >= nvr) row = rows;
+      } else {
+        const long long g = u * CH + t;  // flat vector index
+        row = (long long)((double)g * inv_nvr);
+        long long r = g - row * nvr;
+        if (r < 0) {  // the product rounded up past a row boundary
+          --row;
+          r += nvr;
+        } else if (r >= nvr) {
+          ++row;
+          r -= nvr;
+        }
+        iv = (int)r;
+        if (g >= total) row = rows;
       }
       pv[e] = -1;
       if (row < rows) {
@@ -254,18 +272,18 @@ int launch_pack(int dir, int split, int esize, const void *src, void *dst, const
     const long long rows = (long long)a.n2 * a.n3;
     const int nvr = a.n1 / vpe;
     const int rpu = nvr >= 1024 ? 1 : 1024 / nvr;
-    const int vcpr = (nvr + 1023) / 1024;
-    const long long vunits = rpu > 1 ? (rows + rpu - 1) / rpu : rows * vcpr;
+    const long long vunits = rpu > 1 ? (rows + rpu - 1) / rpu : (rows * nvr + 1023) / 1024;
     const long long vgrid = vunits < (1LL << 20) ? vunits : (1LL << 20);
+    const double inv = 1.0 / (double)nvr;
     if (a.ax == 0)
       hipLaunchKernelGGL((pack_vec_kernel<0>), dim3((unsigned)vgrid), dim3(256), 0, st, a, vpe, vunits,
-                         vcpr, rpu, rows);
+                         rpu, rows, inv);
     else if (a.ax == 1)
       hipLaunchKernelGGL((pack_vec_kernel<1>), dim3((unsigned)vgrid), dim3(256), 0, st, a, vpe, vunits,
-                         vcpr, rpu, rows);
+                         rpu, rows, inv);
     else
       hipLaunchKernelGGL((pack_vec_kernel<2>), dim3((unsigned)vgrid), dim3(256), 0, st, a, vpe, vunits,
-                         vcpr, rpu, rows);
+                         rpu, rows, inv);
     return hip_check(hipGetLastError(), "pack_vec_kernel launch");
   }
   const int cpr = (a.n1 + 1023) / 1024;

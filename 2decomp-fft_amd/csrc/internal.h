@@ -77,9 +77,10 @@ struct AxisDesc {
   const void *bhat;   // DFT_M of the chirp sequence exp(+i pi m^2 / nblue), wrapped
   // Complex side of a line stage (r2c output, c2r input) in the x-pencil
   // send / receive-buffer order of a process grid (a transform fused with
-  // the pack or unpack): npack blocks of a line's points, block m holding
-  // points [b1, b1 + w) described by pack[3m .. 3m+2] = {b1, w, base}, and
-  // point i of line l at base - b1 + i + l * w.  npack = 0: plain layout.
+  // the pack or unpack): block m of a line's points holds [b1, b1 + w) and
+  // puts point i of line l at base - b1 + i + l * w; pack[2i .. 2i+1] =
+  // {base - b1, w} of the block holding point i.  npack = 0: plain layout,
+  // else the number of blocks.
   const long long *pack;
   int npack;
   // Columns of a y- or z-pencil in buffer order (fused pack / unpack):
@@ -105,16 +106,19 @@ struct AxisDesc {
   const void *dtw;
 };
 
-// Index of complex point i of line l in the packed order of d (npack > 0).
+#ifdef __HIPCC__  // (a host compiler, for the host-only sanitizer build, skips these)
+// Index of complex point i of line l in the packed order of d (npack > 0):
+// one 16-byte read of the point's entry {base - b1, w} (those of the block
+// holding i), so that a fused unpack's loads wait on a single cached read.
 __device__ __forceinline__ long long pack_index(const __attribute__((address_space(4))) AxisDesc *d,
                                                 int i, long long l) {
-  const __attribute__((address_space(1))) long long *pk =
-      (const __attribute__((address_space(1))) long long *)d->pack;
-  const int np = d->npack;
-  int m = 0;
-  while (m + 1 < np && (long long)i >= pk[3 * (m + 1)]) ++m;
-  return pk[3 * m + 2] - pk[3 * m] + i + l * pk[3 * m + 1];
+  typedef long long ll2 __attribute__((ext_vector_type(2)));
+  const __attribute__((address_space(1))) ll2 *pk =
+      (const __attribute__((address_space(1))) ll2 *)d->pack;
+  const ll2 e = pk[i];
+  return e.x + i + l * e.y;
 }
+#endif
 
 #ifdef __HIPCC__  // (a host compiler, for the host-only sanitizer build, skips these)
 // Offset of point j of outer column c1 from its plain position in the

@@ -104,9 +104,10 @@ struct d2dhip_plan_s {
   std::vector<hipEvent_t> pool;
   // fused pack / unpack: the x stage's descriptor with a block table of the
   // process grid (d2dhip_exec_r2c_pack, d2dhip_exec_c2r_unpack)
-  long long *d_pack = nullptr;
+  long long *d_pack = nullptr;  // per-point table, pack_cap entries
+  size_t pack_cap = 0;
   AxisDesc *d_pdesc = nullptr;
-  long long h_pack[3 * D2DHIP_MAX_PROCS] = {};
+  std::vector<long long> h_pack;
   AxisDesc h_pdesc{};
   // hipGraph of the exec calls made between capture_begin and capture_end
   hipStream_t cap_stream = nullptr, cap_keep = nullptr;
@@ -1728,15 +1729,29 @@ int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
   long long b1 = 0;
   for (int m = 0; m < np; ++m) {
     if (d->x1dist[m] < 0) return set_error(D2DHIP_ERR_INVALID_VALUE, "negative block size");
-    p->h_pack[3 * m] = b1;
-    p->h_pack[3 * m + 1] = d->x1dist[m];
-    p->h_pack[3 * m + 2] = even ? (long long)m * d->x1count : d->x1disp[m];
     b1 += d->x1dist[m];
   }
   if (b1 != nxc) return set_error(D2DHIP_ERR_INVALID_VALUE, "decomposition does not match the pencil");
-  if (!p->d_pack && hipMalloc(&p->d_pack, sizeof(p->h_pack)) != hipSuccess) {
+  // {base - b1, w} of point i's block, for every complex point i of a line
+  p->h_pack.assign(2 * (size_t)nxc, 0);
+  b1 = 0;
+  for (int m = 0; m < np; ++m) {
+    const long long base = even ? (long long)m * d->x1count : d->x1disp[m];
+    for (long long i = b1; i < b1 + d->x1dist[m]; ++i) {
+      p->h_pack[2 * i] = base - b1;
+      p->h_pack[2 * i + 1] = d->x1dist[m];
+    }
+    b1 += d->x1dist[m];
+  }
+  if (p->pack_cap < p->h_pack.size()) {
+    if (p->d_pack) (void)hipFree(p->d_pack);
     p->d_pack = nullptr;
-    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(pack table)");
+    p->pack_cap = 0;
+    if (hipMalloc(&p->d_pack, p->h_pack.size() * sizeof(long long)) != hipSuccess) {
+      p->d_pack = nullptr;
+      return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(pack table)");
+    }
+    p->pack_cap = p->h_pack.size();
   }
   if (!p->d_pdesc && hipMalloc(&p->d_pdesc, sizeof(AxisDesc)) != hipSuccess) {
     p->d_pdesc = nullptr;
@@ -1745,13 +1760,9 @@ int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
   p->h_pdesc = p->h_desc[idx];
   p->h_pdesc.pack = p->d_pack;
   p->h_pdesc.npack = np;
-  int st = hip_check(hipMemcpyAsync(p->d_pack, p->h_pack, sizeof(long long) * 3 * np,
-                                    hipMemcpyHostToDevice, p->stream),
-                     "hipMemcpyAsync(pack table)");
-  if (!st)
-    st = hip_check(hipMemcpyAsync(p->d_pdesc, &p->h_pdesc, sizeof(AxisDesc), hipMemcpyHostToDevice,
-                                  p->stream),
-                   "hipMemcpyAsync(pack descriptor)");
+  int st = upload_sync(p, p->d_pack, p->h_pack.data(), sizeof(long long) * p->h_pack.size(),
+                       "hipMemcpy(pack table)");
+  if (!st) st = upload_sync(p, p->d_pdesc, &p->h_pdesc, sizeof(AxisDesc), "hipMemcpy(pack descriptor)");
   if (st) return st;
   const AxisIo io = {in, out, 0};
   return p->fast[idx] ? p->fast[idx]->launch(p->d_pdesc, p->h_pdesc, io, p->stream)
@@ -1765,16 +1776,17 @@ int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
 // mem_split_yx, src/transpose_y_to_x.f90:266-323) or its input in the
 // receive order of dir (D2DHIP_Z_TO_Y: mem_merge_zy,
 // src/transpose_z_to_y.f90:489-547; D2DHIP_X_TO_Y: mem_merge_xy,
-// src/transpose_x_to_y.f90:384-441).  All four keep natural order inside
-// blocks of the y axis: point (i, j, k) of block m (b1 <= j < b1 + w) at
-// disp_m + i + (j - b1) n1 + k w n1 (disp_m = m * count under EVEN).  The
-// stage kernel's fused form (" ypk" / " yun") or the general kernel applies
-// that map in its stores / loads; a specialised kernel without a fused form
-// runs unfused through a y-pencil workspace.  Results equal the stage
-// followed by d2dhip_pack (d2dhip_unpack followed by the stage) bit for bit.
-// The block table of map `dir` on the output (pack) or input (unpack) side
-// of stage `stage` into tab (3 entries per block), np blocks.
+// src/transpose_x_to_y.f90:384-441), and the z stages likewise with
+// mem_split_zy / mem_merge_yz.  All keep natural order inside blocks of the
+// stage's own axis (disp_m = m * count under EVEN).  The stage kernel's
+// fused form (" ypk" / " yun" / " yup") or the general kernel applies the
+// map in its stores / loads; a specialised kernel without a fused form runs
+// unfused through a pencil workspace.  Results equal the stage and
+// d2dhip_pack / d2dhip_unpack bit for bit.
 //
+// stage_table: the per-point table ({dj, dc1} for every point of the
+// stage's axis) of map `dir` on the output (pack) or input (unpack) side of
+// stage `stage`.
 // y stages: the four y-pencil maps; z stages (c2c_1m_z on the physical
 // z-pencil, and the spectral one under PHYSICAL_IN_X): mem_split_zy after,
 // mem_merge_yz before.  Every one of these maps cuts its pencil into blocks
