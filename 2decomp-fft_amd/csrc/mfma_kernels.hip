@@ -157,15 +157,45 @@ __device__ __forceinline__ unsigned swap_neg(unsigned w) {
   return __builtin_amdgcn_alignbit(w, w, 16) ^ 0x80000000u;
 }
 
+// The workgroup's LDS copy of the split DFT-16 operands (SW >= 2), part
+// major so that a wave's read of one part is 64 consecutive 16-byte words:
+// entry (which * 3 + part) * 64 + lane, which 0 = Ar, 1 = Ai, part 0 = h,
+// 1 = m, 2 = l.
+constexpr int kW16LdsBytes = 6 * 64 * 16;
+
+// The six products with A read from that LDS copy next to its MFMAs, one
+// part (4 VGPRs) live at a time, small terms first as far as the part order
+// allows: l.h, m.m, m.h, h.l, h.m, h.h (the terms of size 2^-16 of the
+// leading one are all added before it).  The opaque zero keeps the compiler
+// from merging the reads of different groups into registers held across
+// the pass, which would give back the registers this form saves.
+__device__ __forceinline__ f32x4 mfma6_lds(const u32x4 *tab, int which, const Split3 &b) {
+  int z = 0;
+  asm volatile("" : "+s"(z));
+  const u32x4 *p = tab + z + which * 3 * 64;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  const bf16x8 al = __builtin_bit_cast(bf16x8, p[2 * 64]);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b.h, acc, 0, 0, 0);
+  const bf16x8 am = __builtin_bit_cast(bf16x8, p[64]);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, b.m, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, b.h, acc, 0, 0, 0);
+  const bf16x8 ah = __builtin_bit_cast(bf16x8, p[0]);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b.l, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b.m, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b.h, acc, 0, 0, 0);
+}
+
 // Radix-16 DFT of 16 butterflies spread over the wave: x holds this lane's
 // inputs 4*(l/16) .. +3 (already twiddled), y receives its outputs
 // 4*(l/16) .. +3.  SW = 0: yr = Ar x, yi = Ai x.  SW = 1: Ai is not held;
 // yi = Ar x' with x'_s = (im_s, -re_s), since Ai[i][2s] = -Ar[i][2s+1] and
 // Ai[i][2s+1] = Ar[i][2s] (12 fewer registers, 12 more VALU operations per
-// group).
+// group).  SW = 2 / 3: as 0 / 1 with A read from the LDS copy `tab` (this
+// lane's entry of part 0) instead of held in registers.
 template <int SW>
 __device__ __forceinline__ void mf_dft16(const cpx<float> (&x)[4], cpx<float> (&y)[4],
-                                         const Split3 &ar, const Split3 &ai) {
+                                         const Split3 &ar, const Split3 &ai,
+                                         const u32x4 *tab = nullptr) {
   u32x4 h, m, l;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -179,9 +209,9 @@ __device__ __forceinline__ void mf_dft16(const cpx<float> (&x)[4], cpx<float> (&
   bs.h = __builtin_bit_cast(bf16x8, h);
   bs.m = __builtin_bit_cast(bf16x8, m);
   bs.l = __builtin_bit_cast(bf16x8, l);
-  const f32x4 yr = mfma6(ar, bs);
+  const f32x4 yr = SW >= 2 ? mfma6_lds(tab, 0, bs) : mfma6(ar, bs);
   f32x4 yi;
-  if constexpr (SW) {
+  if constexpr (SW == 1 || SW == 3) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       h[e] = swap_neg(h[e]);
@@ -191,9 +221,9 @@ __device__ __forceinline__ void mf_dft16(const cpx<float> (&x)[4], cpx<float> (&
     bs.h = __builtin_bit_cast(bf16x8, h);
     bs.m = __builtin_bit_cast(bf16x8, m);
     bs.l = __builtin_bit_cast(bf16x8, l);
-    yi = mfma6(ar, bs);
+    yi = SW == 3 ? mfma6_lds(tab, 0, bs) : mfma6(ar, bs);
   } else {
-    yi = mfma6(ai, bs);
+    yi = SW == 2 ? mfma6_lds(tab, 1, bs) : mfma6(ai, bs);
   }
 #pragma unroll
   for (int e = 0; e < 4; ++e) y[e] = cpx<float>{yr[e], yi[e]};
@@ -242,6 +272,11 @@ struct MfShape {
   static constexpr size_t tw_lds_off = (C::lds_bytes() + 15) / 16 * 16;
   static constexpr size_t lds_bytes(bool twl) {
     return twl ? tw_lds_off + (size_t)TWN * sizeof(cpx<float>) : C::lds_bytes();
+  }
+  // the DFT-16 operand table of the SW >= 2 kernels, after the tile
+  static constexpr size_t w16_lds_off = tw_lds_off;
+  static constexpr size_t lds_bytes_sw(int sw) {
+    return sw >= 2 ? w16_lds_off + kW16LdsBytes : C::lds_bytes();
   }
 };
 
@@ -390,6 +425,8 @@ __device__ __forceinline__ void mf_run(DescPtr d, void *out, int conj, long long
   auto at = [&](int j, int t) __attribute__((always_inline)) { return C::row(j) * LDJ + t * LDT; };
 
   const cpx<T> *ldtw = reinterpret_cast<const cpx<T> *>(lds_raw + S::tw_lds_off);
+  static_assert(SW < 2 || !TWL, "the operand table and LDS twiddles share the space after the tile");
+  const u32x4 *w16tab = reinterpret_cast<const u32x4 *>(lds_raw + S::w16_lds_off) + lane;
   constexpr int NB16 = N / 16;
   cpx<T> v[E];
   if constexpr (IO == IO_C2R) {
@@ -443,7 +480,7 @@ __device__ __forceinline__ void mf_run(DescPtr d, void *out, int conj, long long
 #pragma unroll
   for (int q = 0; q < GW; ++q) {
     cpx<T> x[4] = {v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]}, y[4];
-    mf_dft16<SW>(x, y, ar, ai);
+    mf_dft16<SW>(x, y, ar, ai, w16tab);
 #pragma unroll
     for (int e = 0; e < 4; ++e) v[4 * q + e] = y[e];
   }
@@ -493,7 +530,7 @@ __device__ __forceinline__ void mf_run(DescPtr d, void *out, int conj, long long
       cpx<T> x[4], y[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) x[e] = cmul(v[4 * q + e], w1[e]);
-      mf_dft16<SW>(x, y, ar, ai);
+      mf_dft16<SW>(x, y, ar, ai, w16tab);
 #pragma unroll
       for (int e = 0; e < 4; ++e) v[4 * q + e] = y[e];
     }
@@ -695,7 +732,20 @@ __device__ __forceinline__ void mf_tile(DescPtr d, const void *in, void *out, in
   // loses most of its memory-level parallelism
   __builtin_amdgcn_sched_barrier(0);
   Split3 ar, ai;
-  w16_operands(threadIdx.x & 63, ar, ai);
+  if constexpr (SW >= 2) {
+    // the operand table into LDS (read before the first butterfly, which
+    // the barrier orders)
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    using S = MfShape<N, TX, NT, IO, LINES, RL, XM>;
+    u32x4 *tab = reinterpret_cast<u32x4 *>(lds_raw + S::w16_lds_off);
+    for (int e = threadIdx.x; e < 6 * 64; e += NT) {
+      const int part = e / 64, lane = e % 64;
+      tab[e] = *reinterpret_cast<const u32x4 *>(&kW16P.w[lane][part / 3][part % 3][0]);
+    }
+    __syncthreads();
+  } else {
+    w16_operands(threadIdx.x & 63, ar, ai);
+  }
   mf_run<N, TX, NT, IO, LINES, RL, XM, SW, false>(d, out, conj, tile, f, ar, ai);
 }
 
@@ -723,42 +773,42 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 
 template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 static int launch_mf(const AxisDesc *dd, const AxisDesc &h, const AxisIo &io, hipStream_t st) {
-  using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
+  using S = MfShape<N, TX, NT, IO, LINES, RL, XM>;
   const long long ntiles = xcd_grid((long long)h.tiles0 * h.C1);
   if (ntiles <= 0) return D2DHIP_SUCCESS;
   const long long gx = ntiles < (1LL << 20) ? ntiles : (1LL << 20);
   const long long gy = (ntiles + gx - 1) / gx;
   if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "axis stage has too many tiles");
   hipLaunchKernelGGL((mf_axis_kernel<N, TX, NT, IO, LINES, RL, XM, WPE, SW>), dim3((unsigned)gx, (unsigned)gy),
-                     dim3(NT), C::lds_bytes(), st, dd, io.in, io.out, io.conj);
+                     dim3(NT), S::lds_bytes_sw(SW), st, dd, io.in, io.out, io.conj);
   return hip_check(hipGetLastError(), "mf_axis_kernel launch");
 }
 
 template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 static hipError_t prepare_mf() {
-  using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
+  using S = MfShape<N, TX, NT, IO, LINES, RL, XM>;
   return hipFuncSetAttribute((const void *)mf_axis_kernel<N, TX, NT, IO, LINES, RL, XM, WPE, SW>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::lds_bytes_sw(SW));
 }
 
 template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 static int launch_mf_multi(const MultiGroup *g, int ng, long long tiles, hipStream_t st) {
-  using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
+  using S = MfShape<N, TX, NT, IO, LINES, RL, XM>;
   const long long nb = xcd_grid(tiles);
   if (nb <= 0) return D2DHIP_SUCCESS;
   const long long gx = nb < (1LL << 20) ? nb : (1LL << 20);
   const long long gy = (nb + gx - 1) / gx;
   if (gy > 65535) return set_error(D2DHIP_ERR_UNSUPPORTED, "launch has too many tiles");
   hipLaunchKernelGGL((mf_multi_kernel<N, TX, NT, IO, LINES, RL, XM, WPE, SW>), dim3((unsigned)gx, (unsigned)gy),
-                     dim3(NT), C::lds_bytes(), st, g, ng, tiles);
+                     dim3(NT), S::lds_bytes_sw(SW), st, g, ng, tiles);
   return hip_check(hipGetLastError(), "mf_multi_kernel launch");
 }
 
 template <int N, int TX, int NT, int IO, bool LINES, class RL, int XM, int WPE, int SW>
 static hipError_t prepare_mf_multi() {
-  using C = FastCfg<float, N, TX, NT, LINES, RL, XM>;
+  using S = MfShape<N, TX, NT, IO, LINES, RL, XM>;
   return hipFuncSetAttribute((const void *)mf_multi_kernel<N, TX, NT, IO, LINES, RL, XM, WPE, SW>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes());
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::lds_bytes_sw(SW));
 }
 
 template <int... Rs>
@@ -777,7 +827,7 @@ static FastVariant make_mf(const char *name) {
   v.n = N;
   v.TX = TX;
   v.NT = NT;
-  v.lds_bytes = C::lds_bytes();
+  v.lds_bytes = MfShape<N, TX, NT, IO, LINES, RL, XM>::lds_bytes_sw(SW);
   v.ldj = C::LDJ;
   v.ldt = C::LDT;
   v.radices = mf_rvec(RL{});
@@ -790,7 +840,7 @@ static FastVariant make_mf(const char *name) {
 }
 
 // c2c, r2c and c2r of one shape; XM = 1 adds " xs", WPE > 1 " wN", SW = 1
-// " sw" to the name
+// or 3 " sw", SW = 2 or 3 " al" to the name
 #define D2D_MF_VARIANTS(N, TX, NT, LINES, RL, XM, WPE, SW, TAG)                                         \
   make_mf<N, TX, NT, IO_C2C, LINES, RL, XM, WPE, SW>("mfma n=" #N " tx=" #TX " nt=" #NT " c2c" TAG),   \
       make_mf<N, TX, NT, IO_R2C, LINES, RL, XM, WPE, SW>("mfma n=" #N " tx=" #TX " nt=" #NT " r2c" TAG), \
@@ -806,6 +856,10 @@ std::vector<FastVariant> mf_variants() {
       D2D_MF_VARIANTS(1024, 8, 512, false, M16_16_4, 1, 6, 1, " xs w6 sw"),
       D2D_MF_VARIANTS(1024, 8, 1024, false, M16_16_4, 0, 8, 1, " w8 sw"),
       D2D_MF_VARIANTS(1024, 16, 1024, false, M16_16_4, 1, 4, 0, " xs"),
+      // the DFT-16 operands read from LDS next to their MFMAs (al)
+      D2D_MF_VARIANTS(1024, 16, 1024, false, M16_16_4, 1, 8, 3, " xs w8 sw al"),
+      D2D_MF_VARIANTS(1024, 16, 1024, false, M16_16_4, 1, 8, 2, " xs w8 al"),
+      D2D_MF_VARIANTS(1024, 8, 512, false, M16_16_4, 1, 6, 2, " xs w6 al"),
       // contiguous lines
       D2D_MF_VARIANTS(1024, 4, 256, true, M16_16_4, 0, 4, 0, ""),
       D2D_MF_VARIANTS(1024, 4, 256, true, M16_16_4, 1, 6, 1, " xs w6 sw"),
