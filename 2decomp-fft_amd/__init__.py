@@ -34,6 +34,7 @@ FLAG_SKIP_X = 0x2
 FLAG_SKIP_Y = 0x4
 FLAG_SKIP_Z = 0x8
 FLAG_TUNE = 0x10
+LAYOUT_PLAIN, LAYOUT_BLOCKED = 0, 1
 R8, C16, R4, C8 = 0, 1, 2, 3
 X_TO_Y, Y_TO_Z, Z_TO_Y, Y_TO_X = 0, 1, 2, 3
 SUCCESS = 0
@@ -128,6 +129,8 @@ def lib():
             "d2dhip_plan_set_stage_kernel": ([pl, ci, ctypes.c_char_p], ci),
             "d2dhip_plan_set_chunking": ([pl, ci, ci], ci),
             "d2dhip_plan_get_chunking": ([pl, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
+            "d2dhip_plan_set_layout": ([pl, ci], ci),
+            "d2dhip_plan_get_layout": ([pl, ctypes.POINTER(ci)], ci),
             "d2dhip_plan_profile": ([pl, ci], ci),
             "d2dhip_plan_profile_report": ([pl, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ci)], ci),
@@ -367,6 +370,16 @@ class FFTEngine:
         f, b = ctypes.c_int(), ctypes.c_int()
         check(lib().d2dhip_plan_get_chunking(self._plan, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+    def set_layout(self, layout):
+        """LAYOUT_PLAIN or LAYOUT_BLOCKED: the spectral pencil's order between
+        the stages of fft_r2c / fft_c2r (PHYSICAL_IN_X whole-grid engines)."""
+        check(lib().d2dhip_plan_set_layout(self._plan, int(layout)))
+
+    def layout(self):
+        v = ctypes.c_int()
+        check(lib().d2dhip_plan_get_layout(self._plan, ctypes.byref(v)))
+        return v.value
 
     STAGES = ("c2c_x", "c2c_y", "c2c_z", "r2c", "spec_y", "spec_last", "c2r")
 

@@ -116,10 +116,13 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   const bool cvalid = PAIRED ? (ca < d->C0r) : (c0 < d->C0);
   // real-to-real stages: the transformed points start at dtt_first
   const long long f = DTT ? d->dtt_first : 0;
+  // (blocked columns: c2c column stages of the blocked spectral layout)
   const long long in_base =
-      ((IO == IO_R2C || DTT) && PAIRED ? ca : c0) * d->in_s0 + c1 * d->in_s1 + f * d->in_sj;
+      ((IO == IO_R2C || DTT) && PAIRED ? ca : c0) * d->in_s0 + c1 * d->in_s1 + f * d->in_sj +
+      (IO == IO_C2C && !LINES ? (c0 >> 4) * d->in_b0 : 0);
   const long long out_base =
-      ((IO == IO_C2R || DTT) && PAIRED ? ca : c0) * d->out_s0 + c1 * d->out_s1 + f * d->out_sj;
+      ((IO == IO_C2R || DTT) && PAIRED ? ca : c0) * d->out_s0 + c1 * d->out_s1 + f * d->out_sj +
+      (IO == IO_C2C && !LINES ? (c0 >> 4) * d->out_b0 : 0);
   const void *tw = d->tw;
   cpx<T> v[E];
 
@@ -294,8 +297,8 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
         long long ia = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
         long long ib = ia + d->in_s0;
         if (LINES && d->npack) {  // receive-buffer order (fused unpack; x stages)
-          ia = pack_index(d, k, cc + c1 * d->C0r);
-          ib = pack_index(d, k, cc + 1 + c1 * d->C0r);
+          ia = pack_index(d, k, cc + c1 * d->pack_l1);
+          ib = pack_index(d, k, cc + 1 + c1 * d->pack_l1);
         }
         if (cc < d->C0r) A[it] = gload<T>(in, ia);
         if (cc + 1 < d->C0r) B[it] = gload<T>(in, ib);
@@ -606,8 +609,8 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
           long long oa_i = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
           long long ob_i = oa_i + d->out_s0;
           if (LINES && d->npack) {  // send-buffer order (fused pack; x stages)
-            oa_i = pack_index(d, k, cc + c1 * d->C0r);
-            ob_i = pack_index(d, k, cc + 1 + c1 * d->C0r);
+            oa_i = pack_index(d, k, cc + c1 * d->pack_l1);
+            ob_i = pack_index(d, k, cc + 1 + c1 * d->pack_l1);
           }
           gstore<T>(out, oa_i, oa[it]);
           if (cc + 1 < d->C0r) gstore<T>(out, ob_i, ob[it]);

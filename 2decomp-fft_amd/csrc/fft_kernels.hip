@@ -44,8 +44,8 @@ __device__ __forceinline__ cpx<T> pload(DescPtr d, const AxisIo &io, long long c
   } else {
     cpx<T> v = {T(0), T(0)};
     if (c0 < d->C0)  // (a c2c stage may read its pencil in unpack order, packj = 2 or 3)
-      v = gload<T>(io.in, c0 * d->in_s0 + c1 * d->in_s1 + (long long)j * d->in_sj +
-                              (d->packj >= 2 ? packj_offset(d, j, c1) : 0));
+      v = gload<T>(io.in, c0 * d->in_s0 + (c0 >> 4) * d->in_b0 + c1 * d->in_s1 +
+                              (long long)j * d->in_sj + (d->packj >= 2 ? packj_offset(d, j, c1) : 0));
     if (io.conj) v.y = -v.y;
     return v;
   }
@@ -64,7 +64,7 @@ __device__ __forceinline__ void pstore(DescPtr d, const AxisIo &io, long long c0
   } else {
     if (c0 < d->C0) {
       if (io.conj) v.y = -v.y;
-      gstore<T>(io.out, c0 * d->out_s0 + c1 * d->out_s1 + (long long)j * d->out_sj +
+      gstore<T>(io.out, c0 * d->out_s0 + (c0 >> 4) * d->out_b0 + c1 * d->out_s1 + (long long)j * d->out_sj +
                             (d->packj == 1 ? packj_offset(d, j, c1) : d->packj == 3 ? packj_offset2(d, j, c1) : 0),
                 v);
     }
@@ -368,8 +368,8 @@ __device__ __forceinline__ void axis_tile(DescPtr d, const AxisIo &io, long long
       const long long base = c1 * d->in_s1 + (long long)k * d->in_sj;
       long long ia = base + ca * d->in_s0, ib = base + (ca + 1) * d->in_s0;
       if (d->npack) {  // receive-buffer order (fused unpack)
-        ia = pack_index(d, k, ca + c1 * d->C0r);
-        ib = pack_index(d, k, ca + 1 + c1 * d->C0r);
+        ia = pack_index(d, k, ca + c1 * d->pack_l1);
+        ib = pack_index(d, k, ca + 1 + c1 * d->pack_l1);
       }
       cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
       if (ca < d->C0r) A = gload<T>(in, ia);
@@ -409,8 +409,8 @@ __device__ __forceinline__ void axis_tile(DescPtr d, const AxisIo &io, long long
       const long long base = c1 * d->out_s1 + (long long)k * d->out_sj;
       long long oa_i = base + ca * d->out_s0, ob_i = base + (ca + 1) * d->out_s0;
       if (d->npack) {  // send-buffer order (fused pack)
-        oa_i = pack_index(d, k, ca + c1 * d->C0r);
-        ob_i = pack_index(d, k, ca + 1 + c1 * d->C0r);
+        oa_i = pack_index(d, k, ca + c1 * d->pack_l1);
+        ob_i = pack_index(d, k, ca + 1 + c1 * d->pack_l1);
       }
       gstore<T>(out, oa_i, cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half});
       if (ca + 1 < d->C0r) gstore<T>(out, ob_i, cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half});
@@ -567,8 +567,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(genera
       const long long base = c1 * d->in_s1 + (long long)k * d->in_sj;
       long long ia = base + ca * d->in_s0, ib = base + (ca + 1) * d->in_s0;
       if (d->npack) {  // receive-buffer order (fused unpack)
-        ia = pack_index(desc_ptr(d), k, ca + c1 * d->C0r);
-        ib = pack_index(desc_ptr(d), k, ca + 1 + c1 * d->C0r);
+        ia = pack_index(desc_ptr(d), k, ca + c1 * d->pack_l1);
+        ib = pack_index(desc_ptr(d), k, ca + 1 + c1 * d->pack_l1);
       }
       cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
       if (ca < d->C0r) A = gload<T>(in, ia);
@@ -626,8 +626,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(genera
       const long long base = c1 * d->out_s1 + (long long)k * d->out_sj;
       long long oa_i = base + ca * d->out_s0, ob_i = base + (ca + 1) * d->out_s0;
       if (d->npack) {  // send-buffer order (fused pack)
-        oa_i = pack_index(desc_ptr(d), k, ca + c1 * d->C0r);
-        ob_i = pack_index(desc_ptr(d), k, ca + 1 + c1 * d->C0r);
+        oa_i = pack_index(desc_ptr(d), k, ca + c1 * d->pack_l1);
+        ob_i = pack_index(desc_ptr(d), k, ca + 1 + c1 * d->pack_l1);
       }
       gstore<T>(out, oa_i, cpx<T>{(zk.x + zm.x) * half, (zk.y - zm.y) * half});
       if (ca + 1 < d->C0r) gstore<T>(out, ob_i, cpx<T>{(zk.y + zm.y) * half, (zm.x - zk.x) * half});

@@ -67,6 +67,9 @@ struct AxisDesc {
   long long C1;       // columns along c1
   long long in_s0, in_s1, in_sj;
   long long out_s0, out_s1, out_sj;
+  // Blocked columns (c2c stages): column c0 sits at c0 * s0 + (c0 >> 4) * b0,
+  // i.e. blocks of 16 columns b0 + 16 * s0 elements apart; 0 = plain.
+  long long in_b0, out_b0;
   int ldj, ldt;       // LDS strides (complex units) of point j and column t
   int nh;             // n/2 + 1 of the line length
   // Bluestein (chirp-z) stages: the line length is nblue, `n` and the passes
@@ -77,12 +80,14 @@ struct AxisDesc {
   const void *bhat;   // DFT_M of the chirp sequence exp(+i pi m^2 / nblue), wrapped
   // Complex side of a line stage (r2c output, c2r input) in the x-pencil
   // send / receive-buffer order of a process grid (a transform fused with
-  // the pack or unpack): block m of a line's points holds [b1, b1 + w) and
-  // puts point i of line l at base - b1 + i + l * w; pack[2i .. 2i+1] =
-  // {base - b1, w} of the block holding point i.  npack = 0: plain layout,
-  // else the number of blocks.
+  // the pack or unpack), or in a plan's blocked spectral layout: block m
+  // of a line's points holds [b1, b1 + w) and puts point i of line l at
+  // base - b1 + i + l * w; pack[2i .. 2i+1] = {base - b1, w} of the block
+  // holding point i.  npack = 0: plain layout, > 0 the number of blocks.
   const long long *pack;
   int npack;
+  long long pack_l1;  // line index l = c0 + c1 * pack_l1 of the packed order (npack != 0)
+  long long pack_bs;  // npack < 0: the blocked layout's block stride (no table)
   // Columns of a y- or z-pencil in buffer order (fused pack / unpack):
   // packj = 1 on the output side, 2 on the input side, 3 on both (input
   // through pack, output through pack2), 0 plain.  The table holds one
@@ -110,8 +115,10 @@ struct AxisDesc {
 // Index of complex point i of line l in the packed order of d (npack > 0):
 // one 16-byte read of the point's entry {base - b1, w} (those of the block
 // holding i), so that a fused unpack's loads wait on a single cached read.
+// npack < 0: the blocked spectral layout, blocks of 16 points pack_bs apart.
 __device__ __forceinline__ long long pack_index(const __attribute__((address_space(4))) AxisDesc *d,
                                                 int i, long long l) {
+  if (d->npack < 0) return (long long)(i >> 4) * d->pack_bs + (i & 15) + 16 * l;
   typedef long long ll2 __attribute__((ext_vector_type(2)));
   const __attribute__((address_space(1))) ll2 *pk =
       (const __attribute__((address_space(1))) ll2 *)d->pack;

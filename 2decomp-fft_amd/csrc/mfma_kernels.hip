@@ -310,8 +310,10 @@ __device__ __forceinline__ void mf_fetch(DescPtr d, const void *in, int conj, lo
     const long long c0 = c0b + t;
     ca = 2 * c0;
     cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
-    ib = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1;
-    ob = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1;
+    ib = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1 +
+         (IO == IO_C2C && !LINES ? (c0 >> 4) * d->in_b0 : 0);
+    ob = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1 +
+         (IO == IO_C2C && !LINES ? (c0 >> 4) * d->out_b0 : 0);
   };
   // LDS element of point j of tile column t
   auto at = [&](int j, int t) __attribute__((always_inline)) { return C::row(j) * LDJ + t * LDT; };
@@ -332,8 +334,8 @@ __device__ __forceinline__ void mf_fetch(DescPtr d, const void *in, int conj, lo
         long long ia = cc * d->in_s0 + c1 * d->in_s1 + (long long)k * d->in_sj;
         long long ib = ia + d->in_s0;
         if (LINES && d->npack) {  // receive-buffer order (fused unpack; x stages)
-          ia = pack_index(d, k, cc + c1 * d->C0r);
-          ib = pack_index(d, k, cc + 1 + c1 * d->C0r);
+          ia = pack_index(d, k, cc + c1 * d->pack_l1);
+          ib = pack_index(d, k, cc + 1 + c1 * d->pack_l1);
         }
         if (cc < d->C0r) f[it] = gload<T>(in, ia);
         if (cc + 1 < d->C0r) f[ITER + it] = gload<T>(in, ib);
@@ -418,8 +420,10 @@ __device__ __forceinline__ void mf_run(DescPtr d, void *out, int conj, long long
     const long long c0 = c0b + t;
     ca = 2 * c0;
     cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
-    ib = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1;
-    ob = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1;
+    ib = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1 +
+         (IO == IO_C2C && !LINES ? (c0 >> 4) * d->in_b0 : 0);
+    ob = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1 +
+         (IO == IO_C2C && !LINES ? (c0 >> 4) * d->out_b0 : 0);
   };
   // LDS element of point j of tile column t
   auto at = [&](int j, int t) __attribute__((always_inline)) { return C::row(j) * LDJ + t * LDT; };
@@ -683,8 +687,8 @@ __device__ __forceinline__ void mf_run(DescPtr d, void *out, int conj, long long
           long long oa_i = cc * d->out_s0 + c1 * d->out_s1 + (long long)k * d->out_sj;
           long long ob_i = oa_i + d->out_s0;
           if (LINES && d->npack) {  // send-buffer order (fused pack; x stages)
-            oa_i = pack_index(d, k, cc + c1 * d->C0r);
-            ob_i = pack_index(d, k, cc + 1 + c1 * d->C0r);
+            oa_i = pack_index(d, k, cc + c1 * d->pack_l1);
+            ob_i = pack_index(d, k, cc + 1 + c1 * d->pack_l1);
           }
           gstore<T>(out, oa_i, oa[it]);
           if (cc + 1 < d->C0r) gstore<T>(out, ob_i, ob[it]);

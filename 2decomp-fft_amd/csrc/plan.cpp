@@ -91,6 +91,24 @@ struct d2dhip_plan_s {
   int chunk_fwd = 0, chunk_bwd = 0;
   AxisDesc h_cdesc[8];
   AxisDesc *d_cdesc = nullptr;
+  // Blocked spectral layout (D2DHIP_LAYOUT_BLOCKED; PHYSICAL_IN_X whole-grid
+  // plans with a workspace): between the x stage and the last stage of
+  // exec_r2c / exec_c2r the spectral pencil sits in the workspace in blocks
+  // of 16 i-columns, element (i, j, k) at
+  //   (i >> 4) * bs + k * 16 * nyb + j * 16 + (i & 15),
+  //   nyb = ny + 1, bs = 16 * nyb * nz,
+  // so that a z-stage tile reads and writes rows of 16 elements lying
+  // 16 * nyb elements apart, several to an address-translation page (the
+  // plain z stage's rows lie a whole i-j plane apart, one page each, and
+  // keep the translation cache busy), and a y-stage tile is one contiguous
+  // run.  The spare j row keeps the row distance off a power of two, which
+  // HBM serves more slowly.  Forward: x (into the blocks), z (in place), y
+  // (blocks -> the plain output); inverse: y, z, x.  Descriptors: 0 x r2c,
+  // 1 z over the whole blocks, 2 z over the last partial block, 3 y forward,
+  // 4 y inverse, 5 x c2r.
+  int layout = D2DHIP_LAYOUT_PLAIN;
+  AxisDesc h_bdesc[6];
+  AxisDesc *d_bdesc = nullptr;
   void *wk = nullptr;  // c2r workspace (spectral pencil)
   size_t wk_bytes = 0;
   // stage timing: an event pair around every launch while enabled
@@ -651,6 +669,7 @@ int make_dtt_stage(d2dhip_plan_s *p, int idx, int axis, int type, int first, int
 }
 
 int upload_chunk_descs(d2dhip_plan_s *p);
+int upload_blocked_descs(d2dhip_plan_s *p);
 
 // A captured graph holds the kernels, grids and chunk offsets of the
 // configuration it was recorded with, while the descriptors it reads are
@@ -679,6 +698,7 @@ int upload_descs(d2dhip_plan_s *p) {
   int st = upload_sync(p, p->d_desc, p->h_desc, sizeof(AxisDesc) * d2dhip_plan_s::kStages,
                        "hipMemcpy(stage descriptors)");
   if (!st) st = upload_chunk_descs(p);
+  if (!st) st = upload_blocked_descs(p);
   return st;
 }
 
@@ -1077,6 +1097,245 @@ int run_pair_chunked(d2dhip_plan_s *p, long long K, int q1, int i1, int q2, int 
   return D2DHIP_SUCCESS;
 }
 
+// ---- blocked spectral layout ----------------------------------------------------
+
+long long blk_nyb(const d2dhip_plan_s *p) { return (long long)p->ny + 1; }
+long long blk_bs(const d2dhip_plan_s *p) { return 16 * blk_nyb(p) * p->nz; }
+// Complex elements of the blocked pencil (whole blocks of 16 columns).
+long long blk_elems(const d2dhip_plan_s *p) { return ((p->nx / 2 + 1 + 15) / 16) * blk_bs(p); }
+
+bool blk_possible(const d2dhip_plan_s *p) {
+  return p->format == D2DHIP_PHYSICAL_IN_X && !p->multi && !(p->flags & D2DHIP_FLAG_INPLACE);
+}
+
+// Stage idx's descriptor (its kernel, twiddles and tile width) on other
+// columns: C0 x C1 columns, column c0 at c0 * s0 + (c0 >> 4) * b0.
+AxisDesc blk_cols(const d2dhip_plan_s *p, int idx, long long C0, long long C1, long long is0,
+                  long long ib0, long long is1, long long isj, long long os0, long long ob0,
+                  long long os1, long long osj) {
+  AxisDesc a = p->h_desc[idx];
+  a.C0 = a.C0r = C0;
+  a.C1 = C1;
+  a.tiles0 = (int)((C0 + a.TX - 1) / a.TX);
+  a.in_s0 = is0;
+  a.in_b0 = ib0;
+  a.in_s1 = is1;
+  a.in_sj = isj;
+  a.out_s0 = os0;
+  a.out_b0 = ob0;
+  a.out_s1 = os1;
+  a.out_sj = osj;
+  return a;
+}
+
+// Rebuilds and uploads the blocked-layout descriptors for the current kernel
+// choices (nothing while the layout is plain).
+int upload_blocked_descs(d2dhip_plan_s *p) {
+  if (p->layout != D2DHIP_LAYOUT_BLOCKED) return D2DHIP_SUCCESS;
+  const long long nxc = p->nx / 2 + 1, ny = p->ny, nz = p->nz;
+  const long long nyb = blk_nyb(p), bs = blk_bs(p), nfull = nxc / 16, r = nxc % 16;
+  // x stages: point i of line l = j + k * nyb at (i >> 4) * bs + (i & 15) + 16 l
+  // (the plain x stages see the ny * nz lines as one run; here lines j of
+  // plane k, so that the line index can skip the spare j row)
+  for (int s = 0; s < 2; ++s) {
+    AxisDesc a = p->h_desc[s ? 6 : 3];
+    a.C0r = ny;
+    a.C0 = (ny + 1) / 2;
+    a.C1 = nz;
+    a.tiles0 = (int)((a.C0 + a.TX - 1) / a.TX);
+    a.in_s1 = s ? nxc * ny : (long long)p->nx * ny;
+    a.out_s1 = s ? (long long)p->nx * ny : nxc * ny;
+    a.pack = nullptr;
+    a.npack = -1;
+    a.pack_l1 = nyb;
+    a.pack_bs = bs;
+    p->h_bdesc[s ? 5 : 0] = a;
+  }
+  // z over whole blocks: the columns (j, i & 15) of a block are contiguous
+  p->h_bdesc[1] = blk_cols(p, 5, 16 * ny, nfull > 0 ? nfull : 1, 1, 0, bs, 16 * nyb, 1, 0, bs,
+                           16 * nyb);
+  // z over the last partial block (its first r columns of each row j),
+  // launched on the block's first element
+  p->h_bdesc[2] = blk_cols(p, 5, r > 0 ? r : 1, ny, 1, 0, 16, 16 * nyb, 1, 0, 16, 16 * nyb);
+  // y: blocks -> plain (nx/2+1, ny, nz) forward, plain -> blocks inverse
+  p->h_bdesc[3] = blk_cols(p, 4, nxc, nz, 1, bs - 16, 16 * nyb, 16, 1, 0, nxc * ny, nxc);
+  p->h_bdesc[4] = blk_cols(p, 4, nxc, nz, 1, 0, nxc * ny, nxc, 1, bs - 16, 16 * nyb, 16);
+  if (!p->d_bdesc && hipMalloc(&p->d_bdesc, sizeof(p->h_bdesc)) != hipSuccess) {
+    p->d_bdesc = nullptr;
+    return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(blocked descriptors) failed");
+  }
+  drop_graph(p);
+  return upload_sync(p, p->d_bdesc, p->h_bdesc, sizeof(p->h_bdesc), "hipMemcpy(blocked descriptors)");
+}
+
+// One launch of stage idx's kernel through blocked descriptor `slot`, the
+// buffers offset by in_off / out_off complex elements; profiled as stage idx
+// (weight w of one stage call).
+int run_blk(d2dhip_plan_s *p, int slot, int idx, const void *in, void *out, int conj,
+            long long in_off = 0, long long out_off = 0, double w = 1.0) {
+  const size_t cb = 2 * (size_t)p->rb;
+  const AxisIo io = {static_cast<const unsigned char *>(in) + in_off * cb,
+                     static_cast<unsigned char *>(out) + out_off * cb, conj};
+  hipEvent_t a = nullptr, b = nullptr;
+  if (p->prof) {
+    a = take_event(p);
+    b = take_event(p);
+    if (!a || !b) return set_error(D2DHIP_ERR_HIP, "hipEventCreate failed");
+    int st = hip_check(hipEventRecord(a, p->stream), "hipEventRecord");
+    if (st) return st;
+  }
+  const AxisDesc *dd = p->d_bdesc + slot;
+  int st = p->fast[idx] ? p->fast[idx]->launch(dd, p->h_bdesc[slot], io, p->stream)
+                        : launch_axis(p->rb, p->geom[idx].io, dd, p->h_bdesc[slot], io,
+                                      p->shape[idx], p->stream);
+  if (p->prof) {
+    if (!st) st = hip_check(hipEventRecord(b, p->stream), "hipEventRecord");
+    p->recs.push_back({idx, a, b, w});
+  }
+  return st;
+}
+
+// The z stage on the blocked pencil in wk (in place): the whole blocks, then
+// the last partial one.
+int run_blk_z(d2dhip_plan_s *p, void *wk, int conj) {
+  const long long nxc = p->nx / 2 + 1, nfull = nxc / 16, r = nxc % 16;
+  int st = D2DHIP_SUCCESS;
+  if (nfull > 0) st = run_blk(p, 1, 5, wk, wk, conj);
+  if (!st && r > 0)
+    st = run_blk(p, 2, 5, wk, wk, conj, nfull * blk_bs(p), nfull * blk_bs(p), nfull > 0 ? 0.0 : 1.0);
+  return st;
+}
+
+// exec_r2c / the c2r stages in the blocked layout (wk: the plan's workspace).
+int run_blocked_r2c(d2dhip_plan_s *p, const void *in_r, void *out_c) {
+  int st = run_blk(p, 0, 3, in_r, p->wk, 0);
+  if (!st) st = run_blk_z(p, p->wk, 0);
+  if (!st) st = run_blk(p, 3, 4, p->wk, out_c, 0);
+  return st;
+}
+
+int run_blocked_c2r(d2dhip_plan_s *p, const void *in_c, void *out_r) {
+  int st = run_blk(p, 4, 4, in_c, p->wk, 1);
+  if (!st) st = run_blk_z(p, p->wk, 1);
+  if (!st) st = run_blk(p, 5, 6, p->wk, out_r, 1);
+  return st;
+}
+
+// Whether an exec call runs in the blocked layout: set, whole stages, none
+// skipped, and the plan's own workspace.
+bool blocked_now(const d2dhip_plan_s *p, int chunk, const void *wk) {
+  return p->layout == D2DHIP_LAYOUT_BLOCKED && chunk == 0 && wk && wk == p->wk &&
+         !(p->flags & (D2DHIP_FLAG_SKIP_X | D2DHIP_FLAG_SKIP_Y | D2DHIP_FLAG_SKIP_Z));
+}
+
+// Workspace bytes a plan needs for its layout.
+size_t wk_need(const d2dhip_plan_s *p) {
+  const long long nsp = (p->format == D2DHIP_PHYSICAL_IN_X)
+                            ? (long long)(p->nx / 2 + 1) * p->ny * p->nz
+                            : (long long)p->nx * p->ny * (p->nz / 2 + 1);
+  const long long e = p->layout == D2DHIP_LAYOUT_BLOCKED ? std::max(nsp, blk_elems(p)) : nsp;
+  return (size_t)e * 2 * p->rb;
+}
+
+// Switches the layout, resizing the workspace to what the layout needs.
+int set_layout(d2dhip_plan_s *p, int layout) {
+  if (layout == D2DHIP_LAYOUT_BLOCKED && !blk_possible(p))
+    return set_error(D2DHIP_ERR_UNSUPPORTED,
+                     "blocked layout: a whole-grid PHYSICAL_IN_X plan with a workspace");
+  if (layout != D2DHIP_LAYOUT_PLAIN && layout != D2DHIP_LAYOUT_BLOCKED)
+    return set_error(D2DHIP_ERR_INVALID_VALUE, "layout must be D2DHIP_LAYOUT_PLAIN or _BLOCKED");
+  if (p->capturing)
+    return set_error(D2DHIP_ERR_UNSUPPORTED, "plan is capturing: its layout is fixed");
+  drop_graph(p);
+  p->layout = layout;
+  if (!(p->flags & D2DHIP_FLAG_INPLACE) && !p->multi) {
+    const size_t need = wk_need(p);
+    if (need > p->wk_bytes) {  // (grows only: switching back keeps the larger one)
+      // the old workspace may still be read by queued work
+      int st = hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+      if (st) return st;
+      if (p->wk) (void)hipFree(p->wk);
+      p->wk = nullptr;
+      p->wk_bytes = 0;
+      if (hipMalloc(&p->wk, need) != hipSuccess) {
+        p->wk = nullptr;
+        p->layout = D2DHIP_LAYOUT_PLAIN;
+        return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(workspace) failed");
+      }
+      p->wk_bytes = need;
+    }
+  }
+  return upload_blocked_descs(p);
+}
+
+// Times the r2c + c2r round trip in each layout on scratch buffers and keeps
+// the faster; cached like the kernel choices.
+int tune_layout(d2dhip_plan_s *p) {
+  if (!blk_possible(p) || (p->flags & (D2DHIP_FLAG_SKIP_Y | D2DHIP_FLAG_SKIP_Z)) || p->chunk_fwd ||
+      p->chunk_bwd)
+    return D2DHIP_SUCCESS;
+  char kb[512];
+  std::snprintf(kb, sizeof(kb), "v1 layout pr=%d n=%d,%d,%d k=%s|%s|%s|%s", p->prec, p->nx, p->ny,
+                p->nz, choice_name(p, 3).c_str(), choice_name(p, 4).c_str(),
+                choice_name(p, 5).c_str(), choice_name(p, 6).c_str());
+  const std::string key(kb);
+  auto it = g_tune.find(key);
+  int lay = 0;
+  if (it != g_tune.end() && std::sscanf(it->second.c_str(), "%d", &lay) == 1)
+    return set_layout(p, lay == D2DHIP_LAYOUT_BLOCKED ? lay : D2DHIP_LAYOUT_PLAIN);
+  const long long nph = (long long)p->nx * p->ny * p->nz;
+  const long long nsp = (long long)(p->nx / 2 + 1) * p->ny * p->nz;
+  void *a = nullptr, *b = nullptr, *c = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = D2DHIP_SUCCESS;
+  if (hipMalloc(&a, (size_t)nph * p->rb) != hipSuccess ||
+      hipMalloc(&b, (size_t)nsp * 2 * p->rb) != hipSuccess ||
+      hipMalloc(&c, (size_t)nph * p->rb) != hipSuccess)
+    st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(tuning buffers) failed");
+  if (!st) st = launch_fill_uniform(a, nph, p->rb == 8 ? D2DHIP_R8 : D2DHIP_R4, 7, p->stream);
+  if (!st) st = hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  if (!st) st = hip_check(hipEventCreate(&e0), "hipEventCreate");
+  if (!st) st = hip_check(hipEventCreate(&e1), "hipEventCreate");
+  const int prof = p->prof;
+  p->prof = 0;
+  float ms[2] = {1e30f, 1e30f};
+  // two rounds, each layout's best kept (one run each is decided by noise
+  // when the two are close)
+  for (int round = 0; round < 2 && !st; ++round) {
+    for (int lay = 0; lay < 2 && !st; ++lay) {
+      if ((st = set_layout(p, lay))) break;
+      auto once = [&]() {
+        int r = lay ? run_blocked_r2c(p, a, b) : d2dhip_exec_r2c(p, a, b);
+        return r ? r : (lay ? run_blocked_c2r(p, b, c) : d2dhip_exec_c2r(p, b, c));
+      };
+      if ((st = once())) break;
+      if ((st = hip_check(hipEventRecord(e0, p->stream), "hipEventRecord"))) break;
+      const int reps = 3;
+      for (int q = 0; q < reps && !st; ++q) st = once();
+      if (st) break;
+      if ((st = hip_check(hipEventRecord(e1, p->stream), "hipEventRecord"))) break;
+      if ((st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize"))) break;
+      float t = 0.f;
+      if ((st = hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))) break;
+      ms[lay] = std::min(ms[lay], t / reps);
+    }
+  }
+  p->prof = prof;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (c) (void)hipFree(c);
+  const int best = ms[1] < ms[0] ? D2DHIP_LAYOUT_BLOCKED : D2DHIP_LAYOUT_PLAIN;
+  if (!st) {
+    char v[16];
+    std::snprintf(v, sizeof(v), "%d", best);
+    tune_store(p, key, v);
+  }
+  int st2 = set_layout(p, best);
+  return st ? st : st2;
+}
+
 // Times the forward pair (r2c x, y) and the inverse pair (y, c2r x) for
 // chunk sizes K in {0, 2, 4, ..., 64} (K < nz) on scratch buffers and keeps
 // the fastest of each; cached like the kernel choices, keyed by the grid and
@@ -1218,6 +1477,17 @@ int plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec,
       st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(c2r workspace) failed");
     }
   }
+  if (!st && blk_possible(p)) {
+    // D2DHIP_LAYOUT=plain|blocked fixes the layout; tuning times both
+    const char *e = std::getenv("D2DHIP_LAYOUT");
+    if (e && std::strcmp(e, "blocked") == 0) {
+      st = set_layout(p, D2DHIP_LAYOUT_BLOCKED);
+    } else if (!(e && std::strcmp(e, "plain") == 0) && tune) {
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      tune_load();
+      st = tune_layout(p);
+    }
+  }
   if (st) {
     d2dhip_plan_destroy(p);
     return st;
@@ -1291,6 +1561,7 @@ int d2dhip_plan_destroy(d2dhip_plan p) {
   if (p->wk) (void)hipFree(p->wk);
   if (p->d_desc) (void)hipFree(p->d_desc);
   if (p->d_cdesc) (void)hipFree(p->d_cdesc);
+  if (p->d_bdesc) (void)hipFree(p->d_bdesc);
   if (p->d_pack) (void)hipFree(p->d_pack);
   if (p->d_pdesc) (void)hipFree(p->d_pdesc);
   if (p->d_ypack) (void)hipFree(p->d_ypack);
@@ -1426,6 +1697,17 @@ int d2dhip_plan_set_chunking(d2dhip_plan p, int fwd_planes, int bwd_planes) {
   return upload_chunk_descs(p);
 }
 
+int d2dhip_plan_set_layout(d2dhip_plan p, int layout) {
+  if (!p) return set_error(D2DHIP_ERR_NULL, "plan is NULL");
+  return set_layout(p, layout);
+}
+
+int d2dhip_plan_get_layout(d2dhip_plan p, int *layout) {
+  if (!p || !layout) return set_error(D2DHIP_ERR_NULL, "NULL argument");
+  *layout = p->layout;
+  return D2DHIP_SUCCESS;
+}
+
 int d2dhip_plan_get_chunking(d2dhip_plan p, int *fwd_planes, int *bwd_planes) {
   if (!p || !fwd_planes || !bwd_planes) return set_error(D2DHIP_ERR_NULL, "NULL argument");
   *fwd_planes = p->chunk_fwd;
@@ -1487,6 +1769,7 @@ int d2dhip_exec_r2c(d2dhip_plan p, const void *in_r, void *out_c) {
                      "plan of one rank of a process grid: the 3D transform needs the exchange "
                      "between its stages (use d2dhip_exec_1m)");
   if (!in_r || !out_c) return set_error(D2DHIP_ERR_NULL, "NULL data pointer");
+  if (blocked_now(p, p->chunk_fwd, p->wk)) return run_blocked_r2c(p, in_r, out_c);
   const bool fx = p->format == D2DHIP_PHYSICAL_IN_X;
   // the r2c axis cannot be skipped (src/fft_generic.f90:222-223, :265-266)
   const int warn = (p->flags & (fx ? D2DHIP_FLAG_SKIP_X : D2DHIP_FLAG_SKIP_Z)) ? 1 : 0;
@@ -1512,6 +1795,7 @@ namespace {
 // The c2r stages with the spectral workspace wk (the plan's own, in_c itself,
 // or the library's static scratch array).
 int c2r_stages(d2dhip_plan_s *p, void *in_c, void *out_r, void *wk) {
+  if (blocked_now(p, p->chunk_bwd, wk)) return run_blocked_c2r(p, in_c, out_r);
   const bool fx = p->format == D2DHIP_PHYSICAL_IN_X;
   const int warn = (p->flags & (fx ? D2DHIP_FLAG_SKIP_X : D2DHIP_FLAG_SKIP_Z)) ? 1 : 0;
   const unsigned skip_first = fx ? D2DHIP_FLAG_SKIP_Z : D2DHIP_FLAG_SKIP_X;
@@ -1760,6 +2044,7 @@ int run_x_stage_packed(d2dhip_plan_s *p, int idx, const void *in, void *out,
   p->h_pdesc = p->h_desc[idx];
   p->h_pdesc.pack = p->d_pack;
   p->h_pdesc.npack = np;
+  p->h_pdesc.pack_l1 = p->h_pdesc.C0r;  // lines j + k * ny of the x-pencil
   int st = upload_sync(p, p->d_pack, p->h_pack.data(), sizeof(long long) * p->h_pack.size(),
                        "hipMemcpy(pack table)");
   if (!st) st = upload_sync(p, p->d_pdesc, &p->h_pdesc, sizeof(AxisDesc), "hipMemcpy(pack descriptor)");

@@ -30,6 +30,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))  # oracle/oracle.py (cpu_baseline only)
 
 PEAK_HBM_GBS = 8000.0
+# spectral pencil order between the stages (the tuner picks; see d2dhip_plan_set_layout)
+LAYOUT_NAMES = {0: "plain", 1: "blocked"}
 PEAK_BF16_FLOPS = 2.5e15  # dense bf16 MFMA peak of MI355X
 
 # HBM bytes per launch of the 1024^3 FP64 stage kernels, from two rocprofv3
@@ -668,7 +670,8 @@ def main():
                         "speedup_vs_fp64": round(ms / ms32, 3),
                         "stage_ms": stage_ms(rep32),
                         "kernels": {k: kernels32[k] for k in ("r2c", "spec_y", "spec_last", "c2r")},
-                        "chunk_planes": {"fwd": eng32.chunking()[0], "bwd": eng32.chunking()[1]}}
+                        "chunk_planes": {"fwd": eng32.chunking()[0], "bwd": eng32.chunking()[1]},
+                        "layout": LAYOUT_NAMES[eng32.layout()]}
     del spec32, y32, eng32
 
     # ---- split-BF16 matrix-core variant (FP32 storage) -------------------------
@@ -697,6 +700,7 @@ def main():
         "mfma_util": round(mf_flops / (mss * 1e-3) / PEAK_BF16_FLOPS, 4),
         "stage_ms": stage_ms(reps),
         "kernels": {k: kernels_s[k] for k in ("r2c", "spec_y", "spec_last", "c2r")},
+        "layout": LAYOUT_NAMES[engs.layout()],
     }
     del x32, specs, ys, engs
 
@@ -725,6 +729,7 @@ def main():
         "stages": stages,
         "kernels": {k: v for k, v in kernels64.items() if k in stages},
         "chunk_planes": {"fwd": eng.chunking()[0], "bwd": eng.chunking()[1]},
+        "layout": LAYOUT_NAMES[eng.layout()],
         "plan_seconds": round(plan_s, 2),
         "roofline": {"bound": "hbm", "kernel": "%s (%s stage)" % (kernels64.get(dom), dom),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
