@@ -1268,21 +1268,48 @@ int set_layout(d2dhip_plan_s *p, int layout) {
   return upload_blocked_descs(p);
 }
 
+// Index of the candidate of stage idx named `name` (-1: "general"; -2: none).
+int cand_index(const d2dhip_plan_s *p, int idx, const std::string &name) {
+  if (name == "general") return -1;
+  const StageGeom &g = p->geom[idx];
+  for (size_t k = 0; k < g.cands.size(); ++k)
+    if (name == g.cands[k]->name) return (int)k;
+  return -2;
+}
+
+// Kernel choices of the y and z stages and the layout, uploaded together.
+int use_layout(d2dhip_plan_s *p, int layout, int c4, int c5) {
+  int st = configure(p, 4, c4);
+  if (!st) st = configure(p, 5, c5);
+  if (!st) st = set_layout(p, layout);
+  if (!st) st = upload_descs(p);
+  return st;
+}
+
 // Times the r2c + c2r round trip in each layout on scratch buffers and keeps
-// the faster; cached like the kernel choices.
+// the faster, each layout with its own y- and z-stage kernels (the blocked
+// layout's picked by timing every candidate in the blocked form); cached like
+// the kernel choices, as "0" or "1|<y kernel>|<z kernel>".
 int tune_layout(d2dhip_plan_s *p) {
   if (!blk_possible(p) || (p->flags & (D2DHIP_FLAG_SKIP_Y | D2DHIP_FLAG_SKIP_Z)) || p->chunk_fwd ||
       p->chunk_bwd)
     return D2DHIP_SUCCESS;
   char kb[512];
-  std::snprintf(kb, sizeof(kb), "v1 layout pr=%d n=%d,%d,%d k=%s|%s|%s|%s", p->prec, p->nx, p->ny,
+  std::snprintf(kb, sizeof(kb), "v2 layout pr=%d n=%d,%d,%d k=%s|%s|%s|%s", p->prec, p->nx, p->ny,
                 p->nz, choice_name(p, 3).c_str(), choice_name(p, 4).c_str(),
                 choice_name(p, 5).c_str(), choice_name(p, 6).c_str());
   const std::string key(kb);
+  const int plain4 = p->geom[4].choice, plain5 = p->geom[5].choice;
   auto it = g_tune.find(key);
-  int lay = 0;
-  if (it != g_tune.end() && std::sscanf(it->second.c_str(), "%d", &lay) == 1)
-    return set_layout(p, lay == D2DHIP_LAYOUT_BLOCKED ? lay : D2DHIP_LAYOUT_PLAIN);
+  if (it != g_tune.end()) {
+    const std::string &v = it->second;
+    const size_t bar = v.size() > 2 && v[0] == '1' && v[1] == '|' ? v.find('|', 2) : std::string::npos;
+    if (bar != std::string::npos) {
+      const int c4 = cand_index(p, 4, v.substr(2, bar - 2)), c5 = cand_index(p, 5, v.substr(bar + 1));
+      if (c4 != -2 && c5 != -2) return use_layout(p, D2DHIP_LAYOUT_BLOCKED, c4, c5);
+    }
+    return set_layout(p, D2DHIP_LAYOUT_PLAIN);
+  }
   const long long nph = (long long)p->nx * p->ny * p->nz;
   const long long nsp = (long long)(p->nx / 2 + 1) * p->ny * p->nz;
   void *a = nullptr, *b = nullptr, *c = nullptr;
@@ -1293,31 +1320,57 @@ int tune_layout(d2dhip_plan_s *p) {
       hipMalloc(&c, (size_t)nph * p->rb) != hipSuccess)
     st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(tuning buffers) failed");
   if (!st) st = launch_fill_uniform(a, nph, p->rb == 8 ? D2DHIP_R8 : D2DHIP_R4, 7, p->stream);
-  if (!st) st = hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
   if (!st) st = hip_check(hipEventCreate(&e0), "hipEventCreate");
   if (!st) st = hip_check(hipEventCreate(&e1), "hipEventCreate");
   const int prof = p->prof;
   p->prof = 0;
+  // ms per call of op over reps calls, after one untimed call
+  auto timed = [&](auto op, int reps, float *ms) {
+    int r = op();
+    if (!r) r = hip_check(hipEventRecord(e0, p->stream), "hipEventRecord");
+    for (int q = 0; q < reps && !r; ++q) r = op();
+    if (!r) r = hip_check(hipEventRecord(e1, p->stream), "hipEventRecord");
+    if (!r) r = hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float t = 0.f;
+    if (!r) r = hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+    *ms = t / reps;
+    return r;
+  };
+  // the blocked layout's kernels: the z stage (in place on the workspace,
+  // refilled by a blocked r2c of the noise before each candidate, so that the
+  // values stay finite), then the y stage (forward)
+  int blk4 = plain4, blk5 = plain5;
+  if (!st) st = use_layout(p, D2DHIP_LAYOUT_BLOCKED, plain4, plain5);
+  for (int idx = 5; idx >= 4 && !st; --idx) {
+    float best_t = 1e30f;
+    int best_c = p->geom[idx].choice;
+    for (int cc = -1; cc < (int)p->geom[idx].cands.size() && !st; ++cc) {
+      if ((st = configure(p, idx, cc)) || (st = upload_descs(p))) break;
+      if ((st = run_blocked_r2c(p, a, b))) break;
+      float t = 0.f;
+      st = timed([&]() { return idx == 5 ? run_blk_z(p, p->wk, 0) : run_blk(p, 3, 4, p->wk, b, 0); },
+                 2, &t);
+      if (!st && t < best_t) {
+        best_t = t;
+        best_c = cc;
+      }
+    }
+    if (!st) st = configure(p, idx, best_c);
+    if (!st) st = upload_descs(p);
+    (idx == 5 ? blk5 : blk4) = best_c;
+  }
+  // round trips, two rounds, each layout's best kept (one run each is
+  // decided by noise when the two are close)
   float ms[2] = {1e30f, 1e30f};
-  // two rounds, each layout's best kept (one run each is decided by noise
-  // when the two are close)
   for (int round = 0; round < 2 && !st; ++round) {
     for (int lay = 0; lay < 2 && !st; ++lay) {
-      if ((st = set_layout(p, lay))) break;
-      auto once = [&]() {
+      if ((st = use_layout(p, lay, lay ? blk4 : plain4, lay ? blk5 : plain5))) break;
+      float t = 0.f;
+      st = timed([&]() {
         int r = lay ? run_blocked_r2c(p, a, b) : d2dhip_exec_r2c(p, a, b);
         return r ? r : (lay ? run_blocked_c2r(p, b, c) : d2dhip_exec_c2r(p, b, c));
-      };
-      if ((st = once())) break;
-      if ((st = hip_check(hipEventRecord(e0, p->stream), "hipEventRecord"))) break;
-      const int reps = 3;
-      for (int q = 0; q < reps && !st; ++q) st = once();
-      if (st) break;
-      if ((st = hip_check(hipEventRecord(e1, p->stream), "hipEventRecord"))) break;
-      if ((st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize"))) break;
-      float t = 0.f;
-      if ((st = hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))) break;
-      ms[lay] = std::min(ms[lay], t / reps);
+      }, 3, &t);
+      if (!st) ms[lay] = std::min(ms[lay], t);
     }
   }
   p->prof = prof;
@@ -1327,12 +1380,9 @@ int tune_layout(d2dhip_plan_s *p) {
   if (b) (void)hipFree(b);
   if (c) (void)hipFree(c);
   const int best = ms[1] < ms[0] ? D2DHIP_LAYOUT_BLOCKED : D2DHIP_LAYOUT_PLAIN;
-  if (!st) {
-    char v[16];
-    std::snprintf(v, sizeof(v), "%d", best);
-    tune_store(p, key, v);
-  }
-  int st2 = set_layout(p, best);
+  int st2 = best ? use_layout(p, best, blk4, blk5) : use_layout(p, best, plain4, plain5);
+  if (!st && !st2)
+    tune_store(p, key, best ? "1|" + choice_name(p, 4) + "|" + choice_name(p, 5) : std::string("0"));
   return st ? st : st2;
 }
 
