@@ -1277,36 +1277,46 @@ int cand_index(const d2dhip_plan_s *p, int idx, const std::string &name) {
   return -2;
 }
 
-// Kernel choices of the y and z stages and the layout, uploaded together.
-int use_layout(d2dhip_plan_s *p, int layout, int c4, int c5) {
-  int st = configure(p, 4, c4);
-  if (!st) st = configure(p, 5, c5);
+// Kernel choices of stages 3-6 (c[0..3]) and the layout, uploaded together.
+int use_layout(d2dhip_plan_s *p, int layout, const int c[4]) {
+  int st = D2DHIP_SUCCESS;
+  for (int k = 0; k < 4 && !st; ++k) st = configure(p, 3 + k, c[k]);
   if (!st) st = set_layout(p, layout);
   if (!st) st = upload_descs(p);
   return st;
 }
 
 // Times the r2c + c2r round trip in each layout on scratch buffers and keeps
-// the faster, each layout with its own y- and z-stage kernels (the blocked
-// layout's picked by timing every candidate in the blocked form); cached like
-// the kernel choices, as "0" or "1|<y kernel>|<z kernel>".
+// the faster, each layout with its own kernels for stages 3-6 (the blocked
+// layout's picked by timing every candidate of each stage in the blocked
+// form); cached like the kernel choices, as "0" or "1|<k3>|<k4>|<k5>|<k6>".
 int tune_layout(d2dhip_plan_s *p) {
   if (!blk_possible(p) || (p->flags & (D2DHIP_FLAG_SKIP_Y | D2DHIP_FLAG_SKIP_Z)) || p->chunk_fwd ||
       p->chunk_bwd)
     return D2DHIP_SUCCESS;
   char kb[512];
-  std::snprintf(kb, sizeof(kb), "v2 layout pr=%d n=%d,%d,%d k=%s|%s|%s|%s", p->prec, p->nx, p->ny,
+  std::snprintf(kb, sizeof(kb), "v3 layout pr=%d n=%d,%d,%d k=%s|%s|%s|%s", p->prec, p->nx, p->ny,
                 p->nz, choice_name(p, 3).c_str(), choice_name(p, 4).c_str(),
                 choice_name(p, 5).c_str(), choice_name(p, 6).c_str());
   const std::string key(kb);
-  const int plain4 = p->geom[4].choice, plain5 = p->geom[5].choice;
+  int plain[4], blk[4];
+  for (int k = 0; k < 4; ++k) plain[k] = blk[k] = p->geom[3 + k].choice;
   auto it = g_tune.find(key);
   if (it != g_tune.end()) {
     const std::string &v = it->second;
-    const size_t bar = v.size() > 2 && v[0] == '1' && v[1] == '|' ? v.find('|', 2) : std::string::npos;
-    if (bar != std::string::npos) {
-      const int c4 = cand_index(p, 4, v.substr(2, bar - 2)), c5 = cand_index(p, 5, v.substr(bar + 1));
-      if (c4 != -2 && c5 != -2) return use_layout(p, D2DHIP_LAYOUT_BLOCKED, c4, c5);
+    if (v.size() > 2 && v[0] == '1' && v[1] == '|') {
+      size_t at = 2;
+      bool ok = true;
+      for (int k = 0; k < 4 && ok; ++k) {
+        const size_t bar = k < 3 ? v.find('|', at) : v.size();
+        ok = bar != std::string::npos;
+        if (ok) {
+          blk[k] = cand_index(p, 3 + k, v.substr(at, bar - at));
+          ok = blk[k] != -2;
+          at = bar + 1;
+        }
+      }
+      if (ok) return use_layout(p, D2DHIP_LAYOUT_BLOCKED, blk);
     }
     return set_layout(p, D2DHIP_LAYOUT_PLAIN);
   }
@@ -1336,20 +1346,28 @@ int tune_layout(d2dhip_plan_s *p) {
     *ms = t / reps;
     return r;
   };
-  // the blocked layout's kernels: the z stage (in place on the workspace,
-  // refilled by a blocked r2c of the noise before each candidate, so that the
-  // values stay finite), then the y stage (forward)
-  int blk4 = plain4, blk5 = plain5;
-  if (!st) st = use_layout(p, D2DHIP_LAYOUT_BLOCKED, plain4, plain5);
-  for (int idx = 5; idx >= 4 && !st; --idx) {
+  // the blocked layout's kernels, stage by stage in the blocked form: z (in
+  // place on the workspace, refilled by a blocked r2c of the noise before
+  // each candidate, so that the values stay finite), y forward, r2c x into
+  // the blocks, c2r x out of them
+  if (!st) st = use_layout(p, D2DHIP_LAYOUT_BLOCKED, blk);
+  const int order[4] = {5, 4, 3, 6};
+  for (int q = 0; q < 4 && !st; ++q) {
+    const int idx = order[q];
     float best_t = 1e30f;
     int best_c = p->geom[idx].choice;
     for (int cc = -1; cc < (int)p->geom[idx].cands.size() && !st; ++cc) {
       if ((st = configure(p, idx, cc)) || (st = upload_descs(p))) break;
       if ((st = run_blocked_r2c(p, a, b))) break;
       float t = 0.f;
-      st = timed([&]() { return idx == 5 ? run_blk_z(p, p->wk, 0) : run_blk(p, 3, 4, p->wk, b, 0); },
-                 2, &t);
+      st = timed([&]() {
+        switch (idx) {
+          case 5: return run_blk_z(p, p->wk, 0);
+          case 4: return run_blk(p, 3, 4, p->wk, b, 0);
+          case 3: return run_blk(p, 0, 3, a, p->wk, 0);
+          default: return run_blk(p, 5, 6, p->wk, c, 1);
+        }
+      }, 2, &t);
       if (!st && t < best_t) {
         best_t = t;
         best_c = cc;
@@ -1357,14 +1375,14 @@ int tune_layout(d2dhip_plan_s *p) {
     }
     if (!st) st = configure(p, idx, best_c);
     if (!st) st = upload_descs(p);
-    (idx == 5 ? blk5 : blk4) = best_c;
+    blk[idx - 3] = best_c;
   }
   // round trips, two rounds, each layout's best kept (one run each is
   // decided by noise when the two are close)
   float ms[2] = {1e30f, 1e30f};
   for (int round = 0; round < 2 && !st; ++round) {
     for (int lay = 0; lay < 2 && !st; ++lay) {
-      if ((st = use_layout(p, lay, lay ? blk4 : plain4, lay ? blk5 : plain5))) break;
+      if ((st = use_layout(p, lay, lay ? blk : plain))) break;
       float t = 0.f;
       st = timed([&]() {
         int r = lay ? run_blocked_r2c(p, a, b) : d2dhip_exec_r2c(p, a, b);
@@ -1380,9 +1398,15 @@ int tune_layout(d2dhip_plan_s *p) {
   if (b) (void)hipFree(b);
   if (c) (void)hipFree(c);
   const int best = ms[1] < ms[0] ? D2DHIP_LAYOUT_BLOCKED : D2DHIP_LAYOUT_PLAIN;
-  int st2 = best ? use_layout(p, best, blk4, blk5) : use_layout(p, best, plain4, plain5);
-  if (!st && !st2)
-    tune_store(p, key, best ? "1|" + choice_name(p, 4) + "|" + choice_name(p, 5) : std::string("0"));
+  int st2 = use_layout(p, best, best ? blk : plain);
+  if (!st && !st2) {
+    std::string v = "0";
+    if (best) {
+      v = "1";
+      for (int k = 3; k <= 6; ++k) v += "|" + choice_name(p, k);
+    }
+    tune_store(p, key, v);
+  }
   return st ? st : st2;
 }
 

@@ -6,8 +6,8 @@ plan's workspace, is blocked, and the y and z passes run in the other order.
 Checked against numpy pocketfft (tolerances as in test_gpu_fft.py: FP64 1e-13,
 FP32 5e-6, split-BF16 1e-6 against the FP64 reference of the FP32-rounded
 input), with the input of c2r preserved, the layout agreeing with the plain
-one to rounding, every candidate kernel of the two spectral stages in the
-blocked form, and the calls that keep the plain layout.
+one to rounding, every candidate kernel of the four stages in the blocked
+form, the tuner's choice, and the calls that keep the plain layout.
 """
 import numpy as np
 import pytest
@@ -82,14 +82,14 @@ def test_blocked_r2c_c2r(d2d, torch_gpu, shape, prec):
 
 @pytest.mark.parametrize("shape,prec", [((34, 64, 128), 64), ((34, 1024, 1024), 64),
                                         ((34, 1024, 1024), 32), ((34, 1024, 1024), 16)])
-def test_blocked_every_spectral_kernel(d2d, torch_gpu, shape, prec):
-    """Every kernel able to run the y or z stage, in the blocked form."""
+def test_blocked_every_kernel(d2d, torch_gpu, shape, prec):
+    """Every kernel able to run the x, y or z stage, in the blocked form."""
     torch = torch_gpu
     x = field(shape, prec, 7)
     ref = reference(x)
     e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, *shape, prec=prec)
     e.set_layout(d2d.LAYOUT_BLOCKED)
-    for stage in ("spec_y", "spec_last"):
+    for stage in ("spec_y", "spec_last", "r2c", "c2r"):
         keep = e.stage_kernels()[stage]
         for name in e.stage_candidates(stage):
             print("blocked", shape, prec, stage, name, flush=True)
@@ -98,6 +98,28 @@ def test_blocked_every_spectral_kernel(d2d, torch_gpu, shape, prec):
             assert rel_l2(spec, ref) <= TOL[prec], (stage, name)
             assert rel_l2(back, x) <= TOL[prec], (stage, name)
         e.set_stage_kernel(stage, keep)
+
+
+def test_tuned_layout(d2d, torch_gpu, tmp_path, monkeypatch):
+    """A tuned plan races both layouts (each with its own kernels) and caches
+    the pick; a second plan of the shape takes it from the cache."""
+    torch = torch_gpu
+    cache = tmp_path / "tune.tsv"
+    monkeypatch.setenv("D2DHIP_TUNE_CACHE", str(cache))
+    shape = (130, 64, 128)
+    x = field(shape, 64, 5)
+    ref = reference(x)
+    e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, *shape, tune=True)
+    spec, back = round_trip(d2d, torch, e, x, 64)
+    assert rel_l2(spec, ref) <= TOL[64] and rel_l2(back, x) <= TOL[64]
+    lines = [ln for ln in cache.read_text().splitlines() if " layout " in ln]
+    assert len(lines) == 1
+    val = lines[0].split("\t")[1]
+    assert val == "0" or (val.startswith("1|") and val.count("|") == 4), val
+    e2 = d2d.FFTEngine(d2d.PHYSICAL_IN_X, *shape, tune=True)
+    assert e2.layout() == e.layout() and e2.stage_kernels() == e.stage_kernels()
+    spec2, back2 = round_trip(d2d, torch, e2, x, 64)
+    assert np.array_equal(spec2, spec) and np.array_equal(back2, back)
 
 
 def test_blocked_plain_fallbacks(d2d, torch_gpu):
