@@ -34,7 +34,7 @@ FLAG_SKIP_X = 0x2
 FLAG_SKIP_Y = 0x4
 FLAG_SKIP_Z = 0x8
 FLAG_TUNE = 0x10
-LAYOUT_PLAIN, LAYOUT_BLOCKED = 0, 1
+LAYOUT_PLAIN, LAYOUT_BLOCKED, LAYOUT_BLOCKED32 = 0, 1, 2
 R8, C16, R4, C8 = 0, 1, 2, 3
 X_TO_Y, Y_TO_Z, Z_TO_Y, Y_TO_X = 0, 1, 2, 3
 SUCCESS = 0
@@ -372,7 +372,7 @@ class FFTEngine:
         return f.value, b.value
 
     def set_layout(self, layout):
-        """LAYOUT_PLAIN or LAYOUT_BLOCKED: the spectral pencil's order between
+        """LAYOUT_PLAIN, LAYOUT_BLOCKED or LAYOUT_BLOCKED32: the spectral pencil's order between
         the stages of fft_r2c / fft_c2r (PHYSICAL_IN_X whole-grid engines)."""
         check(lib().d2dhip_plan_set_layout(self._plan, int(layout)))
 

@@ -119,10 +119,10 @@ __device__ __forceinline__ void fast_tile(DescPtr d, const void *in, void *out,
   // (blocked columns: c2c column stages of the blocked spectral layout)
   const long long in_base =
       ((IO == IO_R2C || DTT) && PAIRED ? ca : c0) * d->in_s0 + c1 * d->in_s1 + f * d->in_sj +
-      (IO == IO_C2C && !LINES ? (c0 >> 4) * d->in_b0 : 0);
+      (IO == IO_C2C && !LINES ? (c0 >> d->bsh) * d->in_b0 : 0);
   const long long out_base =
       ((IO == IO_C2R || DTT) && PAIRED ? ca : c0) * d->out_s0 + c1 * d->out_s1 + f * d->out_sj +
-      (IO == IO_C2C && !LINES ? (c0 >> 4) * d->out_b0 : 0);
+      (IO == IO_C2C && !LINES ? (c0 >> d->bsh) * d->out_b0 : 0);
   const void *tw = d->tw;
   cpx<T> v[E];
 

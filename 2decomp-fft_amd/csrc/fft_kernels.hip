@@ -44,7 +44,7 @@ __device__ __forceinline__ cpx<T> pload(DescPtr d, const AxisIo &io, long long c
   } else {
     cpx<T> v = {T(0), T(0)};
     if (c0 < d->C0)  // (a c2c stage may read its pencil in unpack order, packj = 2 or 3)
-      v = gload<T>(io.in, c0 * d->in_s0 + (c0 >> 4) * d->in_b0 + c1 * d->in_s1 +
+      v = gload<T>(io.in, c0 * d->in_s0 + (c0 >> d->bsh) * d->in_b0 + c1 * d->in_s1 +
                               (long long)j * d->in_sj + (d->packj >= 2 ? packj_offset(d, j, c1) : 0));
     if (io.conj) v.y = -v.y;
     return v;
@@ -64,7 +64,7 @@ __device__ __forceinline__ void pstore(DescPtr d, const AxisIo &io, long long c0
   } else {
     if (c0 < d->C0) {
       if (io.conj) v.y = -v.y;
-      gstore<T>(io.out, c0 * d->out_s0 + (c0 >> 4) * d->out_b0 + c1 * d->out_s1 + (long long)j * d->out_sj +
+      gstore<T>(io.out, c0 * d->out_s0 + (c0 >> d->bsh) * d->out_b0 + c1 * d->out_s1 + (long long)j * d->out_sj +
                             (d->packj == 1 ? packj_offset(d, j, c1) : d->packj == 3 ? packj_offset2(d, j, c1) : 0),
                 v);
     }

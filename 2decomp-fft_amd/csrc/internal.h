@@ -67,9 +67,10 @@ struct AxisDesc {
   long long C1;       // columns along c1
   long long in_s0, in_s1, in_sj;
   long long out_s0, out_s1, out_sj;
-  // Blocked columns (c2c stages): column c0 sits at c0 * s0 + (c0 >> 4) * b0,
-  // i.e. blocks of 16 columns b0 + 16 * s0 elements apart; 0 = plain.
+  // Blocked columns (c2c stages): column c0 sits at c0 * s0 + (c0 >> bsh) * b0,
+  // i.e. blocks of 2^bsh columns b0 + 2^bsh * s0 elements apart; 0 = plain.
   long long in_b0, out_b0;
+  int bsh;
   int ldj, ldt;       // LDS strides (complex units) of point j and column t
   int nh;             // n/2 + 1 of the line length
   // Bluestein (chirp-z) stages: the line length is nblue, `n` and the passes
@@ -115,10 +116,11 @@ struct AxisDesc {
 // Index of complex point i of line l in the packed order of d (npack > 0):
 // one 16-byte read of the point's entry {base - b1, w} (those of the block
 // holding i), so that a fused unpack's loads wait on a single cached read.
-// npack < 0: the blocked spectral layout, blocks of 16 points pack_bs apart.
+// npack < 0: the blocked spectral layout, blocks of 2^bsh points pack_bs apart.
 __device__ __forceinline__ long long pack_index(const __attribute__((address_space(4))) AxisDesc *d,
                                                 int i, long long l) {
-  if (d->npack < 0) return (long long)(i >> 4) * d->pack_bs + (i & 15) + 16 * l;
+  if (d->npack < 0)
+    return (long long)(i >> d->bsh) * d->pack_bs + (i & ((1 << d->bsh) - 1)) + (l << d->bsh);
   typedef long long ll2 __attribute__((ext_vector_type(2)));
   const __attribute__((address_space(1))) ll2 *pk =
       (const __attribute__((address_space(1))) ll2 *)d->pack;

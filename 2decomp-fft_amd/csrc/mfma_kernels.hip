@@ -311,9 +311,9 @@ __device__ __forceinline__ void mf_fetch(DescPtr d, const void *in, int conj, lo
     ca = 2 * c0;
     cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
     ib = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1 +
-         (IO == IO_C2C && !LINES ? (c0 >> 4) * d->in_b0 : 0);
+         (IO == IO_C2C && !LINES ? (c0 >> d->bsh) * d->in_b0 : 0);
     ob = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1 +
-         (IO == IO_C2C && !LINES ? (c0 >> 4) * d->out_b0 : 0);
+         (IO == IO_C2C && !LINES ? (c0 >> d->bsh) * d->out_b0 : 0);
   };
   // LDS element of point j of tile column t
   auto at = [&](int j, int t) __attribute__((always_inline)) { return C::row(j) * LDJ + t * LDT; };
@@ -421,9 +421,9 @@ __device__ __forceinline__ void mf_run(DescPtr d, void *out, int conj, long long
     ca = 2 * c0;
     cvalid = (IO == IO_C2C) ? (c0 < d->C0) : (ca < d->C0r);
     ib = (IO == IO_R2C ? ca : c0) * d->in_s0 + c1 * d->in_s1 +
-         (IO == IO_C2C && !LINES ? (c0 >> 4) * d->in_b0 : 0);
+         (IO == IO_C2C && !LINES ? (c0 >> d->bsh) * d->in_b0 : 0);
     ob = (IO == IO_C2R ? ca : c0) * d->out_s0 + c1 * d->out_s1 +
-         (IO == IO_C2C && !LINES ? (c0 >> 4) * d->out_b0 : 0);
+         (IO == IO_C2C && !LINES ? (c0 >> d->bsh) * d->out_b0 : 0);
   };
   // LDS element of point j of tile column t
   auto at = [&](int j, int t) __attribute__((always_inline)) { return C::row(j) * LDJ + t * LDT; };

@@ -91,18 +91,19 @@ struct d2dhip_plan_s {
   int chunk_fwd = 0, chunk_bwd = 0;
   AxisDesc h_cdesc[8];
   AxisDesc *d_cdesc = nullptr;
-  // Blocked spectral layout (D2DHIP_LAYOUT_BLOCKED; PHYSICAL_IN_X whole-grid
-  // plans with a workspace): between the x stage and the last stage of
-  // exec_r2c / exec_c2r the spectral pencil sits in the workspace in blocks
-  // of 16 i-columns, element (i, j, k) at
-  //   (i >> 4) * bs + k * 16 * nyb + j * 16 + (i & 15),
-  //   nyb = ny + 1, bs = 16 * nyb * nz,
-  // so that a z-stage tile reads and writes rows of 16 elements lying
-  // 16 * nyb elements apart, several to an address-translation page (the
-  // plain z stage's rows lie a whole i-j plane apart, one page each, and
-  // keep the translation cache busy), and a y-stage tile is one contiguous
-  // run.  The spare j row keeps the row distance off a power of two, which
-  // HBM serves more slowly.  Forward: x (into the blocks), z (in place), y
+  // Blocked spectral layout (D2DHIP_LAYOUT_BLOCKED / _BLOCKED32; PHYSICAL_IN_X
+  // whole-grid plans with a workspace): between the x stage and the last
+  // stage of exec_r2c / exec_c2r the spectral pencil sits in the workspace in
+  // blocks of W = 16 or 32 i-columns, element (i, j, k) at
+  //   (i / W) * bs + k * W * nyb + j * W + (i % W),
+  //   nyb = ny + 1, bs = W * nyb * nz,
+  // so that a z-stage tile reads and writes rows lying W * nyb elements
+  // apart, several to an address-translation page (the plain z stage's rows
+  // lie a whole i-j plane apart, one page each, and keep the translation
+  // cache busy), and a y-stage tile is one contiguous run.  An x-stage tile
+  // then touches one page per block, so the wider blocks halve its pages.
+  // The spare j row keeps the row distance off a power of two, which HBM
+  // serves more slowly.  Forward: x (into the blocks), z (in place), y
   // (blocks -> the plain output); inverse: y, z, x.  Descriptors: 0 x r2c,
   // 1 z over the whole blocks, 2 z over the last partial block, 3 y forward,
   // 4 y inverse, 5 x c2r.
@@ -1099,21 +1100,28 @@ int run_pair_chunked(d2dhip_plan_s *p, long long K, int q1, int i1, int q2, int 
 
 // ---- blocked spectral layout ----------------------------------------------------
 
+// Block width (i-columns per block) of the plan's blocked layout.
+int blk_w(const d2dhip_plan_s *p) { return p->layout == D2DHIP_LAYOUT_BLOCKED32 ? 32 : 16; }
 long long blk_nyb(const d2dhip_plan_s *p) { return (long long)p->ny + 1; }
-long long blk_bs(const d2dhip_plan_s *p) { return 16 * blk_nyb(p) * p->nz; }
-// Complex elements of the blocked pencil (whole blocks of 16 columns).
-long long blk_elems(const d2dhip_plan_s *p) { return ((p->nx / 2 + 1 + 15) / 16) * blk_bs(p); }
+long long blk_bs(const d2dhip_plan_s *p) { return blk_w(p) * blk_nyb(p) * p->nz; }
+// Complex elements of the blocked pencil (whole blocks).
+long long blk_elems(const d2dhip_plan_s *p) {
+  const int w = blk_w(p);
+  return ((p->nx / 2 + 1 + w - 1) / w) * blk_bs(p);
+}
 
 bool blk_possible(const d2dhip_plan_s *p) {
   return p->format == D2DHIP_PHYSICAL_IN_X && !p->multi && !(p->flags & D2DHIP_FLAG_INPLACE);
 }
 
 // Stage idx's descriptor (its kernel, twiddles and tile width) on other
-// columns: C0 x C1 columns, column c0 at c0 * s0 + (c0 >> 4) * b0.
+// columns: C0 x C1 columns, column c0 at c0 * s0 + (c0 >> bsh) * b0 with the
+// layout's block width 2^bsh.
 AxisDesc blk_cols(const d2dhip_plan_s *p, int idx, long long C0, long long C1, long long is0,
                   long long ib0, long long is1, long long isj, long long os0, long long ob0,
                   long long os1, long long osj) {
   AxisDesc a = p->h_desc[idx];
+  a.bsh = ilog2(blk_w(p));
   a.C0 = a.C0r = C0;
   a.C1 = C1;
   a.tiles0 = (int)((C0 + a.TX - 1) / a.TX);
@@ -1131,10 +1139,10 @@ AxisDesc blk_cols(const d2dhip_plan_s *p, int idx, long long C0, long long C1, l
 // Rebuilds and uploads the blocked-layout descriptors for the current kernel
 // choices (nothing while the layout is plain).
 int upload_blocked_descs(d2dhip_plan_s *p) {
-  if (p->layout != D2DHIP_LAYOUT_BLOCKED) return D2DHIP_SUCCESS;
-  const long long nxc = p->nx / 2 + 1, ny = p->ny, nz = p->nz;
-  const long long nyb = blk_nyb(p), bs = blk_bs(p), nfull = nxc / 16, r = nxc % 16;
-  // x stages: point i of line l = j + k * nyb at (i >> 4) * bs + (i & 15) + 16 l
+  if (p->layout == D2DHIP_LAYOUT_PLAIN) return D2DHIP_SUCCESS;
+  const long long nxc = p->nx / 2 + 1, ny = p->ny, nz = p->nz, W = blk_w(p);
+  const long long nyb = blk_nyb(p), bs = blk_bs(p), nfull = nxc / W, r = nxc % W;
+  // x stages: point i of line l = j + k * nyb at (i / W) * bs + (i % W) + W l
   // (the plain x stages see the ny * nz lines as one run; here lines j of
   // plane k, so that the line index can skip the spare j row)
   for (int s = 0; s < 2; ++s) {
@@ -1149,17 +1157,18 @@ int upload_blocked_descs(d2dhip_plan_s *p) {
     a.npack = -1;
     a.pack_l1 = nyb;
     a.pack_bs = bs;
+    a.bsh = ilog2((int)W);
     p->h_bdesc[s ? 5 : 0] = a;
   }
-  // z over whole blocks: the columns (j, i & 15) of a block are contiguous
-  p->h_bdesc[1] = blk_cols(p, 5, 16 * ny, nfull > 0 ? nfull : 1, 1, 0, bs, 16 * nyb, 1, 0, bs,
-                           16 * nyb);
+  // z over whole blocks: the columns (j, i % W) of a block are contiguous
+  p->h_bdesc[1] = blk_cols(p, 5, W * ny, nfull > 0 ? nfull : 1, 1, 0, bs, W * nyb, 1, 0, bs,
+                           W * nyb);
   // z over the last partial block (its first r columns of each row j),
   // launched on the block's first element
-  p->h_bdesc[2] = blk_cols(p, 5, r > 0 ? r : 1, ny, 1, 0, 16, 16 * nyb, 1, 0, 16, 16 * nyb);
+  p->h_bdesc[2] = blk_cols(p, 5, r > 0 ? r : 1, ny, 1, 0, W, W * nyb, 1, 0, W, W * nyb);
   // y: blocks -> plain (nx/2+1, ny, nz) forward, plain -> blocks inverse
-  p->h_bdesc[3] = blk_cols(p, 4, nxc, nz, 1, bs - 16, 16 * nyb, 16, 1, 0, nxc * ny, nxc);
-  p->h_bdesc[4] = blk_cols(p, 4, nxc, nz, 1, 0, nxc * ny, nxc, 1, bs - 16, 16 * nyb, 16);
+  p->h_bdesc[3] = blk_cols(p, 4, nxc, nz, 1, bs - W, W * nyb, W, 1, 0, nxc * ny, nxc);
+  p->h_bdesc[4] = blk_cols(p, 4, nxc, nz, 1, 0, nxc * ny, nxc, 1, bs - W, W * nyb, W);
   if (!p->d_bdesc && hipMalloc(&p->d_bdesc, sizeof(p->h_bdesc)) != hipSuccess) {
     p->d_bdesc = nullptr;
     return set_error(D2DHIP_ERR_ALLOC, "hipMalloc(blocked descriptors) failed");
@@ -1198,7 +1207,7 @@ int run_blk(d2dhip_plan_s *p, int slot, int idx, const void *in, void *out, int 
 // The z stage on the blocked pencil in wk (in place): the whole blocks, then
 // the last partial one.
 int run_blk_z(d2dhip_plan_s *p, void *wk, int conj) {
-  const long long nxc = p->nx / 2 + 1, nfull = nxc / 16, r = nxc % 16;
+  const long long nxc = p->nx / 2 + 1, nfull = nxc / blk_w(p), r = nxc % blk_w(p);
   int st = D2DHIP_SUCCESS;
   if (nfull > 0) st = run_blk(p, 1, 5, wk, wk, conj);
   if (!st && r > 0)
@@ -1224,7 +1233,7 @@ int run_blocked_c2r(d2dhip_plan_s *p, const void *in_c, void *out_r) {
 // Whether an exec call runs in the blocked layout: set, whole stages, none
 // skipped, and the plan's own workspace.
 bool blocked_now(const d2dhip_plan_s *p, int chunk, const void *wk) {
-  return p->layout == D2DHIP_LAYOUT_BLOCKED && chunk == 0 && wk && wk == p->wk &&
+  return p->layout != D2DHIP_LAYOUT_PLAIN && chunk == 0 && wk && wk == p->wk &&
          !(p->flags & (D2DHIP_FLAG_SKIP_X | D2DHIP_FLAG_SKIP_Y | D2DHIP_FLAG_SKIP_Z));
 }
 
@@ -1233,17 +1242,19 @@ size_t wk_need(const d2dhip_plan_s *p) {
   const long long nsp = (p->format == D2DHIP_PHYSICAL_IN_X)
                             ? (long long)(p->nx / 2 + 1) * p->ny * p->nz
                             : (long long)p->nx * p->ny * (p->nz / 2 + 1);
-  const long long e = p->layout == D2DHIP_LAYOUT_BLOCKED ? std::max(nsp, blk_elems(p)) : nsp;
+  const long long e = p->layout != D2DHIP_LAYOUT_PLAIN ? std::max(nsp, blk_elems(p)) : nsp;
   return (size_t)e * 2 * p->rb;
 }
 
 // Switches the layout, resizing the workspace to what the layout needs.
 int set_layout(d2dhip_plan_s *p, int layout) {
-  if (layout == D2DHIP_LAYOUT_BLOCKED && !blk_possible(p))
+  if (layout != D2DHIP_LAYOUT_PLAIN && layout != D2DHIP_LAYOUT_BLOCKED &&
+      layout != D2DHIP_LAYOUT_BLOCKED32)
+    return set_error(D2DHIP_ERR_INVALID_VALUE,
+                     "layout must be D2DHIP_LAYOUT_PLAIN, _BLOCKED or _BLOCKED32");
+  if (layout != D2DHIP_LAYOUT_PLAIN && !blk_possible(p))
     return set_error(D2DHIP_ERR_UNSUPPORTED,
                      "blocked layout: a whole-grid PHYSICAL_IN_X plan with a workspace");
-  if (layout != D2DHIP_LAYOUT_PLAIN && layout != D2DHIP_LAYOUT_BLOCKED)
-    return set_error(D2DHIP_ERR_INVALID_VALUE, "layout must be D2DHIP_LAYOUT_PLAIN or _BLOCKED");
   if (p->capturing)
     return set_error(D2DHIP_ERR_UNSUPPORTED, "plan is capturing: its layout is fixed");
   drop_graph(p);
@@ -1286,37 +1297,41 @@ int use_layout(d2dhip_plan_s *p, int layout, const int c[4]) {
   return st;
 }
 
-// Times the r2c + c2r round trip in each layout on scratch buffers and keeps
-// the faster, each layout with its own kernels for stages 3-6 (the blocked
-// layout's picked by timing every candidate of each stage in the blocked
-// form); cached like the kernel choices, as "0" or "1|<k3>|<k4>|<k5>|<k6>".
+// Times the r2c + c2r round trip in each layout (plain, blocks of 16 and of
+// 32 columns) on scratch buffers and keeps the fastest, each layout with its
+// own kernels for stages 3-6 (a blocked layout's picked by timing every
+// candidate of each stage in that layout); cached like the kernel choices, as
+// "0" or "<layout>|<k3>|<k4>|<k5>|<k6>".
 int tune_layout(d2dhip_plan_s *p) {
   if (!blk_possible(p) || (p->flags & (D2DHIP_FLAG_SKIP_Y | D2DHIP_FLAG_SKIP_Z)) || p->chunk_fwd ||
       p->chunk_bwd)
     return D2DHIP_SUCCESS;
   char kb[512];
-  std::snprintf(kb, sizeof(kb), "v3 layout pr=%d n=%d,%d,%d k=%s|%s|%s|%s", p->prec, p->nx, p->ny,
+  std::snprintf(kb, sizeof(kb), "v4 layout pr=%d n=%d,%d,%d k=%s|%s|%s|%s", p->prec, p->nx, p->ny,
                 p->nz, choice_name(p, 3).c_str(), choice_name(p, 4).c_str(),
                 choice_name(p, 5).c_str(), choice_name(p, 6).c_str());
   const std::string key(kb);
-  int plain[4], blk[4];
-  for (int k = 0; k < 4; ++k) plain[k] = blk[k] = p->geom[3 + k].choice;
+  constexpr int kLayouts = 3;  // D2DHIP_LAYOUT_PLAIN, _BLOCKED, _BLOCKED32
+  int ch[kLayouts][4];
+  for (int l = 0; l < kLayouts; ++l)
+    for (int k = 0; k < 4; ++k) ch[l][k] = p->geom[3 + k].choice;
   auto it = g_tune.find(key);
   if (it != g_tune.end()) {
     const std::string &v = it->second;
-    if (v.size() > 2 && v[0] == '1' && v[1] == '|') {
+    const int lay = v.empty() ? 0 : v[0] - '0';
+    if (lay >= 1 && lay < kLayouts && v.size() > 2 && v[1] == '|') {
       size_t at = 2;
       bool ok = true;
       for (int k = 0; k < 4 && ok; ++k) {
         const size_t bar = k < 3 ? v.find('|', at) : v.size();
         ok = bar != std::string::npos;
         if (ok) {
-          blk[k] = cand_index(p, 3 + k, v.substr(at, bar - at));
-          ok = blk[k] != -2;
+          ch[lay][k] = cand_index(p, 3 + k, v.substr(at, bar - at));
+          ok = ch[lay][k] != -2;
           at = bar + 1;
         }
       }
-      if (ok) return use_layout(p, D2DHIP_LAYOUT_BLOCKED, blk);
+      if (ok) return use_layout(p, lay, ch[lay]);
     }
     return set_layout(p, D2DHIP_LAYOUT_PLAIN);
   }
@@ -1346,43 +1361,45 @@ int tune_layout(d2dhip_plan_s *p) {
     *ms = t / reps;
     return r;
   };
-  // the blocked layout's kernels, stage by stage in the blocked form: z (in
-  // place on the workspace, refilled by a blocked r2c of the noise before
-  // each candidate, so that the values stay finite), y forward, r2c x into
-  // the blocks, c2r x out of them
-  if (!st) st = use_layout(p, D2DHIP_LAYOUT_BLOCKED, blk);
+  // each blocked layout's kernels, stage by stage in that layout: z (in place
+  // on the workspace, refilled by a blocked r2c of the noise before each
+  // candidate, so that the values stay finite), y forward, r2c x into the
+  // blocks, c2r x out of them
   const int order[4] = {5, 4, 3, 6};
-  for (int q = 0; q < 4 && !st; ++q) {
-    const int idx = order[q];
-    float best_t = 1e30f;
-    int best_c = p->geom[idx].choice;
-    for (int cc = -1; cc < (int)p->geom[idx].cands.size() && !st; ++cc) {
-      if ((st = configure(p, idx, cc)) || (st = upload_descs(p))) break;
-      if ((st = run_blocked_r2c(p, a, b))) break;
-      float t = 0.f;
-      st = timed([&]() {
-        switch (idx) {
-          case 5: return run_blk_z(p, p->wk, 0);
-          case 4: return run_blk(p, 3, 4, p->wk, b, 0);
-          case 3: return run_blk(p, 0, 3, a, p->wk, 0);
-          default: return run_blk(p, 5, 6, p->wk, c, 1);
+  for (int lay = 1; lay < kLayouts && !st; ++lay) {
+    if ((st = use_layout(p, lay, ch[lay]))) break;
+    for (int q = 0; q < 4 && !st; ++q) {
+      const int idx = order[q];
+      float best_t = 1e30f;
+      int best_c = p->geom[idx].choice;
+      for (int cc = -1; cc < (int)p->geom[idx].cands.size() && !st; ++cc) {
+        if ((st = configure(p, idx, cc)) || (st = upload_descs(p))) break;
+        if ((st = run_blocked_r2c(p, a, b))) break;
+        float t = 0.f;
+        st = timed([&]() {
+          switch (idx) {
+            case 5: return run_blk_z(p, p->wk, 0);
+            case 4: return run_blk(p, 3, 4, p->wk, b, 0);
+            case 3: return run_blk(p, 0, 3, a, p->wk, 0);
+            default: return run_blk(p, 5, 6, p->wk, c, 1);
+          }
+        }, 2, &t);
+        if (!st && t < best_t) {
+          best_t = t;
+          best_c = cc;
         }
-      }, 2, &t);
-      if (!st && t < best_t) {
-        best_t = t;
-        best_c = cc;
       }
+      if (!st) st = configure(p, idx, best_c);
+      if (!st) st = upload_descs(p);
+      ch[lay][idx - 3] = best_c;
     }
-    if (!st) st = configure(p, idx, best_c);
-    if (!st) st = upload_descs(p);
-    blk[idx - 3] = best_c;
   }
   // round trips, two rounds, each layout's best kept (one run each is
-  // decided by noise when the two are close)
-  float ms[2] = {1e30f, 1e30f};
+  // decided by noise when two are close)
+  float ms[kLayouts] = {1e30f, 1e30f, 1e30f};
   for (int round = 0; round < 2 && !st; ++round) {
-    for (int lay = 0; lay < 2 && !st; ++lay) {
-      if ((st = use_layout(p, lay, lay ? blk : plain))) break;
+    for (int lay = 0; lay < kLayouts && !st; ++lay) {
+      if ((st = use_layout(p, lay, ch[lay]))) break;
       float t = 0.f;
       st = timed([&]() {
         int r = lay ? run_blocked_r2c(p, a, b) : d2dhip_exec_r2c(p, a, b);
@@ -1397,12 +1414,14 @@ int tune_layout(d2dhip_plan_s *p) {
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
   if (c) (void)hipFree(c);
-  const int best = ms[1] < ms[0] ? D2DHIP_LAYOUT_BLOCKED : D2DHIP_LAYOUT_PLAIN;
-  int st2 = use_layout(p, best, best ? blk : plain);
+  int best = 0;
+  for (int lay = 1; lay < kLayouts; ++lay)
+    if (ms[lay] < ms[best]) best = lay;
+  int st2 = use_layout(p, best, ch[best]);
   if (!st && !st2) {
     std::string v = "0";
     if (best) {
-      v = "1";
+      v = std::to_string(best);
       for (int k = 3; k <= 6; ++k) v += "|" + choice_name(p, k);
     }
     tune_store(p, key, v);
@@ -1554,8 +1573,8 @@ int plan_create(d2dhip_plan *plan, int format, int nx, int ny, int nz, int prec,
   if (!st && blk_possible(p)) {
     // D2DHIP_LAYOUT=plain|blocked fixes the layout; tuning times both
     const char *e = std::getenv("D2DHIP_LAYOUT");
-    if (e && std::strcmp(e, "blocked") == 0) {
-      st = set_layout(p, D2DHIP_LAYOUT_BLOCKED);
+    if (e && (std::strcmp(e, "blocked") == 0 || std::strcmp(e, "blocked32") == 0)) {
+      st = set_layout(p, e[7] ? D2DHIP_LAYOUT_BLOCKED32 : D2DHIP_LAYOUT_BLOCKED);
     } else if (!(e && std::strcmp(e, "plain") == 0) && tune) {
       std::lock_guard<std::mutex> lock(g_tune_mu);
       tune_load();

@@ -31,7 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))  # oracle/oracle.py (cpu_baseli
 
 PEAK_HBM_GBS = 8000.0
 # spectral pencil order between the stages (the tuner picks; see d2dhip_plan_set_layout)
-LAYOUT_NAMES = {0: "plain", 1: "blocked"}
+LAYOUT_NAMES = {0: "plain", 1: "blocked", 2: "blocked32"}
 PEAK_BF16_FLOPS = 2.5e15  # dense bf16 MFMA peak of MI355X
 
 # HBM bytes per launch of the 1024^3 FP64 stage kernels, from two rocprofv3

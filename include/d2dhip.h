@@ -181,18 +181,20 @@ int d2dhip_plan_get_chunking(d2dhip_plan plan, int *fwd_planes, int *bwd_planes)
 /* Layout of the spectral pencil inside exec_r2c / exec_c2r of a
  * PHYSICAL_IN_X plan (no counterpart in the reference, whose stages work
  * on the natural (i, j, k) order; the inputs and outputs of the calls keep
- * that order either way).  D2DHIP_LAYOUT_BLOCKED keeps the pencil between
- * the x stage and the last stage in the plan's workspace as blocks of 16
- * i-columns and runs the stages in the order x, z, y (forward) and y, z, x
+ * that order either way).  D2DHIP_LAYOUT_BLOCKED (_BLOCKED32) keeps the
+ * pencil between the x stage and the last stage in the plan's workspace as
+ * blocks of 16 (32) i-columns and runs the stages in the order x, z, y
+ * (forward) and y, z, x
  * (inverse), so that no stage walks rows a whole i-j plane apart; the
  * result agrees with the plain layout to rounding (the order of the y and z
  * passes differs).  It needs a whole-grid plan with a workspace (not
  * D2DHIP_FLAG_INPLACE), and exec_c2r_noalloc and chunked or skipped stages
  * keep the plain layout.  Plans created with D2DHIP_FLAG_TUNE pick the
- * layout by timing; the environment variable D2DHIP_LAYOUT=plain|blocked
- * fixes it at plan creation. */
+ * layout by timing; the environment variable
+ * D2DHIP_LAYOUT=plain|blocked|blocked32 fixes it at plan creation. */
 #define D2DHIP_LAYOUT_PLAIN 0
 #define D2DHIP_LAYOUT_BLOCKED 1
+#define D2DHIP_LAYOUT_BLOCKED32 2
 int d2dhip_plan_set_layout(d2dhip_plan plan, int layout);
 int d2dhip_plan_get_layout(d2dhip_plan plan, int *layout);
 /* Device bytes the plan holds (twiddles + workspace). */

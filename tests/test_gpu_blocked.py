@@ -1,4 +1,5 @@
-"""Blocked spectral layout of PHYSICAL_IN_X r2c / c2r plans (d2dhip_plan_set_layout).
+"""Blocked spectral layouts of PHYSICAL_IN_X r2c / c2r plans (d2dhip_plan_set_layout:
+blocks of 16 and of 32 i-columns).
 
 The calls' inputs and outputs keep the reference's natural (i, j, k) order
 (src/fft_common_3d.f90:121-296); only the pencil between the stages, in the
@@ -60,15 +61,16 @@ SHAPES = [(20, 12, 10), (30, 16, 24), (34, 20, 18), (130, 64, 128), (64, 1024, 1
 
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("prec", [64, 32, 16])
-def test_blocked_r2c_c2r(d2d, torch_gpu, shape, prec):
+@pytest.mark.parametrize("layout", [1, 2])
+def test_blocked_r2c_c2r(d2d, torch_gpu, shape, prec, layout):
     if prec == 16 and not (shape[1] in (512, 1024) and shape[2] in (512, 1024)):
         pytest.skip("split-BF16 plans use the matrix cores on 512/1024-point axes only")
-    print("blocked r2c/c2r", shape, "prec", prec, flush=True)
+    print("blocked r2c/c2r", shape, "prec", prec, "layout", layout, flush=True)
     torch = torch_gpu
     x = field(shape, prec, sum(shape) + prec)
     e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, *shape, prec=prec)
-    e.set_layout(d2d.LAYOUT_BLOCKED)
-    assert e.layout() == d2d.LAYOUT_BLOCKED
+    e.set_layout(layout)
+    assert e.layout() == layout
     spec, back = round_trip(d2d, torch, e, x, prec)
     ref = reference(x)
     assert rel_l2(spec, ref) <= TOL[prec]
@@ -80,15 +82,16 @@ def test_blocked_r2c_c2r(d2d, torch_gpu, shape, prec):
     assert rel_l2(back, back0) <= TOL[prec]
 
 
-@pytest.mark.parametrize("shape,prec", [((34, 64, 128), 64), ((34, 1024, 1024), 64),
-                                        ((34, 1024, 1024), 32), ((34, 1024, 1024), 16)])
-def test_blocked_every_kernel(d2d, torch_gpu, shape, prec):
+@pytest.mark.parametrize("shape,prec,layout", [((34, 64, 128), 64, 1), ((34, 1024, 1024), 64, 1),
+                                               ((34, 1024, 1024), 32, 1), ((34, 1024, 1024), 16, 1),
+                                               ((66, 64, 128), 64, 2), ((66, 1024, 1024), 16, 2)])
+def test_blocked_every_kernel(d2d, torch_gpu, shape, prec, layout):
     """Every kernel able to run the x, y or z stage, in the blocked form."""
     torch = torch_gpu
     x = field(shape, prec, 7)
     ref = reference(x)
     e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, *shape, prec=prec)
-    e.set_layout(d2d.LAYOUT_BLOCKED)
+    e.set_layout(layout)
     for stage in ("spec_y", "spec_last", "r2c", "c2r"):
         keep = e.stage_kernels()[stage]
         for name in e.stage_candidates(stage):
@@ -115,7 +118,7 @@ def test_tuned_layout(d2d, torch_gpu, tmp_path, monkeypatch):
     lines = [ln for ln in cache.read_text().splitlines() if " layout " in ln]
     assert len(lines) == 1
     val = lines[0].split("\t")[1]
-    assert val == "0" or (val.startswith("1|") and val.count("|") == 4), val
+    assert val == "0" or (val[0] in "12" and val[1] == "|" and val.count("|") == 4), val
     e2 = d2d.FFTEngine(d2d.PHYSICAL_IN_X, *shape, tune=True)
     assert e2.layout() == e.layout() and e2.stage_kernels() == e.stage_kernels()
     spec2, back2 = round_trip(d2d, torch, e2, x, 64)
@@ -158,7 +161,7 @@ def test_blocked_plain_fallbacks(d2d, torch_gpu):
     with pytest.raises(d2d.D2dError):
         z.set_layout(d2d.LAYOUT_BLOCKED)
     with pytest.raises(d2d.D2dError):
-        e.set_layout(2)
+        e.set_layout(3)
 
 
 def test_blocked_graph_replay(d2d, torch_gpu):
