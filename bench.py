@@ -388,7 +388,7 @@ def bench_config(args, torch, d2d, w):
         extra["stage_ms"] = {k: round(v[0], 4) for k, v in rep.items()}
         extra["kernels"] = {k: v for k, v in e.dtt_stage_kernels().items()}
         # DCT-I both ways on the same grid (all 513 points, extension of 1024)
-        e1 = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, opt_DTT=(1, 1, 1))
+        e1 = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, tune=not args.no_tune, opt_DTT=(1, 1, 1))
 
         def step1():
             e1.dtt_3d_r2r(x, f, d2d.FORWARD)

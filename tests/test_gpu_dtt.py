@@ -148,6 +148,41 @@ def test_specialised_dtt_tiles_match_general(d2d, oracle, torch_gpu, prec, shape
     assert rel_l2(y[sl], _host(xin, shape)[sl]) < tol
 
 
+@pytest.mark.parametrize("shape,dtt,axis", [((9, 513, 17), (1, 1, 1), 1), ((9, 17, 513), (2, 2, 2), 2),
+                                            ((9, 17, 513), (4, 6, 6), 2)])
+def test_every_dtt_column_kernel(d2d, oracle, torch_gpu, shape, dtt, axis):
+    """Every specialised kernel able to run the long axis's real-to-real
+    stages (forward and backward; the 1024-point extension of DCT-I, the
+    512-point tiles of DCT-II / III and DST-II / III) matches the oracle to
+    1e-13 and returns the input."""
+    torch = torch_gpu
+    e = d2d.FFTEngine(1, *shape, opt_DTT=dtt)
+    x = np.asfortranarray(np.random.default_rng(3).uniform(-1, 1, shape))
+    xin = _dev(torch, x, torch.float64)
+    out = torch.empty_like(xin)
+    back = torch.empty_like(xin)
+    want = oracle.dtt3d(x, dtt, -1, 1)
+    fwd, bwd = 7 + axis, 10 + axis
+    cands_f = [n for n in e.stage_candidates(fwd) if n != "general"]
+    cands_b = [n for n in e.stage_candidates(bwd) if n != "general"]
+    assert len(cands_f) >= 2 and len(cands_b) >= 2, (cands_f, cands_b)
+    for name_f in cands_f:
+        for name_b in cands_b:
+            if name_f.split(" dct")[0] != name_b.split(" dct")[0]:
+                continue
+            print("dtt", shape, dtt, name_f, "/", name_b, flush=True)
+            e.set_stage_kernel(fwd, name_f)
+            e.set_stage_kernel(bwd, name_b)
+            e.dtt_3d_r2r(xin, out, d2d.FORWARD)
+            e.dtt_3d_r2r(out, back, d2d.BACKWARD)
+            torch.cuda.synchronize()
+            assert rel_l2(_host(out, shape), want) < 1e-13, (name_f, name_b)
+            y = _host(back, shape) / oracle.dtt_norm(dtt, shape)
+            c = oracle.dtt_config(dtt)
+            sl = tuple(slice(c[3 + a] - 1, c[3 + a] - 1 + shape[a] - c[6 + a]) for a in range(3))
+            assert rel_l2(y[sl], x[sl]) < 1e-13, (name_f, name_b)
+
+
 def test_dtt_on_split_bf16_plan(d2d, oracle, torch_gpu):
     """A split-BF16 plan stores FP32, so its real-to-real stages run the FP32
     tiles (the matrix-core kernels are complex-to-complex only)."""
