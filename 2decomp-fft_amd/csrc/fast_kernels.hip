@@ -1088,10 +1088,8 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(float, 64, 16, 128, true, R8_8),
       // real-to-real stages of (2^k + 1)-point grids (k = 6..10)
       D2D_DTT(double, 1024, 4, 256, false, R16_16_4),
-      D2D_DTT(double, 1024, 8, 512, false, R16_16_4),
       D2D_DTT(double, 1024, 4, 256, true, R16_16_4),
       D2D_DTT(double, 512, 8, 512, false, R8_8_8),
-      D2D_DTT(double, 512, 16, 1024, false, R8_8_8),
       D2D_DTT(double, 512, 4, 256, true, R8_8_8),
       D2D_DTT(double, 256, 8, 128, false, R16_16),
       D2D_DTT(double, 256, 4, 64, true, R16_16),

@@ -165,7 +165,7 @@ def test_every_dtt_column_kernel(d2d, oracle, torch_gpu, shape, dtt, axis):
     fwd, bwd = 7 + axis, 10 + axis
     cands_f = [n for n in e.stage_candidates(fwd) if n != "general"]
     cands_b = [n for n in e.stage_candidates(bwd) if n != "general"]
-    assert len(cands_f) >= 2 and len(cands_b) >= 2, (cands_f, cands_b)
+    assert cands_f and cands_b, (cands_f, cands_b)
     for name_f in cands_f:
         for name_b in cands_b:
             if name_f.split(" dct")[0] != name_b.split(" dct")[0]:
