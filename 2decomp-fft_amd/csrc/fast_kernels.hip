@@ -785,6 +785,15 @@ using R8_8_8 = RList<8, 8, 8>;
 using R16_9_5 = RList<16, 9, 5>;     // 720
 using R4_9_3_5 = RList<4, 9, 3, 5>;  // 540
 using R16_9_3 = RList<16, 9, 3>;     // 432
+using R16_8_3 = RList<16, 8, 3>;     // 384
+using R16_8_5 = RList<16, 8, 5>;     // 640
+using R16_16_3 = RList<16, 16, 3>;   // 768
+using R8_8_5 = RList<8, 8, 5>;       // 320
+using R16_9_4 = RList<16, 9, 4>;     // 576
+using R16_10_5 = RList<16, 10, 5>;   // 800
+using R16_12_5 = RList<16, 12, 5>;   // 960
+using R10_10_10 = RList<10, 10, 10>; // 1000
+using R16_9_8 = RList<16, 9, 8>;     // 1152
 using R16_16 = RList<16, 16>;         // 256
 using R16_8 = RList<16, 8>;           // 128
 using R8_8 = RList<8, 8>;             // 64
@@ -1043,6 +1052,52 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(double, 432, 8, 512, false, R16_9_3),
       D2D_VARIANTS(double, 432, 4, 256, false, R16_9_3),
       D2D_VARIANTS(double, 432, 4, 256, true, R16_9_3),
+      // common mixed-radix lengths 384 = 16*8*3, 640 = 16*8*5, 768 = 16*16*3
+      D2D_VARIANTS(double, 384, 8, 256, false, R16_8_3),
+      D2D_VARIANTS(double, 384, 4, 128, true, R16_8_3),
+      D2D_VARIANTS(double, 640, 8, 512, false, R16_8_5),
+      D2D_VARIANTS(double, 640, 4, 256, false, R16_8_5),
+      D2D_VARIANTS(double, 640, 4, 256, true, R16_8_5),
+      D2D_VARIANTS(double, 768, 8, 512, false, R16_16_3),
+      D2D_VARIANTS(double, 768, 4, 256, false, R16_16_3),
+      D2D_VARIANTS(double, 768, 4, 256, true, R16_16_3),
+      D2D_VARIANTS(float, 384, 16, 512, false, R16_8_3),
+      D2D_VARIANTS(float, 384, 4, 128, true, R16_8_3),
+      D2D_VARIANTS(float, 640, 16, 512, false, R16_8_5),
+      D2D_VARIANTS(float, 640, 4, 256, true, R16_8_5),
+      D2D_VARIANTS(float, 768, 16, 1024, false, R16_16_3),
+      D2D_VARIANTS(float, 768, 8, 512, false, R16_16_3),
+      D2D_VARIANTS(float, 768, 4, 256, true, R16_16_3),
+      // and 320, 576, 800, 960, 1000, 1152
+      D2D_VARIANTS(double, 320, 8, 256, false, R8_8_5),
+      D2D_VARIANTS(double, 320, 4, 128, true, R8_8_5),
+      D2D_VARIANTS(double, 576, 8, 512, false, R16_9_4),
+      D2D_VARIANTS(double, 576, 4, 256, false, R16_9_4),
+      D2D_VARIANTS(double, 576, 4, 256, true, R16_9_4),
+      D2D_VARIANTS(double, 800, 8, 512, false, R16_10_5),
+      D2D_VARIANTS(double, 800, 4, 256, false, R16_10_5),
+      D2D_VARIANTS(double, 800, 4, 256, true, R16_10_5),
+      D2D_VARIANTS(double, 960, 8, 512, false, R16_12_5),
+      D2D_VARIANTS(double, 960, 4, 256, false, R16_12_5),
+      D2D_VARIANTS(double, 960, 4, 256, true, R16_12_5),
+      D2D_VARIANTS(double, 1000, 8, 512, false, R10_10_10),
+      D2D_VARIANTS(double, 1000, 4, 256, false, R10_10_10),
+      D2D_VARIANTS(double, 1000, 4, 256, true, R10_10_10),
+      D2D_VARIANTS(double, 1152, 8, 512, false, R16_9_8),
+      D2D_VARIANTS(double, 1152, 4, 256, false, R16_9_8),
+      D2D_VARIANTS(double, 1152, 4, 256, true, R16_9_8),
+      D2D_VARIANTS(float, 320, 16, 512, false, R8_8_5),
+      D2D_VARIANTS(float, 320, 4, 128, true, R8_8_5),
+      D2D_VARIANTS(float, 576, 16, 512, false, R16_9_4),
+      D2D_VARIANTS(float, 576, 4, 256, true, R16_9_4),
+      D2D_VARIANTS(float, 800, 16, 512, false, R16_10_5),
+      D2D_VARIANTS(float, 800, 4, 256, true, R16_10_5),
+      D2D_VARIANTS(float, 960, 16, 512, false, R16_12_5),
+      D2D_VARIANTS(float, 960, 4, 256, true, R16_12_5),
+      D2D_VARIANTS(float, 1000, 16, 512, false, R10_10_10),
+      D2D_VARIANTS(float, 1000, 4, 256, true, R10_10_10),
+      D2D_VARIANTS(float, 1152, 16, 512, false, R16_9_8),
+      D2D_VARIANTS(float, 1152, 4, 256, true, R16_9_8),
       // short power-of-two lengths: 128-byte rows on strided axes
       D2D_VARIANTS(double, 256, 8, 128, false, R16_16),
       D2D_VARIANTS(double, 256, 4, 64, true, R16_16),

@@ -334,7 +334,12 @@ THIN = [(1024, 6, 5), (6, 1024, 5), (6, 5, 1024), (512, 6, 5), (6, 512, 5), (6, 
         (64, 5, 3), (5, 64, 3), (5, 3, 64), (32, 5, 3), (3, 32, 5), (5, 3, 32),
         (48, 5, 3), (5, 48, 3), (3, 5, 48), (80, 5, 3), (5, 80, 3), (3, 5, 80),
         (96, 5, 3), (5, 96, 3), (3, 5, 96), (160, 5, 3), (5, 160, 3), (3, 5, 160),
-        (192, 5, 3), (5, 192, 3), (3, 5, 192), (160, 48, 20), (96, 80, 36), (2048, 6, 5)]
+        (192, 5, 3), (5, 192, 3), (3, 5, 192), (160, 48, 20), (96, 80, 36), (2048, 6, 5),
+        (384, 5, 3), (5, 384, 3), (3, 5, 384), (640, 5, 3), (5, 640, 3), (3, 5, 640),
+        (768, 5, 3), (5, 768, 3), (3, 5, 768), (320, 5, 3), (5, 320, 3), (3, 5, 320),
+        (576, 5, 3), (5, 576, 3), (3, 5, 576), (800, 5, 3), (5, 800, 3), (3, 5, 800),
+        (960, 5, 3), (5, 960, 3), (3, 5, 960), (1000, 5, 3), (5, 1000, 3), (3, 5, 1000),
+        (1152, 5, 3), (5, 1152, 3), (3, 5, 1152)]
 STAGE_USE = {"c2c_x": "c2c", "c2c_y": "c2c", "c2c_z": "c2c", "r2c": "r2c", "spec_y": "rt",
              "spec_last": "rt", "c2r": "c2r"}
 
