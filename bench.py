@@ -512,6 +512,41 @@ def bench_config(args, torch, d2d, w):
         extra["device_only_ms"] = round(ms_dev, 4)
         extra["pcie_GBps_pinned"] = round(xfer / ((res["pinned"] - ms_dev) * 1e-3) / 1e9, 1)
         extra["pcie_GBps_pageable"] = round(xfer / ((res["pageable"] - ms_dev) * 1e-3) / 1e9, 1)
+    elif w == "sizes":
+        # r2c+c2r FP64 cubes of the mixed-radix lengths with specialised
+        # kernels, each also timed with every stage on the general kernel
+        sizes = {}
+        ms, nbytes, err = 0.0, 0, 0.0
+        for n in (320, 384, 576, 640, 768, 800, 960, 1000, 1152):
+            nxc = n // 2 + 1
+            x = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
+            d2d.fill_uniform(x, x.numel(), d2d.R8, 2)
+            sp = torch.empty(nxc * n * n, dtype=torch.complex128, device="cuda")
+            y = torch.empty_like(x)
+            e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, tune=not args.no_tune)
+
+            def step():
+                e.fft_r2c(x, sp)
+                e.fft_c2r(sp, y)
+            t = timed(step)
+            b = 2 * (n ** 3 * 8 + 5 * nxc * n * n * 16)
+            r = float(torch.linalg.vector_norm(y / n ** 3 - x) / torch.linalg.vector_norm(x))
+            kern = e.stage_kernels()
+            for st in ("r2c", "spec_y", "spec_last", "c2r"):
+                e.set_stage_kernel(st, "general")
+            tg = timed(step)
+            sizes[str(n)] = {"ms": round(t, 4), "GBps_model": round(b / (t * 1e-3) / 1e9, 1),
+                             "general_kernel_ms": round(tg, 4),
+                             "speedup_vs_general": round(tg / t, 2), "round_trip_rel_l2": r,
+                             "kernels": {k: kern[k] for k in ("r2c", "spec_y", "spec_last", "c2r")}}
+            ms += t
+            nbytes += b
+            err = max(err, r)
+            del e, x, sp, y
+            torch.cuda.empty_cache()
+        cfg = {"workload": "r2c+c2r FP64 n^3 PHYSICAL_IN_X, n in 320 384 576 640 768 800 960 1000 "
+                           "1152, uniform seed 2; one step = one round trip of each"}
+        extra["sizes"] = sizes
     elif w == "halo512":
         # update_halo of a 512^3 real8 x-pencil, 2 halo levels, y and z periodic
         n, lev = 512, 2
@@ -569,14 +604,15 @@ def main():
                     help="skip the plan-time autotuner (use the default kernels)")
     ap.add_argument("--workload", choices=["r2c1024", "batch64", "c2c128", "r2c512", "r2cz720",
                                            "pack2048", "r2cpack2048", "dtt513", "halo512",
-                                           "rank1024", "host512"],
+                                           "rank1024", "host512", "sizes"],
                     default="r2c1024",
                     help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids; "
                          "c2c128 / r2c512 / r2cz720 / pack2048: the other parity configurations, "
                          "timed; r2cpack2048: the pack2048 rank's x stage fused with its pack; "
                          "dtt513: 3D DCT-II + DCT-III round trip; halo512: update_halo; "
                          "rank1024: one rank's stages with their packs and unpacks; host512: the "
-                         "512^3 round trip on host arrays, PCIe included")
+                         "512^3 round trip on host arrays, PCIe included; sizes: FP64 round "
+                         "trips of nine mixed-radix cubes against the general kernel")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -592,7 +628,7 @@ def main():
         print(json.dumps(bench_batch(args, torch, d2d)), flush=True)
         return
     if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048", "dtt513",
-                         "halo512", "rank1024", "host512"):
+                         "halo512", "rank1024", "host512", "sizes"):
         print(json.dumps(bench_config(args, torch, d2d, args.workload)), flush=True)
         return
     n = args.n
