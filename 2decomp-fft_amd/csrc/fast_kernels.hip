@@ -794,6 +794,8 @@ using R16_10_5 = RList<16, 10, 5>;   // 800
 using R16_12_5 = RList<16, 12, 5>;   // 960
 using R10_10_10 = RList<10, 10, 10>; // 1000
 using R16_9_8 = RList<16, 9, 8>;     // 1152
+using R16_16_5 = RList<16, 16, 5>;   // 1280
+using R16_16_6 = RList<16, 16, 6>;   // 1536
 using R16_16 = RList<16, 16>;         // 256
 using R16_8 = RList<16, 8>;           // 128
 using R8_8 = RList<8, 8>;             // 64
@@ -1098,6 +1100,21 @@ const std::vector<FastVariant> &fast_registry() {
       D2D_VARIANTS(float, 1000, 4, 256, true, R10_10_10),
       D2D_VARIANTS(float, 1152, 16, 512, false, R16_9_8),
       D2D_VARIANTS(float, 1152, 4, 256, true, R16_9_8),
+      // 1280, 1536 and 2048-point columns (the y and z stages of those cubes)
+      D2D_VARIANTS(double, 1280, 4, 256, false, R16_16_5),
+      D2D_VARIANTS(double, 1280, 4, 256, true, R16_16_5),
+      D2D_SPLIT_C2C(double, 1280, 8, 512, false, R16_16_5, 4),
+      D2D_VARIANTS(double, 1536, 4, 256, false, R16_16_6),
+      D2D_VARIANTS(double, 1536, 2, 256, true, R16_16_6),
+      D2D_SPLIT_C2C(double, 1536, 8, 512, false, R16_16_6, 4),
+      D2D_VARIANTS(double, 2048, 4, 512, false, R16_16_8),
+      D2D_SPLIT_C2C(double, 2048, 8, 1024, false, R16_16_8, 4),
+      D2D_VARIANTS(float, 1280, 8, 512, false, R16_16_5),
+      D2D_VARIANTS(float, 1280, 4, 256, true, R16_16_5),
+      D2D_VARIANTS(float, 1536, 8, 512, false, R16_16_6),
+      D2D_VARIANTS(float, 1536, 4, 256, true, R16_16_6),
+      D2D_VARIANTS(float, 2048, 8, 512, false, R16_16_8),
+      D2D_VARIANTS(float, 2048, 2, 256, true, R16_16_8),
       // short power-of-two lengths: 128-byte rows on strided axes
       D2D_VARIANTS(double, 256, 8, 128, false, R16_16),
       D2D_VARIANTS(double, 256, 4, 64, true, R16_16),

@@ -339,7 +339,8 @@ THIN = [(1024, 6, 5), (6, 1024, 5), (6, 5, 1024), (512, 6, 5), (6, 512, 5), (6, 
         (768, 5, 3), (5, 768, 3), (3, 5, 768), (320, 5, 3), (5, 320, 3), (3, 5, 320),
         (576, 5, 3), (5, 576, 3), (3, 5, 576), (800, 5, 3), (5, 800, 3), (3, 5, 800),
         (960, 5, 3), (5, 960, 3), (3, 5, 960), (1000, 5, 3), (5, 1000, 3), (3, 5, 1000),
-        (1152, 5, 3), (5, 1152, 3), (3, 5, 1152)]
+        (1152, 5, 3), (5, 1152, 3), (3, 5, 1152), (1280, 5, 3), (5, 1280, 3), (3, 5, 1280),
+        (1536, 5, 3), (5, 1536, 3), (3, 5, 1536), (5, 2048, 3), (3, 5, 2048)]
 STAGE_USE = {"c2c_x": "c2c", "c2c_y": "c2c", "c2c_z": "c2c", "r2c": "r2c", "spec_y": "rt",
              "spec_last": "rt", "c2r": "c2r"}
 
