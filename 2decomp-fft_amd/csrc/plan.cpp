@@ -883,6 +883,18 @@ int time_stage(d2dhip_plan_s *p, int idx, const void *in, void *out, hipEvent_t 
   return st;
 }
 
+// Whether `bytes` more of device memory can be taken for tuning scratch, with
+// a sixteenth of the device left free: a plan too large to tune keeps its
+// default choices (and caches nothing) instead of failing.
+bool tune_fits(size_t bytes) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return bytes + tot / 16 <= fr;
+}
+
 int tune_stages(d2dhip_plan_s *p) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   tune_load();
@@ -915,6 +927,7 @@ int tune_stages(d2dhip_plan_s *p) {
       }
     }
     if (!a) {
+      if (!tune_fits(2 * need)) break;  // untuned stages keep their defaults
       if (hipMalloc(&a, need) != hipSuccess || hipMalloc(&b, need) != hipSuccess) {
         st = set_error(D2DHIP_ERR_ALLOC, "hipMalloc(tuning buffers) failed");
         break;
@@ -1337,6 +1350,12 @@ int tune_layout(d2dhip_plan_s *p) {
   }
   const long long nph = (long long)p->nx * p->ny * p->nz;
   const long long nsp = (long long)(p->nx / 2 + 1) * p->ny * p->nz;
+  // scratch, and the workspace's growth to the larger of the blocked layouts
+  const long long nxc = p->nx / 2 + 1, rows = ((long long)p->ny + 1) * p->nz;
+  const size_t blk = (size_t)std::max((nxc + 15) / 16 * 16, (nxc + 31) / 32 * 32) * rows * 2 * p->rb;
+  const size_t grow = blk > p->wk_bytes ? blk - p->wk_bytes : 0;
+  if (!tune_fits((size_t)nph * p->rb * 2 + (size_t)nsp * 2 * p->rb + grow))
+    return set_layout(p, D2DHIP_LAYOUT_PLAIN);
   void *a = nullptr, *b = nullptr, *c = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int st = D2DHIP_SUCCESS;
@@ -1445,6 +1464,7 @@ int tune_chunks(d2dhip_plan_s *p) {
   const long long nph = (long long)p->nx * p->ny * p->nz;
   const long long nsp = (long long)(p->nx / 2 + 1) * p->ny * p->nz;
   const size_t need = (size_t)std::max(nph * p->rb, nsp * 2 * p->rb);
+  if (!tune_fits(2 * need)) return upload_chunk_descs(p);  // whole stages
   void *a = nullptr, *b = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int st = D2DHIP_SUCCESS;
