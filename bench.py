@@ -547,6 +547,35 @@ def bench_config(args, torch, d2d, w):
         cfg = {"workload": "r2c+c2r FP64 n^3 PHYSICAL_IN_X, n in 320 384 576 640 768 800 960 1000 "
                            "1152, uniform seed 2; one step = one round trip of each"}
         extra["sizes"] = sizes
+    elif w == "r2c2048":
+        # the largest cube one MI355X holds in FP64: 2048^3 r2c+c2r, the
+        # c2r working in place on its input (opt_inplace), so that input,
+        # spectrum and output take 206 GB of the 288 GB; the tuner's scratch
+        # does not fit beside them, so the plan keeps its default kernels
+        n = 2048
+        nxc = n // 2 + 1
+        x = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
+        d2d.fill_uniform(x, x.numel(), d2d.R8, 2)
+        sp = torch.empty(nxc * n * n, dtype=torch.complex128, device="cuda")
+        y = torch.empty_like(x)
+        e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, n, n, n, inplace=True, tune=not args.no_tune)
+
+        def step():
+            e.fft_r2c(x, sp)
+            e.fft_c2r(sp, y)
+        ms = timed(step)
+        nbytes = 2 * (n ** 3 * 8 + 5 * nxc * n * n * 16)
+        cfg = {"workload": "r2c+c2r FP64 2048^3 PHYSICAL_IN_X, c2r in place on its input, "
+                           "uniform seed 2"}
+        e.profile(True)
+        step()
+        extra["stage_ms"] = {k: round(v[0] / v[1], 4) for k, v in e.profile_report().items() if v[1]}
+        e.profile(False)
+        extra["kernels"] = e.stage_kernels()
+        extra["device_GB_in_use"] = round(torch.cuda.memory_allocated() / 1e9, 1)
+        # the round trip's error in place (no 69 GB temporaries)
+        y.div_(float(n) ** 3).sub_(x)
+        err = float(torch.linalg.vector_norm(y) / torch.linalg.vector_norm(x))
     elif w == "halo512":
         # update_halo of a 512^3 real8 x-pencil, 2 halo levels, y and z periodic
         n, lev = 512, 2
@@ -604,7 +633,7 @@ def main():
                     help="skip the plan-time autotuner (use the default kernels)")
     ap.add_argument("--workload", choices=["r2c1024", "batch64", "c2c128", "r2c512", "r2cz720",
                                            "pack2048", "r2cpack2048", "dtt513", "halo512",
-                                           "rank1024", "host512", "sizes"],
+                                           "rank1024", "host512", "sizes", "r2c2048"],
                     default="r2c1024",
                     help="r2c1024: the headline round trip; batch64: 64 uneven c2c grids; "
                          "c2c128 / r2c512 / r2cz720 / pack2048: the other parity configurations, "
@@ -612,7 +641,8 @@ def main():
                          "dtt513: 3D DCT-II + DCT-III round trip; halo512: update_halo; "
                          "rank1024: one rank's stages with their packs and unpacks; host512: the "
                          "512^3 round trip on host arrays, PCIe included; sizes: FP64 round "
-                         "trips of nine mixed-radix cubes against the general kernel")
+                         "trips of nine mixed-radix cubes against the general kernel; r2c2048: "
+                         "the 2048^3 FP64 round trip (206 GB on one GPU)")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -628,7 +658,7 @@ def main():
         print(json.dumps(bench_batch(args, torch, d2d)), flush=True)
         return
     if args.workload in ("c2c128", "r2c512", "r2cz720", "pack2048", "r2cpack2048", "dtt513",
-                         "halo512", "rank1024", "host512", "sizes"):
+                         "halo512", "rank1024", "host512", "sizes", "r2c2048"):
         print(json.dumps(bench_config(args, torch, d2d, args.workload)), flush=True)
         return
     n = args.n
