@@ -1,5 +1,5 @@
 // fast_common.h -- compile-time stage shapes and pencil IO shared by the
-// specialised axis kernels (fast_kernels.hip: VALU passes; mfma_kernels.hip:
+// specialised axis kernels (fast_impl.h: VALU passes; mfma_kernels.hip:
 // radix-16 passes on the matrix cores).
 #pragma once
 

@@ -28,7 +28,7 @@
 //
 // Passes with another radix (the 4 of 1024 = 16*16*4, the 2 of
 // 512 = 16*16*2) run on the VALU with the FP32 codelets, one butterfly per
-// thread, as in fast_kernels.hip.
+// thread, as in fast_impl.h.
 //
 // What the instruction count comes to: the lane maps below make every LDS
 // and HBM address a lane base plus a compile-time offset (a first version
@@ -244,7 +244,7 @@ __device__ __forceinline__ void mf_dft16(const cpx<float> (&x)[4], cpx<float> (&
 // the same for every group of the lane, so every LDS address is a lane base
 // plus an immediate and the second pass's twiddles are four loads per lane.
 //
-// VALU pass: one butterfly per thread and q, as in fast_kernels.hip:
+// VALU pass: one butterfly per thread and q, as in fast_impl.h:
 // t = tid%TX (columns) or tid/RPT (lines), b = (tid/TX or tid%RPT) + q*RPT.
 template <int N, int TX, int NT, bool LINES>
 struct MfMap {
