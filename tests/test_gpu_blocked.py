@@ -53,10 +53,11 @@ def round_trip(d2d, torch, e, x, prec):
     return from_dev(spec, (nx // 2 + 1, ny, nz)), from_dev(back, x.shape) / x.size
 
 
-# (nx, ny, nz): nx/2+1 = 11 (no whole block), 16 (whole blocks only), 18, 66,
-# 33 with 1024-point axes (the 1024^3 kernels), 34 with 512/1024 axes
-SHAPES = [(20, 12, 10), (30, 16, 24), (34, 20, 18), (130, 64, 128), (64, 1024, 1024),
-          (66, 512, 1024)]
+# (nx, ny, nz): nx/2+1 = 11 (no whole block), 16 (whole blocks only), 18, 17 with
+# odd nx, ny, nz (an unpaired last line per plane), 66, 33 with 1024-point axes
+# (the 1024^3 kernels), 34 with 512/1024 axes
+SHAPES = [(20, 12, 10), (30, 16, 24), (34, 20, 18), (33, 21, 19), (130, 64, 128),
+          (64, 1024, 1024), (66, 512, 1024)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
