@@ -20,8 +20,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libd2dhip.so")
 # (longest first, so that a parallel build starts them first)
 SOURCES = ["fast_variants_a.hip", "fast_variants_c.hip", "fast_variants_b.hip",
-           "fast_variants_d.hip", "fast_kernels.hip", "mfma_kernels.hip", "fft_kernels.hip",
-           "copy_kernels.hip", "halo_kernels.hip", "plan.cpp", "api.cpp"]
+           "fast_variants_e.hip", "fast_variants_d.hip", "fast_kernels.hip", "mfma_kernels.hip",
+           "fft_kernels.hip", "copy_kernels.hip", "halo_kernels.hip", "plan.cpp", "api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ROCM_LIB = os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "lib")
 FLAGS = [

@@ -2,7 +2,7 @@
 // launchers and the building blocks of the variant registry.  Included by
 // fast_kernels.hip (the persistent batch tiles and the registry) and by the
 // fast_variants_*.hip files, each of which instantiates its own share of the
-// registered variants, so that the four compile in parallel.
+// registered variants, so that they compile in parallel.
 #pragma once
 
 // Compile-time specialised axis stages for hot lengths.

@@ -186,11 +186,13 @@ std::vector<FastVariant> fast_variants_a();
 std::vector<FastVariant> fast_variants_b();
 std::vector<FastVariant> fast_variants_c();
 std::vector<FastVariant> fast_variants_d();
+std::vector<FastVariant> fast_variants_e();
 
 const std::vector<FastVariant> &fast_registry() {
   static const std::vector<FastVariant> reg = [] {
     std::vector<FastVariant> r;
-    for (auto part : {fast_variants_a, fast_variants_b, fast_variants_c, fast_variants_d}) {
+    for (auto part : {fast_variants_a, fast_variants_b, fast_variants_c, fast_variants_d,
+                      fast_variants_e}) {
       const std::vector<FastVariant> v = part();
       r.insert(r.end(), v.begin(), v.end());
     }

@@ -1,4 +1,4 @@
-// fast_variants_a.hip -- registered specialised kernels, part 1 of 4:
+// fast_variants_a.hip -- registered specialised kernels, part 1 of 5:
 // 1024-point FP64 kernels (every exchange form), the fused pack / unpack twins, 512 and the 2048-point lines.
 #include "fast_impl.h"
 

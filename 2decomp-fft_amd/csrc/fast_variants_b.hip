@@ -1,4 +1,4 @@
-// fast_variants_b.hip -- registered specialised kernels, part 2 of 4:
+// fast_variants_b.hip -- registered specialised kernels, part 2 of 5:
 // FP32 1024 / 512, the 720 / 540 / 432 lengths and 384 / 640 / 768.
 #include "fast_impl.h"
 

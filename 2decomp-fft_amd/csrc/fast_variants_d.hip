@@ -1,4 +1,4 @@
-// fast_variants_d.hip -- registered specialised kernels, part 4 of 4:
+// fast_variants_d.hip -- registered specialised kernels, part 4 of 5:
 // the real-to-real (DCT / DST) tiles of (2^k + 1)-point grids.
 #include "fast_impl.h"
 

@@ -1,0 +1,64 @@
+// fast_variants_e.hip -- registered specialised kernels, part 5 of 5: the
+// lengths 100, 120, 200, 240, 300, 360, 400, 480, 500 and 600 (FP64 and FP32;
+// strided columns in 128-256-byte rows, contiguous lines).
+#include "fast_impl.h"
+
+namespace d2dhip {
+
+using R4_5_5 = RList<4, 5, 5>;       // 100
+using R8_15 = RList<8, 15>;          // 120
+using R8_5_5 = RList<8, 5, 5>;       // 200
+using R16_15 = RList<16, 15>;        // 240
+using R12_5_5 = RList<12, 5, 5>;     // 300
+using R8_9_5 = RList<8, 9, 5>;       // 360
+using R16_5_5 = RList<16, 5, 5>;     // 400
+using R16_6_5 = RList<16, 6, 5>;     // 480
+using R4_5_5_5 = RList<4, 5, 5, 5>;  // 500
+using R8_15_5 = RList<8, 15, 5>;     // 600
+
+std::vector<FastVariant> fast_variants_e() {
+  return {
+    D2D_VARIANTS(double, 100, 16, 128, false, R4_5_5),
+    D2D_VARIANTS(double, 100, 16, 128, true, R4_5_5),
+    D2D_VARIANTS(double, 120, 16, 128, false, R8_15),
+    D2D_VARIANTS(double, 120, 16, 128, true, R8_15),
+    D2D_VARIANTS(double, 200, 8, 128, false, R8_5_5),
+    D2D_VARIANTS(double, 200, 8, 128, true, R8_5_5),
+    D2D_VARIANTS(double, 240, 8, 128, false, R16_15),
+    D2D_VARIANTS(double, 240, 8, 128, true, R16_15),
+    D2D_VARIANTS(double, 300, 8, 256, false, R12_5_5),
+    D2D_VARIANTS(double, 300, 4, 128, true, R12_5_5),
+    D2D_VARIANTS(double, 360, 8, 256, false, R8_9_5),
+    D2D_VARIANTS(double, 360, 4, 128, true, R8_9_5),
+    D2D_VARIANTS(double, 400, 8, 256, false, R16_5_5),
+    D2D_VARIANTS(double, 400, 4, 128, true, R16_5_5),
+    D2D_VARIANTS(double, 480, 8, 256, false, R16_6_5),
+    D2D_VARIANTS(double, 480, 4, 128, true, R16_6_5),
+    D2D_VARIANTS(double, 500, 8, 256, false, R4_5_5_5),
+    D2D_VARIANTS(double, 500, 4, 128, true, R4_5_5_5),
+    D2D_VARIANTS(double, 600, 8, 512, false, R8_15_5),
+    D2D_VARIANTS(double, 600, 4, 256, true, R8_15_5),
+    D2D_VARIANTS(float, 100, 32, 256, false, R4_5_5),
+    D2D_VARIANTS(float, 100, 16, 128, true, R4_5_5),
+    D2D_VARIANTS(float, 120, 32, 256, false, R8_15),
+    D2D_VARIANTS(float, 120, 16, 128, true, R8_15),
+    D2D_VARIANTS(float, 200, 16, 256, false, R8_5_5),
+    D2D_VARIANTS(float, 200, 8, 128, true, R8_5_5),
+    D2D_VARIANTS(float, 240, 16, 256, false, R16_15),
+    D2D_VARIANTS(float, 240, 8, 128, true, R16_15),
+    D2D_VARIANTS(float, 300, 16, 512, false, R12_5_5),
+    D2D_VARIANTS(float, 300, 4, 128, true, R12_5_5),
+    D2D_VARIANTS(float, 360, 16, 512, false, R8_9_5),
+    D2D_VARIANTS(float, 360, 4, 128, true, R8_9_5),
+    D2D_VARIANTS(float, 400, 16, 512, false, R16_5_5),
+    D2D_VARIANTS(float, 400, 4, 128, true, R16_5_5),
+    D2D_VARIANTS(float, 480, 16, 512, false, R16_6_5),
+    D2D_VARIANTS(float, 480, 4, 128, true, R16_6_5),
+    D2D_VARIANTS(float, 500, 16, 512, false, R4_5_5_5),
+    D2D_VARIANTS(float, 500, 4, 128, true, R4_5_5_5),
+    D2D_VARIANTS(float, 600, 16, 512, false, R8_15_5),
+    D2D_VARIANTS(float, 600, 4, 256, true, R8_15_5),
+  };
+}
+
+}  // namespace d2dhip

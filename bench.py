@@ -517,7 +517,8 @@ def bench_config(args, torch, d2d, w):
         # kernels, each also timed with every stage on the general kernel
         sizes = {}
         ms, nbytes, err = 0.0, 0, 0.0
-        for n in (320, 384, 576, 640, 768, 800, 960, 1000, 1152):
+        for n in (100, 120, 200, 240, 300, 320, 360, 384, 400, 480, 500, 576, 600, 640, 768, 800,
+                  960, 1000, 1152):
             nxc = n // 2 + 1
             x = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
             d2d.fill_uniform(x, x.numel(), d2d.R8, 2)
@@ -544,8 +545,9 @@ def bench_config(args, torch, d2d, w):
             err = max(err, r)
             del e, x, sp, y
             torch.cuda.empty_cache()
-        cfg = {"workload": "r2c+c2r FP64 n^3 PHYSICAL_IN_X, n in 320 384 576 640 768 800 960 1000 "
-                           "1152, uniform seed 2; one step = one round trip of each"}
+        cfg = {"workload": "r2c+c2r FP64 n^3 PHYSICAL_IN_X, n in 100 120 200 240 300 320 360 384 "
+                           "400 480 500 576 600 640 768 800 960 1000 1152, uniform seed 2; one "
+                           "step = one round trip of each"}
         extra["sizes"] = sizes
     elif w == "r2c2048":
         # the largest cube one MI355X holds in FP64: 2048^3 r2c+c2r, the
@@ -641,7 +643,7 @@ def main():
                          "dtt513: 3D DCT-II + DCT-III round trip; halo512: update_halo; "
                          "rank1024: one rank's stages with their packs and unpacks; host512: the "
                          "512^3 round trip on host arrays, PCIe included; sizes: FP64 round "
-                         "trips of nine mixed-radix cubes against the general kernel; r2c2048: "
+                         "trips of 19 mixed-radix cubes against the general kernel; r2c2048: "
                          "the 2048^3 FP64 round trip (206 GB on one GPU)")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
