@@ -1392,7 +1392,8 @@ int tune_layout(d2dhip_plan_s *p) {
       float best_t = 1e30f;
       int best_c = p->geom[idx].choice;
       for (int cc = -1; cc < (int)p->geom[idx].cands.size() && !st; ++cc) {
-        if ((st = configure(p, idx, cc)) || (st = upload_descs(p))) break;
+        if (configure(p, idx, cc)) continue;  // e.g. the general kernel cannot run this length
+        if ((st = upload_descs(p))) break;
         if ((st = run_blocked_r2c(p, a, b))) break;
         float t = 0.f;
         st = timed([&]() {
