@@ -886,11 +886,16 @@ int time_stage(d2dhip_plan_s *p, int idx, const void *in, void *out, hipEvent_t 
 // Whether `bytes` more of device memory can be taken for tuning scratch, with
 // a sixteenth of the device left free: a plan too large to tune keeps its
 // default choices (and caches nothing) instead of failing.
+// D2DHIP_TUNE_MEM_LIMIT (bytes) caps the free memory this assumes.
 bool tune_fits(size_t bytes) {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
     (void)hipGetLastError();
     return false;
+  }
+  if (const char *e = std::getenv("D2DHIP_TUNE_MEM_LIMIT")) {
+    const unsigned long long cap = std::strtoull(e, nullptr, 10);
+    if (cap < fr) fr = (size_t)cap;
   }
   return bytes + tot / 16 <= fr;
 }

@@ -126,6 +126,24 @@ def test_tuned_layout(d2d, torch_gpu, tmp_path, monkeypatch):
     assert np.array_equal(spec2, spec) and np.array_equal(back2, back)
 
 
+def test_tuning_without_room(d2d, torch_gpu, tmp_path, monkeypatch):
+    """A tuned plan whose tuning scratch does not fit device memory keeps its
+    default kernels and the plain layout, caches nothing, and is exact."""
+    torch = torch_gpu
+    cache = tmp_path / "tune.tsv"
+    monkeypatch.setenv("D2DHIP_TUNE_CACHE", str(cache))
+    monkeypatch.setenv("D2DHIP_TUNE_MEM_LIMIT", "0")
+    shape = (130, 64, 128)
+    x = field(shape, 64, 13)
+    e = d2d.FFTEngine(d2d.PHYSICAL_IN_X, *shape, tune=True)
+    ref = d2d.FFTEngine(d2d.PHYSICAL_IN_X, *shape)
+    assert e.stage_kernels() == ref.stage_kernels()
+    assert e.layout() == d2d.LAYOUT_PLAIN and e.chunking() == (0, 0)
+    assert not cache.exists() or not cache.read_text().strip()
+    spec, back = round_trip(d2d, torch, e, x, 64)
+    assert rel_l2(spec, reference(x)) <= TOL[64] and rel_l2(back, x) <= TOL[64]
+
+
 def test_blocked_plain_fallbacks(d2d, torch_gpu):
     """Chunked or skipped stages and exec_c2r_noalloc keep the plain layout;
     plans without a workspace or of one rank of a process grid refuse it."""
