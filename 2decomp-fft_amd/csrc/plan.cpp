@@ -1280,8 +1280,9 @@ int set_layout(d2dhip_plan_s *p, int layout) {
   if (!(p->flags & D2DHIP_FLAG_INPLACE) && !p->multi) {
     const size_t need = wk_need(p);
     if (need > p->wk_bytes) {  // (grows only: switching back keeps the larger one)
-      // the old workspace may still be read by queued work
-      int st = hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+      // the old workspace may still be read by queued work (on whichever
+      // stream the plan last ran)
+      int st = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
       if (st) return st;
       if (p->wk) (void)hipFree(p->wk);
       p->wk = nullptr;
