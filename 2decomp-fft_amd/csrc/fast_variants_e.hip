@@ -1,6 +1,6 @@
 // fast_variants_e.hip -- registered specialised kernels, part 5 of 5: the
-// lengths 100, 120, 200, 240, 300, 360, 400, 480, 500 and 600 (FP64 and FP32;
-// strided columns in 128-256-byte rows, contiguous lines).
+// lengths 100, 120, 200, 240, 300, 360, 400, 480, 500, 600, 896, 1200 and 1600
+// (FP64 and FP32; strided columns in 64-256-byte rows, contiguous lines).
 #include "fast_impl.h"
 
 namespace d2dhip {
@@ -15,6 +15,9 @@ using R16_5_5 = RList<16, 5, 5>;     // 400
 using R16_6_5 = RList<16, 6, 5>;     // 480
 using R4_5_5_5 = RList<4, 5, 5, 5>;  // 500
 using R8_15_5 = RList<8, 15, 5>;     // 600
+using R16_8_7 = RList<16, 8, 7>;     // 896
+using R16_15_5 = RList<16, 15, 5>;   // 1200
+using R16_10_10 = RList<16, 10, 10>; // 1600
 
 std::vector<FastVariant> fast_variants_e() {
   return {
@@ -58,6 +61,23 @@ std::vector<FastVariant> fast_variants_e() {
     D2D_VARIANTS(float, 500, 4, 128, true, R4_5_5_5),
     D2D_VARIANTS(float, 600, 16, 512, false, R8_15_5),
     D2D_VARIANTS(float, 600, 4, 256, true, R8_15_5),
+    D2D_VARIANTS(double, 896, 8, 512, false, R16_8_7),
+    D2D_VARIANTS(double, 896, 4, 256, false, R16_8_7),
+    D2D_VARIANTS(double, 896, 4, 256, true, R16_8_7),
+    D2D_SPLIT_C2C(double, 1200, 8, 1024, false, R16_15_5, 4),
+    D2D_SPLIT_C2C(double, 1200, 8, 512, false, R16_15_5, 4),
+    D2D_VARIANTS(double, 1200, 4, 256, false, R16_15_5),
+    D2D_VARIANTS(double, 1200, 4, 256, true, R16_15_5),
+    D2D_SPLIT_C2C(double, 1600, 8, 1024, false, R16_10_10, 4),
+    D2D_SPLIT_C2C(double, 1600, 8, 512, false, R16_10_10, 4),
+    D2D_VARIANTS(double, 1600, 4, 256, false, R16_10_10),
+    D2D_VARIANTS(double, 1600, 2, 256, true, R16_10_10),
+    D2D_VARIANTS(float, 896, 16, 512, false, R16_8_7),
+    D2D_VARIANTS(float, 896, 4, 256, true, R16_8_7),
+    D2D_VARIANTS(float, 1200, 8, 512, false, R16_15_5),
+    D2D_VARIANTS(float, 1200, 4, 256, true, R16_15_5),
+    D2D_VARIANTS(float, 1600, 8, 512, false, R16_10_10),
+    D2D_VARIANTS(float, 1600, 4, 256, true, R16_10_10),
   };
 }
 

@@ -341,7 +341,7 @@ THIN = [(1024, 6, 5), (6, 1024, 5), (6, 5, 1024), (512, 6, 5), (6, 512, 5), (6, 
         (960, 5, 3), (5, 960, 3), (3, 5, 960), (1000, 5, 3), (5, 1000, 3), (3, 5, 1000),
         (1152, 5, 3), (5, 1152, 3), (3, 5, 1152), (1280, 5, 3), (5, 1280, 3), (3, 5, 1280),
         (1536, 5, 3), (5, 1536, 3), (3, 5, 1536), (5, 2048, 3), (3, 5, 2048)] + [
-        s for n in (100, 120, 200, 240, 300, 360, 400, 480, 500, 600)
+        s for n in (100, 120, 200, 240, 300, 360, 400, 480, 500, 600, 896, 1200, 1600)
         for s in ((n, 5, 3), (5, n, 3), (3, 5, n))]
 STAGE_USE = {"c2c_x": "c2c", "c2c_y": "c2c", "c2c_z": "c2c", "r2c": "r2c", "spec_y": "rt",
              "spec_last": "rt", "c2r": "c2r"}
