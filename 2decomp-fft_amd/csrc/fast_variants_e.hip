@@ -68,6 +68,7 @@ std::vector<FastVariant> fast_variants_e() {
     D2D_SPLIT_C2C(double, 1200, 8, 512, false, R16_15_5, 4),
     D2D_VARIANTS(double, 1200, 4, 256, false, R16_15_5),
     D2D_VARIANTS(double, 1200, 4, 256, true, R16_15_5),
+    D2D_VARIANTS(double, 1200, 2, 256, true, R16_15_5),
     D2D_SPLIT_C2C(double, 1600, 8, 1024, false, R16_10_10, 4),
     D2D_SPLIT_C2C(double, 1600, 8, 512, false, R16_10_10, 4),
     D2D_VARIANTS(double, 1600, 4, 256, false, R16_10_10),

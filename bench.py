@@ -518,7 +518,7 @@ def bench_config(args, torch, d2d, w):
         sizes = {}
         ms, nbytes, err = 0.0, 0, 0.0
         for n in (100, 120, 200, 240, 300, 320, 360, 384, 400, 480, 500, 576, 600, 640, 768, 800,
-                  896, 960, 1000, 1152, 1200, 1600):
+                  896, 960, 1000, 1152, 1200, 1280, 1536, 1600):
             nxc = n // 2 + 1
             x = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
             d2d.fill_uniform(x, x.numel(), d2d.R8, 2)
@@ -546,7 +546,8 @@ def bench_config(args, torch, d2d, w):
             del e, x, sp, y
             torch.cuda.empty_cache()
         cfg = {"workload": "r2c+c2r FP64 n^3 PHYSICAL_IN_X, n in 100 120 200 240 300 320 360 384 "
-                           "400 480 500 576 600 640 768 800 896 960 1000 1152 1200 1600, "
+                           "400 480 500 576 600 640 768 800 896 960 1000 1152 1200 1280 1536 "
+                           "1600, "
                            "uniform seed 2; one "
                            "step = one round trip of each"}
         extra["sizes"] = sizes
@@ -644,7 +645,7 @@ def main():
                          "dtt513: 3D DCT-II + DCT-III round trip; halo512: update_halo; "
                          "rank1024: one rank's stages with their packs and unpacks; host512: the "
                          "512^3 round trip on host arrays, PCIe included; sizes: FP64 round "
-                         "trips of 22 mixed-radix cubes against the general kernel; r2c2048: "
+                         "trips of 24 mixed-radix cubes against the general kernel; r2c2048: "
                          "the 2048^3 FP64 round trip (206 GB on one GPU)")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
     args = ap.parse_args()
