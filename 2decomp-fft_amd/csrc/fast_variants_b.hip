@@ -36,9 +36,11 @@ std::vector<FastVariant> fast_variants_b() {
     D2D_VARIANTS(double, 640, 8, 512, false, R16_8_5),
     D2D_VARIANTS(double, 640, 4, 256, false, R16_8_5),
     D2D_VARIANTS(double, 640, 4, 256, true, R16_8_5),
+    D2D_SPLIT_C2C(double, 640, 8, 512, false, R16_8_5, 4),
     D2D_VARIANTS(double, 768, 8, 512, false, R16_16_3),
     D2D_VARIANTS(double, 768, 4, 256, false, R16_16_3),
     D2D_VARIANTS(double, 768, 4, 256, true, R16_16_3),
+    D2D_SPLIT_C2C(double, 768, 8, 512, false, R16_16_3, 4),
     D2D_VARIANTS(float, 384, 16, 512, false, R16_8_3),
     D2D_VARIANTS(float, 384, 4, 128, true, R16_8_3),
     D2D_VARIANTS(float, 640, 16, 512, false, R16_8_5),
