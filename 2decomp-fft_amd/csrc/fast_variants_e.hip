@@ -18,6 +18,7 @@ using R8_15_5 = RList<8, 15, 5>;     // 600
 using R16_8_7 = RList<16, 8, 7>;     // 896
 using R16_15_5 = RList<16, 15, 5>;   // 1200
 using R16_10_10 = RList<16, 10, 10>; // 1600
+using R8_8_5_5 = RList<8, 8, 5, 5>;   // 1600, 128 rows: 2 or 3 butterflies a lane
 
 std::vector<FastVariant> fast_variants_e() {
   return {
@@ -70,6 +71,11 @@ std::vector<FastVariant> fast_variants_e() {
     D2D_VARIANTS(double, 1200, 4, 256, true, R16_15_5),
     D2D_VARIANTS(double, 1200, 2, 256, true, R16_15_5),
     D2D_SPLIT_C2C(double, 1600, 8, 1024, false, R16_10_10, 4),
+    make_variant<double, 1600, 8, 1024, IO_C2C, false, R8_8_5_5, 1, 4>(
+        "double n=1600 tx=8 nt=1024 c2c xs w4 r8855"),
+    make_variant<double, 1600, 2, 256, IO_C2C, true, R8_8_5_5>("double n=1600 tx=2 nt=256 c2c r8855"),
+    make_variant<double, 1600, 2, 256, IO_R2C, true, R8_8_5_5>("double n=1600 tx=2 nt=256 r2c r8855"),
+    make_variant<double, 1600, 2, 256, IO_C2R, true, R8_8_5_5>("double n=1600 tx=2 nt=256 c2r r8855"),
     D2D_SPLIT_C2C(double, 1600, 8, 512, false, R16_10_10, 4),
     D2D_VARIANTS(double, 1600, 4, 256, false, R16_10_10),
     D2D_VARIANTS(double, 1600, 2, 256, true, R16_10_10),
