@@ -517,8 +517,14 @@ def bench_config(args, torch, d2d, w):
         # kernels, each also timed with every stage on the general kernel
         sizes = {}
         ms, nbytes, err = 0.0, 0, 0.0
-        for n in (100, 120, 200, 240, 300, 320, 360, 384, 400, 480, 500, 576, 600, 640, 768, 800,
-                  896, 960, 1000, 1152, 1200, 1280, 1536, 1600):
+        every = (100, 120, 200, 240, 300, 320, 360, 384, 400, 480, 500, 576, 600, 640, 768, 800,
+                 896, 960, 1000, 1152, 1200, 1280, 1536, 1600)
+        pick = [int(v) for v in args.cubes.split(",") if v.strip()] if args.cubes else every
+        bad = [v for v in pick if v not in every]
+        if bad:
+            raise SystemExit("bench.py: --cubes %s: not among the sizes workload's lengths %s"
+                             % (bad, list(every)))
+        for n in pick:
             nxc = n // 2 + 1
             x = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
             d2d.fill_uniform(x, x.numel(), d2d.R8, 2)
@@ -545,11 +551,8 @@ def bench_config(args, torch, d2d, w):
             err = max(err, r)
             del e, x, sp, y
             torch.cuda.empty_cache()
-        cfg = {"workload": "r2c+c2r FP64 n^3 PHYSICAL_IN_X, n in 100 120 200 240 300 320 360 384 "
-                           "400 480 500 576 600 640 768 800 896 960 1000 1152 1200 1280 1536 "
-                           "1600, "
-                           "uniform seed 2; one "
-                           "step = one round trip of each"}
+        cfg = {"workload": "r2c+c2r FP64 n^3 PHYSICAL_IN_X, n in %s, uniform seed 2; one step = "
+                           "one round trip of each" % " ".join(str(v) for v in pick)}
         extra["sizes"] = sizes
     elif w == "r2c2048":
         # the largest cube one MI355X holds in FP64: 2048^3 r2c+c2r, the
@@ -648,6 +651,8 @@ def main():
                          "trips of 24 mixed-radix cubes against the general kernel; r2c2048: "
                          "the 2048^3 FP64 round trip (206 GB on one GPU)")
     ap.add_argument("--batch-prec", type=int, choices=[64, 32], default=64)
+    ap.add_argument("--cubes", default="",
+                    help="sizes workload: comma-separated subset of its cube lengths")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the 3D FFT path on one GPU only; "
