@@ -76,6 +76,10 @@ std::vector<FastVariant> fast_variants_e() {
     make_variant<double, 1600, 2, 256, IO_C2C, true, R8_8_5_5>("double n=1600 tx=2 nt=256 c2c r8855"),
     make_variant<double, 1600, 2, 256, IO_R2C, true, R8_8_5_5>("double n=1600 tx=2 nt=256 r2c r8855"),
     make_variant<double, 1600, 2, 256, IO_C2R, true, R8_8_5_5>("double n=1600 tx=2 nt=256 c2r r8855"),
+    make_variant<double, 1600, 2, 256, IO_R2C, true, R8_8_5_5, 1, 4>(
+        "double n=1600 tx=2 nt=256 r2c xs w4 r8855"),
+    make_variant<double, 1600, 2, 256, IO_C2R, true, R8_8_5_5, 1, 4>(
+        "double n=1600 tx=2 nt=256 c2r xs w4 r8855"),
     D2D_SPLIT_C2C(double, 1600, 8, 512, false, R16_10_10, 4),
     D2D_VARIANTS(double, 1600, 4, 256, false, R16_10_10),
     D2D_VARIANTS(double, 1600, 2, 256, true, R16_10_10),
